@@ -1,0 +1,170 @@
+"""Benchmark of the MI355X Stokes hot path.
+
+One step = one Stokes solve (Uzawa to 1e-6 relative residual, reference
+settings) of the 3D random-viscosity workload (96^3 Q1-P0 elements, 16
+particles per cell, log10 eta in [-3, 3]) with all inputs resident in HBM.
+Prints one JSON line (see the "metric" field).  Extra objects:
+
+  roofline      the dominant kernel (the FP64 matrix-free K apply) timed with
+                HIP events on the solver's stream: algorithmic bytes per apply
+                / measured apply time, against the 8 TB/s HBM peak.
+  cpu_baseline  the CPU oracle (assembled CSR + reference Uzawa with the
+                PETSc-default GMRES(30)+ILU(0) inner solve) on a bounded
+                sample of the same generator, timed on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Stokes solve time at 1e-6 residual (learned-PC vs plain Uzawa); HBM GB/s %peak"
+HBM_PEAK_GBS = 8000.0
+
+
+def k_apply_bytes(dim, N, E, P, b=8):
+    """Algorithmic HBM bytes of one K apply: read u and write y (dim*N values
+    each), the particle stream (xi[dim] and omega = w detJ eta, padded to 4
+    values), the element offsets (int32)."""
+    return 2 * b * dim * N + b * 4 * P + 4 * (E + 1)
+
+
+def cpu_baseline(res, threads):
+    import oracle as O
+    from gale_amd import configs
+    cfg = configs.random_viscosity_3d(res=res, ppc=16)
+    pb = O.Problem(**cfg.oracle_kwargs())
+    up = O.uzawa_params(tolerance=1e-6, inner=1, nthreads=threads, inner_max_iterations=100000)
+    t0 = time.perf_counter()
+    st = pb.assemble(nthreads=threads)
+    t_asm = time.perf_counter() - t0
+    st, u, p, rep = pb.uzawa(up)
+    return dict(status=int(st), solve_s=rep.solve_seconds, assemble_s=t_asm, uzawa_iterations=rep.uzawa_iterations,
+                inner_iterations=int(rep.inner_iterations), E=cfg.E, P=cfg.P)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--res", type=int, default=96)
+    ap.add_argument("--ppc", type=int, default=16)
+    ap.add_argument("--apply-reps", type=int, default=20)
+    ap.add_argument("--cpu-res", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--max-outer", type=int, default=20000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding is a design in DESIGN.md, not built).", file=sys.stderr)
+        sys.exit(2)
+
+    import gale_amd as G
+    from gale_amd import configs
+
+    t = time.perf_counter()
+    cfg = configs.random_viscosity_3d(res=args.res, ppc=args.ppc)
+    t_gen = time.perf_counter() - t
+    print(f"# generated {cfg.name} res={args.res} E={cfg.E} P={cfg.P} in {t_gen:.1f}s", file=sys.stderr, flush=True)
+    pb = G.Problem(**cfg.problem_kwargs())
+    params = G.uzawa_params(tolerance=1e-6, maxIterations=args.max_outer, inner_max_iterations=200000)
+    s = G.Stokes_SLE_UzawaSolver(pb, params)
+    s.setup()  # first linear solve of the Picard sequence: no yielding yet
+
+    # roofline of the dominant kernel, timed with HIP events on the solver's stream
+    ms = s.time_apply_K(G.PREC_FP64, args.apply_reps)
+    nbytes = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)
+    achieved = nbytes / (ms * 1e-3) / 1e9
+    ms32 = s.time_apply_K(G.PREC_MIXED, args.apply_reps)
+    nbytes32 = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P, b=4)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_apply_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            tr = json.load(open(tpath))
+            if tr.get("res") == args.res and tr.get("ppc") == args.ppc:
+                traffic = tr.get("bytes_per_apply")
+        except Exception:
+            traffic = None
+    print(f"# K apply fp64 {ms:.4f} ms ({achieved:.0f} GB/s), fp32 {ms32:.4f} ms", file=sys.stderr, flush=True)
+
+    for w in range(args.warmup):
+        s.set_pressure(None)
+        u, p, rep = s.Solve()
+        print(f"# warmup {w}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations}",
+              file=sys.stderr, flush=True)
+    import torch
+    times, reps = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.set_pressure(None)
+        u, p, rep = s.Solve()
+        times.append(rep.solve_seconds)
+        reps.append(rep.as_dict())
+        print(f"# step {k}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations} "
+              f"rel={rep.rel_residual:.3e}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    solve_s = float(np.mean(times))
+    last = reps[-1]
+    n_u = pb.N * cfg.dim
+    dofs = n_u + cfg.E
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        t = time.perf_counter()
+        cb = cpu_baseline(args.cpu_res, threads)
+        print(f"# cpu baseline {cb} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
+        # the GPU on the same sample, for a like-for-like ratio
+        cfg_s = configs.random_viscosity_3d(res=args.cpu_res, ppc=16)
+        ps = G.Problem(**cfg_s.problem_kwargs())
+        ss = G.Stokes_SLE_UzawaSolver(ps, G.uzawa_params(tolerance=1e-6, inner_max_iterations=200000))
+        _, _, rs = ss.Solve()
+        cpu = {"value": cb["solve_s"], "unit": "s", "cores": threads, "kind": "port",
+               "sample": f"random_viscosity_3d at {args.cpu_res}^3 ({cb['E']} elements, {cb['P']} particles): "
+                         f"oracle assembled CSR + reference Uzawa, GMRES(30)+ILU(0) inner (PETSc defaults), "
+                         f"{cb['uzawa_iterations']} Uzawa its; assembly {cb['assemble_s']:.2f}s not included",
+               "gpu_same_sample_s": rs.solve_seconds,
+               "status": cb["status"]}
+    line = {
+        "metric": METRIC,
+        "value": solve_s,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / max(args.steps, 1) * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"random_viscosity_3d {args.res}^3 Q1-P0, {args.ppc} ppc, log10 eta GRF [-3,3] "
+                               f"+ plastic lid (first, linear solve), free slip; plain FP64 Uzawa, diagonal Q, "
+                               f"Jacobi-CG inner at 0.1 x tol",
+                   "elements": cfg.E, "particles": cfg.P, "dofs": dofs,
+                   "uzawa_iterations": last["uzawa_iterations"], "inner_iterations": last["inner_iterations"],
+                   "rel_residual": last["rel_residual"], "dof_per_s": dofs / solve_s,
+                   "setup_s": last["setup_seconds"], "k_applies": last["k_applies"],
+                   "k_apply_fp32_ms": ms32, "k_apply_fp32_gbs": nbytes32 / (ms32 * 1e-3) / 1e9},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "gs::k_apply_K3<double> (8 parity-colour launches + zero fill per apply)",
+                     "ms_per_apply": ms, "algorithmic_bytes": nbytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

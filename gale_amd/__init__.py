@@ -1,0 +1,513 @@
+"""gale_amd -- MI355X Q1-P0 PIC Stokes hot path behind the reference's
+Stokes_SLE / Stokes_SLE_UzawaSolver surface.
+
+The compute path is ``libgale_stokes.so`` (hand-written HIP for gfx950, C ABI
+declared in ``include/gale_stokes.h`` and ``include/gale_generate.h``).  This
+module binds that library with ctypes and mirrors the reference's component
+interface (XML parameter names, Solve / GetRhs / FormResidual, Picard) so that
+callers and tests read like the reference's own.  There is no CPU fallback:
+if the library or a GPU is missing, calls raise ``GaleError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgale_stokes.so")
+
+OK, E_NAN, E_MAXIT, E_ZERO_FORCE, E_STAGNATION, E_HIP, E_INVALID, E_ESCAPED, E_INNER_MAXIT, E_PICARD, E_NO_DEVICE = range(11)
+YIELD_NONE, YIELD_DRUCKER_PRAGER, YIELD_VON_MISES = 0, 1, 2
+PC_NONE, PC_DIAGONAL = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+SHAPE_EVERYWHERE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_BELOW_SINES = 0, 1, 2, 3
+WALLS = {"left": 0, "right": 1, "bottom": 2, "top": 3, "back": 4, "front": 5}
+
+# Exported entry points (must match include/gale_stokes.h and include/gale_generate.h).
+EXPORTS = (
+    "gs_strerror", "gs_version", "gs_device_count", "gs_problem_create", "gs_problem_destroy", "gs_problem_info",
+    "gs_problem_particles", "gs_solver_create", "gs_solver_set_params", "gs_solver_destroy", "gs_solver_setup",
+    "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
+    "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
+    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays",
+    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
+)
+
+
+class GaleError(RuntimeError):
+    def __init__(self, status: int, what: str = ""):
+        self.status = status
+        msg = _strerror(status)
+        super().__init__(f"{what}: {msg} (status {status})" if what else f"{msg} (status {status})")
+
+
+class Material(C.Structure):
+    _fields_ = [
+        ("eta0", C.c_double), ("density", C.c_double), ("yield_type", C.c_int32), ("pad_", C.c_int32),
+        ("cohesion", C.c_double), ("friction", C.c_double), ("min_yield_stress", C.c_double),
+        ("min_viscosity", C.c_double), ("max_strain_rate", C.c_double),
+    ]
+
+
+class ProblemDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("res", C.c_int32 * 3), ("lo", C.c_double * 3), ("hi", C.c_double * 3),
+        ("bc_mask", C.c_void_p), ("bc_value", C.c_void_p), ("gravity", C.c_double), ("ghat", C.c_double * 3),
+        ("eta_clamp_min", C.c_double), ("eta_clamp_max", C.c_double), ("n_materials", C.c_int32),
+        ("pad_", C.c_int32), ("materials", C.c_void_p), ("n_particles", C.c_int64), ("coord", C.c_void_p),
+        ("material", C.c_void_p), ("eta0_particle", C.c_void_p),
+    ]
+
+
+class UzawaParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double), ("max_iterations", C.c_int32), ("min_iterations", C.c_int32),
+        ("use_absolute_tolerance", C.c_int32), ("preconditioner", C.c_int32), ("precision", C.c_int32),
+        ("inner_max_iterations", C.c_int32), ("inner_rtol_factor", C.c_double),
+        ("inner_rtol_override", C.c_double), ("monitor", C.c_int32), ("check_every", C.c_int32),
+    ]
+
+
+class PicardParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double), ("max_iterations", C.c_int32), ("min_iterations", C.c_int32),
+        ("kill_non_convergent", C.c_int32), ("pad_", C.c_int32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("uzawa_iterations", C.c_int32), ("inner_iterations", C.c_int64),
+        ("rel_residual", C.c_double), ("abs_residual", C.c_double), ("div_u", C.c_double),
+        ("momentum_residual", C.c_double), ("solve_seconds", C.c_double), ("setup_seconds", C.c_double),
+        ("picard_iterations", C.c_int32), ("pad_", C.c_int32), ("picard_residual", C.c_double),
+        ("k_applies", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad_"}
+
+
+class Shape(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("material", C.c_int32), ("axis", C.c_int32), ("n_modes", C.c_int32),
+        ("p", C.c_double * 6), ("modes", C.c_double * 32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libgale_stokes.so (built in-tree by ``gale_amd.build_lib``)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GaleError(E_NO_DEVICE, f"{LIB_PATH} is not built (run python -m gale_amd.build_lib)")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, d = C.c_void_p, C.c_int32, C.c_int64, C.c_double
+        sig = {
+            "gs_strerror": (C.c_char_p, [i32]),
+            "gs_version": (i32, []),
+            "gs_device_count": (i32, []),
+            "gs_problem_create": (i32, [vp, vp]),
+            "gs_problem_destroy": (None, [vp]),
+            "gs_problem_info": (i32, [vp, vp]),
+            "gs_problem_particles": (i32, [vp, vp, vp, vp, vp]),
+            "gs_solver_create": (i32, [vp, vp, vp]),
+            "gs_solver_set_params": (i32, [vp, vp]),
+            "gs_solver_destroy": (None, [vp]),
+            "gs_solver_setup": (i32, [vp, vp, vp, i32]),
+            "gs_stokes_solve": (i32, [vp, vp, vp, vp]),
+            "gs_solver_set_pressure": (i32, [vp, vp]),
+            "gs_picard_solve": (i32, [vp, vp, vp, vp, vp]),
+            "gs_get_rhs": (i32, [vp, vp]),
+            "gs_form_residual": (i32, [vp, vp, vp]),
+            "gs_compute_dt": (i32, [vp, d, vp]),
+            "gs_advect_rk2": (i32, [vp, d]),
+            "gs_relocate": (i32, [vp, vp]),
+            "gs_problem_set_coords": (i32, [vp, vp]),
+            "gs_apply_K": (i32, [vp, vp, vp]),
+            "gs_apply_K_fp32": (i32, [vp, vp, vp]),
+            "gs_apply_G": (i32, [vp, vp, vp]),
+            "gs_apply_Gt": (i32, [vp, vp, vp]),
+            "gs_solver_sync": (i32, [vp]),
+            "gs_time_apply_K": (i32, [vp, i32, i32, vp]),
+            "gs_solver_setup_arrays": (i32, [vp, vp, vp, vp, vp, vp]),
+            "gs_rng_stream": (d, [i64, i64, vp]),
+            "gs_layout_random": (i64, [i32, vp, vp, vp, i32, i64, vp]),
+            "gs_layout_gauss": (i64, [i32, vp, vp, vp, vp]),
+            "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
+            "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
+            "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _strerror(status: int) -> str:
+    try:
+        return lib().gs_strerror(status).decode()
+    except Exception:  # library missing
+        return "gale_stokes library unavailable"
+
+
+def _check(status: int, what: str = "", allow=()):
+    if status != OK and status not in allow:
+        raise GaleError(status, what)
+    return status
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _dims(dim, vals, fill):
+    v = list(vals)[:dim] + [fill] * (3 - min(dim, len(vals)))
+    return v[:3]
+
+
+def device_count() -> int:
+    return int(lib().gs_device_count())
+
+
+# ---------------------------------------------------------------------------
+# generation helpers (gale_generate.h)
+# ---------------------------------------------------------------------------
+def rng_stream(seed: int, count: int) -> np.ndarray:
+    out = np.zeros(count)
+    lib().gs_rng_stream(seed, count, _ptr(out))
+    return out
+
+
+def layout_random(dim, res, lo, hi, ppc, seed=13) -> np.ndarray:
+    r = np.asarray(_dims(dim, res, 1), np.int32)
+    l, h = np.asarray(_dims(dim, lo, 0.0)), np.asarray(_dims(dim, hi, 1.0))
+    E = int(np.prod(r[:dim]))
+    coord = np.zeros((E * ppc, 3))
+    n = lib().gs_layout_random(dim, _ptr(r), _ptr(l), _ptr(h), ppc, seed, _ptr(coord))
+    if n < 0:
+        raise GaleError(E_INVALID, "layout_random")
+    return coord
+
+
+def layout_gauss(dim, res, lo, hi) -> np.ndarray:
+    r = np.asarray(_dims(dim, res, 1), np.int32)
+    l, h = np.asarray(_dims(dim, lo, 0.0)), np.asarray(_dims(dim, hi, 1.0))
+    E = int(np.prod(r[:dim]))
+    coord = np.zeros((E * 2 ** dim, 3))
+    lib().gs_layout_gauss(dim, _ptr(r), _ptr(l), _ptr(h), _ptr(coord))
+    return coord
+
+
+def num_nodes(dim, res) -> int:
+    return int(np.prod([r + 1 for r in list(res)[:dim]]))
+
+
+def wall_bc(dim, res, walls, mask=None, values=None):
+    """walls: iterable of (wall name, component, value) in WallVC terms."""
+    r = np.asarray(_dims(dim, res, 1), np.int32)
+    N = num_nodes(dim, res)
+    if mask is None:
+        mask = np.zeros(N * dim, np.uint8)
+    if values is None:
+        values = np.zeros(N * dim)
+    for wall, comp, val in walls:
+        _check(lib().gs_wall_bc(dim, _ptr(r), WALLS[wall], comp, float(val), _ptr(mask), _ptr(values)), "wall_bc")
+    return mask, values
+
+
+def free_slip(dim, res):
+    walls = [("left", 0, 0.0), ("right", 0, 0.0), ("bottom", 1, 0.0), ("top", 1, 0.0)]
+    if dim == 3:
+        walls += [("back", 2, 0.0), ("front", 2, 0.0)]
+    return wall_bc(dim, res, walls)
+
+
+def sphere(material, centre, radius):
+    s = Shape()
+    s.type, s.material = SHAPE_SPHERE, material
+    for i, c in enumerate(list(centre)[:3]):
+        s.p[i] = c
+    s.p[3] = radius
+    return s
+
+
+def box(material, start, end):
+    s = Shape()
+    s.type, s.material = SHAPE_BOX, material
+    for i in range(min(3, len(start))):
+        s.p[i] = start[i]
+        s.p[3 + i] = end[i]
+    return s
+
+
+def everywhere(material):
+    s = Shape()
+    s.type, s.material = SHAPE_EVERYWHERE, material
+    return s
+
+
+def below_sines(material, axis, level, modes=()):
+    """modes: (amplitude, k_a, k_b, phase) for level + sum a sin(2 pi (k_a a + k_b b) + phase)."""
+    s = Shape()
+    s.type, s.material, s.axis = SHAPE_BELOW_SINES, material, axis
+    s.p[0] = level
+    s.n_modes = len(modes)
+    for k, m in enumerate(modes[:8]):
+        for j in range(4):
+            s.modes[4 * k + j] = m[j]
+    return s
+
+
+def shape_layout(dim, coord, shapes, default=0) -> np.ndarray:
+    coord = _f64(coord).reshape(-1, 3)
+    mat = np.full(coord.shape[0], default, np.int32)
+    arr = (Shape * len(shapes))(*shapes)
+    _check(lib().gs_shape_layout(dim, coord.shape[0], _ptr(coord), len(shapes), C.addressof(arr), _ptr(mat)),
+           "shape_layout")
+    return mat
+
+
+def gaussian_field(dim, nodes, spacing, corr, seed, vmin, vmax) -> np.ndarray:
+    n = np.asarray(_dims(dim, nodes, 1), np.int32)
+    sp = np.asarray(_dims(dim, spacing, 1.0))
+    out = np.zeros(int(np.prod(n[:dim])))
+    _check(lib().gs_gaussian_field(dim, _ptr(n), _ptr(sp), corr, seed, vmin, vmax, _ptr(out)), "gaussian_field")
+    return out
+
+
+# ---------------------------------------------------------------------------
+# problem + solver (gale_stokes.h)
+# ---------------------------------------------------------------------------
+def material(eta0=1.0, density=0.0, yield_type=YIELD_NONE, cohesion=0.0, friction=0.0, min_yield_stress=0.0,
+             min_viscosity=0.0, max_strain_rate=0.0) -> dict:
+    return dict(eta0=eta0, density=density, yield_type=yield_type, cohesion=cohesion, friction=friction,
+                min_yield_stress=min_yield_stress, min_viscosity=min_viscosity, max_strain_rate=max_strain_rate)
+
+
+class Problem:
+    """FeMesh + MaterialPointsSwarm + IntegrationPointsSwarm on the GPU."""
+
+    def __init__(self, dim, res, lo, hi, coord, material_index, materials, bc_mask=None, bc_value=None,
+                 gravity=1.0, ghat=(0.0, -1.0, 0.0), eta_clamp=(0.0, 0.0), eta0_particle=None):
+        self.dim = dim
+        self.res = tuple(list(res)[:dim])
+        self.lo, self.hi = tuple(list(lo)[:dim]), tuple(list(hi)[:dim])
+        self.N = num_nodes(dim, res)
+        self.E = int(np.prod(self.res))
+        d = ProblemDesc()
+        d.dim = dim
+        for i, v in enumerate(_dims(dim, res, 1)):
+            d.res[i] = v
+        for i, v in enumerate(_dims(dim, lo, 0.0)):
+            d.lo[i] = v
+        for i, v in enumerate(_dims(dim, hi, 1.0)):
+            d.hi[i] = v
+        for i in range(3):
+            d.ghat[i] = ghat[i]
+        self._keep = []
+        if bc_mask is not None:
+            bc_mask = np.ascontiguousarray(bc_mask, np.uint8).reshape(-1)
+            if bc_mask.size != self.N * dim:
+                raise GaleError(E_INVALID, "bc_mask size")
+            self._keep.append(bc_mask)
+            d.bc_mask = _ptr(bc_mask)
+        if bc_value is not None:
+            bc_value = _f64(bc_value).reshape(-1)
+            self._keep.append(bc_value)
+            d.bc_value = _ptr(bc_value)
+        d.gravity = gravity
+        d.eta_clamp_min, d.eta_clamp_max = eta_clamp
+        mats = (Material * len(materials))()
+        for i, m in enumerate(materials):
+            mats[i] = Material(m.get("eta0", 1.0), m.get("density", 0.0), m.get("yield_type", 0), 0,
+                               m.get("cohesion", 0.0), m.get("friction", 0.0), m.get("min_yield_stress", 0.0),
+                               m.get("min_viscosity", 0.0), m.get("max_strain_rate", 0.0))
+        self._keep.append(mats)
+        d.n_materials = len(materials)
+        d.materials = C.addressof(mats)
+        coord = _f64(coord).reshape(-1, 3)
+        mi = np.ascontiguousarray(material_index, np.int32)
+        self._keep += [coord, mi]
+        d.n_particles = coord.shape[0]
+        d.coord = _ptr(coord)
+        d.material = _ptr(mi)
+        if eta0_particle is not None:
+            e0 = _f64(eta0_particle)
+            self._keep.append(e0)
+            d.eta0_particle = _ptr(e0)
+        self.P = coord.shape[0]
+        h = C.c_void_p()
+        st = lib().gs_problem_create(C.byref(d), C.byref(h))
+        self._h = h
+        self.create_status = st
+        if not h.value:
+            raise GaleError(st, "gs_problem_create")
+        self._keep = []
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gs_problem_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        out = np.zeros(5, np.int64)
+        _check(lib().gs_problem_info(self._h, _ptr(out)))
+        return dict(N=int(out[0]), E=int(out[1]), P=int(out[2]), dim=int(out[3]), escaped=int(out[4]))
+
+    def particles(self):
+        coord = np.zeros((self.P, 3))
+        owner = np.zeros(self.P, np.int32)
+        xi = np.zeros((self.P, self.dim))
+        w = np.zeros(self.P)
+        _check(lib().gs_problem_particles(self._h, _ptr(coord), _ptr(owner), _ptr(xi), _ptr(w)), "particles")
+        return coord, owner, xi, w
+
+    def relocate(self):
+        owner = np.zeros(self.P, np.int32)
+        st = lib().gs_relocate(self._h, _ptr(owner))
+        _check(st, "relocate", allow=(E_ESCAPED,))
+        return st, owner
+
+    def set_coords(self, coord):
+        c = _f64(coord).reshape(-1, 3)
+        st = lib().gs_problem_set_coords(self._h, _ptr(c))
+        _check(st, "set_coords", allow=(E_ESCAPED,))
+        return st
+
+
+def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
+                 Preconditioner=True, precision=PREC_FP64, inner_max_iterations=0, inner_rtol_factor=0.1,
+                 inner_rtol_override=0.0, monitor=False, check_every=4) -> UzawaParams:
+    """Stokes_SLE_UzawaSolver parameters under their XML names
+    (Stokes_SLE_UzawaSolver.cxx:251-257 defaults)."""
+    return UzawaParams(tolerance, maxIterations, minIterations, int(bool(useAbsoluteTolerance)),
+                       PC_DIAGONAL if Preconditioner else PC_NONE, precision, inner_max_iterations,
+                       inner_rtol_factor, inner_rtol_override, int(bool(monitor)), check_every)
+
+
+class Stokes_SLE_UzawaSolver:
+    """The reference component (Stokes_SLE + Stokes_SLE_UzawaSolver) on the GPU.
+
+    ``Solve`` = _Stokes_SLE_UzawaSolver_Solve, ``GetRhs`` / ``FormResidual``
+    the SLE_Solver virtuals, ``NonLinearExecute`` the Picard loop.  Failures
+    raise GaleError with the status that mirrors the reference firewall.
+    """
+
+    def __init__(self, problem: Problem, params: UzawaParams | None = None, **xml):
+        self.problem = problem
+        self.params = params or uzawa_params(**xml)
+        h = C.c_void_p()
+        _check(lib().gs_solver_create(problem._h, C.byref(self.params), C.byref(h)), "gs_solver_create")
+        self._h = h
+        self.n = problem.N * problem.dim
+        self.np = problem.E
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gs_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: UzawaParams):
+        self.params = params
+        _check(lib().gs_solver_set_params(self._h, C.byref(params)))
+
+    def setup(self, u_prev=None, p_prev=None, yield_on=False):
+        u_prev, p_prev = _f64(u_prev), _f64(p_prev)
+        _check(lib().gs_solver_setup(self._h, _ptr(u_prev), _ptr(p_prev), int(bool(yield_on))), "setup")
+
+    def Solve(self, raise_on_error=True):
+        u = np.zeros(self.n)
+        p = np.zeros(self.np)
+        rep = Report()
+        st = lib().gs_stokes_solve(self._h, _ptr(u), _ptr(p), C.byref(rep))
+        if raise_on_error:
+            _check(st, "Solve")
+        return u, p, rep
+
+    def set_pressure(self, p=None):
+        p = _f64(p)
+        _check(lib().gs_solver_set_pressure(self._h, _ptr(p)))
+
+    def NonLinearExecute(self, nonLinearTolerance=0.01, nonLinearMaxIterations=500, nonLinearMinIterations=1,
+                         killNonConvergent=False, raise_on_error=True):
+        pp = PicardParams(nonLinearTolerance, nonLinearMaxIterations, nonLinearMinIterations,
+                          int(bool(killNonConvergent)), 0)
+        u = np.zeros(self.n)
+        p = np.zeros(self.np)
+        rep = Report()
+        st = lib().gs_picard_solve(self._h, C.byref(pp), _ptr(u), _ptr(p), C.byref(rep))
+        if raise_on_error:
+            _check(st, "NonLinearExecute")
+        return u, p, rep
+
+    def GetRhs(self):
+        out = np.zeros(self.np)
+        _check(lib().gs_get_rhs(self._h, _ptr(out)), "GetRhs")
+        return out
+
+    def FormResidual(self, p):
+        p = _f64(p)
+        out = np.zeros(self.np)
+        _check(lib().gs_form_residual(self._h, _ptr(p), _ptr(out)), "FormResidual")
+        return out
+
+    def compute_dt(self, dtFactor=1.0):
+        dt = C.c_double()
+        _check(lib().gs_compute_dt(self._h, dtFactor, C.byref(dt)), "compute_dt")
+        return dt.value
+
+    def advect_rk2(self, dt):
+        st = lib().gs_advect_rk2(self._h, dt)
+        _check(st, "advect_rk2", allow=(E_ESCAPED,))
+        return st
+
+    def setup_arrays(self):
+        P = self.problem.P
+        dg, Q, f, h, eta = np.zeros(self.n), np.zeros(self.np), np.zeros(self.n), np.zeros(self.np), np.zeros(P)
+        _check(lib().gs_solver_setup_arrays(self._h, _ptr(dg), _ptr(Q), _ptr(f), _ptr(h), _ptr(eta)))
+        return dict(diagK=dg, Q=Q, f=f, h=h, eta=eta)
+
+    # device-pointer operator applies (torch tensors on cuda)
+    def apply_K(self, x, y):
+        _check(lib().gs_apply_K(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "apply_K")
+
+    def apply_K_fp32(self, x, y):
+        _check(lib().gs_apply_K_fp32(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "apply_K_fp32")
+
+    def apply_G(self, p, y):
+        _check(lib().gs_apply_G(self._h, C.c_void_p(p.data_ptr()), C.c_void_p(y.data_ptr())), "apply_G")
+
+    def apply_Gt(self, x, q):
+        _check(lib().gs_apply_Gt(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(q.data_ptr())), "apply_Gt")
+
+    def sync(self):
+        _check(lib().gs_solver_sync(self._h))
+
+    def time_apply_K(self, precision=PREC_FP64, n_rep=20):
+        ms = C.c_double()
+        _check(lib().gs_time_apply_K(self._h, precision, n_rep, C.byref(ms)), "time_apply_K")
+        return ms.value

@@ -1,0 +1,960 @@
+// gs_kernels.hip -- CDNA4 (gfx950) kernels of the Q1-P0 PIC Stokes hot path.
+//
+// Element kernels run one thread per element.  Elements are processed in
+// 2^dim parity colours so that no two threads of a launch share a node: the
+// nodal scatter is a plain read-modify-write, deterministic and free of
+// atomics.  Within a colour the block index is remapped so that each of the
+// eight XCDs (blocks are dealt round-robin to XCDs) works on one contiguous
+// range of elements and its L2 sees the neighbouring nodes.
+//
+// K apply (reference: K_e = sum_p w_p detJ B_p^T D_p B_p assembled by
+// ConstitutiveMatrixCartesian.cxx:363-485 and applied by PETSc MatMult) is
+// evaluated matrix-free: the element's nodal velocities are turned into the
+// coefficients of the (bi/tri)linear velocity gradient by a Walsh-Hadamard
+// butterfly, each material point adds its stress times the monomials
+// {1, r, s, t, rs, rt, st} to 30 (3D) / 7 (2D) element moments, and the
+// nodal forces are read back from the moments.  Each particle costs about 70
+// FMA-class operations and 32 bytes of stream.
+#include "gs_internal.h"
+
+#include <math.h>
+
+namespace gs {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long xcd_block(long long b, long long nb) {
+  const long long q = nb / 8, rem = nb % 8;
+  const long long x = b % 8, i = b / 8;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
+// CartesianGenerator.cxx:2174-2215 vertex coordinate (contactDepth 0).
+__device__ __forceinline__ double vcoord(const Geo& g, int d, long long idx) {
+#pragma clang fp contract(off)
+  if (idx <= 0) return g.lo[d];
+  if (idx >= g.res[d]) return g.hi[d];
+  return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
+}
+
+__device__ __forceinline__ int num_approx0(double v) { return v >= -1e-12 && v <= 1e-12; }
+
+// Mesh_RegularAlgorithms.cxx:191-224, bit for bit: per axis
+// index = (unsigned)modf((x-min)/sep); a point within 1e-12 of a face goes to
+// the lower element; within 1e-12 of max goes to the last element.
+__device__ __forceinline__ int search_element(const Geo& g, const double* x, long long* el) {
+#pragma clang fp contract(off)
+  long long ind[3] = {0, 0, 0};
+  for (int d = 0; d < g.dim; d++) {
+    const double sep = (g.hi[d] - g.lo[d]) / (double)g.res[d];
+    if (num_approx0(x[d] - g.hi[d])) {
+      ind[d] = g.res[d] - 1;
+    } else if (x[d] < g.lo[d] || x[d] > g.hi[d]) {
+      return 0;
+    } else {
+      double integer;
+      const double out = (x[d] - g.lo[d]) / sep;
+      const double frac = modf(out, &integer);
+      ind[d] = (long long)(unsigned)integer;
+      if (ind[d] > 0 && num_approx0(frac)) ind[d]--;
+    }
+  }
+  *el = ind[0] + (long long)g.res[0] * (ind[1] + (long long)g.res[1] * ind[2]);
+  return 1;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  return v;
+}
+
+// Block reduction of NV values followed by a deterministic cross-block pass
+// in the last block to finish (partials are summed in block order).  Returns
+// true in the last block, with the totals in tot[] for every thread.
+template <int NV, bool MAX>
+__device__ bool reduce_last(double (&v)[NV], Reducer rd, double (&tot)[NV]) {
+  __shared__ double sh[NV][kBlock / 64];
+  __shared__ int last;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; k++) v[k] = MAX ? wave_max(v[k]) : wave_sum(v[k]);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double s = sh[k][0];
+      for (int w = 1; w < nw; w++) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+      __hip_atomic_store(&rd.partials[blockIdx.x * NV + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    const unsigned prev = atomicAdd(rd.counter, 1u);
+    last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!last) return false;
+  __threadfence();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = MAX ? -INFINITY : 0.0;
+    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+      const double pv = __hip_atomic_load(&rd.partials[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s = MAX ? fmax(s, pv) : s + pv;
+    }
+    v[k] = MAX ? wave_max(s) : wave_sum(s);
+  }
+  __syncthreads();
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NV; k++) sh[k][wid] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; k++) {
+    double s = sh[k][0];
+    for (int w = 1; w < nw; w++) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+    tot[k] = s;
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(rd.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
+}
+
+static inline int reduce_grid(long long n) {
+  long long b = (n + kBlock - 1) / kBlock;
+  if (b > kMaxReduceBlocks) b = kMaxReduceBlocks;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+static inline unsigned grid_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// Element (i,j,k) of parity colour c from thread t.
+__device__ __forceinline__ void colour_elem(const Geo& g, int c, long long t, long long* ijk) {
+  const long long mx = (g.res[0] - (c & 1) + 1) / 2;
+  const long long my = g.dim > 1 ? (g.res[1] - ((c >> 1) & 1) + 1) / 2 : 1;
+  const long long a = t % mx, rem = t / mx;
+  const long long b = rem % my, cc = rem / my;
+  ijk[0] = 2 * a + (c & 1);
+  ijk[1] = 2 * b + ((c >> 1) & 1);
+  ijk[2] = 2 * cc + ((c >> 2) & 1);
+}
+
+// ---------------------------------------------------------------------------
+// fills and vector kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(T* __restrict__ y, long long n, T v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+void launch_fill(double* y, long long n, double v, hipStream_t st) {
+  if (n <= 0) return;
+  unsigned gb = grid_for(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_fill<double>, dim3(gb), dim3(kBlock), 0, st, y, n, v);
+}
+void launch_fill_f(float* y, long long n, float v, hipStream_t st) {
+  if (n <= 0) return;
+  unsigned gb = grid_for(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_fill<float>, dim3(gb), dim3(kBlock), 0, st, y, n, v);
+}
+
+__global__ void k_masked_copy(const double* __restrict__ src, double* __restrict__ dst,
+                              const uint8_t* __restrict__ bc, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = bc[i] ? 0.0 : src[i];
+}
+void launch_masked_copy(const double* src, double* dst, const uint8_t* bcdof, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_masked_copy, dim3(grid_for(n)), dim3(kBlock), 0, st, src, dst, bcdof, n);
+}
+
+// f <- mask(f - K u_bc)   (StiffnessMatrix.cxx:2092-2117 lifting)
+__global__ void k_lift(double* __restrict__ f, const double* __restrict__ Ku, const uint8_t* __restrict__ bc,
+                       long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    f[i] = bc[i] ? 0.0 : f[i] - Ku[i];
+}
+void launch_lift(double* f, const double* Kubc, const uint8_t* bcdof, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_lift, dim3(grid_for(n)), dim3(kBlock), 0, st, f, Kubc, bcdof, n);
+}
+
+__global__ void k_make_dinv(const double* __restrict__ d, const uint8_t* __restrict__ bc, double* __restrict__ di,
+                            long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    di[i] = bc[i] ? 0.0 : 1.0 / d[i];
+}
+void launch_make_dinv(const double* diag, const uint8_t* bcdof, double* dinv, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_make_dinv, dim3(grid_for(n)), dim3(kBlock), 0, st, diag, bcdof, dinv, n);
+}
+
+__global__ void k_recip(const double* __restrict__ a, double* __restrict__ b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = 1.0 / a[i];
+}
+void launch_recip(const double* a, double* b, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_recip, dim3(grid_for(n)), dim3(kBlock), 0, st, a, b, n);
+}
+
+__global__ void k_scale(double* __restrict__ y, const double* __restrict__ x, double a, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = a * x[i];
+}
+void launch_scale(double* y, const double* x, double a, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, a, n);
+}
+
+__global__ void k_axpby(double* __restrict__ y, const double* __restrict__ x, double a, double b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = a * x[i] + b * y[i];
+}
+void launch_axpby(double* y, const double* x, double a, double b, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kBlock), 0, st, y, x, a, b, n);
+}
+
+__global__ void k_add(double* __restrict__ y, const double* __restrict__ a, const double* __restrict__ b,
+                      long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = a[i] + b[i];
+}
+void launch_add(double* y, const double* a, const double* b, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, st, y, a, b, n);
+}
+
+__global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long long n, double* S, int slot,
+                      Reducer rd) {
+  double v[1] = {0.0}, tot[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    v[0] += a[i] * b[i];
+  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) S[slot] = tot[0];
+}
+void launch_dot(const double* a, const double* b, long long n, double* S, int slot, Reducer r, hipStream_t st) {
+  hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kBlock), 0, st, a, b, n, S, slot, r);
+}
+
+__global__ void k_vmax(const double* __restrict__ u, long long nnode, int dim, double* S, int slot, Reducer rd) {
+  double v[1] = {0.0}, tot[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nnode;
+       i += (long long)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int d = 0; d < dim; d++) s += u[i * dim + d] * u[i * dim + d];
+    v[0] = fmax(v[0], sqrt(s));
+  }
+  if (reduce_last<1, true>(v, rd, tot) && threadIdx.x == 0) S[slot] = tot[0];
+}
+void launch_vmax(const double* u, long long nnode, int dim, double* S, int slot, Reducer rd, hipStream_t st) {
+  hipLaunchKernelGGL(k_vmax, dim3(reduce_grid(nnode)), dim3(kBlock), 0, st, u, nnode, dim, S, slot, rd);
+}
+
+// ---------------------------------------------------------------------------
+// Inner velocity CG with Jacobi preconditioner.  Scalars stay on the device.
+// ---------------------------------------------------------------------------
+__global__ void k_cg_init(const double* __restrict__ b, const double* __restrict__ dinv, double* __restrict__ x,
+                          double* __restrict__ r, double* __restrict__ p, long long n, double* S, Reducer rd) {
+  double v[2] = {0.0, 0.0}, tot[2];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double bi = b[i], zi = dinv[i] * bi;
+    x[i] = 0.0;
+    r[i] = bi;
+    p[i] = zi;
+    v[0] += bi * zi;
+    v[1] += bi * bi;
+  }
+  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
+    S[S_RZ] = tot[0];
+    S[S_BB] = tot[1];
+    S[S_RR] = tot[1];
+  }
+}
+void launch_cg_init(const double* b, const double* dinv, double* x, double* r, double* p, long long n, double* S,
+                    Reducer rd, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_init, dim3(reduce_grid(n)), dim3(kBlock), 0, st, b, dinv, x, r, p, n, S, rd);
+}
+
+__global__ void k_cg_update(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                            const double* __restrict__ Ap, const double* __restrict__ dinv, long long n, double* S,
+                            Reducer rd) {
+  const double rz = S[S_RZ], pap = S[S_PAP];
+  const double alpha = (pap != 0.0 && isfinite(pap)) ? rz / pap : 0.0;
+  double v[2] = {0.0, 0.0}, tot[2];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    v[0] += ri * (dinv[i] * ri);
+    v[1] += ri * ri;
+  }
+  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
+    S[S_BETA] = rz != 0.0 ? tot[0] / rz : 0.0;
+    S[S_RZ] = tot[0];
+    S[S_RR] = tot[1];
+  }
+}
+void launch_cg_update(double* x, double* r, const double* p, const double* Ap, const double* dinv, long long n,
+                      double* S, Reducer rd, hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, x, r, p, Ap, dinv, n, S, rd);
+}
+
+__global__ void k_cg_pupdate(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ dinv,
+                             long long n, const double* S) {
+  const double beta = S[S_BETA];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = dinv[i] * r[i] + beta * p[i];
+}
+void launch_cg_pupdate(double* p, const double* r, const double* dinv, long long n, const double* S,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_cg_pupdate, dim3(grid_for(n)), dim3(kBlock), 0, st, p, r, dinv, n, S);
+}
+
+// ---------------------------------------------------------------------------
+// Uzawa outer loop vector steps (Stokes_SLE_UzawaSolver.cxx:664-837).
+// ---------------------------------------------------------------------------
+__global__ void k_uz_precond(const double* __restrict__ r, const double* __restrict__ Qinv, double* __restrict__ z,
+                             long long n, double* S, Reducer rd) {
+  double v[1] = {0.0}, tot[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double zi = Qinv ? r[i] * Qinv[i] : r[i];
+    z[i] = zi;
+    v[0] += zi * r[i];
+  }
+  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
+    const double prev = S[S_ZR_PREV];
+    S[S_UBETA] = prev != 0.0 ? tot[0] / prev : 0.0;
+    S[S_ZR] = tot[0];
+  }
+}
+void launch_uz_precond(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_uz_precond, dim3(reduce_grid(n)), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+}
+
+__global__ void k_uz_sdir(double* __restrict__ s, const double* __restrict__ z, long long n, const double* S,
+                          int first) {
+  const double beta = first ? 0.0 : S[S_UBETA];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    s[i] = first ? z[i] : z[i] + beta * s[i];
+}
+void launch_uz_sdir(double* s, const double* z, long long n, const double* S, int first, hipStream_t st) {
+  hipLaunchKernelGGL(k_uz_sdir, dim3(grid_for(n)), dim3(kBlock), 0, st, s, z, n, S, first);
+}
+
+__global__ void k_uz_update(double* __restrict__ p, const double* __restrict__ s, double* __restrict__ r,
+                            const double* __restrict__ q, long long n, double* S, Reducer rd) {
+  const double alpha = S[S_ZR] / S[S_SQ];
+  double v[1] = {0.0}, tot[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    p[i] += alpha * s[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+  }
+  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
+    S[S_ORR] = tot[0];
+    S[S_ZR_PREV] = S[S_ZR];
+    S[S_TMP0] = alpha;
+  }
+}
+void launch_uz_update(double* p, const double* s, double* r, const double* q, long long n, double* S, Reducer rd,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(k_uz_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, p, s, r, q, n, S, rd);
+}
+
+__global__ void k_uz_uupdate(double* __restrict__ u, const double* __restrict__ v, long long n, const double* S) {
+  const double alpha = S[S_TMP0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    u[i] -= alpha * v[i];
+}
+void launch_uz_uupdate(double* u, const double* v, long long n, const double* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_uz_uupdate, dim3(grid_for(n)), dim3(kBlock), 0, st, u, v, n, S);
+}
+
+// ---------------------------------------------------------------------------
+// K apply
+// ---------------------------------------------------------------------------
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+  using type = double4;
+};
+template <>
+struct Vec4<float> {
+  using type = float4;
+};
+
+// 3D: one thread per element of colour c.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mcount, const T* __restrict__ x,
+                                                     T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
+                                                     const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
+  const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
+  const T ix = (T)(0.25 / (vcoord(g, 0, ijk[0] + 1) - vcoord(g, 0, ijk[0])));
+  const T iy = (T)(0.25 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
+  const T iz = (T)(0.25 / (vcoord(g, 2, ijk[2] + 1) - vcoord(g, 2, ijk[2])));
+  const long long sx = 1, sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+  long long nid[8];
+#pragma unroll
+  for (int a = 0; a < 8; a++) nid[a] = n0 + (a & 1) * sx + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
+  // gradient coefficients: L[c][0] = A[c] . (1,s,t,st), L[c][1] = B[c] . (1,r,t,rt), L[c][2] = C[c] . (1,r,s,rs)
+  T A[3][4], B[3][4], C[3][4];
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) {
+    T v[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) v[a] = x[nid[a] * 3 + cc];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int bit = 1 << d;
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+        if (!(a & bit)) {
+          const T lo = v[a], hi = v[a | bit];
+          v[a] = lo + hi;
+          v[a | bit] = hi - lo;
+        }
+    }
+    A[cc][0] = ix * v[1]; A[cc][1] = ix * v[3]; A[cc][2] = ix * v[5]; A[cc][3] = ix * v[7];
+    B[cc][0] = iy * v[2]; B[cc][1] = iy * v[3]; B[cc][2] = iy * v[6]; B[cc][3] = iy * v[7];
+    C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
+  }
+  // moments
+  T mxx0 = 0, mxxs = 0, mxxt = 0, mxxst = 0;
+  T myy0 = 0, myyr = 0, myyt = 0, myyrt = 0;
+  T mzz0 = 0, mzzr = 0, mzzs = 0, mzzrs = 0;
+  T txy0 = 0, txyr = 0, txys = 0, txyt = 0, txyrt = 0, txyst = 0;
+  T txz0 = 0, txzr = 0, txzs = 0, txzt = 0, txzrs = 0, txzst = 0;
+  T tyz0 = 0, tyzr = 0, tyzs = 0, tyzt = 0, tyzrs = 0, tyzrt = 0;
+  const int pb = off[e], pe = off[e + 1];
+  for (int p = pb; p < pe; p++) {
+    const typename Vec4<T>::type q = pk[p];
+    const T r = q.x, s = q.y, tt = q.z, w = q.w;
+    const T st = s * tt, rt = r * tt, rs = r * s;
+    const T uxx = A[0][0] + A[0][1] * s + A[0][2] * tt + A[0][3] * st;
+    const T uxy = B[0][0] + B[0][1] * r + B[0][2] * tt + B[0][3] * rt;
+    const T uxz = C[0][0] + C[0][1] * r + C[0][2] * s + C[0][3] * rs;
+    const T uyx = A[1][0] + A[1][1] * s + A[1][2] * tt + A[1][3] * st;
+    const T uyy = B[1][0] + B[1][1] * r + B[1][2] * tt + B[1][3] * rt;
+    const T uyz = C[1][0] + C[1][1] * r + C[1][2] * s + C[1][3] * rs;
+    const T uzx = A[2][0] + A[2][1] * s + A[2][2] * tt + A[2][3] * st;
+    const T uzy = B[2][0] + B[2][1] * r + B[2][2] * tt + B[2][3] * rt;
+    const T uzz = C[2][0] + C[2][1] * r + C[2][2] * s + C[2][3] * rs;
+    const T w2 = w + w;
+    const T sxx = w2 * uxx, syy = w2 * uyy, szz = w2 * uzz;
+    const T sxy = w * (uxy + uyx), sxz = w * (uxz + uzx), syz = w * (uyz + uzy);
+    mxx0 += sxx; mxxs += sxx * s; mxxt += sxx * tt; mxxst += sxx * st;
+    myy0 += syy; myyr += syy * r; myyt += syy * tt; myyrt += syy * rt;
+    mzz0 += szz; mzzr += szz * r; mzzs += szz * s; mzzrs += szz * rs;
+    txy0 += sxy; txyr += sxy * r; txys += sxy * s; txyt += sxy * tt; txyrt += sxy * rt; txyst += sxy * st;
+    txz0 += sxz; txzr += sxz * r; txzs += sxz * s; txzt += sxz * tt; txzrs += sxz * rs; txzst += sxz * st;
+    tyz0 += syz; tyzr += syz * r; tyzs += syz * s; tyzt += syz * tt; tyzrs += syz * rs; tyzrt += syz * rt;
+  }
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
+    const T yx = ix * ax * (mxx0 + ay * mxxs + az * mxxt + ay * az * mxxst) +
+                 iy * ay * (txy0 + ax * txyr + az * txyt + ax * az * txyrt) +
+                 iz * az * (txz0 + ax * txzr + ay * txzs + ax * ay * txzrs);
+    const T yy = iy * ay * (myy0 + ax * myyr + az * myyt + ax * az * myyrt) +
+                 ix * ax * (txy0 + ay * txys + az * txyt + ay * az * txyst) +
+                 iz * az * (tyz0 + ax * tyzr + ay * tyzs + ax * ay * tyzrs);
+    const T yz = iz * az * (mzz0 + ax * mzzr + ay * mzzs + ax * ay * mzzrs) +
+                 ix * ax * (txz0 + ay * txzs + az * txzt + ay * az * txzst) +
+                 iy * ay * (tyz0 + ax * tyzr + az * tyzt + ax * az * tyzrt);
+    const unsigned bf = bcflag ? bcflag[nid[a]] : 0u;
+    T* yp = y + nid[a] * 3;
+    if (!(bf & 1u)) yp[0] += yx;
+    if (!(bf & 2u)) yp[1] += yy;
+    if (!(bf & 4u)) yp[2] += yz;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_apply_K2(Geo g, int c, long long mcount, const T* __restrict__ x,
+                                                     T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
+                                                     const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
+  const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * ijk[1];
+  const T ix = (T)(0.5 / (vcoord(g, 0, ijk[0] + 1) - vcoord(g, 0, ijk[0])));
+  const T iy = (T)(0.5 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
+  const long long sy = g.nn[0];
+  const long long n0 = ijk[0] + sy * ijk[1];
+  long long nid[4];
+#pragma unroll
+  for (int a = 0; a < 4; a++) nid[a] = n0 + (a & 1) + ((a >> 1) & 1) * sy;
+  T A[2][2], B[2][2];
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const T u0 = x[nid[0] * 2 + cc], u1 = x[nid[1] * 2 + cc], u2 = x[nid[2] * 2 + cc], u3 = x[nid[3] * 2 + cc];
+    const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
+    A[cc][0] = ix * t1; A[cc][1] = ix * t3;
+    B[cc][0] = iy * t2; B[cc][1] = iy * t3;
+  }
+  T mxx0 = 0, mxxs = 0, myy0 = 0, myyr = 0, txy0 = 0, txyr = 0, txys = 0;
+  const int pb = off[e], pe = off[e + 1];
+  for (int p = pb; p < pe; p++) {
+    const typename Vec4<T>::type q = pk[p];
+    const T r = q.x, s = q.y, w = q.w;
+    const T uxx = A[0][0] + A[0][1] * s, uxy = B[0][0] + B[0][1] * r;
+    const T uyx = A[1][0] + A[1][1] * s, uyy = B[1][0] + B[1][1] * r;
+    const T w2 = w + w;
+    const T sxx = w2 * uxx, syy = w2 * uyy, sxy = w * (uxy + uyx);
+    mxx0 += sxx; mxxs += sxx * s;
+    myy0 += syy; myyr += syy * r;
+    txy0 += sxy; txyr += sxy * r; txys += sxy * s;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
+    const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
+    const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
+    const unsigned bf = bcflag ? bcflag[nid[a]] : 0u;
+    T* yp = y + nid[a] * 2;
+    if (!(bf & 1u)) yp[0] += yx;
+    if (!(bf & 2u)) yp[1] += yy;
+  }
+}
+
+template <typename T>
+static void apply_K_impl(const Geo& g, const T* x, T* y, const typename Vec4<T>::type* pk, const int* off,
+                         const uint8_t* bcflag, hipStream_t st) {
+  const int ncol = g.dim == 2 ? 4 : 8;
+  for (int c = 0; c < ncol; c++) {
+    const long long m = colour_count(g, c);
+    if (m <= 0) continue;
+    if (g.dim == 3)
+      hipLaunchKernelGGL(k_apply_K3<T>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
+    else
+      hipLaunchKernelGGL(k_apply_K2<T>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
+  }
+}
+
+void launch_apply_K(const Geo& g, const double* x, double* y, const double4* pk, const int* off,
+                    const uint8_t* bcflag, hipStream_t st) {
+  launch_fill(y, g.N * g.dim, 0.0, st);
+  apply_K_impl<double>(g, x, y, pk, off, bcflag, st);
+}
+void launch_apply_K32(const Geo& g, const float* x, float* y, const float4* pk, const int* off,
+                      const uint8_t* bcflag, hipStream_t st) {
+  launch_fill_f(y, g.N * g.dim, 0.0f, st);
+  apply_K_impl<float>(g, x, y, pk, off, bcflag, st);
+}
+
+__global__ void k_pack32(const double4* __restrict__ a, float4* __restrict__ b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double4 v = a[i];
+    b[i] = make_float4((float)v.x, (float)v.y, (float)v.z, (float)v.w);
+  }
+}
+void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st) {
+  if (P > 0) hipLaunchKernelGGL(k_pack32, dim3(grid_for(P)), dim3(kBlock), 0, st, pk, pk32, P);
+}
+
+// ---------------------------------------------------------------------------
+// G p (node-centric gather) and G^T u (element-centric).  G_e[a,d] is the
+// closed form of GradientStiffnessMatrixTerm.cxx:225-245 on an affine
+// element: -int_e dN_a/dx_d = -s_ad * prod_{d'!=d} h_d' / 2^(dim-1).
+// ---------------------------------------------------------------------------
+__global__ void k_apply_G(Geo g, const double* __restrict__ p, double* __restrict__ y,
+                          const uint8_t* __restrict__ bcflag) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const long long i = n % g.nn[0], j = (n / g.nn[0]) % g.nn[1], k = n / (g.nn[0] * g.nn[1]);
+  double acc[3] = {0.0, 0.0, 0.0};
+  const long long k0 = g.dim == 3 ? k - 1 : 0, k1 = g.dim == 3 ? k : 0;
+  for (long long kk = k0; kk <= k1; kk++) {
+    if (kk < 0 || kk >= g.res[2]) continue;
+    const double hz = g.dim == 3 ? vcoord(g, 2, kk + 1) - vcoord(g, 2, kk) : 1.0;
+    const double sz = (kk == k - 1) ? 1.0 : -1.0;
+    for (long long jj = j - 1; jj <= j; jj++) {
+      if (jj < 0 || jj >= g.res[1]) continue;
+      const double hy = vcoord(g, 1, jj + 1) - vcoord(g, 1, jj);
+      const double sy = (jj == j - 1) ? 1.0 : -1.0;
+      for (long long ii = i - 1; ii <= i; ii++) {
+        if (ii < 0 || ii >= g.res[0]) continue;
+        const double hx = vcoord(g, 0, ii + 1) - vcoord(g, 0, ii);
+        const double sx = (ii == i - 1) ? 1.0 : -1.0;
+        const double pe = p[ii + (long long)g.res[0] * (jj + (long long)g.res[1] * kk)];
+        if (g.dim == 2) {
+          acc[0] -= sx * (hy * 0.5) * pe;
+          acc[1] -= sy * (hx * 0.5) * pe;
+        } else {
+          acc[0] -= sx * (hy * hz * 0.25) * pe;
+          acc[1] -= sy * (hx * hz * 0.25) * pe;
+          acc[2] -= sz * (hx * hy * 0.25) * pe;
+        }
+      }
+    }
+  }
+  const unsigned bf = bcflag ? bcflag[n] : 0u;
+  for (int d = 0; d < g.dim; d++) y[n * g.dim + d] = (bf >> d) & 1u ? 0.0 : acc[d];
+}
+void launch_apply_G(const Geo& g, const double* p, double* y, const uint8_t* bcflag, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_G, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g, p, y, bcflag);
+}
+
+__global__ void k_apply_Gt(Geo g, const double* __restrict__ x, double* __restrict__ q) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= g.E) return;
+  const long long i = e % g.res[0], j = (e / g.res[0]) % g.res[1], k = g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0;
+  const double hx = vcoord(g, 0, i + 1) - vcoord(g, 0, i);
+  const double hy = vcoord(g, 1, j + 1) - vcoord(g, 1, j);
+  const long long n0 = i + g.nn[0] * (j + g.nn[1] * k);
+  const int dim = g.dim;
+  if (dim == 2) {
+    const long long sy = g.nn[0];
+    const double* u0 = x + n0 * 2;
+    const double* u1 = x + (n0 + 1) * 2;
+    const double* u2 = x + (n0 + sy) * 2;
+    const double* u3 = x + (n0 + sy + 1) * 2;
+    const double fx = (u1[0] - u0[0]) + (u3[0] - u2[0]);
+    const double fy = (u2[1] - u0[1]) + (u3[1] - u1[1]);
+    q[e] = -(hy * 0.5) * fx - (hx * 0.5) * fy;
+  } else {
+    const double hz = vcoord(g, 2, k + 1) - vcoord(g, 2, k);
+    const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
+      const double* u = x + nd * 3;
+      fx += (a & 1) ? u[0] : -u[0];
+      fy += (a & 2) ? u[1] : -u[1];
+      fz += (a & 4) ? u[2] : -u[2];
+    }
+    q[e] = -(hy * hz * 0.25) * fx - (hx * hz * 0.25) * fy - (hx * hy * 0.25) * fz;
+  }
+}
+void launch_apply_Gt(const Geo& g, const double* x, double* q, hipStream_t st) {
+  hipLaunchKernelGGL(k_apply_Gt, dim3(grid_for(g.E)), dim3(kBlock), 0, st, g, x, q);
+}
+
+// ---------------------------------------------------------------------------
+// Fused constitutive pass (one per linear solve): effective viscosity at each
+// material point (MaterialViscosity -> YieldRheology -> VonMises /
+// DruckerPrager), omega_p = w_p detJ eta_p written into the particle stream,
+// element diag(K_e), Q_e = sum G_e^2 / K_e,vv (UzawaPreconditionerTerm.cxx
+// :195-215) and the buoyancy force (BuoyancyForceTerm.cxx:474-533).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sym_inv(int dim, const double* t) {
+  double v;
+  if (dim == 2)
+    v = 0.5 * (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
+  else
+    v = 0.5 * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) + t[3] * t[3] + t[4] * t[4] + t[5] * t[5];
+  return sqrt(v);
+}
+
+// Yield-limited viscosity (YieldRheology.cxx:203-231, VonMises.cxx:207-291,
+// DruckerPrager.cxx:296-445).  L = velocity gradient at the point.
+__device__ double yield_viscosity(int dim, const MatDev& m, double eta, const double (&L)[3][3], double pres,
+                                  int* yielded) {
+  double sr[6], stv[6];
+  const int nst = dim == 2 ? 3 : 6;
+  if (dim == 2) {
+    sr[0] = L[0][0];
+    sr[1] = L[1][1];
+    sr[2] = 0.5 * (L[0][1] + L[1][0]);
+  } else {
+    sr[0] = L[0][0];
+    sr[1] = L[1][1];
+    sr[2] = L[2][2];
+    sr[3] = 0.5 * (L[0][1] + L[1][0]);
+    sr[4] = 0.5 * (L[0][2] + L[2][0]);
+    sr[5] = 0.5 * (L[1][2] + L[2][1]);
+  }
+  for (int k = 0; k < nst; k++) stv[k] = (k < dim ? 2.0 * eta : eta * 2.0) * sr[k];
+  double trs = 0.0, trr = 0.0;
+  for (int k = 0; k < dim; k++) {
+    trs += stv[k];
+    trr += sr[k];
+  }
+  for (int k = 0; k < dim; k++) {
+    sr[k] -= trr / dim;
+    stv[k] -= trs / dim;
+  }
+  const double srII = sym_inv(dim, sr), tauII = sym_inv(dim, stv);
+  *yielded = 0;
+  if (m.yield_type == 1) {
+    const double mu = m.friction, C = m.cohesion;
+    double factor, fs;
+    if (dim == 2) {
+      factor = 1 / sqrt(1 + mu * mu);
+      fs = mu * pres * factor + C * factor;
+    } else {
+      const double cphi = 1 / sqrt(1 + mu * mu);
+      const double sphi = mu * cphi;
+      factor = 2 * cphi / (sqrt(3.0) * (3 - sphi));
+      fs = mu * factor * pres + C * 3 * factor;
+    }
+    double minys = m.min_yield_stress;
+    if (minys == 0.0) minys = C;
+    if (fs < minys * factor) fs = minys * factor;
+    if (fs < tauII) {
+      double v = (m.max_strain_rate > 0 && srII > m.max_strain_rate) ? fs / (2 * m.max_strain_rate)
+                                                                      : fs / (2 * srII);
+      if (v < m.min_viscosity) v = m.min_viscosity;
+      *yielded = 1;
+      return v;
+    }
+  } else if (m.yield_type == 2) {
+    const double crit = m.cohesion;
+    if (crit < tauII) {
+      const double beta = 1.0 - crit / tauII;
+      double corr = -eta * beta;
+      if (eta + corr < m.min_viscosity) corr = m.min_viscosity - eta;
+      *yielded = 1;
+      return eta + corr;
+    }
+  }
+  return eta;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void k_setup(Geo g, int c, long long mcount, SetupArgs a, double4* __restrict__ pk,
+                                                  const int* __restrict__ off, double* __restrict__ diag,
+                                                  double* __restrict__ f, double* __restrict__ Q,
+                                                  double* __restrict__ eta_out, uint8_t* __restrict__ yielded) {
+  constexpr int NEN = DIM == 2 ? 4 : 8;
+  const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
+  double h[3] = {1.0, 1.0, 1.0};
+  for (int d = 0; d < DIM; d++) h[d] = vcoord(g, d, ijk[d] + 1) - vcoord(g, d, ijk[d]);
+  const double detJ = DIM == 2 ? (h[0] * 0.5) * (h[1] * 0.5) : (h[0] * 0.5) * (h[1] * 0.5) * (h[2] * 0.5);
+  double gi[3];
+  for (int d = 0; d < DIM; d++) gi[d] = 2.0 / h[d];
+  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+  long long nid[NEN];
+  for (int q = 0; q < NEN; q++) nid[q] = n0 + (q & 1) + ((q >> 1) & 1) * sy + ((q >> 2) & 1) * sz;
+  double ue[NEN][3];
+  const bool need_u = a.yield_on && a.u_prev != nullptr;
+  if (need_u)
+    for (int q = 0; q < NEN; q++)
+      for (int d = 0; d < DIM; d++) ue[q][d] = a.u_prev[nid[q] * DIM + d];
+  const double pres = a.p_prev ? a.p_prev[e] : 0.0;
+  double dg[NEN][3], fe[NEN][3];
+  for (int q = 0; q < NEN; q++)
+    for (int d = 0; d < 3; d++) dg[q][d] = fe[q][d] = 0.0;
+  const int pb = off[e], pe = off[e + 1];
+  const double w = (DIM == 2 ? 4.0 : 8.0) / (double)(pe - pb);
+  for (int p = pb; p < pe; p++) {
+    double4 q4 = pk[p];
+    const double xi[3] = {q4.x, q4.y, q4.z};
+    const int mi = a.mat[p];
+    const MatDev m = a.mats[mi];
+    double eta = a.eta0p ? a.eta0p[p] : m.eta0;
+    // shape functions and global derivatives (affine element)
+    double N[NEN], dN[3][NEN];
+    for (int q = 0; q < NEN; q++) {
+      double fct[3], sg[3];
+      for (int d = 0; d < DIM; d++) {
+        sg[d] = ((q >> d) & 1) ? 1.0 : -1.0;
+        fct[d] = 1.0 + sg[d] * xi[d];
+      }
+      const double cN = DIM == 2 ? 0.25 : 0.125;
+      double nv = cN;
+      for (int d = 0; d < DIM; d++) nv *= fct[d];
+      N[q] = nv;
+      for (int d = 0; d < DIM; d++) {
+        double v = cN * sg[d] * gi[d];
+        for (int d2 = 0; d2 < DIM; d2++)
+          if (d2 != d) v *= fct[d2];
+        dN[d][q] = v;
+      }
+    }
+    int yl = 0;
+    if (need_u && m.yield_type != 0) {
+      double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+      for (int q = 0; q < NEN; q++)
+        for (int i = 0; i < DIM; i++)
+          for (int j = 0; j < DIM; j++) L[i][j] += dN[j][q] * ue[q][i];
+      eta = yield_viscosity(DIM, m, eta, L, pres, &yl);
+    }
+    if (a.clamp_min > 0 && eta < a.clamp_min) eta = a.clamp_min;
+    if (a.clamp_max > 0 && eta > a.clamp_max) eta = a.clamp_max;
+    const double cw = w * detJ;
+    const double om = cw * eta;
+    q4.w = om;
+    pk[p] = q4;
+    if (eta_out) eta_out[p] = eta;
+    if (yielded) yielded[p] = (uint8_t)yl;
+    const double fr = cw * m.density;
+    for (int q = 0; q < NEN; q++) {
+      double sq = 0.0, d2[3];
+      for (int d = 0; d < DIM; d++) {
+        d2[d] = dN[d][q] * dN[d][q];
+        sq += d2[d];
+      }
+      for (int d = 0; d < DIM; d++) {
+        dg[q][d] += om * (sq + d2[d]);
+        fe[q][d] += a.g[d] * fr * N[q];
+      }
+    }
+  }
+  // closed-form element G and Q_e
+  double qe = 0.0;
+  for (int q = 0; q < NEN; q++)
+    for (int d = 0; d < DIM; d++) {
+      double area = 1.0;
+      for (int d2 = 0; d2 < DIM; d2++)
+        if (d2 != d) area *= h[d2];
+      const double s = ((q >> d) & 1) ? 1.0 : -1.0;
+      const double ge = -s * area * (DIM == 2 ? 0.5 : 0.25);
+      qe += ge * (ge / dg[q][d]);
+    }
+  Q[e] = qe;
+  for (int q = 0; q < NEN; q++)
+    for (int d = 0; d < DIM; d++) {
+      diag[nid[q] * DIM + d] += dg[q][d];
+      f[nid[q] * DIM + d] += fe[q][d];
+    }
+}
+
+void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f, double* Q,
+                  double* eta_out, uint8_t* yielded, hipStream_t st) {
+  launch_fill(diag, g.N * g.dim, 0.0, st);
+  launch_fill(f, g.N * g.dim, 0.0, st);
+  const int ncol = g.dim == 2 ? 4 : 8;
+  for (int c = 0; c < ncol; c++) {
+    const long long m = colour_count(g, c);
+    if (m <= 0) continue;
+    if (g.dim == 3)
+      hipLaunchKernelGGL(k_setup<3>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, a, pk, off, diag, f, Q,
+                         eta_out, yielded);
+    else
+      hipLaunchKernelGGL(k_setup<2>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, a, pk, off, diag, f, Q,
+                         eta_out, yielded);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Particles: location (K9), offsets, remap (K10), RK2 advection (K8).
+// ---------------------------------------------------------------------------
+__global__ void k_locate(Geo g, const double* __restrict__ coord, long long P, int* __restrict__ keys,
+                         int* __restrict__ vals, unsigned* esc) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const double x[3] = {coord[3 * i], coord[3 * i + 1], coord[3 * i + 2]};
+  long long el;
+  if (search_element(g, x, &el)) {
+    keys[i] = (int)el;
+  } else {
+    keys[i] = (int)g.E;
+    atomicAdd(esc, 1u);
+  }
+  vals[i] = (int)i;
+}
+void launch_locate(const Geo& g, const double* coord, long long P, int* keys, int* vals, double* S,
+                   hipStream_t st) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_locate, dim3(grid_for(P)), dim3(kBlock), 0, st, g, coord, P, keys, vals,
+                     reinterpret_cast<unsigned*>(S + S_ESC));
+}
+
+__global__ void k_offsets(const int* __restrict__ keys, long long P, long long E, int* __restrict__ off) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i > P) return;
+  const long long kprev = i == 0 ? -1 : keys[i - 1];
+  long long kcur = i < P ? keys[i] : E;
+  if (kcur > E) kcur = E;
+  for (long long e = kprev + 1; e <= kcur; e++) off[e] = (int)i;
+}
+void launch_offsets(const int* keys_sorted, long long P, long long E, int* off, hipStream_t st) {
+  hipLaunchKernelGGL(k_offsets, dim3(grid_for(P + 1)), dim3(kBlock), 0, st, keys_sorted, P, E, off);
+}
+
+// CoincidentMapper global -> local: for an affine element the Newton
+// iteration of ElementType.cxx:221-333 converges to xi = (x - centre) * 2/h.
+__global__ void k_remap(Geo g, const double* __restrict__ coord, const int* __restrict__ off,
+                        double4* __restrict__ pk, long long E) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
+                            g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
+  double xc[3] = {0, 0, 0}, sc[3] = {0, 0, 0};
+  for (int d = 0; d < g.dim; d++) {
+    const double a = vcoord(g, d, ijk[d]), b = vcoord(g, d, ijk[d] + 1);
+    xc[d] = 0.5 * (a + b);
+    sc[d] = 2.0 / (b - a);
+  }
+  for (int p = off[e]; p < off[e + 1]; p++) {
+    double xi[3] = {0, 0, 0};
+    for (int d = 0; d < g.dim; d++) xi[d] = (coord[3 * (long long)p + d] - xc[d]) * sc[d];
+    pk[p] = make_double4(xi[0], xi[1], xi[2], 0.0);
+  }
+}
+void launch_remap(const Geo& g, const double* coord, const int* off, double4* pk, long long E, hipStream_t st) {
+  hipLaunchKernelGGL(k_remap, dim3(grid_for(E)), dim3(kBlock), 0, st, g, coord, off, pk, E);
+}
+
+// Velocity at a global point (FeVariable_InterpolateValueAt): 0 if outside.
+__device__ int interp_velocity(const Geo& g, const double* __restrict__ u, const double* x, double* v) {
+  long long e;
+  if (!search_element(g, x, &e)) return 0;
+  const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
+                            g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
+  double xi[3] = {0, 0, 0};
+  for (int d = 0; d < g.dim; d++) {
+    const double a = vcoord(g, d, ijk[d]), b = vcoord(g, d, ijk[d] + 1);
+    xi[d] = (x[d] - 0.5 * (a + b)) * (2.0 / (b - a));
+  }
+  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+  const int nen = g.dim == 2 ? 4 : 8;
+  for (int d = 0; d < g.dim; d++) v[d] = 0.0;
+  for (int q = 0; q < nen; q++) {
+    double nv = g.dim == 2 ? 0.25 : 0.125;
+    for (int d = 0; d < g.dim; d++) nv *= 1.0 + (((q >> d) & 1) ? 1.0 : -1.0) * xi[d];
+    const long long nd = n0 + (q & 1) + ((q >> 1) & 1) * sy + ((q >> 2) & 1) * sz;
+    for (int d = 0; d < g.dim; d++) v[d] += nv * u[nd * g.dim + d];
+  }
+  for (int d = 0; d < g.dim; d++)
+    if (isinf(v[d])) return 0;
+  return 1;
+}
+
+// TimeIntegrand.cxx:265-335: x_half = x0 + 0.5 dt v(x0); x1 = x0 + dt v(x_half),
+// falling back to x0 + dt v(x0) when the midpoint lookup fails.  Position
+// updates are not contracted into FMAs so that they round as the reference's
+// separate multiply and add.
+__global__ void k_advect(Geo g, double* __restrict__ coord, long long P, const double* __restrict__ u, double dt,
+                         unsigned* esc) {
+#pragma clang fp contract(off)
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  double x0[3] = {coord[3 * i], coord[3 * i + 1], coord[3 * i + 2]};
+  double v1[3], v2[3], xh[3] = {x0[0], x0[1], x0[2]};
+  if (!interp_velocity(g, u, x0, v1)) {
+    atomicAdd(esc, 1u);
+    return;
+  }
+  for (int d = 0; d < g.dim; d++) xh[d] = x0[d] + 0.5 * dt * v1[d];
+  const int ok = interp_velocity(g, u, xh, v2);
+  for (int d = 0; d < g.dim; d++) coord[3 * i + d] = x0[d] + dt * (ok ? v2[d] : v1[d]);
+}
+void launch_advect_rk2(const Geo& g, double* coord, long long P, const double* u, double dt, double* S,
+                       hipStream_t st) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_advect, dim3(grid_for(P)), dim3(kBlock), 0, st, g, coord, P, u, dt,
+                     reinterpret_cast<unsigned*>(S + S_ESC));
+}
+
+}  // namespace gs

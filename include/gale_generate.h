@@ -1,0 +1,68 @@
+/*
+ * gale_generate.h -- host-side problem generation for the Stokes hot path:
+ * particle layouts, wall boundary conditions and material shapes, the parts
+ * of the reference's XML component graph that run once before the first
+ * solve.  Exported by libgale_stokes.so next to gale_stokes.h.
+ */
+#ifndef GALE_GENERATE_H
+#define GALE_GENERATE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Park-Miller stream with the reference's seed mask; replaces
+ * Swarm_Random_Seed / Swarm_Random_Random (StgDomain/Swarm/src/Random.cxx:45-74).
+ * Writes count raw draws to out (may be NULL) and returns the last one. */
+double gs_rng_stream(int64_t seed, int64_t count, double* out);
+
+/* RandomParticleLayout (StgDomain/Swarm/src/RandomParticleLayout.cxx:196-216)
+ * driven cell by cell as PerCellParticleLayout (PerCellParticleLayout.cxx:147-168):
+ * ppc particles per element, coordinates P*3.  Returns P (or -1). */
+int64_t gs_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi, int32_t ppc,
+                         int64_t seed, double* coord);
+
+/* GaussParticleLayout with 2 points per axis (GaussParticleLayout.cxx:175-180)
+ * placed as global coordinates; with ConstantWeights they carry weight 1. */
+int64_t gs_layout_gauss(int32_t dim, const int32_t* res, const double* lo, const double* hi, double* coord);
+
+/* WallVC (StgDomain/Utils/src/WallVC.cxx): wall 0 left, 1 right, 2 bottom,
+ * 3 top, 4 back, 5 front; marks component comp of every node on the wall in
+ * mask (N*dim) and writes value into values (N*dim, may be NULL). */
+int32_t gs_wall_bc(int32_t dim, const int32_t* res, int32_t wall, int32_t comp, double value, uint8_t* mask,
+                   double* values);
+
+enum {
+  GS_SHAPE_EVERYWHERE = 0,   /* Everywhere */
+  GS_SHAPE_SPHERE = 1,       /* Sphere (Sphere.cxx:189-208): p[0..2] centre, p[3] radius */
+  GS_SHAPE_BOX = 2,          /* Box (Box.cxx:209-221): p[0..2] start, p[3..5] end */
+  GS_SHAPE_BELOW_SINES = 3   /* below x_axis = p[0] + sum_k modes[4k] sin(2pi(modes[4k+1] a + modes[4k+2] b) + modes[4k+3]) */
+};
+
+typedef struct gs_shape {
+  int32_t type;
+  int32_t material;
+  int32_t axis;       /* vertical axis for GS_SHAPE_BELOW_SINES */
+  int32_t n_modes;
+  double p[6];
+  double modes[32];
+} gs_shape;
+
+/* Material_Layout (PICellerator/MaterialPoints/src/Material.cxx:210-228):
+ * shapes are applied in order and the last one containing a particle sets
+ * its material. */
+int32_t gs_shape_layout(int32_t dim, int64_t P, const double* coord, int32_t n_shapes, const gs_shape* shapes,
+                        int32_t* material);
+
+/* Seeded Gaussian random field on a node grid (white noise filtered by a
+ * separable Gaussian with covariance length corr), min-max scaled to
+ * [vmin, vmax].  No reference counterpart: used to build viscosity fields. */
+int32_t gs_gaussian_field(int32_t dim, const int32_t* nodes, const double* spacing, double corr, int64_t seed,
+                          double vmin, double vmax, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1384 @@
+/*
+ * gale_oracle.c -- CPU restatement of Gale's Q1-P0 Stokes hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see gale_oracle.h).  This is the checker that the
+ * HIP implementation in gale_amd/ is compared against.  It follows the
+ * reference algorithm step by step, assembling K, G and Q element by element
+ * into CSR exactly as the reference's PETSc path does, and solving with the
+ * reference's Uzawa loop.
+ *
+ * What pins each part (the tests under tests/ that check this file):
+ *   - Park-Miller stream: the textbook known answer (seed 1 -> 10000th value
+ *     1043618065, Park & Miller 1988) through the reference's seed mask.
+ *   - K element integration + global assembly: the reference's own PETSc
+ *     binary fixtures testIsoviscousSiffness.matrix (3x3 Q1) and
+ *     testMaterialViscosity2D.matrix (10x10 Q1), evaluated at 2x2 Gauss
+ *     points; plus rigid-body nullity and symmetry in 3D.
+ *   - D-matrix / stress conventions: testConstitutiveMatrix_Cartesian2D.expected.
+ *   - G: the divergence-theorem closed form (G^T u)_e = -(flux of u out of e).
+ *   - Uzawa solve: a dense direct solve of the saddle-point system (numpy),
+ *     and the hydrostatic closed form (u = 0, p = rho g (H - y) + c).
+ *   - Drucker-Prager / von Mises viscosity: closed-form pure-shear cases.
+ *   - Ownership: brute force over face/edge/corner lattices and +-1e-12 /
+ *     +-1 ulp offsets.
+ *   - RK2: rigid rotation closed form and the linear-field exactness of the
+ *     midpoint rule.
+ * Functions without a check of their own are listed as "parity unpinned" in
+ * DESIGN.md.
+ */
+#include "gale_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define MAXNEN 8
+#define MAXDOF 24
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Park-Miller minimal standard with Schrage's trick.                       */
+/* StgDomain/Swarm/src/Random.cxx:45-74                                      */
+/* ------------------------------------------------------------------------ */
+static const long PM_M = 2147483647L, PM_A = 16807L, PM_Q = 127773L, PM_R = 2836L;
+static const long PM_MASK = 123459876L;
+
+void orc_rng_seed(orc_rng* g, long seed) { g->state = seed ^ PM_MASK; }
+
+double orc_rng_next(orc_rng* g) {
+  const long tmp = g->state / PM_Q;
+  g->state = PM_A * (g->state - tmp * PM_Q) - PM_R * tmp;
+  if (g->state < 0) g->state += PM_M;
+  return (double)g->state;
+}
+
+/* Random.cxx:76-78 Swarm_Random_Random_WithMinMax */
+double orc_rng_range(orc_rng* g, double mn, double mx) {
+  return (mx - mn) * orc_rng_next(g) / (double)PM_M + mn;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh: CartesianGenerator vertex coordinates (CartesianGenerator.cxx:     */
+/* 2174-2215, contactDepth = 0) and regular element search                   */
+/* (Mesh_RegularAlgorithms.cxx:155-224, Num_Approx epsilon 1e-12 from        */
+/* StGermain/Base/Foundation/src/Numerics.cxx:36-41).                        */
+/* ------------------------------------------------------------------------ */
+double orc_vertex_coord(int32_t n, double lo, double hi, int32_t idx) {
+  if (idx <= 0) return lo;
+  if (idx >= n) return hi;
+  return lo + ((double)idx / (double)n) * (hi - lo);
+}
+
+static int num_approx(double v, double val) { return v >= val - 1e-12 && v <= val + 1e-12; }
+
+int32_t orc_search_element(int32_t dim, const int32_t* res, const double* lo, const double* hi,
+                           const double* x, int32_t* elInd) {
+  int32_t inds[3] = {0, 0, 0};
+  for (int d = 0; d < dim; d++) {
+    const double sep = (hi[d] - lo[d]) / res[d];
+    if (num_approx(x[d] - hi[d], 0.0)) {
+      inds[d] = res[d] - 1;
+    } else if (x[d] < lo[d] || x[d] > hi[d]) {
+      return 0;
+    } else {
+      double integer;
+      const double out = (x[d] - lo[d]) / sep;
+      const double frac = modf(out, &integer);
+      inds[d] = (int32_t)(unsigned)integer;
+      if (inds[d] > 0 && num_approx(frac, 0.0)) inds[d]--;
+    }
+  }
+  /* Grid_Project: x fastest */
+  *elInd = inds[0] + res[0] * (inds[1] + (dim == 3 ? res[1] * inds[2] : 0));
+  return 1;
+}
+
+/* RandomParticleLayout.cxx:196-216 driven cell by cell as in
+ * PerCellParticleLayout.cxx:147-168.  Three draws per particle, also in 2D
+ * (the third coordinate is discarded here). */
+int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi,
+                          int32_t ppc, long seed, double* coord) {
+  orc_rng g;
+  orc_rng_seed(&g, seed);
+  const int64_t E = (int64_t)res[0] * res[1] * (dim == 3 ? res[2] : 1);
+  int64_t p = 0;
+  for (int64_t e = 0; e < E; e++) {
+    int32_t ijk[3] = {(int32_t)(e % res[0]), (int32_t)((e / res[0]) % res[1]),
+                      dim == 3 ? (int32_t)(e / ((int64_t)res[0] * res[1])) : 0};
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+      mn[d] = orc_vertex_coord(res[d], lo[d], hi[d], ijk[d]);
+      mx[d] = orc_vertex_coord(res[d], lo[d], hi[d], ijk[d] + 1);
+    }
+    for (int c = 0; c < ppc; c++, p++) {
+      for (int d = 0; d < 3; d++) {
+        double v = orc_rng_range(&g, mn[d], mx[d]);
+        coord[p * 3 + d] = d < dim ? v : 0.0;
+      }
+    }
+  }
+  return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Problem                                                                   */
+/* ------------------------------------------------------------------------ */
+struct orc_problem {
+  int dim, nen, ndof;
+  int32_t res[3];
+  int64_t nn[3];
+  double lo[3], hi[3];
+  int64_t N, E, P;
+  uint8_t* bc_mask;
+  double* bc_value;
+  double gravity, ghat[3];
+  double clamp_min, clamp_max;
+  int nmat;
+  orc_material* mats;
+  double* coord;
+  int32_t* material;
+  double* eta0p;
+  int32_t* owner;
+  double* xi;
+  double* weight;
+  int64_t* cell_off;
+  int64_t* cell_list;
+  double* eta;
+  uint8_t* yielded;
+  int has_visc;
+  /* reduced system */
+  int32_t* eq;
+  int64_t n_u;
+  int64_t* Kptr;
+  int32_t* Kcol;
+  double* Kval;
+  int64_t* Gptr;
+  int32_t* Gcol;
+  double* Gval;
+  double* f;
+  double* h;
+  double* Q;
+  double* Kdiag;
+  double* ilu; /* ILU(0) factor values on the K pattern */
+  int64_t* Kdiagpos;
+  int assembled;
+};
+
+int64_t orc_num_nodes(const orc_problem* pb) { return pb->N; }
+int64_t orc_num_elements(const orc_problem* pb) { return pb->E; }
+int64_t orc_num_particles(const orc_problem* pb) { return pb->P; }
+
+static void elem_ijk(const orc_problem* pb, int64_t e, int32_t* ijk) {
+  ijk[0] = (int32_t)(e % pb->res[0]);
+  ijk[1] = (int32_t)((e / pb->res[0]) % pb->res[1]);
+  ijk[2] = pb->dim == 3 ? (int32_t)(e / ((int64_t)pb->res[0] * pb->res[1])) : 0;
+}
+
+/* CartesianGenerator.cxx:894-955: element vertices in lexicographic order,
+ * x fastest. */
+static void elem_nodes(const orc_problem* pb, int64_t e, int64_t* nodes) {
+  int32_t ijk[3];
+  elem_ijk(pb, e, ijk);
+  for (int a = 0; a < pb->nen; a++) {
+    const int64_t i = ijk[0] + (a & 1), j = ijk[1] + ((a >> 1) & 1), k = ijk[2] + ((a >> 2) & 1);
+    nodes[a] = i + pb->nn[0] * (j + pb->nn[1] * k);
+  }
+}
+
+static void node_coord(const orc_problem* pb, int64_t n, double* x) {
+  const int64_t i = n % pb->nn[0], j = (n / pb->nn[0]) % pb->nn[1], k = n / (pb->nn[0] * pb->nn[1]);
+  const int64_t ijk[3] = {i, j, k};
+  for (int d = 0; d < pb->dim; d++) x[d] = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], (int32_t)ijk[d]);
+}
+
+/* BilinearElementType.cxx:213-243, TrilinearElementType.cxx:250-300:
+ * N_a = c * prod_d (1 + s_ad xi_d), local node a lexicographic. */
+static void shape_funcs(int dim, const double* xi, double* N) {
+  if (dim == 2) {
+    N[0] = 0.25 * (1.0 - xi[0]) * (1.0 - xi[1]);
+    N[1] = 0.25 * (1.0 + xi[0]) * (1.0 - xi[1]);
+    N[2] = 0.25 * (1.0 - xi[0]) * (1.0 + xi[1]);
+    N[3] = 0.25 * (1.0 + xi[0]) * (1.0 + xi[1]);
+  } else {
+    for (int a = 0; a < 8; a++) {
+      const double sx = (a & 1) ? 1.0 : -1.0, sy = (a & 2) ? 1.0 : -1.0, sz = (a & 4) ? 1.0 : -1.0;
+      N[a] = 0.125 * (1.0 + sx * xi[0]) * (1.0 + sy * xi[1]) * (1.0 + sz * xi[2]);
+    }
+  }
+}
+
+static void shape_local_derivs(int dim, const double* xi, double GNi[3][MAXNEN]) {
+  const int nen = dim == 2 ? 4 : 8;
+  const double c = dim == 2 ? 0.25 : 0.125;
+  for (int a = 0; a < nen; a++) {
+    double s[3], f[3];
+    for (int d = 0; d < dim; d++) {
+      s[d] = ((a >> d) & 1) ? 1.0 : -1.0;
+      f[d] = 1.0 + s[d] * xi[d];
+    }
+    for (int d = 0; d < dim; d++) {
+      double v = c * s[d];
+      for (int d2 = 0; d2 < dim; d2++)
+        if (d2 != d) v *= f[d2];
+      GNi[d][a] = v;
+    }
+  }
+}
+
+/* ElementType.cxx:339-474 ElementType_ShapeFunctionsGlobalDerivs */
+static double global_derivs(const orc_problem* pb, int64_t e, const double* xi, double GNx[3][MAXNEN]) {
+  const int dim = pb->dim, nen = pb->nen;
+  int64_t nodes[MAXNEN];
+  double GNi[3][MAXNEN], jac[3][3] = {{0}}, inv[3][3], D;
+  elem_nodes(pb, e, nodes);
+  shape_local_derivs(dim, xi, GNi);
+  for (int n = 0; n < nen; n++) {
+    double x[3];
+    node_coord(pb, nodes[n], x);
+    for (int i = 0; i < dim; i++)
+      for (int j = 0; j < dim; j++) jac[i][j] += GNi[i][n] * x[j];
+  }
+  if (dim == 2) {
+    D = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+    inv[0][0] = jac[1][1] / D;
+    inv[1][1] = jac[0][0] / D;
+    inv[0][1] = -jac[0][1] / D;
+    inv[1][0] = -jac[1][0] / D;
+  } else {
+    D = jac[0][0] * (jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1]) -
+        jac[0][1] * (jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]) +
+        jac[0][2] * (jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0]);
+    double cof[3][3];
+    cof[0][0] = jac[1][1] * jac[2][2] - jac[1][2] * jac[2][1];
+    cof[1][0] = -(jac[1][0] * jac[2][2] - jac[1][2] * jac[2][0]);
+    cof[2][0] = jac[1][0] * jac[2][1] - jac[1][1] * jac[2][0];
+    cof[0][1] = -(jac[0][1] * jac[2][2] - jac[0][2] * jac[2][1]);
+    cof[1][1] = jac[0][0] * jac[2][2] - jac[0][2] * jac[2][0];
+    cof[2][1] = -(jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
+    cof[0][2] = jac[0][1] * jac[1][2] - jac[0][2] * jac[1][1];
+    cof[1][2] = -(jac[0][0] * jac[1][2] - jac[0][2] * jac[1][0]);
+    cof[2][2] = jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) inv[i][j] = cof[i][j] / D;
+  }
+  for (int dx = 0; dx < dim; dx++)
+    for (int n = 0; n < nen; n++) {
+      double v = 0.0;
+      for (int dxi = 0; dxi < dim; dxi++) v += GNi[dxi][n] * inv[dx][dxi];
+      GNx[dx][n] = v;
+    }
+  return D;
+}
+
+static void solve_small(int dim, double J[3][3], const double* b, double* x) {
+  if (dim == 2) {
+    const double D = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    x[0] = (b[0] * J[1][1] - J[0][1] * b[1]) / D;
+    x[1] = (J[0][0] * b[1] - b[0] * J[1][0]) / D;
+  } else {
+    const double D = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                     J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                     J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    for (int c = 0; c < 3; c++) {
+      double M[3][3];
+      memcpy(M, J, sizeof(M));
+      for (int r = 0; r < 3; r++) M[r][c] = b[r];
+      x[c] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+              M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+              M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / D;
+    }
+  }
+}
+
+/* ElementType.cxx:221-333 Newton-Raphson global -> local, initial guess 0,
+ * stop when max |dxi| < 1e-4 or after 100 iterations. */
+static void global_to_local(const orc_problem* pb, int64_t e, const double* x, double* xi) {
+  const int dim = pb->dim, nen = pb->nen;
+  int64_t nodes[MAXNEN];
+  double xn[MAXNEN][3];
+  elem_nodes(pb, e, nodes);
+  for (int n = 0; n < nen; n++) node_coord(pb, nodes[n], xn[n]);
+  for (int d = 0; d < dim; d++) xi[d] = 0.0;
+  for (int it = 0; it < 100; it++) {
+    double N[MAXNEN], GNi[3][MAXNEN], J[3][3] = {{0}}, rhs[3] = {0, 0, 0}, inc[3];
+    shape_funcs(dim, xi, N);
+    shape_local_derivs(dim, xi, GNi);
+    for (int n = 0; n < nen; n++)
+      for (int i = 0; i < dim; i++) {
+        for (int j = 0; j < dim; j++) J[i][j] += GNi[j][n] * xn[n][i];
+        rhs[i] -= N[n] * xn[n][i];
+      }
+    for (int i = 0; i < dim; i++) rhs[i] += x[i];
+    solve_small(dim, J, rhs, inc);
+    double mr = 0.0;
+    for (int d = 0; d < dim; d++) {
+      xi[d] += inc[d];
+      if (fabs(inc[d]) > mr) mr = fabs(inc[d]);
+    }
+    if (mr < 1e-4) break;
+  }
+}
+
+/* Stable counting sort of particles by owning element: the cell tables that
+ * CoincidentMapper builds by visiting material points in index order
+ * (CoincidentMapper.cxx:207-240), weights from ConstantWeights.cxx:178-186
+ * with cellLocalVolume = 2^dim (WeightsCalculator.cxx:84). */
+static int relocate_all(orc_problem* pb) {
+  const int dim = pb->dim;
+  for (int64_t p = 0; p < pb->P; p++) {
+    int32_t el;
+    if (!orc_search_element(dim, pb->res, pb->lo, pb->hi, pb->coord + 3 * p, &el)) return ORC_E_ESCAPED;
+    pb->owner[p] = el;
+  }
+  memset(pb->cell_off, 0, sizeof(int64_t) * (pb->E + 1));
+  for (int64_t p = 0; p < pb->P; p++) pb->cell_off[pb->owner[p] + 1]++;
+  for (int64_t e = 0; e < pb->E; e++) pb->cell_off[e + 1] += pb->cell_off[e];
+  int64_t* fill = (int64_t*)malloc(sizeof(int64_t) * pb->E);
+  memcpy(fill, pb->cell_off, sizeof(int64_t) * pb->E);
+  for (int64_t p = 0; p < pb->P; p++) pb->cell_list[fill[pb->owner[p]]++] = p;
+  free(fill);
+  const double vol = dim == 2 ? 4.0 : 8.0;
+  for (int64_t e = 0; e < pb->E; e++) {
+    const int64_t cnt = pb->cell_off[e + 1] - pb->cell_off[e];
+    for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+      const int64_t p = pb->cell_list[q];
+      pb->weight[p] = vol / (double)cnt;
+      global_to_local(pb, e, pb->coord + 3 * p, pb->xi + dim * p);
+    }
+  }
+  pb->assembled = 0;
+  return ORC_OK;
+}
+
+orc_problem* orc_problem_new(const orc_problem_desc* d) {
+  if (!d || (d->dim != 2 && d->dim != 3)) return NULL;
+  orc_problem* pb = (orc_problem*)calloc(1, sizeof(orc_problem));
+  pb->dim = d->dim;
+  pb->nen = d->dim == 2 ? 4 : 8;
+  pb->ndof = pb->nen * d->dim;
+  pb->N = 1;
+  pb->E = 1;
+  for (int i = 0; i < 3; i++) {
+    pb->res[i] = i < d->dim ? d->res[i] : 1;
+    pb->nn[i] = i < d->dim ? d->res[i] + 1 : 1;
+    pb->lo[i] = d->lo[i];
+    pb->hi[i] = d->hi[i];
+    pb->N *= pb->nn[i];
+    pb->E *= pb->res[i];
+  }
+  const int64_t ND = pb->N * pb->dim;
+  pb->bc_mask = (uint8_t*)calloc(ND, 1);
+  pb->bc_value = (double*)calloc(ND, sizeof(double));
+  if (d->bc_mask) memcpy(pb->bc_mask, d->bc_mask, ND);
+  if (d->bc_value) memcpy(pb->bc_value, d->bc_value, ND * sizeof(double));
+  pb->gravity = d->gravity;
+  for (int i = 0; i < 3; i++) pb->ghat[i] = d->ghat[i];
+  pb->clamp_min = d->eta_clamp_min;
+  pb->clamp_max = d->eta_clamp_max;
+  pb->nmat = d->n_materials;
+  pb->mats = (orc_material*)malloc(sizeof(orc_material) * (pb->nmat > 0 ? pb->nmat : 1));
+  memcpy(pb->mats, d->materials, sizeof(orc_material) * pb->nmat);
+  pb->P = d->n_particles;
+  pb->coord = (double*)malloc(sizeof(double) * 3 * pb->P);
+  memcpy(pb->coord, d->coord, sizeof(double) * 3 * pb->P);
+  pb->material = (int32_t*)malloc(sizeof(int32_t) * pb->P);
+  memcpy(pb->material, d->material, sizeof(int32_t) * pb->P);
+  if (d->eta0_particle) {
+    pb->eta0p = (double*)malloc(sizeof(double) * pb->P);
+    memcpy(pb->eta0p, d->eta0_particle, sizeof(double) * pb->P);
+  }
+  pb->owner = (int32_t*)malloc(sizeof(int32_t) * pb->P);
+  pb->xi = (double*)malloc(sizeof(double) * pb->dim * pb->P);
+  pb->weight = (double*)malloc(sizeof(double) * pb->P);
+  pb->cell_off = (int64_t*)malloc(sizeof(int64_t) * (pb->E + 1));
+  pb->cell_list = (int64_t*)malloc(sizeof(int64_t) * (pb->P > 0 ? pb->P : 1));
+  pb->eta = (double*)calloc(pb->P > 0 ? pb->P : 1, sizeof(double));
+  pb->yielded = (uint8_t*)calloc(pb->P > 0 ? pb->P : 1, 1);
+  for (int64_t p = 0; p < pb->P; p++)
+    if (pb->material[p] < 0 || pb->material[p] >= pb->nmat) {
+      orc_problem_free(pb);
+      return NULL;
+    }
+  if (relocate_all(pb) != ORC_OK) {
+    orc_problem_free(pb);
+    return NULL;
+  }
+  return pb;
+}
+
+static void free_system(orc_problem* pb) {
+  free(pb->eq); free(pb->Kptr); free(pb->Kcol); free(pb->Kval);
+  free(pb->Gptr); free(pb->Gcol); free(pb->Gval);
+  free(pb->f); free(pb->h); free(pb->Q); free(pb->Kdiag); free(pb->ilu); free(pb->Kdiagpos);
+  pb->eq = NULL; pb->Kptr = NULL; pb->Kcol = NULL; pb->Kval = NULL;
+  pb->Gptr = NULL; pb->Gcol = NULL; pb->Gval = NULL;
+  pb->f = NULL; pb->h = NULL; pb->Q = NULL; pb->Kdiag = NULL; pb->ilu = NULL; pb->Kdiagpos = NULL;
+  pb->assembled = 0;
+}
+
+void orc_problem_free(orc_problem* pb) {
+  if (!pb) return;
+  free_system(pb);
+  free(pb->bc_mask); free(pb->bc_value); free(pb->mats); free(pb->coord); free(pb->material);
+  free(pb->eta0p); free(pb->owner); free(pb->xi); free(pb->weight); free(pb->cell_off);
+  free(pb->cell_list); free(pb->eta); free(pb->yielded);
+  free(pb);
+}
+
+void orc_get_particles(const orc_problem* pb, int32_t* owner, double* xi, double* weight, double* coord) {
+  if (owner) memcpy(owner, pb->owner, sizeof(int32_t) * pb->P);
+  if (xi) memcpy(xi, pb->xi, sizeof(double) * pb->dim * pb->P);
+  if (weight) memcpy(weight, pb->weight, sizeof(double) * pb->P);
+  if (coord) memcpy(coord, pb->coord, sizeof(double) * 3 * pb->P);
+}
+
+int32_t orc_set_coords(orc_problem* pb, const double* coord) {
+  memcpy(pb->coord, coord, sizeof(double) * 3 * pb->P);
+  return relocate_all(pb);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Constitutive evaluation per material point                               */
+/*   MaterialViscosity.cxx:129-147, YieldRheology.cxx:190-231,               */
+/*   VonMises.cxx:180-291, DruckerPrager.cxx:296-465,                        */
+/*   TensorMath.cxx:431-459 (second invariant)                              */
+/* ------------------------------------------------------------------------ */
+/* ConstitutiveMatrix_SetIsotropicViscosity (ConstitutiveMatrix.cxx:358-364)
+ * with the Cartesian isotropic correction (ConstitutiveMatrixCartesian.cxx
+ * :536-556): 2D D = diag(2eta, 2eta, eta), 3D diag(2eta x3, eta x3), in the
+ * engineering-shear (Voigt) layout.  D is nst x nst row-major. */
+void orc_isotropic_D(int32_t dim, double eta, double* D) {
+  const int nst = dim == 2 ? 3 : 6;
+  for (int i = 0; i < nst * nst; i++) D[i] = 0.0;
+  for (int k = 0; k < nst; k++) D[k * nst + k] = k < dim ? 2.0 * eta : eta;
+}
+
+/* ConstitutiveMatrixCartesian{2D,3D}_CalculateStress, diagonal branch
+ * (ConstitutiveMatrixCartesian.cxx:727-775): the strain-rate tensor's shear
+ * components enter doubled.  Layouts: 2D [00,11,01], 3D [00,11,22,01,02,12]. */
+void orc_isotropic_stress(int32_t dim, double eta, const double* sr, double* st) {
+  const int nst = dim == 2 ? 3 : 6;
+  double D[36];
+  orc_isotropic_D(dim, eta, D);
+  for (int k = 0; k < nst; k++) st[k] = k < dim ? D[k * nst + k] * sr[k] : D[k * nst + k] * 2.0 * sr[k];
+}
+
+static double sym_invariant(int dim, const double* t) {
+  /* t layout: 2D [00,11,01]; 3D [00,11,22,01,02,12] (TensorMath.h:94-98) */
+  double inv;
+  if (dim == 2)
+    inv = 0.5 * (t[0] * t[0] + t[1] * t[1]) + t[2] * t[2];
+  else
+    inv = 0.5 * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]) + t[3] * t[3] + t[4] * t[4] + t[5] * t[5];
+  return sqrt(inv);
+}
+
+static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, const double* u_prev,
+                                 const double* p_prev, int yield_on, uint8_t* yielded) {
+  const orc_material* m = &pb->mats[pb->material[p]];
+  double eta = pb->eta0p ? pb->eta0p[p] : m->eta0;
+  *yielded = 0;
+  if (yield_on && m->yield_type != ORC_YIELD_NONE && u_prev) {
+    const int dim = pb->dim, nen = pb->nen;
+    const double* xi = pb->xi + dim * p;
+    double GNx[3][MAXNEN];
+    int64_t nodes[MAXNEN];
+    global_derivs(pb, e, xi, GNx);
+    elem_nodes(pb, e, nodes);
+    double L[3][3] = {{0}};
+    for (int a = 0; a < nen; a++)
+      for (int i = 0; i < dim; i++)
+        for (int j = 0; j < dim; j++) L[i][j] += GNx[j][a] * u_prev[nodes[a] * dim + i];
+    double sr[6], st[6];
+    if (dim == 2) {
+      sr[0] = L[0][0];
+      sr[1] = L[1][1];
+      sr[2] = 0.5 * (L[0][1] + L[1][0]);
+    } else {
+      sr[0] = L[0][0];
+      sr[1] = L[1][1];
+      sr[2] = L[2][2];
+      sr[3] = 0.5 * (L[0][1] + L[1][0]);
+      sr[4] = 0.5 * (L[0][2] + L[2][0]);
+      sr[5] = 0.5 * (L[1][2] + L[2][1]);
+    }
+    /* ConstitutiveMatrixCartesian CalculateStress, isotropic D:
+     * normal stress = 2 eta e_ii, shear = eta * 2 * e_ij */
+    orc_isotropic_stress(dim, eta, sr, st);
+    double trs = 0.0, trr = 0.0;
+    for (int k = 0; k < dim; k++) {
+      trs += st[k];
+      trr += sr[k];
+    }
+    for (int k = 0; k < dim; k++) {
+      sr[k] -= trr / dim;
+      st[k] -= trs / dim;
+    }
+    const double srII = sym_invariant(dim, sr);
+    const double tauII = sym_invariant(dim, st);
+    double crit;
+    if (m->yield_type == ORC_YIELD_DRUCKER_PRAGER) {
+      const double pres = p_prev ? p_prev[e] : 0.0;
+      const double mu = m->friction, C = m->cohesion;
+      double factor, fs;
+      if (dim == 2) {
+        factor = 1 / sqrt(1 + mu * mu);
+        fs = mu * pres * factor + C * factor;
+      } else {
+        const double cphi = 1 / sqrt(1 + mu * mu);
+        const double sphi = mu * cphi;
+        factor = 2 * cphi / (sqrt(3.0) * (3 - sphi));
+        fs = mu * factor * pres + C * 3 * factor;
+      }
+      double minys = m->min_yield_stress;
+      if (minys == 0.0) minys = C;
+      if (fs < minys * factor) fs = minys * factor;
+      crit = fs;
+      if (crit < tauII) {
+        double v;
+        if (m->max_strain_rate > 0 && srII > m->max_strain_rate)
+          v = crit / (2 * m->max_strain_rate);
+        else
+          v = crit / (2 * srII);
+        if (v < m->min_viscosity) v = m->min_viscosity;
+        eta = v;
+        *yielded = 1;
+      }
+    } else {
+      crit = m->cohesion;
+      if (crit < tauII) {
+        const double beta = 1.0 - crit / tauII;
+        double corr = -eta * beta;
+        if (eta + corr < m->min_viscosity) corr = m->min_viscosity - eta;
+        eta = eta + corr;
+        *yielded = 1;
+      }
+    }
+  }
+  if (pb->clamp_min > 0 && eta < pb->clamp_min) eta = pb->clamp_min;
+  if (pb->clamp_max > 0 && eta > pb->clamp_max) eta = pb->clamp_max;
+  return eta;
+}
+
+int32_t orc_particle_viscosity(orc_problem* pb, const double* u_prev_full, const double* p_prev,
+                               int32_t yield_on, double* eta, uint8_t* yielded) {
+  for (int64_t e = 0; e < pb->E; e++)
+    for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+      const int64_t p = pb->cell_list[q];
+      pb->eta[p] = particle_viscosity(pb, p, e, u_prev_full, p_prev, yield_on, &pb->yielded[p]);
+      if (!isfinite(pb->eta[p])) return ORC_E_NAN;
+    }
+  pb->has_visc = 1;
+  pb->assembled = 0;
+  if (eta) memcpy(eta, pb->eta, sizeof(double) * pb->P);
+  if (yielded) memcpy(yielded, pb->yielded, pb->P);
+  return ORC_OK;
+}
+
+static void ensure_visc(orc_problem* pb) {
+  if (!pb->has_visc) orc_particle_viscosity(pb, NULL, NULL, 0, NULL, NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Element matrices                                                          */
+/* ------------------------------------------------------------------------ */
+/* ConstitutiveMatrixCartesian.cxx:363-485 with the isotropic D of
+ * ConstitutiveMatrixCartesian.cxx:536-556 and D.B of :631-724.  Row-major
+ * Ke[(col node*dim + i)*ndof + row node*dim + j]. */
+int32_t orc_element_K(orc_problem* pb, int64_t e, double* Ke) {
+  ensure_visc(pb);
+  const int dim = pb->dim, nen = pb->nen, nd = pb->ndof, nst = dim == 2 ? 3 : 6;
+  memset(Ke, 0, sizeof(double) * nd * nd);
+  for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+    const int64_t p = pb->cell_list[q];
+    double GNx[3][MAXNEN];
+    const double detJ = global_derivs(pb, e, pb->xi + dim * p, GNx);
+    const double eta = pb->eta[p];
+    double Dm[36], Dd[6];
+    orc_isotropic_D(dim, eta, Dm);
+    for (int k = 0; k < nst; k++) Dd[k] = Dm[k * nst + k] * (detJ * pb->weight[p]);
+    for (int r = 0; r < nen; r++) {
+      /* B_r (nst x dim) */
+      double Br[6][3] = {{0}};
+      if (dim == 2) {
+        Br[0][0] = GNx[0][r]; Br[1][1] = GNx[1][r]; Br[2][0] = GNx[1][r]; Br[2][1] = GNx[0][r];
+      } else {
+        Br[0][0] = GNx[0][r]; Br[1][1] = GNx[1][r]; Br[2][2] = GNx[2][r];
+        Br[3][0] = GNx[1][r]; Br[3][1] = GNx[0][r];
+        Br[4][0] = GNx[2][r]; Br[4][2] = GNx[0][r];
+        Br[5][1] = GNx[2][r]; Br[5][2] = GNx[1][r];
+      }
+      for (int c = 0; c < nen; c++) {
+        double Bc[6][3] = {{0}};
+        if (dim == 2) {
+          Bc[0][0] = GNx[0][c]; Bc[1][1] = GNx[1][c]; Bc[2][0] = GNx[1][c]; Bc[2][1] = GNx[0][c];
+        } else {
+          Bc[0][0] = GNx[0][c]; Bc[1][1] = GNx[1][c]; Bc[2][2] = GNx[2][c];
+          Bc[3][0] = GNx[1][c]; Bc[3][1] = GNx[0][c];
+          Bc[4][0] = GNx[2][c]; Bc[4][2] = GNx[0][c];
+          Bc[5][1] = GNx[2][c]; Bc[5][2] = GNx[1][c];
+        }
+        for (int i = 0; i < dim; i++)
+          for (int j = 0; j < dim; j++) {
+            double s = 0.0;
+            for (int k = 0; k < nst; k++) s += Bc[k][i] * (Dd[k] * Br[k][j]);
+            Ke[(c * dim + i) * nd + r * dim + j] += s;
+          }
+      }
+    }
+  }
+  return ORC_OK;
+}
+
+/* GradientStiffnessMatrixTerm.cxx:161-249 on the 2x2(x2) Gauss swarm
+ * (GaussParticleLayout, weights 1):  G_e[a*dim+d] += w * (-detJ) * dN_a/dx_d */
+int32_t orc_element_G(orc_problem* pb, int64_t e, double* Ge) {
+  const int dim = pb->dim, nen = pb->nen, ng = dim == 2 ? 4 : 8;
+  const double g = 1.0 / sqrt(3.0);
+  memset(Ge, 0, sizeof(double) * pb->ndof);
+  for (int q = 0; q < ng; q++) {
+    double xi[3];
+    for (int d = 0; d < dim; d++) xi[d] = ((q >> d) & 1) ? g : -g;
+    double GNx[3][MAXNEN];
+    const double detJ = global_derivs(pb, e, xi, GNx);
+    for (int a = 0; a < nen; a++)
+      for (int d = 0; d < dim; d++) Ge[a * dim + d] += 1.0 * (-detJ) * (GNx[d][a] * 1.0);
+  }
+  return ORC_OK;
+}
+
+/* BuoyancyForceTerm.cxx:474-533 (no temperature, no hydrostatic term,
+ * adjust off), gravity vector gHat*gravity (:535-541). */
+int32_t orc_element_f(orc_problem* pb, int64_t e, double* fe) {
+  const int dim = pb->dim, nen = pb->nen;
+  memset(fe, 0, sizeof(double) * pb->ndof);
+  for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+    const int64_t p = pb->cell_list[q];
+    double GNx[3][MAXNEN], N[MAXNEN];
+    const double detJ = global_derivs(pb, e, pb->xi + dim * p, GNx);
+    shape_funcs(dim, pb->xi + dim * p, N);
+    const double rho = pb->mats[pb->material[p]].density;
+    const double factor = detJ * pb->weight[p] * 1.0 * rho;
+    for (int a = 0; a < nen; a++)
+      for (int d = 0; d < dim; d++) fe[a * dim + d] += (pb->ghat[d] * pb->gravity) * factor * N[a];
+  }
+  return ORC_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Global assembly with Dirichlet elimination                                */
+/*   StiffnessMatrix.cxx:2016-2200 (element loop, lifting f -= K_e u_bc,      */
+/*   h -= G_e^T u_bc, MatSetValues ADD), ForceVector.cxx:303-520,            */
+/*   UzawaPreconditionerTerm.cxx:156-239 (Q_ee = sum_v G_v^2 / K_vv).        */
+/* Free dofs are numbered node by node, dof by dof (FeEquationNumber).       */
+/* ------------------------------------------------------------------------ */
+static int64_t find_col(const int32_t* cols, int64_t b, int64_t e, int32_t c) {
+  while (b < e) {
+    const int64_t m = (b + e) / 2;
+    if (cols[m] < c)
+      b = m + 1;
+    else
+      e = m;
+  }
+  return b;
+}
+
+int32_t orc_assemble(orc_problem* pb, const double* u_prev_full, const double* p_prev, int32_t yield_on,
+                     int32_t nthreads) {
+  const int dim = pb->dim, nen = pb->nen, nd = pb->ndof;
+  int32_t st = orc_particle_viscosity(pb, u_prev_full, p_prev, yield_on, NULL, NULL);
+  if (st) return st;
+  free_system(pb);
+  const int64_t ND = pb->N * dim;
+  pb->eq = (int32_t*)malloc(sizeof(int32_t) * ND);
+  int64_t nu = 0;
+  for (int64_t k = 0; k < ND; k++) pb->eq[k] = pb->bc_mask[k] ? -1 : (int32_t)nu++;
+  pb->n_u = nu;
+  /* K pattern: neighbour nodes (3^dim) x dofs, free only, ascending. */
+  pb->Kptr = (int64_t*)calloc(nu + 1, sizeof(int64_t));
+  const int64_t nx = pb->nn[0], ny = pb->nn[1], nz = pb->nn[2];
+  for (int pass = 0; pass < 2; pass++) {
+    int64_t nnz = 0;
+    for (int64_t n = 0; n < pb->N; n++) {
+      const int64_t i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+      for (int c = 0; c < dim; c++) {
+        const int32_t row = pb->eq[n * dim + c];
+        if (row < 0) continue;
+        if (pass == 1) pb->Kptr[row] = nnz;
+        for (int64_t kk = k - 1; kk <= k + 1; kk++) {
+          if (kk < 0 || kk >= nz) continue;
+          for (int64_t jj = j - 1; jj <= j + 1; jj++) {
+            if (jj < 0 || jj >= ny) continue;
+            for (int64_t ii = i - 1; ii <= i + 1; ii++) {
+              if (ii < 0 || ii >= nx) continue;
+              const int64_t m = ii + nx * (jj + ny * kk);
+              for (int c2 = 0; c2 < dim; c2++) {
+                const int32_t col = pb->eq[m * dim + c2];
+                if (col < 0) continue;
+                if (pass == 1) pb->Kcol[nnz] = col;
+                nnz++;
+              }
+            }
+          }
+        }
+      }
+    }
+    if (pass == 0) {
+      pb->Kcol = (int32_t*)malloc(sizeof(int32_t) * (nnz > 0 ? nnz : 1));
+      pb->Kval = (double*)calloc(nnz > 0 ? nnz : 1, sizeof(double));
+    } else {
+      pb->Kptr[nu] = nnz;
+    }
+  }
+  /* G pattern: each free velocity row touches its (up to 2^dim) elements. */
+  pb->Gptr = (int64_t*)calloc(nu + 1, sizeof(int64_t));
+  for (int pass = 0; pass < 2; pass++) {
+    int64_t nnz = 0;
+    for (int64_t n = 0; n < pb->N; n++) {
+      const int64_t i = n % nx, j = (n / nx) % ny, k = n / (nx * ny);
+      for (int c = 0; c < dim; c++) {
+        const int32_t row = pb->eq[n * dim + c];
+        if (row < 0) continue;
+        if (pass == 1) pb->Gptr[row] = nnz;
+        const int64_t k0 = dim == 3 ? k - 1 : 0, k1 = dim == 3 ? k : 0;
+        for (int64_t kk = k0; kk <= k1; kk++) {
+          if (kk < 0 || kk >= pb->res[2]) continue;
+          for (int64_t jj = j - 1; jj <= j; jj++) {
+            if (jj < 0 || jj >= pb->res[1]) continue;
+            for (int64_t ii = i - 1; ii <= i; ii++) {
+              if (ii < 0 || ii >= pb->res[0]) continue;
+              if (pass == 1) pb->Gcol[nnz] = (int32_t)(ii + pb->res[0] * (jj + pb->res[1] * kk));
+              nnz++;
+            }
+          }
+        }
+      }
+    }
+    if (pass == 0) {
+      pb->Gcol = (int32_t*)malloc(sizeof(int32_t) * (nnz > 0 ? nnz : 1));
+      pb->Gval = (double*)calloc(nnz > 0 ? nnz : 1, sizeof(double));
+    } else {
+      pb->Gptr[nu] = nnz;
+    }
+  }
+  pb->f = (double*)calloc(nu > 0 ? nu : 1, sizeof(double));
+  pb->h = (double*)calloc(pb->E, sizeof(double));
+  pb->Q = (double*)calloc(pb->E, sizeof(double));
+  /* element loop coloured by index parity so that scatters never collide */
+  const int ncol = dim == 2 ? 4 : 8;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+  for (int colour = 0; colour < ncol; colour++) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t e = 0; e < pb->E; e++) {
+      int32_t ijk[3];
+      elem_ijk(pb, e, ijk);
+      const int ec = (ijk[0] & 1) | ((ijk[1] & 1) << 1) | ((ijk[2] & 1) << 2);
+      if (ec != colour) continue;
+      double Ke[MAXDOF * MAXDOF], Ge[MAXDOF], fe[MAXDOF];
+      int64_t nodes[MAXNEN];
+      int32_t leq[MAXDOF];
+      double ubc[MAXDOF];
+      orc_element_K(pb, e, Ke);
+      orc_element_G(pb, e, Ge);
+      orc_element_f(pb, e, fe);
+      elem_nodes(pb, e, nodes);
+      for (int a = 0; a < nen; a++)
+        for (int d = 0; d < dim; d++) {
+          leq[a * dim + d] = pb->eq[nodes[a] * dim + d];
+          ubc[a * dim + d] = pb->bc_mask[nodes[a] * dim + d] ? pb->bc_value[nodes[a] * dim + d] : 0.0;
+        }
+      double q = 0.0, hl = 0.0;
+      for (int r = 0; r < nd; r++) {
+        q += Ge[r] * (Ge[r] / Ke[r * nd + r]);
+        const int32_t row = leq[r];
+        if (row < 0) {
+          hl -= ubc[r] * Ge[r];
+          continue;
+        }
+        double lift = 0.0;
+        for (int c = 0; c < nd; c++) {
+          const int32_t col = leq[c];
+          if (col < 0) {
+            lift -= ubc[c] * Ke[r * nd + c];
+            continue;
+          }
+          const int64_t pos = find_col(pb->Kcol, pb->Kptr[row], pb->Kptr[row + 1], col);
+          pb->Kval[pos] += Ke[r * nd + c];
+        }
+        pb->f[row] += fe[r] + lift;
+        const int64_t gpos = find_col(pb->Gcol, pb->Gptr[row], pb->Gptr[row + 1], (int32_t)e);
+        pb->Gval[gpos] += Ge[r];
+      }
+      pb->h[e] += hl;
+      pb->Q[e] += q;
+    }
+  }
+  pb->Kdiag = (double*)malloc(sizeof(double) * (nu > 0 ? nu : 1));
+  pb->Kdiagpos = (int64_t*)malloc(sizeof(int64_t) * (nu > 0 ? nu : 1));
+  for (int64_t r = 0; r < nu; r++) {
+    const int64_t pos = find_col(pb->Kcol, pb->Kptr[r], pb->Kptr[r + 1], (int32_t)r);
+    pb->Kdiagpos[r] = pos;
+    pb->Kdiag[r] = pb->Kval[pos];
+  }
+  pb->assembled = 1;
+  return ORC_OK;
+}
+
+void orc_system_sizes(const orc_problem* pb, int64_t* out4) {
+  out4[0] = pb->n_u;
+  out4[1] = pb->E;
+  out4[2] = pb->Kptr ? pb->Kptr[pb->n_u] : 0;
+  out4[3] = pb->Gptr ? pb->Gptr[pb->n_u] : 0;
+}
+
+void orc_system_export(const orc_problem* pb, int64_t* Kptr, int32_t* Kcol, double* Kval, int64_t* Gptr,
+                       int32_t* Gcol, double* Gval, double* f, double* h, double* Q, int32_t* eq) {
+  const int64_t nu = pb->n_u, nk = pb->Kptr[nu], ng = pb->Gptr[nu];
+  if (Kptr) memcpy(Kptr, pb->Kptr, sizeof(int64_t) * (nu + 1));
+  if (Kcol) memcpy(Kcol, pb->Kcol, sizeof(int32_t) * nk);
+  if (Kval) memcpy(Kval, pb->Kval, sizeof(double) * nk);
+  if (Gptr) memcpy(Gptr, pb->Gptr, sizeof(int64_t) * (nu + 1));
+  if (Gcol) memcpy(Gcol, pb->Gcol, sizeof(int32_t) * ng);
+  if (Gval) memcpy(Gval, pb->Gval, sizeof(double) * ng);
+  if (f) memcpy(f, pb->f, sizeof(double) * nu);
+  if (h) memcpy(h, pb->h, sizeof(double) * pb->E);
+  if (Q) memcpy(Q, pb->Q, sizeof(double) * pb->E);
+  if (eq) memcpy(eq, pb->eq, sizeof(int32_t) * pb->N * pb->dim);
+}
+
+/* Unconstrained element-by-element operators on full nodal vectors. */
+void orc_apply_K_full(orc_problem* pb, const double* x, double* y) {
+  const int dim = pb->dim, nen = pb->nen, nd = pb->ndof;
+  memset(y, 0, sizeof(double) * pb->N * dim);
+  for (int64_t e = 0; e < pb->E; e++) {
+    double Ke[MAXDOF * MAXDOF];
+    int64_t nodes[MAXNEN];
+    orc_element_K(pb, e, Ke);
+    elem_nodes(pb, e, nodes);
+    for (int r = 0; r < nd; r++) {
+      double s = 0.0;
+      for (int c = 0; c < nd; c++) s += Ke[r * nd + c] * x[nodes[c / dim] * dim + c % dim];
+      y[nodes[r / dim] * dim + r % dim] += s;
+    }
+  }
+  (void)nen;
+}
+
+void orc_diag_K_full(orc_problem* pb, double* dg) {
+  const int dim = pb->dim, nd = pb->ndof;
+  memset(dg, 0, sizeof(double) * pb->N * dim);
+  for (int64_t e = 0; e < pb->E; e++) {
+    double Ke[MAXDOF * MAXDOF];
+    int64_t nodes[MAXNEN];
+    orc_element_K(pb, e, Ke);
+    elem_nodes(pb, e, nodes);
+    for (int r = 0; r < nd; r++) dg[nodes[r / dim] * dim + r % dim] += Ke[r * nd + r];
+  }
+}
+
+void orc_apply_G_full(orc_problem* pb, const double* p, double* y) {
+  const int dim = pb->dim, nd = pb->ndof;
+  memset(y, 0, sizeof(double) * pb->N * dim);
+  for (int64_t e = 0; e < pb->E; e++) {
+    double Ge[MAXDOF];
+    int64_t nodes[MAXNEN];
+    orc_element_G(pb, e, Ge);
+    elem_nodes(pb, e, nodes);
+    for (int r = 0; r < nd; r++) y[nodes[r / dim] * dim + r % dim] += Ge[r] * p[e];
+  }
+}
+
+void orc_apply_Gt_full(orc_problem* pb, const double* x, double* q) {
+  const int dim = pb->dim, nd = pb->ndof;
+  for (int64_t e = 0; e < pb->E; e++) {
+    double Ge[MAXDOF];
+    int64_t nodes[MAXNEN];
+    orc_element_G(pb, e, Ge);
+    elem_nodes(pb, e, nodes);
+    double s = 0.0;
+    for (int r = 0; r < nd; r++) s += Ge[r] * x[nodes[r / dim] * dim + r % dim];
+    q[e] = s;
+  }
+}
+
+void orc_uzawa_Q(orc_problem* pb, double* Q) {
+  const int nd = pb->ndof;
+  for (int64_t e = 0; e < pb->E; e++) {
+    double Ke[MAXDOF * MAXDOF], Ge[MAXDOF];
+    orc_element_K(pb, e, Ke);
+    orc_element_G(pb, e, Ge);
+    double q = 0.0;
+    for (int r = 0; r < nd; r++) q += Ge[r] * (Ge[r] / Ke[r * nd + r]);
+    Q[e] = q;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sparse kernels and inner solvers                                          */
+/* ------------------------------------------------------------------------ */
+static void spmv(int64_t n, const int64_t* ptr, const int32_t* col, const double* val, const double* x,
+                 double* y) {
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; r++) {
+    double s = 0.0;
+    for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) s += val[k] * x[col[k]];
+    y[r] = s;
+  }
+}
+
+/* y = G^T x  (G is n_u x E in CSR by rows) */
+static void spmv_t(int64_t nrows, int64_t ncols, const int64_t* ptr, const int32_t* col, const double* val,
+                   const double* x, double* y) {
+  memset(y, 0, sizeof(double) * ncols);
+  for (int64_t r = 0; r < nrows; r++)
+    for (int64_t k = ptr[r]; k < ptr[r + 1]; k++) y[col[k]] += val[k] * x[r];
+}
+
+static double dot(int64_t n, const double* a, const double* b) {
+  double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+  for (int64_t i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+static double nrm2(int64_t n, const double* a) { return sqrt(dot(n, a, a)); }
+
+/* CG with Jacobi preconditioner, zero initial guess, stop on
+ * |r|_2 <= rtol |b|_2. */
+static int cg_jacobi(orc_problem* pb, const double* b, double* x, double rtol, int maxit, int64_t* its) {
+  const int64_t n = pb->n_u;
+  double *r = (double*)malloc(sizeof(double) * n * 4 + 8), *z = r + n, *p = z + n, *Ap = p + n;
+  memset(x, 0, sizeof(double) * n);
+  memcpy(r, b, sizeof(double) * n);
+  const double bn = nrm2(n, b);
+  int k = 0;
+  int st = ORC_OK;
+  if (bn == 0.0) goto done;
+  for (int64_t i = 0; i < n; i++) z[i] = r[i] / pb->Kdiag[i];
+  memcpy(p, z, sizeof(double) * n);
+  double rz = dot(n, r, z);
+  for (k = 0; k < maxit; k++) {
+    if (nrm2(n, r) <= rtol * bn) break;
+    spmv(n, pb->Kptr, pb->Kcol, pb->Kval, p, Ap);
+    const double alpha = rz / dot(n, p, Ap);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * Ap[i];
+      z[i] = r[i] / pb->Kdiag[i];
+    }
+    const double rz2 = dot(n, r, z);
+    const double beta = rz2 / rz;
+    rz = rz2;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) p[i] = z[i] + beta * p[i];
+  }
+  if (k >= maxit && nrm2(n, r) > rtol * bn) st = ORC_E_INNER_MAXIT;
+done:
+  *its += k;
+  free(r);
+  return st;
+}
+
+/* ILU(0) on the pattern of K (PETSc's serial default preconditioner). */
+static void ilu0_factor(orc_problem* pb) {
+  const int64_t n = pb->n_u;
+  if (pb->ilu) return;
+  pb->ilu = (double*)malloc(sizeof(double) * (pb->Kptr[n] > 0 ? pb->Kptr[n] : 1));
+  memcpy(pb->ilu, pb->Kval, sizeof(double) * pb->Kptr[n]);
+  double* a = pb->ilu;
+  int64_t* pos = (int64_t*)malloc(sizeof(int64_t) * (n > 0 ? n : 1));
+  for (int64_t i = 0; i < n; i++) pos[i] = -1;
+  for (int64_t i = 0; i < n; i++) {
+    for (int64_t k = pb->Kptr[i]; k < pb->Kptr[i + 1]; k++) pos[pb->Kcol[k]] = k;
+    for (int64_t kk = pb->Kptr[i]; kk < pb->Kptr[i + 1]; kk++) {
+      const int32_t k = pb->Kcol[kk];
+      if (k >= i) break;
+      a[kk] /= a[pb->Kdiagpos[k]];
+      for (int64_t jj = pb->Kdiagpos[k] + 1; jj < pb->Kptr[k + 1]; jj++) {
+        const int64_t p = pos[pb->Kcol[jj]];
+        if (p >= 0) a[p] -= a[kk] * a[jj];
+      }
+    }
+    for (int64_t k = pb->Kptr[i]; k < pb->Kptr[i + 1]; k++) pos[pb->Kcol[k]] = -1;
+  }
+  free(pos);
+}
+
+static void ilu0_solve(const orc_problem* pb, const double* b, double* x) {
+  const int64_t n = pb->n_u;
+  const double* a = pb->ilu;
+  for (int64_t i = 0; i < n; i++) {
+    double s = b[i];
+    for (int64_t k = pb->Kptr[i]; k < pb->Kdiagpos[i]; k++) s -= a[k] * x[pb->Kcol[k]];
+    x[i] = s;
+  }
+  for (int64_t i = n - 1; i >= 0; i--) {
+    double s = x[i];
+    for (int64_t k = pb->Kdiagpos[i] + 1; k < pb->Kptr[i + 1]; k++) s -= a[k] * x[pb->Kcol[k]];
+    x[i] = s / a[pb->Kdiagpos[i]];
+  }
+}
+
+/* Restarted GMRES(30), left preconditioned by ILU(0), modified Gram-Schmidt,
+ * zero initial guess, stop when |M^-1 r| <= rtol |M^-1 b| (PETSc 3.2 KSP
+ * defaults used by the reference's velocity solve). */
+static int gmres_ilu(orc_problem* pb, const double* b, double* x, double rtol, int maxit, int64_t* its) {
+  const int64_t n = pb->n_u;
+  const int m = 30;
+  ilu0_factor(pb);
+  double* V = (double*)malloc(sizeof(double) * n * (m + 1));
+  double* w = (double*)malloc(sizeof(double) * n * 2);
+  double* t = w + n;
+  double H[31][30], cs[30], sn[30], g[31], y[30];
+  memset(x, 0, sizeof(double) * n);
+  ilu0_solve(pb, b, t);
+  const double r0 = nrm2(n, t);
+  int total = 0, st = ORC_OK;
+  if (r0 == 0.0) goto done;
+  for (;;) {
+    /* r = M^-1 (b - K x) */
+    spmv(n, pb->Kptr, pb->Kcol, pb->Kval, x, w);
+    for (int64_t i = 0; i < n; i++) w[i] = b[i] - w[i];
+    ilu0_solve(pb, w, t);
+    double beta = nrm2(n, t);
+    if (beta <= rtol * r0) break;
+    if (total >= maxit) {
+      st = ORC_E_INNER_MAXIT;
+      break;
+    }
+    for (int64_t i = 0; i < n; i++) V[i] = t[i] / beta;
+    memset(g, 0, sizeof(g));
+    g[0] = beta;
+    int j;
+    int conv = 0;
+    for (j = 0; j < m && total < maxit; j++) {
+      spmv(n, pb->Kptr, pb->Kcol, pb->Kval, V + (int64_t)j * n, w);
+      ilu0_solve(pb, w, t);
+      for (int i = 0; i <= j; i++) {
+        H[i][j] = dot(n, t, V + (int64_t)i * n);
+        const double hij = H[i][j];
+        const double* vi = V + (int64_t)i * n;
+        for (int64_t k = 0; k < n; k++) t[k] -= hij * vi[k];
+      }
+      H[j + 1][j] = nrm2(n, t);
+      if (H[j + 1][j] != 0.0) {
+        double* vj = V + (int64_t)(j + 1) * n;
+        for (int64_t k = 0; k < n; k++) vj[k] = t[k] / H[j + 1][j];
+      }
+      for (int i = 0; i < j; i++) {
+        const double tmp = cs[i] * H[i][j] + sn[i] * H[i + 1][j];
+        H[i + 1][j] = -sn[i] * H[i][j] + cs[i] * H[i + 1][j];
+        H[i][j] = tmp;
+      }
+      const double den = sqrt(H[j][j] * H[j][j] + H[j + 1][j] * H[j + 1][j]);
+      cs[j] = den == 0.0 ? 1.0 : H[j][j] / den;
+      sn[j] = den == 0.0 ? 0.0 : H[j + 1][j] / den;
+      H[j][j] = cs[j] * H[j][j] + sn[j] * H[j + 1][j];
+      H[j + 1][j] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      total++;
+      if (fabs(g[j + 1]) <= rtol * r0) {
+        j++;
+        conv = 1;
+        break;
+      }
+    }
+    for (int i = j - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int k = i + 1; k < j; k++) s -= H[i][k] * y[k];
+      y[i] = s / H[i][i];
+    }
+    for (int i = 0; i < j; i++) {
+      const double* vi = V + (int64_t)i * n;
+      for (int64_t k = 0; k < n; k++) x[k] += y[i] * vi[k];
+    }
+    if (conv) break;
+  }
+done:
+  *its += total;
+  free(V);
+  free(w);
+  return st;
+}
+
+static int inner_solve(orc_problem* pb, const orc_uzawa_params* up, const double* b, double* x, double rtol,
+                       int64_t* its) {
+  const int maxit = up->inner_max_iterations > 0 ? up->inner_max_iterations : 100000;
+  if (up->inner == 1) return gmres_ilu(pb, b, x, rtol, maxit, its);
+  return cg_jacobi(pb, b, x, rtol, maxit, its);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Uzawa: Stokes_SLE_UzawaSolver.cxx:393-938                                 */
+/* ------------------------------------------------------------------------ */
+static int good(double v) { return isfinite(v); }
+
+static void expand_u(const orc_problem* pb, const double* uf, double* ufull) {
+  const int64_t ND = pb->N * pb->dim;
+  for (int64_t k = 0; k < ND; k++) ufull[k] = pb->eq[k] < 0 ? pb->bc_value[k] : uf[pb->eq[k]];
+}
+
+static int uzawa_reduced(orc_problem* pb, const orc_uzawa_params* up, double* u, double* q, orc_report* rep) {
+  const int64_t nu = pb->n_u, np = pb->E;
+  const double tol = up->tolerance;
+  double* buf = (double*)calloc((size_t)(3 * np + 2 * nu + 8), sizeof(double));
+  double *qTemp = buf, *r = qTemp + np, *s = r + np, *fTemp = s + np, *vStar = fTemp + nu;
+  int64_t inner = 0;
+  int st = ORC_OK;
+  double zr_prev = 1.0, relRes = 0.0, absRes = 0.0;
+  int it = 0;
+  const double qScale = sqrt((double)np), qRecip = 1.0 / qScale;
+  const double t0 = now_s();
+  /* :476-493 zero force firewall */
+  if (nrm2(nu, pb->f) / sqrt((double)nu) <= 1e-99) {
+    st = ORC_E_ZERO_FORCE;
+    goto out;
+  }
+  /* STEP 1 :553-565 */
+  {
+    const double rt = up->inner_rtol_override > 0 ? up->inner_rtol_override : tol;
+    int s1 = inner_solve(pb, up, pb->f, vStar, rt, &inner);
+    if (s1) { st = s1; goto out; }
+  }
+  spmv_t(nu, np, pb->Gptr, pb->Gcol, pb->Gval, vStar, qTemp);
+  for (int64_t i = 0; i < np; i++) qTemp[i] -= pb->h[i];
+  /* STEP 2 :587-605 */
+  const double irt = up->inner_rtol_override > 0 ? up->inner_rtol_override
+                                                 : (up->inner_rtol_factor > 0 ? up->inner_rtol_factor : 0.1) * tol;
+  const double uzawaRhsScale = nrm2(np, qTemp) * qRecip;
+  if (!good(uzawaRhsScale)) { st = ORC_E_NAN; goto out; }
+  /* STEP 3 :616-645  fTemp = f - G q ; u = K^-1 fTemp ; r = G^T u - h */
+  spmv(nu, pb->Gptr, pb->Gcol, pb->Gval, q, fTemp);
+  for (int64_t i = 0; i < nu; i++) fTemp[i] = pb->f[i] - fTemp[i];
+  {
+    int s3 = inner_solve(pb, up, fTemp, u, irt, &inner);
+    if (s3) { st = s3; goto out; }
+  }
+  spmv_t(nu, np, pb->Gptr, pb->Gcol, pb->Gval, u, r);
+  {
+    const double divU = nrm2(np, r) / nrm2(nu, u);
+    if (!good(divU)) { st = ORC_E_NAN; goto out; }
+  }
+  for (int64_t i = 0; i < np; i++) r[i] -= pb->h[i];
+  /* STEP 4 :664-837 */
+  do {
+    if (up->use_preconditioner)
+      for (int64_t i = 0; i < np; i++) qTemp[i] = r[i] / pb->Q[i];
+    else
+      memcpy(qTemp, r, sizeof(double) * np);
+    const double zr = dot(np, qTemp, r);
+    if (it == 0)
+      memcpy(s, qTemp, sizeof(double) * np);
+    else {
+      const double beta = zr / zr_prev;
+      for (int64_t i = 0; i < np; i++) s[i] = qTemp[i] + beta * s[i];
+    }
+    spmv(nu, pb->Gptr, pb->Gcol, pb->Gval, s, fTemp);
+    {
+      int s4 = inner_solve(pb, up, fTemp, vStar, irt, &inner);
+      if (s4) { st = s4; goto out; }
+    }
+    spmv_t(nu, np, pb->Gptr, pb->Gcol, pb->Gval, vStar, qTemp);
+    const double sq = dot(np, s, qTemp);
+    const double alpha = zr / sq;
+    if (!good(zr) || !good(sq) || !good(alpha)) { st = ORC_E_NAN; goto out; }
+    for (int64_t i = 0; i < np; i++) q[i] += alpha * s[i];
+    for (int64_t i = 0; i < nu; i++) u[i] -= alpha * vStar[i];
+    for (int64_t i = 0; i < np; i++) r[i] -= alpha * qTemp[i];
+    zr_prev = zr;
+    absRes = nrm2(np, r) * qRecip;
+    relRes = absRes / uzawaRhsScale;
+    if (!good(absRes)) { st = ORC_E_NAN; goto out; }
+    if (!(it < up->max_iterations)) { st = ORC_E_MAXIT; goto out; }
+    it++;
+  } while (((up->use_absolute_tolerance ? absRes : relRes) > tol) || (it < up->min_iterations));
+out:
+  rep->status = st;
+  rep->uzawa_iterations = it;
+  rep->inner_iterations += inner;
+  rep->rel_residual = relRes;
+  rep->abs_residual = absRes;
+  rep->solve_seconds += now_s() - t0;
+  if (st == ORC_OK) {
+    /* STEP 5 summary :844-870 */
+    spmv_t(nu, np, pb->Gptr, pb->Gcol, pb->Gval, u, r);
+    rep->div_u = nrm2(np, r) / nrm2(nu, u);
+    spmv(nu, pb->Gptr, pb->Gcol, pb->Gval, q, vStar);
+    spmv(nu, pb->Kptr, pb->Kcol, pb->Kval, u, fTemp);
+    for (int64_t i = 0; i < nu; i++) fTemp[i] = pb->f[i] - (fTemp[i] + vStar[i]);
+    rep->momentum_residual = nrm2(nu, fTemp) / nrm2(nu, pb->f);
+  }
+  free(buf);
+  return st;
+}
+
+int32_t orc_uzawa(orc_problem* pb, const orc_uzawa_params* up, const double* p0, double* u_full, double* p,
+                  orc_report* rep) {
+  memset(rep, 0, sizeof(*rep));
+  if (!pb->assembled) {
+    const double ta = now_s();
+    int32_t st = orc_assemble(pb, NULL, NULL, 0, up->nthreads);
+    rep->assemble_seconds = now_s() - ta;
+    if (st) return rep->status = st;
+  }
+#ifdef _OPENMP
+  if (up->nthreads > 0) omp_set_num_threads(up->nthreads);
+#endif
+  double* uf = (double*)calloc(pb->n_u > 0 ? pb->n_u : 1, sizeof(double));
+  double* q = (double*)calloc(pb->E, sizeof(double));
+  if (p0) memcpy(q, p0, sizeof(double) * pb->E);
+  int32_t st = uzawa_reduced(pb, up, uf, q, rep);
+  if (u_full) expand_u(pb, uf, u_full);
+  if (p) memcpy(p, q, sizeof(double) * pb->E);
+  free(uf);
+  free(q);
+  return st;
+}
+
+/* Picard loop: SystemLinearEquations.cxx:735-920.  Iteration 0 is linear
+ * (YieldRheology.cxx:203-204); the pressure vector carries over between
+ * linear solves as the Uzawa initial guess; the residual is
+ * |u_prev - u| / |u| over the free velocity dofs. */
+int32_t orc_picard(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp, double* u_full,
+                   double* p, orc_report* rep) {
+  memset(rep, 0, sizeof(*rep));
+  const int64_t ND = pb->N * pb->dim;
+  double* uf_full = (double*)calloc(ND, sizeof(double));
+  double* q = (double*)calloc(pb->E, sizeof(double));
+  double *uprev = NULL, *ucur = NULL;
+  orc_report r1;
+  int32_t st;
+  double ta = now_s();
+  st = orc_assemble(pb, NULL, NULL, 0, up->nthreads);
+  rep->assemble_seconds += now_s() - ta;
+  if (st) goto out;
+  memset(&r1, 0, sizeof(r1));
+  st = orc_uzawa(pb, up, q, uf_full, q, &r1);
+  rep->inner_iterations += r1.inner_iterations;
+  rep->uzawa_iterations += r1.uzawa_iterations;
+  rep->solve_seconds += r1.solve_seconds;
+  rep->rel_residual = r1.rel_residual;
+  if (st) goto out;
+  {
+    const int64_t nu = pb->n_u;
+    uprev = (double*)malloc(sizeof(double) * (nu > 0 ? nu : 1));
+    ucur = (double*)malloc(sizeof(double) * (nu > 0 ? nu : 1));
+    int converged = 0;
+    int it;
+    double res = 0.0;
+    for (it = 1; it < pp->max_iterations; it++) {
+      for (int64_t k = 0; k < ND; k++)
+        if (pb->eq[k] >= 0) uprev[pb->eq[k]] = uf_full[k];
+      ta = now_s();
+      st = orc_assemble(pb, uf_full, q, 1, up->nthreads);
+      rep->assemble_seconds += now_s() - ta;
+      if (st) goto out;
+      st = orc_uzawa(pb, up, q, uf_full, q, &r1);
+      rep->inner_iterations += r1.inner_iterations;
+      rep->uzawa_iterations += r1.uzawa_iterations;
+      rep->solve_seconds += r1.solve_seconds;
+      rep->rel_residual = r1.rel_residual;
+      if (st) goto out;
+      for (int64_t k = 0; k < ND; k++)
+        if (pb->eq[k] >= 0) ucur[pb->eq[k]] = uf_full[k];
+      double dn = 0.0, cn = 0.0;
+      for (int64_t i = 0; i < nu; i++) {
+        dn += (uprev[i] - ucur[i]) * (uprev[i] - ucur[i]);
+        cn += ucur[i] * ucur[i];
+      }
+      res = sqrt(dn) / sqrt(cn);
+      converged = res < pp->tolerance;
+      if (converged && it >= pp->min_iterations) break;
+    }
+    rep->picard_iterations = it < pp->max_iterations ? it + 1 : it;
+    rep->picard_residual = res;
+    if (!converged && pp->kill_non_convergent) st = ORC_E_PICARD;
+  }
+out:
+  rep->status = st;
+  if (u_full) memcpy(u_full, uf_full, sizeof(double) * ND);
+  if (p) memcpy(p, q, sizeof(double) * pb->E);
+  free(uf_full);
+  free(q);
+  free(uprev);
+  free(ucur);
+  return st;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dt and advection                                                           */
+/* ------------------------------------------------------------------------ */
+/* UpdateDt.cxx:53-84: dt = dtFactor * 0.5 * minSeparation / max|v| */
+double orc_compute_dt(orc_problem* pb, const double* u, double dt_factor) {
+  double vmax = 0.0;
+  for (int64_t n = 0; n < pb->N; n++) {
+    double s = 0.0;
+    for (int d = 0; d < pb->dim; d++) s += u[n * pb->dim + d] * u[n * pb->dim + d];
+    s = sqrt(s);
+    if (s > vmax) vmax = s;
+  }
+  double minsep = INFINITY;
+  for (int d = 0; d < pb->dim; d++)
+    for (int32_t i = 0; i < pb->res[d]; i++) {
+      const double h = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], i + 1) -
+                       orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], i);
+      if (h < minsep) minsep = h;
+    }
+  if (vmax == 0.0) return 0.0;
+  return dt_factor * 0.5 * minsep / vmax;
+}
+
+/* FeVariable.cxx:904-950 + InterpolateWithinElement: 0 if outside. */
+int32_t orc_interpolate_velocity(orc_problem* pb, const double* u, const double* x, double* v) {
+  int32_t e;
+  if (!orc_search_element(pb->dim, pb->res, pb->lo, pb->hi, x, &e)) return 0;
+  double xi[3], N[MAXNEN];
+  int64_t nodes[MAXNEN];
+  global_to_local(pb, e, x, xi);
+  shape_funcs(pb->dim, xi, N);
+  elem_nodes(pb, e, nodes);
+  for (int d = 0; d < pb->dim; d++) {
+    double s = 0.0;
+    for (int a = 0; a < pb->nen; a++) s += N[a] * u[nodes[a] * pb->dim + d];
+    v[d] = s;
+    if (isinf(s)) return 0;
+  }
+  return 1;
+}
+
+/* TimeIntegrand.cxx:265-335 midpoint RK2 with first-order fallback
+ * (:317-331, _TimeIntegrand_RewindToStartAndApplyFirstOrderUpdate :459-483)
+ * and SwarmAdvector.cxx:261-280 as the time derivative. */
+int32_t orc_advect_rk2(orc_problem* pb, const double* u, double dt, double* coord_out) {
+  const int dim = pb->dim;
+  for (int64_t p = 0; p < pb->P; p++) {
+    double x0[3], x[3], v[3];
+    memcpy(x0, pb->coord + 3 * p, sizeof(x0));
+    memcpy(x, x0, sizeof(x));
+    if (!orc_interpolate_velocity(pb, u, x, v)) return ORC_E_ESCAPED;
+    for (int d = 0; d < dim; d++) x[d] = x0[d] + 0.5 * dt * v[d];
+    if (orc_interpolate_velocity(pb, u, x, v)) {
+      for (int d = 0; d < dim; d++) x[d] = x0[d] + dt * v[d];
+    } else {
+      orc_interpolate_velocity(pb, u, x0, v);
+      for (int d = 0; d < dim; d++) x[d] = x0[d] + dt * v[d];
+    }
+    memcpy(coord_out + 3 * p, x, sizeof(x));
+  }
+  return ORC_OK;
+}

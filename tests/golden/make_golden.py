@@ -1,0 +1,61 @@
+"""Writes the golden fixtures under tests/golden/ from data files that the
+reference's own tests hold (run once in a tree where the reference source is
+available; the outputs are committed and the tests never read the reference).
+
+  k_isoviscous_3x3.npz        StgFEM/Assembly/tests/expected/testIsoviscousSiffness.matrix
+  k_material_viscosity_10x10.npz
+                              Underworld/Rheology/tests/expected/testMaterialViscosity2D.matrix
+  constitutive_2d.json        Underworld/Rheology/tests/expected/testConstitutiveMatrix_Cartesian2D.expected
+
+The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
+1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
+they are parsed here as data.
+"""
+import json
+import os
+import re
+import sys
+
+import numpy as np
+
+REF = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def load_petsc_aij(path):
+    b = open(path, "rb").read()
+    cls, M, N, nnz = np.frombuffer(b[:16], dtype=">i4")
+    assert cls == 1211216
+    off = 16
+    rl = np.frombuffer(b[off:off + 4 * M], dtype=">i4").astype(np.int64)
+    off += 4 * M
+    cols = np.frombuffer(b[off:off + 4 * nnz], dtype=">i4").astype(np.int32)
+    off += 4 * nnz
+    vals = np.frombuffer(b[off:off + 8 * nnz], dtype=">f8").astype(np.float64)
+    indptr = np.concatenate([[0], np.cumsum(rl)])
+    return dict(shape=np.array([M, N]), indptr=indptr, indices=cols, data=vals)
+
+
+def main():
+    pairs = [
+        ("StgFEM/Assembly/tests/expected/testIsoviscousSiffness.matrix", "k_isoviscous_3x3.npz"),
+        ("Underworld/Rheology/tests/expected/testMaterialViscosity2D.matrix", "k_material_viscosity_10x10.npz"),
+    ]
+    for src, dst in pairs:
+        m = load_petsc_aij(os.path.join(REF, src))
+        np.savez_compressed(os.path.join(OUT, dst), **m)
+        print(dst, m["shape"], len(m["data"]))
+    txt = open(os.path.join(REF, "Underworld/Rheology/tests/expected/testConstitutiveMatrix_Cartesian2D.expected")).read()
+    block = txt.split("SetMatrixWithViscosity7")[1].split("SetMatrixWithSecondViscosity2")[0]
+    D = [[float(v) for v in re.findall(r"matrixData\[ %d \]\[ \d \] = ([-\d.]+)" % i, block)] for i in range(3)]
+    visc = float(re.search(r"Viscosity =\s+([-\d.]+)", block).group(1))
+    nums = [float(v) for v in re.findall(r"^\s+([-\d.]+)\s+([-\d.]+)\s*$", block, re.M) for v in v]
+    strain = [[nums[0], nums[1]], [nums[2], nums[3]]]
+    stress = [[nums[4], nums[5]], [nums[6], nums[7]]]
+    json.dump(dict(viscosity=visc, D=D, strain_rate=strain, stress=stress),
+              open(os.path.join(OUT, "constitutive_2d.json"), "w"), indent=1)
+    print("constitutive_2d.json", visc, D, strain, stress)
+
+
+if __name__ == "__main__":
+    main()
