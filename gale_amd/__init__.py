@@ -22,6 +22,8 @@ OK, E_NAN, E_MAXIT, E_ZERO_FORCE, E_STAGNATION, E_HIP, E_INVALID, E_ESCAPED, E_I
 YIELD_NONE, YIELD_DRUCKER_PRAGER, YIELD_VON_MISES = 0, 1, 2
 PC_NONE, PC_DIAGONAL = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
+INNER_JACOBI, INNER_MULTIGRID = 0, 1
+APPLY_PARTICLES_FP64, APPLY_PARTICLES_FP32, APPLY_MOMENTS_FP64, APPLY_MOMENTS_FP32 = 0, 1, 2, 3
 SHAPE_EVERYWHERE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_BELOW_SINES = 0, 1, 2, 3
 WALLS = {"left": 0, "right": 1, "bottom": 2, "top": 3, "back": 4, "front": 5}
 
@@ -31,7 +33,7 @@ EXPORTS = (
     "gs_problem_particles", "gs_solver_create", "gs_solver_set_params", "gs_solver_destroy", "gs_solver_setup",
     "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
-    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays",
+    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
 )
 
@@ -67,6 +69,7 @@ class UzawaParams(C.Structure):
         ("use_absolute_tolerance", C.c_int32), ("preconditioner", C.c_int32), ("precision", C.c_int32),
         ("inner_max_iterations", C.c_int32), ("inner_rtol_factor", C.c_double),
         ("inner_rtol_override", C.c_double), ("monitor", C.c_int32), ("check_every", C.c_int32),
+        ("inner_preconditioner", C.c_int32), ("mg_smooth", C.c_int32),
     ]
 
 
@@ -131,6 +134,7 @@ def lib():
             "gs_problem_set_coords": (i32, [vp, vp]),
             "gs_apply_K": (i32, [vp, vp, vp]),
             "gs_apply_K_fp32": (i32, [vp, vp, vp]),
+            "gs_apply_K_moments": (i32, [vp, vp, vp]),
             "gs_apply_G": (i32, [vp, vp, vp]),
             "gs_apply_Gt": (i32, [vp, vp, vp]),
             "gs_solver_sync": (i32, [vp]),
@@ -395,12 +399,14 @@ class Problem:
 
 def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
                  Preconditioner=True, precision=PREC_FP64, inner_max_iterations=0, inner_rtol_factor=0.1,
-                 inner_rtol_override=0.0, monitor=False, check_every=4) -> UzawaParams:
+                 inner_rtol_override=0.0, monitor=False, check_every=4, inner_preconditioner=INNER_MULTIGRID,
+                 mg_smooth=2) -> UzawaParams:
     """Stokes_SLE_UzawaSolver parameters under their XML names
-    (Stokes_SLE_UzawaSolver.cxx:251-257 defaults)."""
+    (Stokes_SLE_UzawaSolver.cxx:251-257 defaults) plus the inner-solve controls."""
     return UzawaParams(tolerance, maxIterations, minIterations, int(bool(useAbsoluteTolerance)),
                        PC_DIAGONAL if Preconditioner else PC_NONE, precision, inner_max_iterations,
-                       inner_rtol_factor, inner_rtol_override, int(bool(monitor)), check_every)
+                       inner_rtol_factor, inner_rtol_override, int(monitor), check_every,
+                       inner_preconditioner, mg_smooth)
 
 
 class Stokes_SLE_UzawaSolver:
@@ -498,6 +504,10 @@ class Stokes_SLE_UzawaSolver:
     def apply_K_fp32(self, x, y):
         _check(lib().gs_apply_K_fp32(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "apply_K_fp32")
 
+    def apply_K_moments(self, x, y):
+        _check(lib().gs_apply_K_moments(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
+               "apply_K_moments")
+
     def apply_G(self, p, y):
         _check(lib().gs_apply_G(self._h, C.c_void_p(p.data_ptr()), C.c_void_p(y.data_ptr())), "apply_G")
 
@@ -507,7 +517,7 @@ class Stokes_SLE_UzawaSolver:
     def sync(self):
         _check(lib().gs_solver_sync(self._h))
 
-    def time_apply_K(self, precision=PREC_FP64, n_rep=20):
+    def time_apply_K(self, form=APPLY_PARTICLES_FP64, n_rep=20):
         ms = C.c_double()
-        _check(lib().gs_time_apply_K(self._h, precision, n_rep, C.byref(ms)), "time_apply_K")
+        _check(lib().gs_time_apply_K(self._h, form, n_rep, C.byref(ms)), "time_apply_K")
         return ms.value

@@ -17,6 +17,7 @@
 
 #include "../../include/gale_stokes.h"
 #include "gs_internal.h"
+#include "gs_mg.h"
 
 using namespace gs;
 
@@ -84,6 +85,37 @@ struct gs_problem {
   bool min_one_particle = true;
 };
 
+
+// One multigrid level: moments (FP64 + FP32 copies), Dirichlet flags,
+// Jacobi diagonal, Chebyshev bound and work vectors in both precisions.
+struct MGLevel {
+  Geo g{};
+  long long n = 0;
+  double* M64 = nullptr;
+  float* M32 = nullptr;
+  uint8_t* bcflag = nullptr;
+  uint8_t* bcdof = nullptr;
+  bool own_bc = false;
+  double* diag = nullptr;
+  double* dinv64 = nullptr;
+  float* dinv32 = nullptr;
+  double lmax = 2.0;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *Ax = nullptr;
+  float *bf = nullptr, *xf = nullptr, *rf = nullptr, *df = nullptr, *Axf = nullptr;
+};
+
+struct MGData {
+  bool structured = false;
+  int L = 0;  // index of the coarsest level
+  std::vector<MGLevel> lv;
+  int nfc = 0;
+  int* freeidx = nullptr;
+  double* Kinv64 = nullptr;
+  float* Kinv32 = nullptr;
+};
+
+static void mg_free(MGData& mg);
+
 struct gs_solver {
   gs_problem* pb = nullptr;
   gs_uzawa_params up{};
@@ -97,6 +129,8 @@ struct gs_solver {
   double* S = nullptr;
   double* hS = nullptr;
   Reducer rd{};
+  double* cg_z = nullptr;
+  MGData mg;
   bool setup_done = false;
   bool has_solution = false;
   int has_executed = 0;
@@ -406,6 +440,8 @@ static gs_uzawa_params default_params() {
   p.inner_rtol_override = 0.0;
   p.monitor = 0;
   p.check_every = 4;
+  p.inner_preconditioner = GS_INNER_MULTIGRID;
+  p.mg_smooth = 2;
   return p;
 }
 
@@ -415,6 +451,10 @@ extern "C" int32_t gs_solver_set_params(gs_solver* s, const gs_uzawa_params* up)
   if (s->up.check_every <= 0) s->up.check_every = 4;
   if (s->up.inner_max_iterations <= 0) s->up.inner_max_iterations = 50000;
   if (s->up.inner_rtol_factor <= 0) s->up.inner_rtol_factor = 0.1;
+  if (s->up.mg_smooth <= 0) s->up.mg_smooth = 2;
+  if (s->up.inner_preconditioner != GS_INNER_JACOBI && s->up.inner_preconditioner != GS_INNER_MULTIGRID)
+    return GS_E_INVALID;
+  s->setup_done = false;
   if (!(s->up.tolerance > 0)) return GS_E_INVALID;
   return GS_OK;
 }
@@ -438,7 +478,7 @@ extern "C" int32_t gs_solver_create(gs_problem* pb, const gs_uzawa_params* up, g
   if (dalloc(&s->f, n) || dalloc(&s->diag, n) || dalloc(&s->dinv, n) || dalloc(&s->h, np) || dalloc(&s->Q, np) ||
       dalloc(&s->Qinv, np) || dalloc(&s->u, n) || dalloc(&s->ufull, n) || dalloc(&s->p, np) || dalloc(&s->r, np) ||
       dalloc(&s->s, np) || dalloc(&s->z, np) || dalloc(&s->q, np) || dalloc(&s->fT, n) || dalloc(&s->vstar, n) ||
-      dalloc(&s->cg_r, n) || dalloc(&s->cg_p, n) || dalloc(&s->cg_Ap, n) || dalloc(&s->tmpN, n) ||
+      dalloc(&s->cg_r, n) || dalloc(&s->cg_p, n) || dalloc(&s->cg_z, n) || dalloc(&s->cg_Ap, n) || dalloc(&s->tmpN, n) ||
       dalloc(&s->tmpE, np) || dalloc(&s->eta, P) || dalloc(&s->uprev, n) || dalloc(&s->yielded, P) ||
       dalloc(&s->S, S_COUNT))
     return fail(GS_E_HIP);
@@ -460,6 +500,8 @@ extern "C" void gs_solver_destroy(gs_solver* s) {
   if (s->pb && s->pb->st) (void)hipStreamSynchronize(s->pb->st);
   dfree(s->f); dfree(s->diag); dfree(s->dinv); dfree(s->h); dfree(s->Q); dfree(s->Qinv); dfree(s->u);
   dfree(s->ufull); dfree(s->p); dfree(s->r); dfree(s->s); dfree(s->z); dfree(s->q); dfree(s->fT); dfree(s->vstar);
+  mg_free(s->mg);
+  dfree(s->cg_z);
   dfree(s->cg_r); dfree(s->cg_p); dfree(s->cg_Ap); dfree(s->tmpN); dfree(s->tmpE); dfree(s->eta); dfree(s->uprev);
   dfree(s->yielded); dfree(s->x32); dfree(s->y32); dfree(s->S); dfree(s->rd.partials); dfree(s->rd.counter);
   if (s->hS) (void)hipHostFree(s->hS);
@@ -478,6 +520,316 @@ static void apply_K_masked(gs_solver* s, const double* x, double* y) {
   gs_problem* pb = s->pb;
   launch_apply_K(pb->g, x, y, pb->pk, pb->off, pb->bcflag, pb->st);
   s->k_applies++;
+}
+
+// ---------------------------------------------------------------------------
+// Geometric multigrid (levels from the particle moments, gs_mg.hip).
+// ---------------------------------------------------------------------------
+static void mg_free(MGData& mg) {
+  for (auto& L : mg.lv) {
+    dfree(L.M64); dfree(L.M32); dfree(L.diag); dfree(L.dinv64); dfree(L.dinv32);
+    dfree(L.b); dfree(L.x); dfree(L.r); dfree(L.d); dfree(L.Ax);
+    dfree(L.bf); dfree(L.xf); dfree(L.rf); dfree(L.df); dfree(L.Axf);
+    if (L.own_bc) {
+      dfree(L.bcflag);
+      dfree(L.bcdof);
+    }
+  }
+  mg.lv.clear();
+  dfree(mg.freeidx); dfree(mg.Kinv64); dfree(mg.Kinv32);
+  mg.structured = false;
+  mg.L = 0;
+  mg.nfc = 0;
+}
+
+static int32_t mg_alloc_level(MGLevel& L) {
+  const int nm = n_moments(L.g.dim);
+  L.n = L.g.N * L.g.dim;
+  CK(dalloc(&L.M64, L.g.E * nm)); CK(dalloc(&L.M32, L.g.E * nm));
+  CK(dalloc(&L.diag, L.n)); CK(dalloc(&L.dinv64, L.n)); CK(dalloc(&L.dinv32, L.n));
+  CK(dalloc(&L.b, L.n)); CK(dalloc(&L.x, L.n)); CK(dalloc(&L.r, L.n)); CK(dalloc(&L.d, L.n)); CK(dalloc(&L.Ax, L.n));
+  CK(dalloc(&L.bf, L.n)); CK(dalloc(&L.xf, L.n)); CK(dalloc(&L.rf, L.n)); CK(dalloc(&L.df, L.n));
+  CK(dalloc(&L.Axf, L.n));
+  return GS_OK;
+}
+
+// Level geometry and Dirichlet flags.  A coarse node takes the flags of the
+// fine node it coincides with; coarsening stops where that would not make P
+// map coarse free dofs onto fine free dofs exactly (then the Galerkin
+// identity P^T K P = K_coarse would not hold for the constrained operator).
+static int32_t mg_structure(gs_solver* s) {
+  gs_problem* pb = s->pb;
+  MGData& mg = s->mg;
+  mg_free(mg);
+  const int dim = pb->dim;
+  MGLevel L0;
+  L0.g = pb->g;
+  L0.bcflag = pb->bcflag;
+  L0.bcdof = pb->bcdof;
+  mg.lv.push_back(L0);
+  CKS(mg_alloc_level(mg.lv.back()));
+  std::vector<uint8_t> flag(pb->g.N);
+  CK(hipMemcpy(flag.data(), pb->bcflag, pb->g.N, hipMemcpyDeviceToHost));
+  for (int lev = 0; lev < 16; lev++) {
+    const Geo& gf = mg.lv.back().g;
+    bool ok = true;
+    for (int d = 0; d < dim; d++) ok = ok && gf.res[d] % 2 == 0 && gf.res[d] / 2 >= 2;
+    if (!ok) break;
+    Geo gc = gf;
+    gc.N = 1;
+    gc.E = 1;
+    for (int d = 0; d < 3; d++) {
+      gc.res[d] = d < dim ? gf.res[d] / 2 : 1;
+      gc.nn[d] = d < dim ? gc.res[d] + 1 : 1;
+      gc.N *= gc.nn[d];
+      gc.E *= gc.res[d];
+    }
+    std::vector<uint8_t> cflag(gc.N);
+    for (long long K = 0; K < gc.nn[2]; K++)
+      for (long long J = 0; J < gc.nn[1]; J++)
+        for (long long I = 0; I < gc.nn[0]; I++)
+          cflag[I + gc.nn[0] * (J + gc.nn[1] * K)] = flag[2 * I + gf.nn[0] * (2 * J + gf.nn[1] * (dim == 3 ? 2 * K : 0))];
+    // consistency: a fine dof is constrained iff all its interpolation parents are
+    for (long long k = 0; k < gf.nn[2] && ok; k++)
+      for (long long j = 0; j < gf.nn[1] && ok; j++)
+        for (long long i = 0; i < gf.nn[0] && ok; i++) {
+          unsigned all = 7u;
+          for (int a = 0; a < ((dim == 3 && (k & 1)) ? 2 : 1); a++)
+            for (int b = 0; b < ((j & 1) ? 2 : 1); b++)
+              for (int c = 0; c < ((i & 1) ? 2 : 1); c++) {
+                const long long I = (i >> 1) + c, J = (j >> 1) + b, K = dim == 3 ? (k >> 1) + a : 0;
+                all &= cflag[I + gc.nn[0] * (J + gc.nn[1] * K)];
+              }
+          const unsigned want = all & ((1u << dim) - 1u);
+          if (want != flag[i + gf.nn[0] * (j + gf.nn[1] * k)]) ok = false;
+        }
+    if (!ok) break;
+    MGLevel Lc;
+    Lc.g = gc;
+    Lc.own_bc = true;
+    CK(dalloc(&Lc.bcflag, gc.N));
+    CK(dalloc(&Lc.bcdof, gc.N * dim));
+    std::vector<uint8_t> cdof(gc.N * dim);
+    for (long long n = 0; n < gc.N; n++)
+      for (int d = 0; d < dim; d++) cdof[n * dim + d] = (cflag[n] >> d) & 1u;
+    CK(hipMemcpy(Lc.bcflag, cflag.data(), gc.N, hipMemcpyHostToDevice));
+    CK(hipMemcpy(Lc.bcdof, cdof.data(), gc.N * dim, hipMemcpyHostToDevice));
+    mg.lv.push_back(Lc);
+    CKS(mg_alloc_level(mg.lv.back()));
+    flag.swap(cflag);
+  }
+  mg.L = (int)mg.lv.size() - 1;
+  // coarsest free dofs
+  const MGLevel& C = mg.lv[mg.L];
+  std::vector<int> fidx;
+  for (long long n = 0; n < C.g.N; n++)
+    for (int d = 0; d < dim; d++)
+      if (!((flag[n] >> d) & 1u)) fidx.push_back((int)(n * dim + d));
+  mg.nfc = (int)fidx.size();
+  if (mg.nfc <= 2000) {
+    CK(dalloc(&mg.freeidx, mg.nfc));
+    CK(hipMemcpy(mg.freeidx, fidx.data(), sizeof(int) * mg.nfc, hipMemcpyHostToDevice));
+    CK(dalloc(&mg.Kinv64, (long long)mg.nfc * mg.nfc));
+    CK(dalloc(&mg.Kinv32, (long long)mg.nfc * mg.nfc));
+  } else {
+    mg.nfc = -1;  // no dense coarse solve: Chebyshev smoothing on the coarsest level
+  }
+  mg.structured = true;
+  return GS_OK;
+}
+
+// In-place inverse of an SPD matrix by Cholesky (host, coarsest level only).
+static bool spd_inverse(std::vector<double>& A, int n) {
+  std::vector<double> Lm(A);
+  for (int j = 0; j < n; j++) {
+    double d = Lm[(size_t)j * n + j];
+    for (int k = 0; k < j; k++) d -= Lm[(size_t)j * n + k] * Lm[(size_t)j * n + k];
+    if (!(d > 0)) return false;
+    d = std::sqrt(d);
+    Lm[(size_t)j * n + j] = d;
+    for (int i = j + 1; i < n; i++) {
+      double v = Lm[(size_t)i * n + j];
+      for (int k = 0; k < j; k++) v -= Lm[(size_t)i * n + k] * Lm[(size_t)j * n + k];
+      Lm[(size_t)i * n + j] = v / d;
+    }
+  }
+  // inverse of L, then A^-1 = L^-T L^-1
+  std::vector<double> Li((size_t)n * n, 0.0);
+  for (int c = 0; c < n; c++) {
+    for (int i = c; i < n; i++) {
+      double v = (i == c) ? 1.0 : 0.0;
+      for (int k = c; k < i; k++) v -= Lm[(size_t)i * n + k] * Li[(size_t)k * n + c];
+      Li[(size_t)i * n + c] = v / Lm[(size_t)i * n + i];
+    }
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j <= i; j++) {
+      double v = 0.0;
+      for (int k = i; k < n; k++) v += Li[(size_t)k * n + i] * Li[(size_t)k * n + j];
+      A[(size_t)i * n + j] = v;
+      A[(size_t)j * n + i] = v;
+    }
+  return true;
+}
+
+// Per-setup numeric part: moments of every level, Jacobi diagonals, the
+// Chebyshev bounds (power iteration on D^-1 K) and the dense coarse inverse.
+static int32_t mg_numeric(gs_solver* s) {
+  gs_problem* pb = s->pb;
+  hipStream_t st = pb->st;
+  MGData& mg = s->mg;
+  if (!mg.structured) CKS(mg_structure(s));
+  const int dim = pb->dim, nm = n_moments(dim);
+  launch_moments_fine(mg.lv[0].g, pb->pk, pb->off, mg.lv[0].M64, st);
+  for (int l = 1; l <= mg.L; l++) launch_moments_coarsen(mg.lv[l].g, mg.lv[l - 1].g, mg.lv[l - 1].M64, mg.lv[l].M64, st);
+  for (int l = 0; l <= mg.L; l++) {
+    MGLevel& L = mg.lv[l];
+    launch_d2f(L.M64, L.M32, L.g.E * nm, st);
+    launch_mom_diag(L.g, L.M64, L.diag, st);
+    launch_make_dinv(L.diag, L.bcdof, L.dinv64, L.n, st);
+    launch_d2f(L.dinv64, L.dinv32, L.n, st);
+  }
+  // Chebyshev upper bound: power iteration on D^-1 K
+  for (int l = 0; l < mg.L || (l == mg.L && mg.nfc < 0); l++) {
+    MGLevel& L = mg.lv[l];
+    launch_fill(L.x, L.n, 1.0, st);
+    launch_masked_copy(L.x, L.x, L.bcdof, L.n, st);
+    double lam = 1.0;
+    for (int it = 0; it < 12; it++) {
+      launch_mom_apply<double>(L.g, L.x, L.Ax, L.M64, L.bcflag, st);
+      // x <- D^-1 K x / |.|
+      launch_mul(L.dinv64, L.Ax, L.d, L.n, st);
+      launch_dot(L.d, L.d, L.n, s->S, S_TMP1, s->rd, st);
+      launch_dot(L.x, L.x, L.n, s->S, S_TMP2, s->rd, st);
+      CKS(read_scalars(s));
+      const double dn = std::sqrt(s->hS[S_TMP1]), xn = std::sqrt(s->hS[S_TMP2]);
+      if (!(xn > 0) || !std::isfinite(dn)) break;
+      lam = dn / xn;
+      launch_scale(L.x, L.d, 1.0 / dn, L.n, st);
+    }
+    L.lmax = lam;
+  }
+  // dense coarse operator by colour probing (nodes 3 apart do not interact)
+  if (mg.nfc > 0) {
+    MGLevel& C = mg.lv[mg.L];
+    const long long N = C.g.N;
+    const int nf = mg.nfc;
+    std::vector<int> fidx(nf);
+    CK(hipMemcpy(fidx.data(), mg.freeidx, sizeof(int) * nf, hipMemcpyDeviceToHost));
+    std::vector<int> pos(C.n, -1);
+    for (int i = 0; i < nf; i++) pos[fidx[i]] = i;
+    std::vector<double> Kd((size_t)nf * nf, 0.0), v(C.n), y(C.n);
+    const int ncz = dim == 3 ? 3 : 1;
+    for (int cz = 0; cz < ncz; cz++)
+      for (int cy = 0; cy < 3; cy++)
+        for (int cx = 0; cx < 3; cx++)
+          for (int d = 0; d < dim; d++) {
+            std::fill(v.begin(), v.end(), 0.0);
+            bool any = false;
+            for (long long n = 0; n < N; n++) {
+              const long long i = n % C.g.nn[0], j = (n / C.g.nn[0]) % C.g.nn[1], k = n / (C.g.nn[0] * C.g.nn[1]);
+              if (i % 3 == cx && j % 3 == cy && k % 3 == cz && pos[n * dim + d] >= 0) {
+                v[n * dim + d] = 1.0;
+                any = true;
+              }
+            }
+            if (!any) continue;
+            CK(hipMemcpyAsync(C.x, v.data(), sizeof(double) * C.n, hipMemcpyHostToDevice, st));
+            launch_mom_apply<double>(C.g, C.x, C.Ax, C.M64, C.bcflag, st);
+            CK(hipMemcpyAsync(y.data(), C.Ax, sizeof(double) * C.n, hipMemcpyDeviceToHost, st));
+            CK(hipStreamSynchronize(st));
+            for (long long n = 0; n < N; n++) {
+              if (v[n * dim + d] == 0.0) continue;
+              const int col = pos[n * dim + d];
+              const long long i = n % C.g.nn[0], j = (n / C.g.nn[0]) % C.g.nn[1], k = n / (C.g.nn[0] * C.g.nn[1]);
+              for (long long kk = k - 1; kk <= k + 1; kk++)
+                for (long long jj = j - 1; jj <= j + 1; jj++)
+                  for (long long ii = i - 1; ii <= i + 1; ii++) {
+                    if (ii < 0 || jj < 0 || kk < 0 || ii >= C.g.nn[0] || jj >= C.g.nn[1] || kk >= C.g.nn[2]) continue;
+                    const long long m = ii + C.g.nn[0] * (jj + C.g.nn[1] * kk);
+                    for (int d2 = 0; d2 < dim; d2++) {
+                      const int row = pos[m * dim + d2];
+                      if (row >= 0) Kd[(size_t)row * nf + col] = y[m * dim + d2];
+                    }
+                  }
+            }
+          }
+    if (!spd_inverse(Kd, nf)) return GS_E_NAN;
+    std::vector<float> Kf(Kd.begin(), Kd.end());
+    CK(hipMemcpyAsync(mg.Kinv64, Kd.data(), sizeof(double) * Kd.size(), hipMemcpyHostToDevice, st));
+    CK(hipMemcpyAsync(mg.Kinv32, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice, st));
+    CK(hipStreamSynchronize(st));
+  }
+  return GS_OK;
+}
+
+template <typename T>
+struct LvPtr {
+  T *b, *x, *r, *d, *Ax;
+  const T *M, *dinv;
+};
+static LvPtr<double> lvp(MGLevel& L, double*) { return {L.b, L.x, L.r, L.d, L.Ax, L.M64, L.dinv64}; }
+static LvPtr<float> lvp(MGLevel& L, float*) { return {L.bf, L.xf, L.rf, L.df, L.Axf, L.M32, L.dinv32}; }
+
+template <typename T>
+static void cheb_smooth(MGLevel& L, int nu, bool zero, hipStream_t st) {
+  LvPtr<T> P = lvp(L, (T*)nullptr);
+  const double emax = 1.1 * L.lmax, emin = 0.1 * L.lmax;
+  const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin), sigma = theta / delta;
+  double rho_prev = 1.0 / sigma;
+  if (zero) {
+    launch_cheb<T>(P.b, nullptr, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 1, st);
+  } else {
+    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+    launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 0, st);
+  }
+  for (int k = 1; k < nu; k++) {
+    const double rho = 1.0 / (2.0 * sigma - rho_prev);
+    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+    launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)(rho * rho_prev), (T)(2.0 * rho / delta), 0, st);
+    rho_prev = rho;
+  }
+}
+
+template <typename T>
+static void vcycle(gs_solver* s, int l) {
+  MGData& mg = s->mg;
+  MGLevel& L = mg.lv[l];
+  hipStream_t st = s->pb->st;
+  const int nu = s->up.mg_smooth > 0 ? s->up.mg_smooth : 2;
+  LvPtr<T> P = lvp(L, (T*)nullptr);
+  if (l == mg.L) {
+    if (mg.nfc > 0)
+      launch_dense_solve<T>(sizeof(T) == 8 ? (const T*)(void*)mg.Kinv64 : (const T*)(void*)mg.Kinv32, mg.freeidx,
+                            mg.nfc, P.b, P.x, L.n, st);
+    else
+      cheb_smooth<T>(L, 8, true, st);
+    return;
+  }
+  MGLevel& C = mg.lv[l + 1];
+  LvPtr<T> Q = lvp(C, (T*)nullptr);
+  cheb_smooth<T>(L, nu, true, st);
+  launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+  launch_resid<T>(P.b, P.Ax, P.r, L.n, st);
+  launch_restrict<T>(C.g, L.g, P.r, Q.b, C.bcflag, st);
+  vcycle<T>(s, l + 1);
+  launch_prolong_add<T>(C.g, L.g, Q.x, P.x, L.bcflag, st);
+  cheb_smooth<T>(L, nu, false, st);
+}
+
+// z = M^-1 r by one V-cycle (FP64, or FP32 in the mixed-precision mode).
+static void mg_precond(gs_solver* s, const double* r, double* z) {
+  MGLevel& L = s->mg.lv[0];
+  hipStream_t st = s->pb->st;
+  if (s->up.precision == GS_PREC_MIXED) {
+    launch_d2f(r, L.bf, L.n, st);
+    vcycle<float>(s, 0);
+    launch_f2d(L.xf, z, L.n, st);
+  } else {
+    (void)hipMemcpyAsync(L.b, r, sizeof(double) * L.n, hipMemcpyDeviceToDevice, st);
+    vcycle<double>(s, 0);
+    (void)hipMemcpyAsync(z, L.x, sizeof(double) * L.n, hipMemcpyDeviceToDevice, st);
+  }
 }
 
 // The fused constitutive pass + diagonal operators + right-hand sides.
@@ -508,6 +860,7 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
   } else {
     launch_fill(s->h, s->np, 0.0, st);
   }
+  if (s->up.inner_preconditioner == GS_INNER_MULTIGRID || s->up.precision == GS_PREC_MIXED) CKS(mg_numeric(s));
   s->setup_done = true;
   return GS_OK;
 }
@@ -539,7 +892,7 @@ extern "C" int32_t gs_solver_setup(gs_solver* s, const double* u_prev, const dou
 
 // Inner velocity solve K x = b (masked), Jacobi-preconditioned CG from a zero
 // guess, stopping at |r| <= rtol |b|.
-static int32_t inner_solve(gs_solver* s, const double* b, double* x, double rtol, long long* its) {
+static int32_t inner_solve_jacobi(gs_solver* s, const double* b, double* x, double rtol, long long* its) {
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
   const long long n = s->n;
@@ -569,6 +922,59 @@ static int32_t inner_solve(gs_solver* s, const double* b, double* x, double rtol
     if (s->hS[S_RR] > target) return GS_E_INNER_MAXIT;
   }
   return GS_OK;
+}
+
+// Inner velocity solve with the multigrid V-cycle as a (possibly FP32, hence
+// slightly variable) preconditioner of FP64 flexible CG.  The operator is the
+// FP64 moment form of K; vectors, dot products and the stopping test
+// |r| <= rtol |b| stay FP64.
+static int32_t inner_solve_mg(gs_solver* s, const double* b, double* x, double rtol, long long* its) {
+  gs_problem* pb = s->pb;
+  hipStream_t st = pb->st;
+  const long long n = s->n;
+  MGLevel& L0 = s->mg.lv[0];
+  launch_fill(x, n, 0.0, st);
+  CK(hipMemcpyAsync(s->cg_r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  launch_dot(b, b, n, s->S, S_BB, s->rd, st);
+  CKS(read_scalars(s));
+  const double bb = s->hS[S_BB];
+  if (bb == 0.0) return GS_OK;
+  if (!std::isfinite(bb)) return GS_E_NAN;
+  const double target = rtol * rtol * bb;
+  mg_precond(s, s->cg_r, s->cg_z);
+  CK(hipMemcpyAsync(s->cg_p, s->cg_z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  launch_dot(s->cg_r, s->cg_z, n, s->S, S_RZ, s->rd, st);
+  const int maxit = s->up.inner_max_iterations;
+  const int every = s->up.check_every > 0 ? std::min(s->up.check_every, 2) : 1;
+  int k = 0;
+  bool conv = false;
+  while (k < maxit) {
+    launch_mom_apply<double>(L0.g, s->cg_p, s->cg_Ap, L0.M64, pb->bcflag, st);
+    s->k_applies++;
+    launch_dot(s->cg_p, s->cg_Ap, n, s->S, S_PAP, s->rd, st);
+    launch_fcg_update(x, s->cg_r, s->cg_p, s->cg_Ap, n, s->S, s->rd, st);
+    k++;
+    if (k % every == 0 || k >= maxit) {
+      CKS(read_scalars(s));
+      const double rr = s->hS[S_RR];
+      if (!std::isfinite(rr)) return GS_E_NAN;
+      if (rr <= target) {
+        conv = true;
+        break;
+      }
+    }
+    mg_precond(s, s->cg_r, s->cg_z);
+    launch_fcg_dots(s->cg_z, s->cg_r, s->cg_Ap, n, s->S, s->rd, st);
+    launch_fcg_p(s->cg_p, s->cg_z, n, s->S, st);
+  }
+  *its += k;
+  if (!conv) return GS_E_INNER_MAXIT;
+  return GS_OK;
+}
+
+static int32_t inner_solve(gs_solver* s, const double* b, double* x, double rtol, long long* its) {
+  if (s->up.inner_preconditioner == GS_INNER_MULTIGRID) return inner_solve_mg(s, b, x, rtol, its);
+  return inner_solve_jacobi(s, b, x, rtol, its);
 }
 
 static double dot_host(gs_solver* s, const double* a, const double* b, long long n, int32_t* err) {
@@ -639,7 +1045,8 @@ static int32_t uzawa(gs_solver* s, gs_report* rep) {
       err = GS_E_NAN;
       break;
     }
-    if (s->up.monitor) std::fprintf(stderr, "  %04d uzawa residual norm %12.13e\n", it + 1, relRes);
+    if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
+      std::fprintf(stderr, "  %04d uzawa residual norm %12.13e\n", it + 1, relRes);
     if (!(it < s->up.max_iterations)) {
       err = GS_E_MAXIT;
       break;
@@ -878,29 +1285,45 @@ extern "C" int32_t gs_solver_sync(gs_solver* s) {
   return GS_OK;
 }
 
-extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t precision, int32_t n_rep, double* ms_per_apply) {
-  if (!s || n_rep <= 0 || !ms_per_apply) return GS_E_INVALID;
+extern "C" int32_t gs_apply_K_moments(gs_solver* s, const double* x, double* y) {
+  if (!s || !x || !y) return GS_E_INVALID;
+  if (!s->setup_done || !s->mg.structured) {
+    CKS(setup_dev(s, nullptr, nullptr, 0));
+    if (!s->mg.structured) CKS(mg_numeric(s));
+  }
+  launch_mom_apply<double>(s->mg.lv[0].g, x, y, s->mg.lv[0].M64, nullptr, s->pb->st);
+  s->k_applies++;
+  return GS_OK;
+}
+
+extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, double* ms_per_apply) {
+  if (!s || n_rep <= 0 || !ms_per_apply || form < 0 || form > 3) return GS_E_INVALID;
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
   if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
-  float ms = 0.f;
-  if (precision == GS_PREC_FP64) {
-    launch_fill(s->cg_p, s->n, 1.0, st);
-    launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, st);
-    CK(hipEventRecord(s->e0, st));
-    for (int i = 0; i < n_rep; i++) launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, st);
-    CK(hipEventRecord(s->e1, st));
-  } else {
-    CKS(ensure_pk32(s));
+  if (form >= 2 && !s->mg.structured) CKS(mg_numeric(s));
+  if (form == 1 || form == 3) {
+    if (form == 1) CKS(ensure_pk32(s));
     if (!s->x32) CK(dalloc(&s->x32, s->n));
     if (!s->y32) CK(dalloc(&s->y32, s->n));
     launch_fill_f(s->x32, s->n, 1.0f, st);
-    launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, st);
-    CK(hipEventRecord(s->e0, st));
-    for (int i = 0; i < n_rep; i++) launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, st);
-    CK(hipEventRecord(s->e1, st));
+  } else {
+    launch_fill(s->cg_p, s->n, 1.0, st);
   }
+  auto one = [&]() {
+    switch (form) {
+      case 0: launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, st); break;
+      case 1: launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, st); break;
+      case 2: launch_mom_apply<double>(s->mg.lv[0].g, s->cg_p, s->cg_Ap, s->mg.lv[0].M64, nullptr, st); break;
+      default: launch_mom_apply<float>(s->mg.lv[0].g, s->x32, s->y32, s->mg.lv[0].M32, nullptr, st); break;
+    }
+  };
+  one();
+  CK(hipEventRecord(s->e0, st));
+  for (int i = 0; i < n_rep; i++) one();
+  CK(hipEventRecord(s->e1, st));
   CK(hipEventSynchronize(s->e1));
+  float ms = 0.f;
   CK(hipEventElapsedTime(&ms, s->e0, s->e1));
   *ms_per_apply = (double)ms / n_rep;
   s->k_applies += n_rep + 1;

@@ -32,6 +32,7 @@ enum Slot {
   S_RR = 2,      // r . r (inner)
   S_BB = 3,      // b . b (inner)
   S_BETA = 4,    // inner beta
+  S_ALPHA = 5,   // inner alpha (flexible CG)
   S_ZR = 8,      // Uzawa z . r
   S_ZR_PREV = 9,
   S_UBETA = 10,  // Uzawa beta
@@ -88,6 +89,7 @@ void launch_recip(const double* a, double* b, long long n, hipStream_t st);
 void launch_scale(double* y, const double* x, double a, long long n, hipStream_t st);  // y = a*x
 void launch_axpby(double* y, const double* x, double a, double b, long long n, hipStream_t st); // y = a x + b y
 void launch_add(double* y, const double* a, const double* b, long long n, hipStream_t st);  // y = a + b
+void launch_mul(const double* a, const double* b, double* c, long long n, hipStream_t st);  // c = a * b
 
 // Inner CG (Jacobi).
 void launch_cg_init(const double* b, const double* dinv, double* x, double* r, double* p, long long n,
@@ -96,6 +98,13 @@ void launch_cg_update(double* x, double* r, const double* p, const double* Ap, c
                       double* S, Reducer rd, hipStream_t st);
 void launch_cg_pupdate(double* p, const double* r, const double* dinv, long long n, const double* S,
                        hipStream_t st);
+
+// Flexible (Polak-Ribiere) CG with a variable preconditioner.
+void launch_fcg_update(double* x, double* r, const double* p, const double* Ap, long long n, double* S, Reducer rd,
+                       hipStream_t st);
+void launch_fcg_dots(const double* z, const double* r, const double* Ap, long long n, double* S, Reducer rd,
+                     hipStream_t st);
+void launch_fcg_p(double* p, const double* z, long long n, const double* S, hipStream_t st);
 
 // Uzawa outer loop.
 void launch_uz_precond(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,

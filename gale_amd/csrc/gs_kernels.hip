@@ -228,6 +228,14 @@ void launch_add(double* y, const double* a, const double* b, long long n, hipStr
   hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, st, y, a, b, n);
 }
 
+__global__ void k_mul(const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ c, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    c[i] = a[i] * b[i];
+}
+void launch_mul(const double* a, const double* b, double* c, long long n, hipStream_t st) {
+  hipLaunchKernelGGL(k_mul, dim3(grid_for(n)), dim3(kBlock), 0, st, a, b, c, n);
+}
+
 __global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long long n, double* S, int slot,
                       Reducer rd) {
   double v[1] = {0.0}, tot[1];
@@ -313,6 +321,57 @@ void launch_cg_pupdate(double* p, const double* r, const double* dinv, long long
   hipLaunchKernelGGL(k_cg_pupdate, dim3(grid_for(n)), dim3(kBlock), 0, st, p, r, dinv, n, S);
 }
 
+// Flexible CG: alpha = rz / pAp; x += alpha p; r -= alpha Ap.
+__global__ void k_fcg_update(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
+                             const double* __restrict__ Ap, long long n, double* S, Reducer rd) {
+  const double rz = S[S_RZ], pap = S[S_PAP];
+  const double alpha = (pap != 0.0 && isfinite(pap)) ? rz / pap : 0.0;
+  double v[1] = {0.0}, tot[1];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * Ap[i];
+    r[i] = ri;
+    v[0] += ri * ri;
+  }
+  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
+    S[S_RR] = tot[0];
+    S[S_ALPHA] = alpha;
+  }
+}
+void launch_fcg_update(double* x, double* r, const double* p, const double* Ap, long long n, double* S, Reducer rd,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(k_fcg_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, x, r, p, Ap, n, S, rd);
+}
+
+// z.r and z.Ap; beta = (z.(r_new - r_old)) / rz_old = -alpha z.Ap / rz_old (Polak-Ribiere).
+__global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ Ap,
+                           long long n, double* S, Reducer rd) {
+  double v[2] = {0.0, 0.0}, tot[2];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double zi = z[i];
+    v[0] += zi * r[i];
+    v[1] += zi * Ap[i];
+  }
+  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
+    const double rz = S[S_RZ];
+    S[S_BETA] = rz != 0.0 ? -S[S_ALPHA] * tot[1] / rz : 0.0;
+    S[S_RZ] = tot[0];
+  }
+}
+void launch_fcg_dots(const double* z, const double* r, const double* Ap, long long n, double* S, Reducer rd,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_fcg_dots, dim3(reduce_grid(n)), dim3(kBlock), 0, st, z, r, Ap, n, S, rd);
+}
+
+__global__ void k_fcg_p(double* __restrict__ p, const double* __restrict__ z, long long n, const double* S) {
+  const double beta = S[S_BETA];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = z[i] + beta * p[i];
+}
+void launch_fcg_p(double* p, const double* z, long long n, const double* S, hipStream_t st) {
+  hipLaunchKernelGGL(k_fcg_p, dim3(grid_for(n)), dim3(kBlock), 0, st, p, z, n, S);
+}
+
 // ---------------------------------------------------------------------------
 // Uzawa outer loop vector steps (Stokes_SLE_UzawaSolver.cxx:664-837).
 // ---------------------------------------------------------------------------
@@ -389,12 +448,27 @@ struct Vec4<float> {
   using type = float4;
 };
 
-// 3D: one thread per element of colour c.
+// Lanes per element: the particles of one element are split over LPE
+// adjacent lanes (contiguous 32-byte records -> coalesced loads), the 30 / 7
+// element moments are summed across the lanes with two xor-shuffles, and each
+// lane scatters its share of the element's nodes.
+constexpr int kLPE = 4;
+
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v) {
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  return v;
+}
+
+// 3D: LPE lanes per element of colour c.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mcount, const T* __restrict__ x,
                                                      T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
                                                      const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
-  const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const long long gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const long long t = gt / kLPE;
+  const int lane = (int)(gt % kLPE);
   if (t >= mcount) return;
   long long ijk[3];
   colour_elem(g, c, t, ijk);
@@ -402,18 +476,15 @@ __global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mco
   const T ix = (T)(0.25 / (vcoord(g, 0, ijk[0] + 1) - vcoord(g, 0, ijk[0])));
   const T iy = (T)(0.25 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
   const T iz = (T)(0.25 / (vcoord(g, 2, ijk[2] + 1) - vcoord(g, 2, ijk[2])));
-  const long long sx = 1, sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
   const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
-  long long nid[8];
-#pragma unroll
-  for (int a = 0; a < 8; a++) nid[a] = n0 + (a & 1) * sx + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
   // gradient coefficients: L[c][0] = A[c] . (1,s,t,st), L[c][1] = B[c] . (1,r,t,rt), L[c][2] = C[c] . (1,r,s,rs)
   T A[3][4], B[3][4], C[3][4];
 #pragma unroll
   for (int cc = 0; cc < 3; cc++) {
     T v[8];
 #pragma unroll
-    for (int a = 0; a < 8; a++) v[a] = x[nid[a] * 3 + cc];
+    for (int a = 0; a < 8; a++) v[a] = x[(n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz) * 3 + cc];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
       const int bit = 1 << d;
@@ -430,14 +501,11 @@ __global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mco
     C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
   }
   // moments
-  T mxx0 = 0, mxxs = 0, mxxt = 0, mxxst = 0;
-  T myy0 = 0, myyr = 0, myyt = 0, myyrt = 0;
-  T mzz0 = 0, mzzr = 0, mzzs = 0, mzzrs = 0;
-  T txy0 = 0, txyr = 0, txys = 0, txyt = 0, txyrt = 0, txyst = 0;
-  T txz0 = 0, txzr = 0, txzs = 0, txzt = 0, txzrs = 0, txzst = 0;
-  T tyz0 = 0, tyzr = 0, tyzs = 0, tyzt = 0, tyzrs = 0, tyzrt = 0;
+  T m[30];
+#pragma unroll
+  for (int k = 0; k < 30; k++) m[k] = 0;
   const int pb = off[e], pe = off[e + 1];
-  for (int p = pb; p < pe; p++) {
+  for (int p = pb + lane; p < pe; p += kLPE) {
     const typename Vec4<T>::type q = pk[p];
     const T r = q.x, s = q.y, tt = q.z, w = q.w;
     const T st = s * tt, rt = r * tt, rs = r * s;
@@ -453,15 +521,24 @@ __global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mco
     const T w2 = w + w;
     const T sxx = w2 * uxx, syy = w2 * uyy, szz = w2 * uzz;
     const T sxy = w * (uxy + uyx), sxz = w * (uxz + uzx), syz = w * (uyz + uzy);
-    mxx0 += sxx; mxxs += sxx * s; mxxt += sxx * tt; mxxst += sxx * st;
-    myy0 += syy; myyr += syy * r; myyt += syy * tt; myyrt += syy * rt;
-    mzz0 += szz; mzzr += szz * r; mzzs += szz * s; mzzrs += szz * rs;
-    txy0 += sxy; txyr += sxy * r; txys += sxy * s; txyt += sxy * tt; txyrt += sxy * rt; txyst += sxy * st;
-    txz0 += sxz; txzr += sxz * r; txzs += sxz * s; txzt += sxz * tt; txzrs += sxz * rs; txzst += sxz * st;
-    tyz0 += syz; tyzr += syz * r; tyzs += syz * s; tyzt += syz * tt; tyzrs += syz * rs; tyzrt += syz * rt;
+    m[0] += sxx; m[1] += sxx * s; m[2] += sxx * tt; m[3] += sxx * st;
+    m[4] += syy; m[5] += syy * r; m[6] += syy * tt; m[7] += syy * rt;
+    m[8] += szz; m[9] += szz * r; m[10] += szz * s; m[11] += szz * rs;
+    m[12] += sxy; m[13] += sxy * r; m[14] += sxy * s; m[15] += sxy * tt; m[16] += sxy * rt; m[17] += sxy * st;
+    m[18] += sxz; m[19] += sxz * r; m[20] += sxz * s; m[21] += sxz * tt; m[22] += sxz * rs; m[23] += sxz * st;
+    m[24] += syz; m[25] += syz * r; m[26] += syz * s; m[27] += syz * tt; m[28] += syz * rs; m[29] += syz * rt;
   }
 #pragma unroll
-  for (int a = 0; a < 8; a++) {
+  for (int k = 0; k < 30; k++) m[k] = quad_sum(m[k]);
+  const T mxx0 = m[0], mxxs = m[1], mxxt = m[2], mxxst = m[3];
+  const T myy0 = m[4], myyr = m[5], myyt = m[6], myyrt = m[7];
+  const T mzz0 = m[8], mzzr = m[9], mzzs = m[10], mzzrs = m[11];
+  const T txy0 = m[12], txyr = m[13], txys = m[14], txyt = m[15], txyrt = m[16], txyst = m[17];
+  const T txz0 = m[18], txzr = m[19], txzs = m[20], txzt = m[21], txzrs = m[22], txzst = m[23];
+  const T tyz0 = m[24], tyzr = m[25], tyzs = m[26], tyzt = m[27], tyzrs = m[28], tyzrt = m[29];
+#pragma unroll
+  for (int k = 0; k < 8 / kLPE; k++) {
+    const int a = lane * (8 / kLPE) + k;
     const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
     const T yx = ix * ax * (mxx0 + ay * mxxs + az * mxxt + ay * az * mxxst) +
                  iy * ay * (txy0 + ax * txyr + az * txyt + ax * az * txyrt) +
@@ -472,8 +549,9 @@ __global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mco
     const T yz = iz * az * (mzz0 + ax * mzzr + ay * mzzs + ax * ay * mzzrs) +
                  ix * ax * (txz0 + ay * txzs + az * txzt + ay * az * txzst) +
                  iy * ay * (tyz0 + ax * tyzr + az * tyzt + ax * az * tyzrt);
-    const unsigned bf = bcflag ? bcflag[nid[a]] : 0u;
-    T* yp = y + nid[a] * 3;
+    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
+    const unsigned bf = bcflag ? bcflag[nd] : 0u;
+    T* yp = y + nd * 3;
     if (!(bf & 1u)) yp[0] += yx;
     if (!(bf & 2u)) yp[1] += yy;
     if (!(bf & 4u)) yp[2] += yz;
@@ -484,7 +562,9 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_apply_K2(Geo g, int c, long long mcount, const T* __restrict__ x,
                                                      T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
                                                      const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
-  const long long t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const long long gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const long long t = gt / kLPE;
+  const int lane = (int)(gt % kLPE);
   if (t >= mcount) return;
   long long ijk[3];
   colour_elem(g, c, t, ijk);
@@ -493,20 +573,17 @@ __global__ __launch_bounds__(kBlock) void k_apply_K2(Geo g, int c, long long mco
   const T iy = (T)(0.5 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
   const long long sy = g.nn[0];
   const long long n0 = ijk[0] + sy * ijk[1];
-  long long nid[4];
-#pragma unroll
-  for (int a = 0; a < 4; a++) nid[a] = n0 + (a & 1) + ((a >> 1) & 1) * sy;
   T A[2][2], B[2][2];
 #pragma unroll
   for (int cc = 0; cc < 2; cc++) {
-    const T u0 = x[nid[0] * 2 + cc], u1 = x[nid[1] * 2 + cc], u2 = x[nid[2] * 2 + cc], u3 = x[nid[3] * 2 + cc];
+    const T u0 = x[n0 * 2 + cc], u1 = x[(n0 + 1) * 2 + cc], u2 = x[(n0 + sy) * 2 + cc], u3 = x[(n0 + sy + 1) * 2 + cc];
     const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
     A[cc][0] = ix * t1; A[cc][1] = ix * t3;
     B[cc][0] = iy * t2; B[cc][1] = iy * t3;
   }
   T mxx0 = 0, mxxs = 0, myy0 = 0, myyr = 0, txy0 = 0, txyr = 0, txys = 0;
   const int pb = off[e], pe = off[e + 1];
-  for (int p = pb; p < pe; p++) {
+  for (int p = pb + lane; p < pe; p += kLPE) {
     const typename Vec4<T>::type q = pk[p];
     const T r = q.x, s = q.y, w = q.w;
     const T uxx = A[0][0] + A[0][1] * s, uxy = B[0][0] + B[0][1] * r;
@@ -517,16 +594,17 @@ __global__ __launch_bounds__(kBlock) void k_apply_K2(Geo g, int c, long long mco
     myy0 += syy; myyr += syy * r;
     txy0 += sxy; txyr += sxy * r; txys += sxy * s;
   }
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
-    const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
-    const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
-    const unsigned bf = bcflag ? bcflag[nid[a]] : 0u;
-    T* yp = y + nid[a] * 2;
-    if (!(bf & 1u)) yp[0] += yx;
-    if (!(bf & 2u)) yp[1] += yy;
-  }
+  mxx0 = quad_sum(mxx0); mxxs = quad_sum(mxxs); myy0 = quad_sum(myy0); myyr = quad_sum(myyr);
+  txy0 = quad_sum(txy0); txyr = quad_sum(txyr); txys = quad_sum(txys);
+  const int a = lane;
+  const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
+  const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
+  const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
+  const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy;
+  const unsigned bf = bcflag ? bcflag[nd] : 0u;
+  T* yp = y + nd * 2;
+  if (!(bf & 1u)) yp[0] += yx;
+  if (!(bf & 2u)) yp[1] += yy;
 }
 
 template <typename T>
@@ -536,10 +614,11 @@ static void apply_K_impl(const Geo& g, const T* x, T* y, const typename Vec4<T>:
   for (int c = 0; c < ncol; c++) {
     const long long m = colour_count(g, c);
     if (m <= 0) continue;
+    const unsigned gb = grid_for(m * kLPE);
     if (g.dim == 3)
-      hipLaunchKernelGGL(k_apply_K3<T>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
+      hipLaunchKernelGGL(k_apply_K3<T>, dim3(gb), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
     else
-      hipLaunchKernelGGL(k_apply_K2<T>, dim3(grid_for(m)), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
+      hipLaunchKernelGGL(k_apply_K2<T>, dim3(gb), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
   }
 }
 

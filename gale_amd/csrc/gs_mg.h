@@ -1,0 +1,29 @@
+// gs_mg.h -- launchers of the multigrid kernels (gs_mg.hip).
+#pragma once
+
+#include "gs_internal.h"
+
+namespace gs {
+
+void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double* M, hipStream_t st);
+void launch_moments_coarsen(const Geo& gc, const Geo& gf, const double* Mf, double* Mc, hipStream_t st);
+void launch_d2f(const double* a, float* b, long long n, hipStream_t st);
+void launch_f2d(const float* a, double* b, long long n, hipStream_t st);
+template <typename T>
+void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, hipStream_t st);
+void launch_mom_diag(const Geo& g, const double* Mom, double* diag, hipStream_t st);
+template <typename T>
+void launch_restrict(const Geo& gc, const Geo& gf, const T* rf, T* bc, const uint8_t* bcflag_c, hipStream_t st);
+template <typename T>
+void launch_prolong_add(const Geo& gc, const Geo& gf, const T* xc, T* xf, const uint8_t* bcflag_f, hipStream_t st);
+template <typename T>
+void launch_cheb(const T* b, const T* Ax, const T* dinv, T* d, T* x, long long n, T alpha, T beta, int first,
+                 hipStream_t st);
+template <typename T>
+void launch_resid(const T* b, const T* Ax, T* r, long long n, hipStream_t st);
+template <typename T>
+void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, const T* b, T* x, long long n, hipStream_t st);
+
+inline int n_moments(int dim) { return dim == 2 ? 9 : 27; }
+
+}  // namespace gs

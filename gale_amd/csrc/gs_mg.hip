@@ -1,0 +1,531 @@
+// gs_mg.hip -- geometric multigrid for the inner velocity solve.
+//
+// Because the Q1 spaces of a regular mesh are nested and the coarse basis
+// functions are exactly representable on the fine mesh, the Galerkin coarse
+// operator P^T K P equals the coarse Q1 operator integrated with the SAME
+// material points and weights omega_p = w_p detJ eta_p.  Its element matrix
+// depends on the points only through 27 (3D) / 9 (2D) weighted moments
+// M[a,b,c] = sum_p omega_p r^a s^b t^c (a,b,c <= 2) in element-local
+// coordinates, since every entry of B^T D B is a polynomial of degree <= 2 in
+// each local coordinate.  The fine moments come from one pass over the
+// particles; each coarser level's moments are an exact polynomial change of
+// variables of its children's.  Every level is applied matrix-free from its
+// moments (216 B per 3D element instead of 32 B per particle), smoothed by
+// Chebyshev-Jacobi, and the coarsest level is solved with a dense inverse.
+#include "gs_internal.h"
+#include "gs_mg.h"
+
+#include <math.h>
+
+namespace gs {
+
+__device__ __forceinline__ long long xcd_block_mg(long long b, long long nb) {
+  const long long q = nb / 8, rem = nb % 8;
+  const long long x = b % 8, i = b / 8;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
+__device__ __forceinline__ double vcoord_mg(const Geo& g, int d, long long idx) {
+#pragma clang fp contract(off)
+  if (idx <= 0) return g.lo[d];
+  if (idx >= g.res[d]) return g.hi[d];
+  return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
+}
+
+__device__ __forceinline__ void colour_elem_mg(const Geo& g, int c, long long t, long long* ijk) {
+  const long long mx = (g.res[0] - (c & 1) + 1) / 2;
+  const long long my = g.dim > 1 ? (g.res[1] - ((c >> 1) & 1) + 1) / 2 : 1;
+  const long long a = t % mx, rem = t / mx;
+  const long long b = rem % my, cc = rem / my;
+  ijk[0] = 2 * a + (c & 1);
+  ijk[1] = 2 * b + ((c >> 1) & 1);
+  ijk[2] = 2 * cc + ((c >> 2) & 1);
+}
+
+static inline unsigned gridn(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+// ---------------------------------------------------------------------------
+// moments
+// ---------------------------------------------------------------------------
+template <int DIM>
+__global__ void k_moments_fine(Geo g, const double4* __restrict__ pk, const int* __restrict__ off,
+                               double* __restrict__ M) {
+  constexpr int NM = DIM == 2 ? 9 : 27;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= g.E) return;
+  double m[NM];
+#pragma unroll
+  for (int k = 0; k < NM; k++) m[k] = 0.0;
+  for (int p = off[e]; p < off[e + 1]; p++) {
+    const double4 q = pk[p];
+    const double r[3] = {1.0, q.x, q.x * q.x};
+    const double s[3] = {1.0, q.y, q.y * q.y};
+    if (DIM == 2) {
+#pragma unroll
+      for (int b = 0; b < 3; b++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) m[a + 3 * b] += q.w * r[a] * s[b];
+    } else {
+      const double t[3] = {1.0, q.z, q.z * q.z};
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+          const double ws = q.w * s[b] * t[c];
+#pragma unroll
+          for (int a = 0; a < 3; a++) m[a + 3 * b + 9 * c] += ws * r[a];
+        }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NM; k++) M[e * NM + k] = m[k];
+}
+
+// Parent moments from children: child local r_f maps to parent r_c = (r_f + sg)/2,
+// sg = -1 for the lower child, +1 for the upper.  Per axis r_c^a = T[a][k] r_f^k.
+template <int DIM>
+__global__ void k_moments_coarsen(Geo gc, Geo gf, const double* __restrict__ Mf, double* __restrict__ Mc) {
+  constexpr int NM = DIM == 2 ? 9 : 27;
+  const long long E = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (E >= gc.E) return;
+  const long long I = E % gc.res[0], J = (E / gc.res[0]) % gc.res[1], K = DIM == 3 ? E / ((long long)gc.res[0] * gc.res[1]) : 0;
+  double acc[NM];
+#pragma unroll
+  for (int k = 0; k < NM; k++) acc[k] = 0.0;
+  for (int ch = 0; ch < (DIM == 2 ? 4 : 8); ch++) {
+    const int ox = ch & 1, oy = (ch >> 1) & 1, oz = (ch >> 2) & 1;
+    const long long e = (2 * I + ox) + (long long)gf.res[0] * ((2 * J + oy) + (long long)gf.res[1] * (2 * K + oz));
+    double T[3][3][3];  // T[axis][a][k]
+    const double sg[3] = {ox ? 1.0 : -1.0, oy ? 1.0 : -1.0, oz ? 1.0 : -1.0};
+    for (int d = 0; d < 3; d++) {
+      const double s = sg[d];
+      T[d][0][0] = 1.0; T[d][0][1] = 0.0; T[d][0][2] = 0.0;
+      T[d][1][0] = 0.5 * s; T[d][1][1] = 0.5; T[d][1][2] = 0.0;
+      T[d][2][0] = 0.25; T[d][2][1] = 0.5 * s; T[d][2][2] = 0.25;  // (r+s)^2/4 = (r^2 + 2 s r + 1)/4
+    }
+    const double* m = Mf + e * NM;
+    if (DIM == 2) {
+      for (int b = 0; b < 3; b++)
+        for (int a = 0; a < 3; a++) {
+          double v = 0.0;
+          for (int kb = 0; kb <= b; kb++)
+            for (int ka = 0; ka <= a; ka++) v += T[0][a][ka] * T[1][b][kb] * m[ka + 3 * kb];
+          acc[a + 3 * b] += v;
+        }
+    } else {
+      for (int c = 0; c < 3; c++)
+        for (int b = 0; b < 3; b++)
+          for (int a = 0; a < 3; a++) {
+            double v = 0.0;
+            for (int kc = 0; kc <= c; kc++)
+              for (int kb = 0; kb <= b; kb++)
+                for (int ka = 0; ka <= a; ka++)
+                  v += T[0][a][ka] * T[1][b][kb] * T[2][c][kc] * m[ka + 3 * kb + 9 * kc];
+            acc[a + 3 * b + 9 * c] += v;
+          }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NM; k++) Mc[E * NM + k] = acc[k];
+}
+
+void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double* M, hipStream_t st) {
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_moments_fine<3>, dim3(gridn(g.E)), dim3(kBlock), 0, st, g, pk, off, M);
+  else
+    hipLaunchKernelGGL(k_moments_fine<2>, dim3(gridn(g.E)), dim3(kBlock), 0, st, g, pk, off, M);
+}
+void launch_moments_coarsen(const Geo& gc, const Geo& gf, const double* Mf, double* Mc, hipStream_t st) {
+  if (gc.dim == 3)
+    hipLaunchKernelGGL(k_moments_coarsen<3>, dim3(gridn(gc.E)), dim3(kBlock), 0, st, gc, gf, Mf, Mc);
+  else
+    hipLaunchKernelGGL(k_moments_coarsen<2>, dim3(gridn(gc.E)), dim3(kBlock), 0, st, gc, gf, Mf, Mc);
+}
+
+template <typename S, typename T>
+__global__ void k_convert(const S* __restrict__ a, T* __restrict__ b, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = (T)a[i];
+}
+void launch_d2f(const double* a, float* b, long long n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL((k_convert<double, float>), dim3(gridn(n) < 8192 ? gridn(n) : 8192), dim3(kBlock), 0, st, a, b, n);
+}
+void launch_f2d(const float* a, double* b, long long n, hipStream_t st) {
+  if (n > 0) hipLaunchKernelGGL((k_convert<float, double>), dim3(gridn(n) < 8192 ? gridn(n) : 8192), dim3(kBlock), 0, st, a, b, n);
+}
+
+// ---------------------------------------------------------------------------
+// moment-based K apply (3D): gradient coefficients from the nodal values, the
+// 30 stress moments as contractions with the 27 particle moments, then the
+// same nodal read-back as the particle kernel.
+// ---------------------------------------------------------------------------
+#define MI(a, b, c) ((a) + 3 * (b) + 9 * (c))
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mom_apply3(Geo g, int c, long long mcount, const T* __restrict__ x,
+                                                       T* __restrict__ y, const T* __restrict__ Mom,
+                                                       const uint8_t* __restrict__ bcflag) {
+  const long long t = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem_mg(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
+  const T ix = (T)(0.25 / (vcoord_mg(g, 0, ijk[0] + 1) - vcoord_mg(g, 0, ijk[0])));
+  const T iy = (T)(0.25 / (vcoord_mg(g, 1, ijk[1] + 1) - vcoord_mg(g, 1, ijk[1])));
+  const T iz = (T)(0.25 / (vcoord_mg(g, 2, ijk[2] + 1) - vcoord_mg(g, 2, ijk[2])));
+  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+  T A[3][4], B[3][4], C[3][4];
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) {
+    T v[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) v[a] = x[(n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz) * 3 + cc];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int bit = 1 << d;
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+        if (!(a & bit)) {
+          const T lo = v[a], hi = v[a | bit];
+          v[a] = lo + hi;
+          v[a | bit] = hi - lo;
+        }
+    }
+    A[cc][0] = ix * v[1]; A[cc][1] = ix * v[3]; A[cc][2] = ix * v[5]; A[cc][3] = ix * v[7];
+    B[cc][0] = iy * v[2]; B[cc][1] = iy * v[3]; B[cc][2] = iy * v[6]; B[cc][3] = iy * v[7];
+    C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
+  }
+  T M[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) M[k] = Mom[e * 27 + k];
+  // exponents of the trial monomials: X = {1,s,t,st}, Y = {1,r,t,rt}, Z = {1,r,s,rs}
+  constexpr int XE[4][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}};
+  constexpr int YE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}};
+  constexpr int ZE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
+  constexpr int PXY[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}};
+  constexpr int PXZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}};
+  constexpr int PYZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}};
+  T m[30];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    T vx = 0, vy = 0, vz = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      vx += A[0][k] * M[MI(XE[k][0] + XE[j][0], XE[k][1] + XE[j][1], XE[k][2] + XE[j][2])];
+      vy += B[1][k] * M[MI(YE[k][0] + YE[j][0], YE[k][1] + YE[j][1], YE[k][2] + YE[j][2])];
+      vz += C[2][k] * M[MI(ZE[k][0] + ZE[j][0], ZE[k][1] + ZE[j][1], ZE[k][2] + ZE[j][2])];
+    }
+    m[j] = (T)2 * vx;
+    m[4 + j] = (T)2 * vy;
+    m[8 + j] = (T)2 * vz;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    T vxy = 0, vxz = 0, vyz = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      vxy += B[0][k] * M[MI(YE[k][0] + PXY[j][0], YE[k][1] + PXY[j][1], YE[k][2] + PXY[j][2])] +
+             A[1][k] * M[MI(XE[k][0] + PXY[j][0], XE[k][1] + PXY[j][1], XE[k][2] + PXY[j][2])];
+      vxz += C[0][k] * M[MI(ZE[k][0] + PXZ[j][0], ZE[k][1] + PXZ[j][1], ZE[k][2] + PXZ[j][2])] +
+             A[2][k] * M[MI(XE[k][0] + PXZ[j][0], XE[k][1] + PXZ[j][1], XE[k][2] + PXZ[j][2])];
+      vyz += C[1][k] * M[MI(ZE[k][0] + PYZ[j][0], ZE[k][1] + PYZ[j][1], ZE[k][2] + PYZ[j][2])] +
+             B[2][k] * M[MI(YE[k][0] + PYZ[j][0], YE[k][1] + PYZ[j][1], YE[k][2] + PYZ[j][2])];
+    }
+    m[12 + j] = vxy;
+    m[18 + j] = vxz;
+    m[24 + j] = vyz;
+  }
+  const T mxx0 = m[0], mxxs = m[1], mxxt = m[2], mxxst = m[3];
+  const T myy0 = m[4], myyr = m[5], myyt = m[6], myyrt = m[7];
+  const T mzz0 = m[8], mzzr = m[9], mzzs = m[10], mzzrs = m[11];
+  const T txy0 = m[12], txyr = m[13], txys = m[14], txyt = m[15], txyrt = m[16], txyst = m[17];
+  const T txz0 = m[18], txzr = m[19], txzs = m[20], txzt = m[21], txzrs = m[22], txzst = m[23];
+  const T tyz0 = m[24], tyzr = m[25], tyzs = m[26], tyzt = m[27], tyzrs = m[28], tyzrt = m[29];
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
+    const T yx = ix * ax * (mxx0 + ay * mxxs + az * mxxt + ay * az * mxxst) +
+                 iy * ay * (txy0 + ax * txyr + az * txyt + ax * az * txyrt) +
+                 iz * az * (txz0 + ax * txzr + ay * txzs + ax * ay * txzrs);
+    const T yy = iy * ay * (myy0 + ax * myyr + az * myyt + ax * az * myyrt) +
+                 ix * ax * (txy0 + ay * txys + az * txyt + ay * az * txyst) +
+                 iz * az * (tyz0 + ax * tyzr + ay * tyzs + ax * ay * tyzrs);
+    const T yz = iz * az * (mzz0 + ax * mzzr + ay * mzzs + ax * ay * mzzrs) +
+                 ix * ax * (txz0 + ay * txzs + az * txzt + ay * az * txzst) +
+                 iy * ay * (tyz0 + ax * tyzr + az * tyzt + ax * az * tyzrt);
+    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
+    const unsigned bf = bcflag ? bcflag[nd] : 0u;
+    T* yp = y + nd * 3;
+    if (!(bf & 1u)) yp[0] += yx;
+    if (!(bf & 2u)) yp[1] += yy;
+    if (!(bf & 4u)) yp[2] += yz;
+  }
+}
+
+// 2D: moments M[a + 3b]; trial X = {1,s}, Y = {1,r}; test sets {1,s}, {1,r}, {1,r,s}.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mom_apply2(Geo g, int c, long long mcount, const T* __restrict__ x,
+                                                       T* __restrict__ y, const T* __restrict__ Mom,
+                                                       const uint8_t* __restrict__ bcflag) {
+  const long long t = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem_mg(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * ijk[1];
+  const T ix = (T)(0.5 / (vcoord_mg(g, 0, ijk[0] + 1) - vcoord_mg(g, 0, ijk[0])));
+  const T iy = (T)(0.5 / (vcoord_mg(g, 1, ijk[1] + 1) - vcoord_mg(g, 1, ijk[1])));
+  const long long sy = g.nn[0];
+  const long long n0 = ijk[0] + sy * ijk[1];
+  T A[2][2], B[2][2];
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const T u0 = x[n0 * 2 + cc], u1 = x[(n0 + 1) * 2 + cc], u2 = x[(n0 + sy) * 2 + cc], u3 = x[(n0 + sy + 1) * 2 + cc];
+    const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
+    A[cc][0] = ix * t1; A[cc][1] = ix * t3;
+    B[cc][0] = iy * t2; B[cc][1] = iy * t3;
+  }
+  T M[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) M[k] = Mom[e * 9 + k];
+  // uxx = A00 + A01 s ; uyy = B10 + B11 r ; uxy = B00 + B01 r ; uyx = A10 + A11 s
+  const T mxx0 = 2 * (A[0][0] * M[0] + A[0][1] * M[3]);
+  const T mxxs = 2 * (A[0][0] * M[3] + A[0][1] * M[6]);
+  const T myy0 = 2 * (B[1][0] * M[0] + B[1][1] * M[1]);
+  const T myyr = 2 * (B[1][0] * M[1] + B[1][1] * M[2]);
+  const T txy0 = B[0][0] * M[0] + B[0][1] * M[1] + A[1][0] * M[0] + A[1][1] * M[3];
+  const T txyr = B[0][0] * M[1] + B[0][1] * M[2] + A[1][0] * M[1] + A[1][1] * M[4];
+  const T txys = B[0][0] * M[3] + B[0][1] * M[4] + A[1][0] * M[3] + A[1][1] * M[6];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
+    const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
+    const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
+    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy;
+    const unsigned bf = bcflag ? bcflag[nd] : 0u;
+    T* yp = y + nd * 2;
+    if (!(bf & 1u)) yp[0] += yx;
+    if (!(bf & 2u)) yp[1] += yy;
+  }
+}
+
+template <typename T>
+__global__ void k_fill_mg(T* __restrict__ y, long long n, T v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    y[i] = v;
+}
+
+template <typename T>
+void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, hipStream_t st) {
+  const long long n = g.N * g.dim;
+  unsigned gb = gridn(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_fill_mg<T>, dim3(gb), dim3(kBlock), 0, st, y, n, (T)0);
+  const int ncol = g.dim == 2 ? 4 : 8;
+  for (int c = 0; c < ncol; c++) {
+    const long long m = colour_count(g, c);
+    if (m <= 0) continue;
+    if (g.dim == 3)
+      hipLaunchKernelGGL(k_mom_apply3<T>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, x, y, Mom, bcflag);
+    else
+      hipLaunchKernelGGL(k_mom_apply2<T>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, x, y, Mom, bcflag);
+  }
+}
+template void launch_mom_apply<double>(const Geo&, const double*, double*, const double*, const uint8_t*, hipStream_t);
+template void launch_mom_apply<float>(const Geo&, const float*, float*, const float*, const uint8_t*, hipStream_t);
+
+// diag(K) from moments: node a, component d: sum_p omega (|grad N_a|^2 + (dN_a/dx_d)^2).
+template <int DIM>
+__global__ void k_mom_diag(Geo g, int c, long long mcount, const double* __restrict__ Mom, double* __restrict__ diag) {
+  constexpr int NEN = DIM == 2 ? 4 : 8;
+  constexpr int NM = DIM == 2 ? 9 : 27;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= mcount) return;
+  long long ijk[3];
+  colour_elem_mg(g, c, t, ijk);
+  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
+  double inv[3];
+  for (int d = 0; d < DIM; d++) inv[d] = 2.0 / (vcoord_mg(g, d, ijk[d] + 1) - vcoord_mg(g, d, ijk[d]));
+  const double cN = DIM == 2 ? 0.25 : 0.125;
+  const double* M = Mom + e * NM;
+  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+  for (int a = 0; a < NEN; a++) {
+    // (dN_a/dx_d)^2 = (cN inv_d)^2 prod_{d' != d} (1 + s_d' x_d')^2 ; (1 + s x)^2 = 1 + 2 s x + x^2
+    double sq[3];
+    for (int d = 0; d < DIM; d++) {
+      double v = 0.0;
+      // polynomial coefficients per axis: axis d contributes only degree 0 (coefficient 1)
+      double cf[3][3];
+      for (int d2 = 0; d2 < 3; d2++) {
+        const double s = ((a >> d2) & 1) ? 1.0 : -1.0;
+        if (d2 >= DIM || d2 == d) {
+          cf[d2][0] = 1.0; cf[d2][1] = 0.0; cf[d2][2] = 0.0;
+        } else {
+          cf[d2][0] = 1.0; cf[d2][1] = 2.0 * s; cf[d2][2] = 1.0;
+        }
+      }
+      if (DIM == 2) {
+        for (int b = 0; b < 3; b++)
+          for (int aa = 0; aa < 3; aa++) v += cf[0][aa] * cf[1][b] * M[aa + 3 * b];
+      } else {
+        for (int cc = 0; cc < 3; cc++)
+          for (int b = 0; b < 3; b++)
+            for (int aa = 0; aa < 3; aa++) v += cf[0][aa] * cf[1][b] * cf[2][cc] * M[aa + 3 * b + 9 * cc];
+      }
+      sq[d] = (cN * inv[d]) * (cN * inv[d]) * v;
+    }
+    double tot = 0.0;
+    for (int d = 0; d < DIM; d++) tot += sq[d];
+    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
+    for (int d = 0; d < DIM; d++) diag[nd * DIM + d] += tot + sq[d];
+  }
+}
+
+void launch_mom_diag(const Geo& g, const double* Mom, double* diag, hipStream_t st) {
+  const long long n = g.N * g.dim;
+  unsigned gb = gridn(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_fill_mg<double>, dim3(gb), dim3(kBlock), 0, st, diag, n, 0.0);
+  const int ncol = g.dim == 2 ? 4 : 8;
+  for (int c = 0; c < ncol; c++) {
+    const long long m = colour_count(g, c);
+    if (m <= 0) continue;
+    if (g.dim == 3)
+      hipLaunchKernelGGL(k_mom_diag<3>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, Mom, diag);
+    else
+      hipLaunchKernelGGL(k_mom_diag<2>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, Mom, diag);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// transfers: P = Q1 interpolation coarse -> fine, R = P^T (full weighting)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ bcflag_c) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= gc.N) return;
+  const int dim = gc.dim;
+  const long long I = n % gc.nn[0], J = (n / gc.nn[0]) % gc.nn[1], K = n / (gc.nn[0] * gc.nn[1]);
+  T acc[3] = {0, 0, 0};
+  const int kz0 = dim == 3 ? -1 : 0, kz1 = dim == 3 ? 1 : 0;
+  for (int dz = kz0; dz <= kz1; dz++) {
+    const long long k = 2 * K + dz;
+    if (k < 0 || k >= gf.nn[2]) continue;
+    const T wz = dz == 0 ? (T)1 : (T)0.5;
+    for (int dy = -1; dy <= 1; dy++) {
+      const long long j = 2 * J + dy;
+      if (j < 0 || j >= gf.nn[1]) continue;
+      const T wy = dy == 0 ? (T)1 : (T)0.5;
+      for (int dx = -1; dx <= 1; dx++) {
+        const long long i = 2 * I + dx;
+        if (i < 0 || i >= gf.nn[0]) continue;
+        const T w = wz * wy * (dx == 0 ? (T)1 : (T)0.5);
+        const long long f = i + gf.nn[0] * (j + gf.nn[1] * k);
+        for (int d = 0; d < dim; d++) acc[d] += w * rf[f * dim + d];
+      }
+    }
+  }
+  const unsigned bf = bcflag_c ? bcflag_c[n] : 0u;
+  for (int d = 0; d < dim; d++) bc[n * dim + d] = ((bf >> d) & 1u) ? (T)0 : acc[d];
+}
+
+template <typename T>
+__global__ void k_prolong_add(Geo gc, Geo gf, const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ bcflag_f) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= gf.N) return;
+  const int dim = gf.dim;
+  const long long i = n % gf.nn[0], j = (n / gf.nn[0]) % gf.nn[1], k = n / (gf.nn[0] * gf.nn[1]);
+  T acc[3] = {0, 0, 0};
+  const int nz = (dim == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
+  const T wz = nz == 2 ? (T)0.5 : (T)1, wy = ny == 2 ? (T)0.5 : (T)1, wx = nx == 2 ? (T)0.5 : (T)1;
+  for (int a = 0; a < nz; a++) {
+    const long long K = dim == 3 ? (k >> 1) + a : 0;
+    for (int b = 0; b < ny; b++) {
+      const long long J = (j >> 1) + b;
+      for (int cc = 0; cc < nx; cc++) {
+        const long long I = (i >> 1) + cc;
+        const long long cn = I + gc.nn[0] * (J + gc.nn[1] * K);
+        for (int d = 0; d < dim; d++) acc[d] += wx * wy * wz * xc[cn * dim + d];
+      }
+    }
+  }
+  const unsigned bf = bcflag_f ? bcflag_f[n] : 0u;
+  for (int d = 0; d < dim; d++)
+    if (!((bf >> d) & 1u)) xf[n * dim + d] += acc[d];
+}
+
+template <typename T>
+void launch_restrict(const Geo& gc, const Geo& gf, const T* rf, T* bc, const uint8_t* bcflag_c, hipStream_t st) {
+  hipLaunchKernelGGL(k_restrict<T>, dim3(gridn(gc.N)), dim3(kBlock), 0, st, gc, gf, rf, bc, bcflag_c);
+}
+template <typename T>
+void launch_prolong_add(const Geo& gc, const Geo& gf, const T* xc, T* xf, const uint8_t* bcflag_f, hipStream_t st) {
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(gridn(gf.N)), dim3(kBlock), 0, st, gc, gf, xc, xf, bcflag_f);
+}
+template void launch_restrict<double>(const Geo&, const Geo&, const double*, double*, const uint8_t*, hipStream_t);
+template void launch_restrict<float>(const Geo&, const Geo&, const float*, float*, const uint8_t*, hipStream_t);
+template void launch_prolong_add<double>(const Geo&, const Geo&, const double*, double*, const uint8_t*, hipStream_t);
+template void launch_prolong_add<float>(const Geo&, const Geo&, const float*, float*, const uint8_t*, hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Chebyshev-Jacobi step: r = b - Ax ; d = alpha d + beta Dinv r ; x += d
+// (with x == 0 on entry, "first" uses Ax = 0 and skips reading x / Ax).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_cheb(const T* __restrict__ b, const T* __restrict__ Ax, const T* __restrict__ dinv, T* __restrict__ d,
+                       T* __restrict__ x, long long n, T alpha, T beta, int first) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const T r = first ? b[i] : b[i] - Ax[i];
+    const T di = (first ? (T)0 : alpha * d[i]) + beta * dinv[i] * r;
+    d[i] = di;
+    x[i] = first ? di : x[i] + di;
+  }
+}
+template <typename T>
+void launch_cheb(const T* b, const T* Ax, const T* dinv, T* d, T* x, long long n, T alpha, T beta, int first, hipStream_t st) {
+  unsigned gb = gridn(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_cheb<T>, dim3(gb), dim3(kBlock), 0, st, b, Ax, dinv, d, x, n, alpha, beta, first);
+}
+template void launch_cheb<double>(const double*, const double*, const double*, double*, double*, long long, double, double, int, hipStream_t);
+template void launch_cheb<float>(const float*, const float*, const float*, float*, float*, long long, float, float, int, hipStream_t);
+
+// r = b - Ax
+template <typename T>
+__global__ void k_resid(const T* __restrict__ b, const T* __restrict__ Ax, T* __restrict__ r, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    r[i] = b[i] - Ax[i];
+}
+template <typename T>
+void launch_resid(const T* b, const T* Ax, T* r, long long n, hipStream_t st) {
+  unsigned gb = gridn(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_resid<T>, dim3(gb), dim3(kBlock), 0, st, b, Ax, r, n);
+}
+template void launch_resid<double>(const double*, const double*, double*, long long, hipStream_t);
+template void launch_resid<float>(const float*, const float*, float*, long long, hipStream_t);
+
+// Dense coarse solve: x[free[i]] = sum_j Kinv[i][j] b[free[j]], one wavefront per row.
+template <typename T>
+__global__ void k_dense_solve(const T* __restrict__ Kinv, const int* __restrict__ freeidx, int nf, const T* __restrict__ b,
+                              T* __restrict__ x) {
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nf) return;
+  T s = 0;
+  for (int j = lane; j < nf; j += 64) s += Kinv[(long long)row * nf + j] * b[freeidx[j]];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) x[freeidx[row]] = s;
+}
+template <typename T>
+void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, const T* b, T* x, long long n, hipStream_t st) {
+  unsigned gb = gridn(n);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(k_fill_mg<T>, dim3(gb), dim3(kBlock), 0, st, x, n, (T)0);
+  if (nf > 0) hipLaunchKernelGGL(k_dense_solve<T>, dim3((nf + 3) / 4), dim3(kBlock), 0, st, Kinv, freeidx, nf, b, x);
+}
+template void launch_dense_solve<double>(const double*, const int*, int, const double*, double*, long long, hipStream_t);
+template void launch_dense_solve<float>(const float*, const int*, int, const float*, float*, long long, hipStream_t);
+
+}  // namespace gs

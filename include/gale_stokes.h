@@ -37,6 +37,11 @@ typedef enum gs_status {
 enum { GS_YIELD_NONE = 0, GS_YIELD_DRUCKER_PRAGER = 1, GS_YIELD_VON_MISES = 2 };
 enum { GS_PC_NONE = 0, GS_PC_DIAGONAL = 1 };
 enum { GS_PREC_FP64 = 0, GS_PREC_MIXED = 1 };
+/* Inner velocity preconditioner: Jacobi, or a geometric multigrid V-cycle whose
+ * levels are the Galerkin coarse operators of the particle-integrated K. */
+enum { GS_INNER_JACOBI = 0, GS_INNER_MULTIGRID = 1 };
+/* Operator forms timed by gs_time_apply_K. */
+enum { GS_APPLY_PARTICLES_FP64 = 0, GS_APPLY_PARTICLES_FP32 = 1, GS_APPLY_MOMENTS_FP64 = 2, GS_APPLY_MOMENTS_FP32 = 3 };
 
 /* One RheologyMaterial: MaterialViscosity (+ optional yield rheology) and
  * the density used by BuoyancyForceTerm.  XML keys in comments. */
@@ -85,8 +90,10 @@ typedef struct gs_uzawa_params {
   int32_t inner_max_iterations; /* cap per inner velocity solve (0 = 50000) */
   double inner_rtol_factor;     /* inner rtol = factor * tolerance (reference: 0.1, :591,596) */
   double inner_rtol_override;   /* > 0: this rtol for every inner solve */
-  int32_t monitor;              /* "monitor": print outer residuals to stderr */
+  int32_t monitor;              /* "monitor": print the outer residual to stderr every monitor iterations (0 = off) */
   int32_t check_every;          /* inner CG convergence check interval (0 = 4) */
+  int32_t inner_preconditioner; /* GS_INNER_JACOBI or GS_INNER_MULTIGRID */
+  int32_t mg_smooth;            /* Chebyshev-Jacobi steps before and after each coarse correction (0 = 2) */
 } gs_uzawa_params;
 
 /* SystemLinearEquations nonlinear keys (SystemLinearEquations.cxx:346-356). */
@@ -190,9 +197,12 @@ int32_t gs_apply_K_fp32(gs_solver* s, const float* x_dev, float* y_dev);
 int32_t gs_apply_G(gs_solver* s, const double* p_dev, double* y_dev);
 int32_t gs_apply_Gt(gs_solver* s, const double* x_dev, double* q_dev);
 int32_t gs_solver_sync(gs_solver* s);
-/* Average device time (ms) of the last n_rep K applies timed by
- * gs_time_apply_K with HIP events on the solver's stream. */
-int32_t gs_time_apply_K(gs_solver* s, int32_t precision, int32_t n_rep, double* ms_per_apply);
+/* Average device time (ms) of n_rep K applies of the given form
+ * (GS_APPLY_*), timed with HIP events on the solver's stream. */
+int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, double* ms_per_apply);
+/* K apply from the per-element particle moments (the form the solver uses;
+ * equal to gs_apply_K up to rounding).  Device pointers, full nodal vectors. */
+int32_t gs_apply_K_moments(gs_solver* s, const double* x_dev, double* y_dev);
 
 /* Diagnostics of the last setup (host arrays): diag(K) on all dofs (N*dim),
  * Q (E), f after lifting (N*dim, zero on Dirichlet dofs), h (E),

@@ -122,6 +122,12 @@ def test_operator_applies_match_oracle(gale, oracle, dim, res):
     np.testing.assert_array_equal(yk, y2.cpu().numpy())  # deterministic scatter
     assert rel(gy.cpu().numpy(), o.apply_G(p)) < 1e-13
     assert rel(q.cpu().numpy(), o.apply_Gt(x)) < 1e-13
+    # moment form of K (what the solver applies): the same operator up to rounding
+    ym = torch.zeros_like(xd)
+    torch.cuda.synchronize()
+    s.apply_K_moments(xd, ym)
+    s.sync()
+    assert rel(ym.cpu().numpy(), yk) < 1e-12
     # FP32 apply (the mixed-precision preconditioner)
     x32 = xd.float()
     y32 = torch.zeros_like(x32)
@@ -131,11 +137,14 @@ def test_operator_applies_match_oracle(gale, oracle, dim, res):
     assert rel(y32.double().cpu().numpy(), yk) < 2e-5
 
 
-@pytest.mark.parametrize("dim,res", [(2, 16), (3, 5)])
-def test_uzawa_parity_tight(gale, oracle, dim, res):
+@pytest.mark.parametrize("dim,res,inner,prec", [(2, 16, 1, 0), (3, 5, 1, 0), (2, 16, 0, 0), (3, 6, 0, 0),
+                                                (2, 16, 1, 1), (3, 8, 1, 1)])
+def test_uzawa_parity_tight(gale, oracle, dim, res, inner, prec):
+    """Jacobi-CG, multigrid-FCG and the FP32-V-cycle mixed form all reach the
+    oracle's discrete solution at tight tolerances."""
     cfg = small_sinker(dim, res)
     g, o = pair(gale, oracle, cfg)
-    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale))
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale, inner_preconditioner=inner, precision=prec))
     u, p, rep = s.Solve()
     st, ou, op, orep = o.uzawa(otight(oracle))
     assert st == 0 and rep.status == 0
