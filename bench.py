@@ -117,9 +117,8 @@ def main():
         u, p, rep = s.Solve()
         print(f"# warmup {w}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations}",
               file=sys.stderr, flush=True)
-    import torch
     times, reps = [], []
-    torch.cuda.synchronize()
+    s.sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
         s.set_pressure(None)
@@ -128,7 +127,7 @@ def main():
         reps.append(rep.as_dict())
         print(f"# step {k}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations} "
               f"rel={rep.rel_residual:.3e}", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
+    s.sync()
     wall = time.perf_counter() - t0
     solve_s = float(np.mean(times))
     last = reps[-1]

@@ -18,6 +18,7 @@
 #include "../../include/gale_stokes.h"
 #include "gs_internal.h"
 #include "gs_mg.h"
+#include "gs_tile.h"
 
 using namespace gs;
 
@@ -83,6 +84,8 @@ struct gs_problem {
   double* hS = nullptr;  // pinned
   Reducer rd{};
   bool min_one_particle = true;
+  double* halo64 = nullptr;  // tiled K apply scratch (fine level)
+  float* halo32 = nullptr;
 };
 
 
@@ -102,6 +105,8 @@ struct MGLevel {
   double lmax = 2.0;
   double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *Ax = nullptr;
   float *bf = nullptr, *xf = nullptr, *rf = nullptr, *df = nullptr, *Axf = nullptr;
+  double* h64 = nullptr;  // tiled-apply halo scratch
+  float* h32 = nullptr;
 };
 
 struct MGData {
@@ -305,7 +310,8 @@ extern "C" int32_t gs_problem_create(const gs_problem_desc* d, gs_problem** out)
       dalloc(&pb->ubc, ND) || dalloc(&pb->pk, P) || dalloc(&pb->coord, 3 * P) || dalloc(&pb->coord2, 3 * P) ||
       dalloc(&pb->mat, P) || dalloc(&pb->mat2, P) || dalloc(&pb->pid, P) || dalloc(&pb->pid2, P) ||
       dalloc(&pb->off, g.E + 1) || dalloc(&pb->keys, P) || dalloc(&pb->keys2, P) || dalloc(&pb->vals, P) ||
-      dalloc(&pb->vals2, P) || dalloc(&pb->S, S_COUNT))
+      dalloc(&pb->vals2, P) || dalloc(&pb->S, S_COUNT) || dalloc(&pb->halo64, tile_halo_size(g)) ||
+      dalloc(&pb->halo32, tile_halo_size(g)))
     return fail(GS_E_HIP);
   if (d->eta0_particle && (dalloc(&pb->eta0p, P) || dalloc(&pb->eta0p2, P))) return fail(GS_E_HIP);
   if (hipHostMalloc(reinterpret_cast<void**>(&pb->hS), sizeof(double) * S_COUNT) != hipSuccess) return fail(GS_E_HIP);
@@ -334,7 +340,8 @@ extern "C" void gs_problem_destroy(gs_problem* pb) {
   dfree(pb->mats_d); dfree(pb->bcdof); dfree(pb->bcflag); dfree(pb->ubc); dfree(pb->pk); dfree(pb->pk32);
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->off); dfree(pb->keys); dfree(pb->keys2); dfree(pb->vals);
-  dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->rd.counter);
+  dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->rd.counter); dfree(pb->halo64);
+  dfree(pb->halo32);
   if (pb->sort_tmp) (void)hipFree(pb->sort_tmp);
   if (pb->hS) (void)hipHostFree(pb->hS);
   if (pb->st) (void)hipStreamDestroy(pb->st);
@@ -518,7 +525,7 @@ static int32_t read_scalars(gs_solver* s) {
 
 static void apply_K_masked(gs_solver* s, const double* x, double* y) {
   gs_problem* pb = s->pb;
-  launch_apply_K(pb->g, x, y, pb->pk, pb->off, pb->bcflag, pb->st);
+  launch_apply_K(pb->g, x, y, pb->pk, pb->off, pb->bcflag, pb->halo64, pb->st);
   s->k_applies++;
 }
 
@@ -529,7 +536,7 @@ static void mg_free(MGData& mg) {
   for (auto& L : mg.lv) {
     dfree(L.M64); dfree(L.M32); dfree(L.diag); dfree(L.dinv64); dfree(L.dinv32);
     dfree(L.b); dfree(L.x); dfree(L.r); dfree(L.d); dfree(L.Ax);
-    dfree(L.bf); dfree(L.xf); dfree(L.rf); dfree(L.df); dfree(L.Axf);
+    dfree(L.bf); dfree(L.xf); dfree(L.rf); dfree(L.df); dfree(L.Axf); dfree(L.h64); dfree(L.h32);
     if (L.own_bc) {
       dfree(L.bcflag);
       dfree(L.bcdof);
@@ -550,6 +557,7 @@ static int32_t mg_alloc_level(MGLevel& L) {
   CK(dalloc(&L.b, L.n)); CK(dalloc(&L.x, L.n)); CK(dalloc(&L.r, L.n)); CK(dalloc(&L.d, L.n)); CK(dalloc(&L.Ax, L.n));
   CK(dalloc(&L.bf, L.n)); CK(dalloc(&L.xf, L.n)); CK(dalloc(&L.rf, L.n)); CK(dalloc(&L.df, L.n));
   CK(dalloc(&L.Axf, L.n));
+  CK(dalloc(&L.h64, tile_halo_size(L.g))); CK(dalloc(&L.h32, tile_halo_size(L.g)));
   return GS_OK;
 }
 
@@ -696,7 +704,7 @@ static int32_t mg_numeric(gs_solver* s) {
     launch_masked_copy(L.x, L.x, L.bcdof, L.n, st);
     double lam = 1.0;
     for (int it = 0; it < 12; it++) {
-      launch_mom_apply<double>(L.g, L.x, L.Ax, L.M64, L.bcflag, st);
+      launch_mom_apply<double>(L.g, L.x, L.Ax, L.M64, L.bcflag, L.h64, st);
       // x <- D^-1 K x / |.|
       launch_mul(L.dinv64, L.Ax, L.d, L.n, st);
       launch_dot(L.d, L.d, L.n, s->S, S_TMP1, s->rd, st);
@@ -735,7 +743,7 @@ static int32_t mg_numeric(gs_solver* s) {
             }
             if (!any) continue;
             CK(hipMemcpyAsync(C.x, v.data(), sizeof(double) * C.n, hipMemcpyHostToDevice, st));
-            launch_mom_apply<double>(C.g, C.x, C.Ax, C.M64, C.bcflag, st);
+            launch_mom_apply<double>(C.g, C.x, C.Ax, C.M64, C.bcflag, C.h64, st);
             CK(hipMemcpyAsync(y.data(), C.Ax, sizeof(double) * C.n, hipMemcpyDeviceToHost, st));
             CK(hipStreamSynchronize(st));
             for (long long n = 0; n < N; n++) {
@@ -767,9 +775,10 @@ template <typename T>
 struct LvPtr {
   T *b, *x, *r, *d, *Ax;
   const T *M, *dinv;
+  T* h;
 };
-static LvPtr<double> lvp(MGLevel& L, double*) { return {L.b, L.x, L.r, L.d, L.Ax, L.M64, L.dinv64}; }
-static LvPtr<float> lvp(MGLevel& L, float*) { return {L.bf, L.xf, L.rf, L.df, L.Axf, L.M32, L.dinv32}; }
+static LvPtr<double> lvp(MGLevel& L, double*) { return {L.b, L.x, L.r, L.d, L.Ax, L.M64, L.dinv64, L.h64}; }
+static LvPtr<float> lvp(MGLevel& L, float*) { return {L.bf, L.xf, L.rf, L.df, L.Axf, L.M32, L.dinv32, L.h32}; }
 
 template <typename T>
 static void cheb_smooth(MGLevel& L, int nu, bool zero, hipStream_t st) {
@@ -780,12 +789,12 @@ static void cheb_smooth(MGLevel& L, int nu, bool zero, hipStream_t st) {
   if (zero) {
     launch_cheb<T>(P.b, nullptr, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 1, st);
   } else {
-    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
     launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 0, st);
   }
   for (int k = 1; k < nu; k++) {
     const double rho = 1.0 / (2.0 * sigma - rho_prev);
-    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
     launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)(rho * rho_prev), (T)(2.0 * rho / delta), 0, st);
     rho_prev = rho;
   }
@@ -809,7 +818,7 @@ static void vcycle(gs_solver* s, int l) {
   MGLevel& C = mg.lv[l + 1];
   LvPtr<T> Q = lvp(C, (T*)nullptr);
   cheb_smooth<T>(L, nu, true, st);
-  launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, st);
+  launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
   launch_resid<T>(P.b, P.Ax, P.r, L.n, st);
   launch_restrict<T>(C.g, L.g, P.r, Q.b, C.bcflag, st);
   vcycle<T>(s, l + 1);
@@ -852,7 +861,7 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
   launch_make_dinv(s->diag, pb->bcdof, s->dinv, s->n, st);
   launch_recip(s->Q, s->Qinv, s->np, st);
   if (pb->has_bc) {
-    launch_apply_K(pb->g, pb->ubc, s->tmpN, pb->pk, pb->off, nullptr, st);
+    launch_apply_K(pb->g, pb->ubc, s->tmpN, pb->pk, pb->off, nullptr, pb->halo64, st);
     s->k_applies++;
     launch_lift(s->f, s->tmpN, pb->bcdof, s->n, st);
     launch_apply_Gt(pb->g, pb->ubc, s->h, st);
@@ -949,7 +958,7 @@ static int32_t inner_solve_mg(gs_solver* s, const double* b, double* x, double r
   int k = 0;
   bool conv = false;
   while (k < maxit) {
-    launch_mom_apply<double>(L0.g, s->cg_p, s->cg_Ap, L0.M64, pb->bcflag, st);
+    launch_mom_apply<double>(L0.g, s->cg_p, s->cg_Ap, L0.M64, pb->bcflag, L0.h64, st);
     s->k_applies++;
     launch_dot(s->cg_p, s->cg_Ap, n, s->S, S_PAP, s->rd, st);
     launch_fcg_update(x, s->cg_r, s->cg_p, s->cg_Ap, n, s->S, s->rd, st);
@@ -1243,7 +1252,7 @@ extern "C" int32_t gs_advect_rk2(gs_solver* s, double dt) {
 extern "C" int32_t gs_apply_K(gs_solver* s, const double* x, double* y) {
   if (!s || !x || !y) return GS_E_INVALID;
   if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
-  launch_apply_K(s->pb->g, x, y, s->pb->pk, s->pb->off, nullptr, s->pb->st);
+  launch_apply_K(s->pb->g, x, y, s->pb->pk, s->pb->off, nullptr, s->pb->halo64, s->pb->st);
   s->k_applies++;
   return GS_OK;
 }
@@ -1262,7 +1271,7 @@ static int32_t ensure_pk32(gs_solver* s) {
 extern "C" int32_t gs_apply_K_fp32(gs_solver* s, const float* x, float* y) {
   if (!s || !x || !y) return GS_E_INVALID;
   CKS(ensure_pk32(s));
-  launch_apply_K32(s->pb->g, x, y, s->pb->pk32, s->pb->off, nullptr, s->pb->st);
+  launch_apply_K32(s->pb->g, x, y, s->pb->pk32, s->pb->off, nullptr, s->pb->halo32, s->pb->st);
   s->k_applies++;
   return GS_OK;
 }
@@ -1291,7 +1300,7 @@ extern "C" int32_t gs_apply_K_moments(gs_solver* s, const double* x, double* y) 
     CKS(setup_dev(s, nullptr, nullptr, 0));
     if (!s->mg.structured) CKS(mg_numeric(s));
   }
-  launch_mom_apply<double>(s->mg.lv[0].g, x, y, s->mg.lv[0].M64, nullptr, s->pb->st);
+  launch_mom_apply<double>(s->mg.lv[0].g, x, y, s->mg.lv[0].M64, nullptr, s->mg.lv[0].h64, s->pb->st);
   s->k_applies++;
   return GS_OK;
 }
@@ -1312,10 +1321,10 @@ extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, do
   }
   auto one = [&]() {
     switch (form) {
-      case 0: launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, st); break;
-      case 1: launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, st); break;
-      case 2: launch_mom_apply<double>(s->mg.lv[0].g, s->cg_p, s->cg_Ap, s->mg.lv[0].M64, nullptr, st); break;
-      default: launch_mom_apply<float>(s->mg.lv[0].g, s->x32, s->y32, s->mg.lv[0].M32, nullptr, st); break;
+      case 0: launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, pb->halo64, st); break;
+      case 1: launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, pb->halo32, st); break;
+      case 2: launch_mom_apply<double>(s->mg.lv[0].g, s->cg_p, s->cg_Ap, s->mg.lv[0].M64, nullptr, s->mg.lv[0].h64, st); break;
+      default: launch_mom_apply<float>(s->mg.lv[0].g, s->x32, s->y32, s->mg.lv[0].M32, nullptr, s->mg.lv[0].h32, st); break;
     }
   };
   one();

@@ -70,10 +70,11 @@ struct SetupArgs {
 void launch_fill(double* y, long long n, double v, hipStream_t st);
 void launch_fill_f(float* y, long long n, float v, hipStream_t st);
 // K apply on full nodal vectors; bcflag (per node bitmask) null = unconstrained.
+// halo: tile_halo_size(g) values of scratch (gs_tile.h).
 void launch_apply_K(const Geo& g, const double* x, double* y, const double4* pk, const int* off,
-                    const uint8_t* bcflag, hipStream_t st);
+                    const uint8_t* bcflag, double* halo, hipStream_t st);
 void launch_apply_K32(const Geo& g, const float* x, float* y, const float4* pk, const int* off,
-                      const uint8_t* bcflag, hipStream_t st);
+                      const uint8_t* bcflag, float* halo, hipStream_t st);
 void launch_apply_G(const Geo& g, const double* p, double* y, const uint8_t* bcflag, hipStream_t st);
 void launch_apply_Gt(const Geo& g, const double* x, double* q, hipStream_t st);
 void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f,

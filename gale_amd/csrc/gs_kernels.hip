@@ -1,21 +1,20 @@
 // gs_kernels.hip -- CDNA4 (gfx950) kernels of the Q1-P0 PIC Stokes hot path.
 //
-// Element kernels run one thread per element.  Elements are processed in
-// 2^dim parity colours so that no two threads of a launch share a node: the
-// nodal scatter is a plain read-modify-write, deterministic and free of
-// atomics.  Within a colour the block index is remapped so that each of the
-// eight XCDs (blocks are dealt round-robin to XCDs) works on one contiguous
-// range of elements and its L2 sees the neighbouring nodes.
+// The constitutive setup pass runs one thread per element in 2^dim parity
+// colours so that no two threads of a launch share a node: its nodal scatter
+// is a plain read-modify-write, deterministic and free of atomics.  Within a
+// colour the block index is remapped so that each of the eight XCDs (blocks
+// are dealt round-robin to XCDs) works on one contiguous range of elements.
 //
 // K apply (reference: K_e = sum_p w_p detJ B_p^T D_p B_p assembled by
 // ConstitutiveMatrixCartesian.cxx:363-485 and applied by PETSc MatMult) is
-// evaluated matrix-free: the element's nodal velocities are turned into the
-// coefficients of the (bi/tri)linear velocity gradient by a Walsh-Hadamard
-// butterfly, each material point adds its stress times the monomials
-// {1, r, s, t, rs, rt, st} to 30 (3D) / 7 (2D) element moments, and the
-// nodal forces are read back from the moments.  Each particle costs about 70
-// FMA-class operations and 32 bytes of stream.
+// evaluated matrix-free in gs_tile.hip: the element's nodal velocities are
+// turned into the coefficients of the (bi/tri)linear velocity gradient by a
+// Walsh-Hadamard butterfly, each material point adds its stress times the
+// monomials {1, r, s, t, rs, rt, st} to 30 (3D) / 7 (2D) element moments, and
+// the nodal forces are read back from the moments.
 #include "gs_internal.h"
+#include "gs_tile.h"
 
 #include <math.h>
 
@@ -435,202 +434,15 @@ void launch_uz_uupdate(double* u, const double* v, long long n, const double* S,
 }
 
 // ---------------------------------------------------------------------------
-// K apply
+// K apply: the tiled kernels of gs_tile.hip.
 // ---------------------------------------------------------------------------
-template <typename T>
-struct Vec4;
-template <>
-struct Vec4<double> {
-  using type = double4;
-};
-template <>
-struct Vec4<float> {
-  using type = float4;
-};
-
-// Lanes per element: the particles of one element are split over LPE
-// adjacent lanes (contiguous 32-byte records -> coalesced loads), the 30 / 7
-// element moments are summed across the lanes with two xor-shuffles, and each
-// lane scatters its share of the element's nodes.
-constexpr int kLPE = 4;
-
-template <typename T>
-__device__ __forceinline__ T quad_sum(T v) {
-  v += __shfl_xor(v, 1, 64);
-  v += __shfl_xor(v, 2, 64);
-  return v;
-}
-
-// 3D: LPE lanes per element of colour c.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_apply_K3(Geo g, int c, long long mcount, const T* __restrict__ x,
-                                                     T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
-                                                     const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
-  const long long gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const long long t = gt / kLPE;
-  const int lane = (int)(gt % kLPE);
-  if (t >= mcount) return;
-  long long ijk[3];
-  colour_elem(g, c, t, ijk);
-  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
-  const T ix = (T)(0.25 / (vcoord(g, 0, ijk[0] + 1) - vcoord(g, 0, ijk[0])));
-  const T iy = (T)(0.25 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
-  const T iz = (T)(0.25 / (vcoord(g, 2, ijk[2] + 1) - vcoord(g, 2, ijk[2])));
-  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
-  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
-  // gradient coefficients: L[c][0] = A[c] . (1,s,t,st), L[c][1] = B[c] . (1,r,t,rt), L[c][2] = C[c] . (1,r,s,rs)
-  T A[3][4], B[3][4], C[3][4];
-#pragma unroll
-  for (int cc = 0; cc < 3; cc++) {
-    T v[8];
-#pragma unroll
-    for (int a = 0; a < 8; a++) v[a] = x[(n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz) * 3 + cc];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const int bit = 1 << d;
-#pragma unroll
-      for (int a = 0; a < 8; a++)
-        if (!(a & bit)) {
-          const T lo = v[a], hi = v[a | bit];
-          v[a] = lo + hi;
-          v[a | bit] = hi - lo;
-        }
-    }
-    A[cc][0] = ix * v[1]; A[cc][1] = ix * v[3]; A[cc][2] = ix * v[5]; A[cc][3] = ix * v[7];
-    B[cc][0] = iy * v[2]; B[cc][1] = iy * v[3]; B[cc][2] = iy * v[6]; B[cc][3] = iy * v[7];
-    C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
-  }
-  // moments
-  T m[30];
-#pragma unroll
-  for (int k = 0; k < 30; k++) m[k] = 0;
-  const int pb = off[e], pe = off[e + 1];
-  for (int p = pb + lane; p < pe; p += kLPE) {
-    const typename Vec4<T>::type q = pk[p];
-    const T r = q.x, s = q.y, tt = q.z, w = q.w;
-    const T st = s * tt, rt = r * tt, rs = r * s;
-    const T uxx = A[0][0] + A[0][1] * s + A[0][2] * tt + A[0][3] * st;
-    const T uxy = B[0][0] + B[0][1] * r + B[0][2] * tt + B[0][3] * rt;
-    const T uxz = C[0][0] + C[0][1] * r + C[0][2] * s + C[0][3] * rs;
-    const T uyx = A[1][0] + A[1][1] * s + A[1][2] * tt + A[1][3] * st;
-    const T uyy = B[1][0] + B[1][1] * r + B[1][2] * tt + B[1][3] * rt;
-    const T uyz = C[1][0] + C[1][1] * r + C[1][2] * s + C[1][3] * rs;
-    const T uzx = A[2][0] + A[2][1] * s + A[2][2] * tt + A[2][3] * st;
-    const T uzy = B[2][0] + B[2][1] * r + B[2][2] * tt + B[2][3] * rt;
-    const T uzz = C[2][0] + C[2][1] * r + C[2][2] * s + C[2][3] * rs;
-    const T w2 = w + w;
-    const T sxx = w2 * uxx, syy = w2 * uyy, szz = w2 * uzz;
-    const T sxy = w * (uxy + uyx), sxz = w * (uxz + uzx), syz = w * (uyz + uzy);
-    m[0] += sxx; m[1] += sxx * s; m[2] += sxx * tt; m[3] += sxx * st;
-    m[4] += syy; m[5] += syy * r; m[6] += syy * tt; m[7] += syy * rt;
-    m[8] += szz; m[9] += szz * r; m[10] += szz * s; m[11] += szz * rs;
-    m[12] += sxy; m[13] += sxy * r; m[14] += sxy * s; m[15] += sxy * tt; m[16] += sxy * rt; m[17] += sxy * st;
-    m[18] += sxz; m[19] += sxz * r; m[20] += sxz * s; m[21] += sxz * tt; m[22] += sxz * rs; m[23] += sxz * st;
-    m[24] += syz; m[25] += syz * r; m[26] += syz * s; m[27] += syz * tt; m[28] += syz * rs; m[29] += syz * rt;
-  }
-#pragma unroll
-  for (int k = 0; k < 30; k++) m[k] = quad_sum(m[k]);
-  const T mxx0 = m[0], mxxs = m[1], mxxt = m[2], mxxst = m[3];
-  const T myy0 = m[4], myyr = m[5], myyt = m[6], myyrt = m[7];
-  const T mzz0 = m[8], mzzr = m[9], mzzs = m[10], mzzrs = m[11];
-  const T txy0 = m[12], txyr = m[13], txys = m[14], txyt = m[15], txyrt = m[16], txyst = m[17];
-  const T txz0 = m[18], txzr = m[19], txzs = m[20], txzt = m[21], txzrs = m[22], txzst = m[23];
-  const T tyz0 = m[24], tyzr = m[25], tyzs = m[26], tyzt = m[27], tyzrs = m[28], tyzrt = m[29];
-#pragma unroll
-  for (int k = 0; k < 8 / kLPE; k++) {
-    const int a = lane * (8 / kLPE) + k;
-    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
-    const T yx = ix * ax * (mxx0 + ay * mxxs + az * mxxt + ay * az * mxxst) +
-                 iy * ay * (txy0 + ax * txyr + az * txyt + ax * az * txyrt) +
-                 iz * az * (txz0 + ax * txzr + ay * txzs + ax * ay * txzrs);
-    const T yy = iy * ay * (myy0 + ax * myyr + az * myyt + ax * az * myyrt) +
-                 ix * ax * (txy0 + ay * txys + az * txyt + ay * az * txyst) +
-                 iz * az * (tyz0 + ax * tyzr + ay * tyzs + ax * ay * tyzrs);
-    const T yz = iz * az * (mzz0 + ax * mzzr + ay * mzzs + ax * ay * mzzrs) +
-                 ix * ax * (txz0 + ay * txzs + az * txzt + ay * az * txzst) +
-                 iy * ay * (tyz0 + ax * tyzr + az * tyzt + ax * az * tyzrt);
-    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
-    const unsigned bf = bcflag ? bcflag[nd] : 0u;
-    T* yp = y + nd * 3;
-    if (!(bf & 1u)) yp[0] += yx;
-    if (!(bf & 2u)) yp[1] += yy;
-    if (!(bf & 4u)) yp[2] += yz;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_apply_K2(Geo g, int c, long long mcount, const T* __restrict__ x,
-                                                     T* __restrict__ y, const typename Vec4<T>::type* __restrict__ pk,
-                                                     const int* __restrict__ off, const uint8_t* __restrict__ bcflag) {
-  const long long gt = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const long long t = gt / kLPE;
-  const int lane = (int)(gt % kLPE);
-  if (t >= mcount) return;
-  long long ijk[3];
-  colour_elem(g, c, t, ijk);
-  const long long e = ijk[0] + (long long)g.res[0] * ijk[1];
-  const T ix = (T)(0.5 / (vcoord(g, 0, ijk[0] + 1) - vcoord(g, 0, ijk[0])));
-  const T iy = (T)(0.5 / (vcoord(g, 1, ijk[1] + 1) - vcoord(g, 1, ijk[1])));
-  const long long sy = g.nn[0];
-  const long long n0 = ijk[0] + sy * ijk[1];
-  T A[2][2], B[2][2];
-#pragma unroll
-  for (int cc = 0; cc < 2; cc++) {
-    const T u0 = x[n0 * 2 + cc], u1 = x[(n0 + 1) * 2 + cc], u2 = x[(n0 + sy) * 2 + cc], u3 = x[(n0 + sy + 1) * 2 + cc];
-    const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
-    A[cc][0] = ix * t1; A[cc][1] = ix * t3;
-    B[cc][0] = iy * t2; B[cc][1] = iy * t3;
-  }
-  T mxx0 = 0, mxxs = 0, myy0 = 0, myyr = 0, txy0 = 0, txyr = 0, txys = 0;
-  const int pb = off[e], pe = off[e + 1];
-  for (int p = pb + lane; p < pe; p += kLPE) {
-    const typename Vec4<T>::type q = pk[p];
-    const T r = q.x, s = q.y, w = q.w;
-    const T uxx = A[0][0] + A[0][1] * s, uxy = B[0][0] + B[0][1] * r;
-    const T uyx = A[1][0] + A[1][1] * s, uyy = B[1][0] + B[1][1] * r;
-    const T w2 = w + w;
-    const T sxx = w2 * uxx, syy = w2 * uyy, sxy = w * (uxy + uyx);
-    mxx0 += sxx; mxxs += sxx * s;
-    myy0 += syy; myyr += syy * r;
-    txy0 += sxy; txyr += sxy * r; txys += sxy * s;
-  }
-  mxx0 = quad_sum(mxx0); mxxs = quad_sum(mxxs); myy0 = quad_sum(myy0); myyr = quad_sum(myyr);
-  txy0 = quad_sum(txy0); txyr = quad_sum(txyr); txys = quad_sum(txys);
-  const int a = lane;
-  const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
-  const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
-  const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
-  const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy;
-  const unsigned bf = bcflag ? bcflag[nd] : 0u;
-  T* yp = y + nd * 2;
-  if (!(bf & 1u)) yp[0] += yx;
-  if (!(bf & 2u)) yp[1] += yy;
-}
-
-template <typename T>
-static void apply_K_impl(const Geo& g, const T* x, T* y, const typename Vec4<T>::type* pk, const int* off,
-                         const uint8_t* bcflag, hipStream_t st) {
-  const int ncol = g.dim == 2 ? 4 : 8;
-  for (int c = 0; c < ncol; c++) {
-    const long long m = colour_count(g, c);
-    if (m <= 0) continue;
-    const unsigned gb = grid_for(m * kLPE);
-    if (g.dim == 3)
-      hipLaunchKernelGGL(k_apply_K3<T>, dim3(gb), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
-    else
-      hipLaunchKernelGGL(k_apply_K2<T>, dim3(gb), dim3(kBlock), 0, st, g, c, m, x, y, pk, off, bcflag);
-  }
-}
-
 void launch_apply_K(const Geo& g, const double* x, double* y, const double4* pk, const int* off,
-                    const uint8_t* bcflag, hipStream_t st) {
-  launch_fill(y, g.N * g.dim, 0.0, st);
-  apply_K_impl<double>(g, x, y, pk, off, bcflag, st);
+                    const uint8_t* bcflag, double* halo, hipStream_t st) {
+  tile_apply_particles(g, x, y, halo, pk, off, bcflag, st);
 }
 void launch_apply_K32(const Geo& g, const float* x, float* y, const float4* pk, const int* off,
-                      const uint8_t* bcflag, hipStream_t st) {
-  launch_fill_f(y, g.N * g.dim, 0.0f, st);
-  apply_K_impl<float>(g, x, y, pk, off, bcflag, st);
+                      const uint8_t* bcflag, float* halo, hipStream_t st) {
+  tile_apply_particles32(g, x, y, halo, pk, off, bcflag, st);
 }
 
 __global__ void k_pack32(const double4* __restrict__ a, float4* __restrict__ b, long long n) {

@@ -9,8 +9,10 @@ void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double
 void launch_moments_coarsen(const Geo& gc, const Geo& gf, const double* Mf, double* Mc, hipStream_t st);
 void launch_d2f(const double* a, float* b, long long n, hipStream_t st);
 void launch_f2d(const float* a, double* b, long long n, hipStream_t st);
+// Moment-form K apply (tiled, gs_tile.hip); Mom is SoA, M[k * E + e]; halo
+// needs tile_halo_size(g) values.
 template <typename T>
-void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, hipStream_t st);
+void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, T* halo, hipStream_t st);
 void launch_mom_diag(const Geo& g, const double* Mom, double* diag, hipStream_t st);
 template <typename T>
 void launch_restrict(const Geo& gc, const Geo& gf, const T* rf, T* bc, const uint8_t* bcflag_c, hipStream_t st);

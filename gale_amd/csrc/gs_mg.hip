@@ -14,6 +14,7 @@
 // Chebyshev-Jacobi, and the coarsest level is solved with a dense inverse.
 #include "gs_internal.h"
 #include "gs_mg.h"
+#include "gs_tile.h"
 
 #include <math.h>
 
@@ -78,7 +79,7 @@ __global__ void k_moments_fine(Geo g, const double4* __restrict__ pk, const int*
     }
   }
 #pragma unroll
-  for (int k = 0; k < NM; k++) M[e * NM + k] = m[k];
+  for (int k = 0; k < NM; k++) M[k * g.E + e] = m[k];
 }
 
 // Parent moments from children: child local r_f maps to parent r_c = (r_f + sg)/2,
@@ -103,7 +104,8 @@ __global__ void k_moments_coarsen(Geo gc, Geo gf, const double* __restrict__ Mf,
       T[d][1][0] = 0.5 * s; T[d][1][1] = 0.5; T[d][1][2] = 0.0;
       T[d][2][0] = 0.25; T[d][2][1] = 0.5 * s; T[d][2][2] = 0.25;  // (r+s)^2/4 = (r^2 + 2 s r + 1)/4
     }
-    const double* m = Mf + e * NM;
+    double m[NM];
+    for (int k = 0; k < NM; k++) m[k] = Mf[k * gf.E + e];
     if (DIM == 2) {
       for (int b = 0; b < 3; b++)
         for (int a = 0; a < 3; a++) {
@@ -126,7 +128,7 @@ __global__ void k_moments_coarsen(Geo gc, Geo gf, const double* __restrict__ Mf,
     }
   }
 #pragma unroll
-  for (int k = 0; k < NM; k++) Mc[E * NM + k] = acc[k];
+  for (int k = 0; k < NM; k++) Mc[k * gc.E + E] = acc[k];
 }
 
 void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double* M, hipStream_t st) {
@@ -154,185 +156,22 @@ void launch_f2d(const float* a, double* b, long long n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL((k_convert<float, double>), dim3(gridn(n) < 8192 ? gridn(n) : 8192), dim3(kBlock), 0, st, a, b, n);
 }
 
-// ---------------------------------------------------------------------------
-// moment-based K apply (3D): gradient coefficients from the nodal values, the
-// 30 stress moments as contractions with the 27 particle moments, then the
-// same nodal read-back as the particle kernel.
-// ---------------------------------------------------------------------------
-#define MI(a, b, c) ((a) + 3 * (b) + 9 * (c))
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mom_apply3(Geo g, int c, long long mcount, const T* __restrict__ x,
-                                                       T* __restrict__ y, const T* __restrict__ Mom,
-                                                       const uint8_t* __restrict__ bcflag) {
-  const long long t = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= mcount) return;
-  long long ijk[3];
-  colour_elem_mg(g, c, t, ijk);
-  const long long e = ijk[0] + (long long)g.res[0] * (ijk[1] + (long long)g.res[1] * ijk[2]);
-  const T ix = (T)(0.25 / (vcoord_mg(g, 0, ijk[0] + 1) - vcoord_mg(g, 0, ijk[0])));
-  const T iy = (T)(0.25 / (vcoord_mg(g, 1, ijk[1] + 1) - vcoord_mg(g, 1, ijk[1])));
-  const T iz = (T)(0.25 / (vcoord_mg(g, 2, ijk[2] + 1) - vcoord_mg(g, 2, ijk[2])));
-  const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
-  const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
-  T A[3][4], B[3][4], C[3][4];
-#pragma unroll
-  for (int cc = 0; cc < 3; cc++) {
-    T v[8];
-#pragma unroll
-    for (int a = 0; a < 8; a++) v[a] = x[(n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz) * 3 + cc];
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const int bit = 1 << d;
-#pragma unroll
-      for (int a = 0; a < 8; a++)
-        if (!(a & bit)) {
-          const T lo = v[a], hi = v[a | bit];
-          v[a] = lo + hi;
-          v[a | bit] = hi - lo;
-        }
-    }
-    A[cc][0] = ix * v[1]; A[cc][1] = ix * v[3]; A[cc][2] = ix * v[5]; A[cc][3] = ix * v[7];
-    B[cc][0] = iy * v[2]; B[cc][1] = iy * v[3]; B[cc][2] = iy * v[6]; B[cc][3] = iy * v[7];
-    C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
-  }
-  T M[27];
-#pragma unroll
-  for (int k = 0; k < 27; k++) M[k] = Mom[e * 27 + k];
-  // exponents of the trial monomials: X = {1,s,t,st}, Y = {1,r,t,rt}, Z = {1,r,s,rs}
-  constexpr int XE[4][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}};
-  constexpr int YE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}};
-  constexpr int ZE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
-  constexpr int PXY[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}};
-  constexpr int PXZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}};
-  constexpr int PYZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}};
-  T m[30];
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    T vx = 0, vy = 0, vz = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      vx += A[0][k] * M[MI(XE[k][0] + XE[j][0], XE[k][1] + XE[j][1], XE[k][2] + XE[j][2])];
-      vy += B[1][k] * M[MI(YE[k][0] + YE[j][0], YE[k][1] + YE[j][1], YE[k][2] + YE[j][2])];
-      vz += C[2][k] * M[MI(ZE[k][0] + ZE[j][0], ZE[k][1] + ZE[j][1], ZE[k][2] + ZE[j][2])];
-    }
-    m[j] = (T)2 * vx;
-    m[4 + j] = (T)2 * vy;
-    m[8 + j] = (T)2 * vz;
-  }
-#pragma unroll
-  for (int j = 0; j < 6; j++) {
-    T vxy = 0, vxz = 0, vyz = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      vxy += B[0][k] * M[MI(YE[k][0] + PXY[j][0], YE[k][1] + PXY[j][1], YE[k][2] + PXY[j][2])] +
-             A[1][k] * M[MI(XE[k][0] + PXY[j][0], XE[k][1] + PXY[j][1], XE[k][2] + PXY[j][2])];
-      vxz += C[0][k] * M[MI(ZE[k][0] + PXZ[j][0], ZE[k][1] + PXZ[j][1], ZE[k][2] + PXZ[j][2])] +
-             A[2][k] * M[MI(XE[k][0] + PXZ[j][0], XE[k][1] + PXZ[j][1], XE[k][2] + PXZ[j][2])];
-      vyz += C[1][k] * M[MI(ZE[k][0] + PYZ[j][0], ZE[k][1] + PYZ[j][1], ZE[k][2] + PYZ[j][2])] +
-             B[2][k] * M[MI(YE[k][0] + PYZ[j][0], YE[k][1] + PYZ[j][1], YE[k][2] + PYZ[j][2])];
-    }
-    m[12 + j] = vxy;
-    m[18 + j] = vxz;
-    m[24 + j] = vyz;
-  }
-  const T mxx0 = m[0], mxxs = m[1], mxxt = m[2], mxxst = m[3];
-  const T myy0 = m[4], myyr = m[5], myyt = m[6], myyrt = m[7];
-  const T mzz0 = m[8], mzzr = m[9], mzzs = m[10], mzzrs = m[11];
-  const T txy0 = m[12], txyr = m[13], txys = m[14], txyt = m[15], txyrt = m[16], txyst = m[17];
-  const T txz0 = m[18], txzr = m[19], txzs = m[20], txzt = m[21], txzrs = m[22], txzst = m[23];
-  const T tyz0 = m[24], tyzr = m[25], tyzs = m[26], tyzt = m[27], tyzrs = m[28], tyzrt = m[29];
-#pragma unroll
-  for (int a = 0; a < 8; a++) {
-    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
-    const T yx = ix * ax * (mxx0 + ay * mxxs + az * mxxt + ay * az * mxxst) +
-                 iy * ay * (txy0 + ax * txyr + az * txyt + ax * az * txyrt) +
-                 iz * az * (txz0 + ax * txzr + ay * txzs + ax * ay * txzrs);
-    const T yy = iy * ay * (myy0 + ax * myyr + az * myyt + ax * az * myyrt) +
-                 ix * ax * (txy0 + ay * txys + az * txyt + ay * az * txyst) +
-                 iz * az * (tyz0 + ax * tyzr + ay * tyzs + ax * ay * tyzrs);
-    const T yz = iz * az * (mzz0 + ax * mzzr + ay * mzzs + ax * ay * mzzrs) +
-                 ix * ax * (txz0 + ay * txzs + az * txzt + ay * az * txzst) +
-                 iy * ay * (tyz0 + ax * tyzr + az * tyzt + ax * az * tyzrt);
-    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy + ((a >> 2) & 1) * sz;
-    const unsigned bf = bcflag ? bcflag[nd] : 0u;
-    T* yp = y + nd * 3;
-    if (!(bf & 1u)) yp[0] += yx;
-    if (!(bf & 2u)) yp[1] += yy;
-    if (!(bf & 4u)) yp[2] += yz;
-  }
-}
-
-// 2D: moments M[a + 3b]; trial X = {1,s}, Y = {1,r}; test sets {1,s}, {1,r}, {1,r,s}.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_mom_apply2(Geo g, int c, long long mcount, const T* __restrict__ x,
-                                                       T* __restrict__ y, const T* __restrict__ Mom,
-                                                       const uint8_t* __restrict__ bcflag) {
-  const long long t = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= mcount) return;
-  long long ijk[3];
-  colour_elem_mg(g, c, t, ijk);
-  const long long e = ijk[0] + (long long)g.res[0] * ijk[1];
-  const T ix = (T)(0.5 / (vcoord_mg(g, 0, ijk[0] + 1) - vcoord_mg(g, 0, ijk[0])));
-  const T iy = (T)(0.5 / (vcoord_mg(g, 1, ijk[1] + 1) - vcoord_mg(g, 1, ijk[1])));
-  const long long sy = g.nn[0];
-  const long long n0 = ijk[0] + sy * ijk[1];
-  T A[2][2], B[2][2];
-#pragma unroll
-  for (int cc = 0; cc < 2; cc++) {
-    const T u0 = x[n0 * 2 + cc], u1 = x[(n0 + 1) * 2 + cc], u2 = x[(n0 + sy) * 2 + cc], u3 = x[(n0 + sy + 1) * 2 + cc];
-    const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
-    A[cc][0] = ix * t1; A[cc][1] = ix * t3;
-    B[cc][0] = iy * t2; B[cc][1] = iy * t3;
-  }
-  T M[9];
-#pragma unroll
-  for (int k = 0; k < 9; k++) M[k] = Mom[e * 9 + k];
-  // uxx = A00 + A01 s ; uyy = B10 + B11 r ; uxy = B00 + B01 r ; uyx = A10 + A11 s
-  const T mxx0 = 2 * (A[0][0] * M[0] + A[0][1] * M[3]);
-  const T mxxs = 2 * (A[0][0] * M[3] + A[0][1] * M[6]);
-  const T myy0 = 2 * (B[1][0] * M[0] + B[1][1] * M[1]);
-  const T myyr = 2 * (B[1][0] * M[1] + B[1][1] * M[2]);
-  const T txy0 = B[0][0] * M[0] + B[0][1] * M[1] + A[1][0] * M[0] + A[1][1] * M[3];
-  const T txyr = B[0][0] * M[1] + B[0][1] * M[2] + A[1][0] * M[1] + A[1][1] * M[4];
-  const T txys = B[0][0] * M[3] + B[0][1] * M[4] + A[1][0] * M[3] + A[1][1] * M[6];
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
-    const T yx = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
-    const T yy = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
-    const long long nd = n0 + (a & 1) + ((a >> 1) & 1) * sy;
-    const unsigned bf = bcflag ? bcflag[nd] : 0u;
-    T* yp = y + nd * 2;
-    if (!(bf & 1u)) yp[0] += yx;
-    if (!(bf & 2u)) yp[1] += yy;
-  }
-}
-
 template <typename T>
 __global__ void k_fill_mg(T* __restrict__ y, long long n, T v) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     y[i] = v;
 }
 
-template <typename T>
-void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, hipStream_t st) {
-  const long long n = g.N * g.dim;
-  unsigned gb = gridn(n);
-  if (gb > 8192) gb = 8192;
-  hipLaunchKernelGGL(k_fill_mg<T>, dim3(gb), dim3(kBlock), 0, st, y, n, (T)0);
-  const int ncol = g.dim == 2 ? 4 : 8;
-  for (int c = 0; c < ncol; c++) {
-    const long long m = colour_count(g, c);
-    if (m <= 0) continue;
-    if (g.dim == 3)
-      hipLaunchKernelGGL(k_mom_apply3<T>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, x, y, Mom, bcflag);
-    else
-      hipLaunchKernelGGL(k_mom_apply2<T>, dim3(gridn(m)), dim3(kBlock), 0, st, g, c, m, x, y, Mom, bcflag);
-  }
+template <>
+void launch_mom_apply<double>(const Geo& g, const double* x, double* y, const double* Mom, const uint8_t* bcflag,
+                              double* halo, hipStream_t st) {
+  tile_apply_moments(g, x, y, halo, Mom, bcflag, st);
 }
-template void launch_mom_apply<double>(const Geo&, const double*, double*, const double*, const uint8_t*, hipStream_t);
-template void launch_mom_apply<float>(const Geo&, const float*, float*, const float*, const uint8_t*, hipStream_t);
+template <>
+void launch_mom_apply<float>(const Geo& g, const float* x, float* y, const float* Mom, const uint8_t* bcflag,
+                             float* halo, hipStream_t st) {
+  tile_apply_moments32(g, x, y, halo, Mom, bcflag, st);
+}
 
 // diag(K) from moments: node a, component d: sum_p omega (|grad N_a|^2 + (dN_a/dx_d)^2).
 template <int DIM>
@@ -347,7 +186,8 @@ __global__ void k_mom_diag(Geo g, int c, long long mcount, const double* __restr
   double inv[3];
   for (int d = 0; d < DIM; d++) inv[d] = 2.0 / (vcoord_mg(g, d, ijk[d] + 1) - vcoord_mg(g, d, ijk[d]));
   const double cN = DIM == 2 ? 0.25 : 0.125;
-  const double* M = Mom + e * NM;
+  double M[NM];
+  for (int k = 0; k < NM; k++) M[k] = Mom[k * g.E + e];
   const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
   const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
   for (int a = 0; a < NEN; a++) {

@@ -1,0 +1,483 @@
+// gs_tile.hip -- tiled matrix-free K apply (particle form and moment form).
+//
+// One workgroup owns a tile of TX x TY (x TZ) elements.  It stages the
+// tile's nodal velocities in LDS with coalesced row loads, evaluates each
+// element (one thread per element), accumulates the element forces into an
+// LDS node array in 2^dim parity phases (deterministic, no atomics), and then
+// stores every node it owns straight to y.  Nodes on a tile's upper faces are
+// owned by the next tile: their partial sums go to a per-tile halo buffer and
+// a second, node-parallel kernel adds them in a fixed order.  So one apply is
+// two launches, every y entry is written once (plus one read-modify-write on
+// tile faces), and the result is bitwise reproducible.
+#include "gs_internal.h"
+#include "gs_tile.h"
+
+#include <math.h>
+
+namespace gs {
+
+namespace {
+
+__device__ __forceinline__ long long xcd_remap(long long b, long long nb) {
+  const long long q = nb / 8, rem = nb % 8;
+  const long long x = b % 8, i = b / 8;
+  return x * q + (x < rem ? x : rem) + i;
+}
+
+__device__ __forceinline__ double vcoord_t(const Geo& g, int d, long long idx) {
+#pragma clang fp contract(off)
+  if (idx <= 0) return g.lo[d];
+  if (idx >= g.res[d]) return g.hi[d];
+  return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
+}
+
+template <typename T>
+struct V4;
+template <>
+struct V4<double> {
+  using type = double4;
+};
+template <>
+struct V4<float> {
+  using type = float4;
+};
+
+#define MI(a, b, c) ((a) + 3 * (b) + 9 * (c))
+
+// ---- 3D element kernels: gradient coefficients from the 8 nodal vectors ----
+template <typename T>
+__device__ __forceinline__ void grad_coeffs3(const T (&u)[8][3], T ix, T iy, T iz, T (&A)[3][4], T (&B)[3][4],
+                                             T (&C)[3][4]) {
+#pragma unroll
+  for (int cc = 0; cc < 3; cc++) {
+    T v[8];
+#pragma unroll
+    for (int a = 0; a < 8; a++) v[a] = u[a][cc];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      const int bit = 1 << d;
+#pragma unroll
+      for (int a = 0; a < 8; a++)
+        if (!(a & bit)) {
+          const T lo = v[a], hi = v[a | bit];
+          v[a] = lo + hi;
+          v[a | bit] = hi - lo;
+        }
+    }
+    A[cc][0] = ix * v[1]; A[cc][1] = ix * v[3]; A[cc][2] = ix * v[5]; A[cc][3] = ix * v[7];
+    B[cc][0] = iy * v[2]; B[cc][1] = iy * v[3]; B[cc][2] = iy * v[6]; B[cc][3] = iy * v[7];
+    C[cc][0] = iz * v[4]; C[cc][1] = iz * v[5]; C[cc][2] = iz * v[6]; C[cc][3] = iz * v[7];
+  }
+}
+
+// Nodal forces from the 30 stress moments (see gs_kernels.hip for the layout).
+template <typename T>
+__device__ __forceinline__ void epilogue3(const T (&m)[30], T ix, T iy, T iz, T (&ye)[8][3]) {
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
+    ye[a][0] = ix * ax * (m[0] + ay * m[1] + az * m[2] + ay * az * m[3]) +
+               iy * ay * (m[12] + ax * m[13] + az * m[15] + ax * az * m[16]) +
+               iz * az * (m[18] + ax * m[19] + ay * m[20] + ax * ay * m[22]);
+    ye[a][1] = iy * ay * (m[4] + ax * m[5] + az * m[6] + ax * az * m[7]) +
+               ix * ax * (m[12] + ay * m[14] + az * m[15] + ay * az * m[17]) +
+               iz * az * (m[24] + ax * m[25] + ay * m[26] + ax * ay * m[28]);
+    ye[a][2] = iz * az * (m[8] + ax * m[9] + ay * m[10] + ax * ay * m[11]) +
+               ix * ax * (m[18] + ay * m[20] + az * m[21] + ay * az * m[23]) +
+               iy * ay * (m[24] + ax * m[25] + az * m[27] + ax * az * m[29]);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void element_particles3(const T (&A)[3][4], const T (&B)[3][4], const T (&C)[3][4],
+                                                   const typename V4<T>::type* __restrict__ pk, int pb, int pe,
+                                                   T (&m)[30]) {
+#pragma unroll
+  for (int k = 0; k < 30; k++) m[k] = 0;
+  for (int p = pb; p < pe; p++) {
+    const typename V4<T>::type q = pk[p];
+    const T r = q.x, s = q.y, tt = q.z, w = q.w;
+    const T st = s * tt, rt = r * tt, rs = r * s;
+    const T uxx = A[0][0] + A[0][1] * s + A[0][2] * tt + A[0][3] * st;
+    const T uxy = B[0][0] + B[0][1] * r + B[0][2] * tt + B[0][3] * rt;
+    const T uxz = C[0][0] + C[0][1] * r + C[0][2] * s + C[0][3] * rs;
+    const T uyx = A[1][0] + A[1][1] * s + A[1][2] * tt + A[1][3] * st;
+    const T uyy = B[1][0] + B[1][1] * r + B[1][2] * tt + B[1][3] * rt;
+    const T uyz = C[1][0] + C[1][1] * r + C[1][2] * s + C[1][3] * rs;
+    const T uzx = A[2][0] + A[2][1] * s + A[2][2] * tt + A[2][3] * st;
+    const T uzy = B[2][0] + B[2][1] * r + B[2][2] * tt + B[2][3] * rt;
+    const T uzz = C[2][0] + C[2][1] * r + C[2][2] * s + C[2][3] * rs;
+    const T w2 = w + w;
+    const T sxx = w2 * uxx, syy = w2 * uyy, szz = w2 * uzz;
+    const T sxy = w * (uxy + uyx), sxz = w * (uxz + uzx), syz = w * (uyz + uzy);
+    m[0] += sxx; m[1] += sxx * s; m[2] += sxx * tt; m[3] += sxx * st;
+    m[4] += syy; m[5] += syy * r; m[6] += syy * tt; m[7] += syy * rt;
+    m[8] += szz; m[9] += szz * r; m[10] += szz * s; m[11] += szz * rs;
+    m[12] += sxy; m[13] += sxy * r; m[14] += sxy * s; m[15] += sxy * tt; m[16] += sxy * rt; m[17] += sxy * st;
+    m[18] += sxz; m[19] += sxz * r; m[20] += sxz * s; m[21] += sxz * tt; m[22] += sxz * rs; m[23] += sxz * st;
+    m[24] += syz; m[25] += syz * r; m[26] += syz * s; m[27] += syz * tt; m[28] += syz * rs; m[29] += syz * rt;
+  }
+}
+
+// Stress moments as contractions of the gradient coefficients with the 27
+// particle moments (moments stored SoA: M[k * E + e]).
+template <typename T>
+__device__ __forceinline__ void element_moments3(const T (&A)[3][4], const T (&B)[3][4], const T (&C)[3][4],
+                                                 const T* __restrict__ Mom, long long E, long long e, T (&m)[30]) {
+  T M[27];
+#pragma unroll
+  for (int k = 0; k < 27; k++) M[k] = Mom[k * E + e];
+  constexpr int XE[4][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}};
+  constexpr int YE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}};
+  constexpr int ZE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
+  constexpr int PXY[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {0, 1, 1}};
+  constexpr int PXZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {0, 1, 1}};
+  constexpr int PYZ[6][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}};
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    T vx = 0, vy = 0, vz = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      vx += A[0][k] * M[MI(XE[k][0] + XE[j][0], XE[k][1] + XE[j][1], XE[k][2] + XE[j][2])];
+      vy += B[1][k] * M[MI(YE[k][0] + YE[j][0], YE[k][1] + YE[j][1], YE[k][2] + YE[j][2])];
+      vz += C[2][k] * M[MI(ZE[k][0] + ZE[j][0], ZE[k][1] + ZE[j][1], ZE[k][2] + ZE[j][2])];
+    }
+    m[j] = (T)2 * vx;
+    m[4 + j] = (T)2 * vy;
+    m[8 + j] = (T)2 * vz;
+  }
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    T vxy = 0, vxz = 0, vyz = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      vxy += B[0][k] * M[MI(YE[k][0] + PXY[j][0], YE[k][1] + PXY[j][1], YE[k][2] + PXY[j][2])] +
+             A[1][k] * M[MI(XE[k][0] + PXY[j][0], XE[k][1] + PXY[j][1], XE[k][2] + PXY[j][2])];
+      vxz += C[0][k] * M[MI(ZE[k][0] + PXZ[j][0], ZE[k][1] + PXZ[j][1], ZE[k][2] + PXZ[j][2])] +
+             A[2][k] * M[MI(XE[k][0] + PXZ[j][0], XE[k][1] + PXZ[j][1], XE[k][2] + PXZ[j][2])];
+      vyz += C[1][k] * M[MI(ZE[k][0] + PYZ[j][0], ZE[k][1] + PYZ[j][1], ZE[k][2] + PYZ[j][2])] +
+             B[2][k] * M[MI(YE[k][0] + PYZ[j][0], YE[k][1] + PYZ[j][1], YE[k][2] + PYZ[j][2])];
+    }
+    m[12 + j] = vxy;
+    m[18 + j] = vxz;
+    m[24 + j] = vyz;
+  }
+}
+
+// ---- 2D element kernels ----
+template <typename T>
+__device__ __forceinline__ void element2(const T (&u)[4][2], T ix, T iy, int form, const T* __restrict__ Mom,
+                                         long long E, long long e, const typename V4<T>::type* __restrict__ pk,
+                                         int pb, int pe, T (&ye)[4][2]) {
+  T A[2][2], B[2][2];
+#pragma unroll
+  for (int cc = 0; cc < 2; cc++) {
+    const T u0 = u[0][cc], u1 = u[1][cc], u2 = u[2][cc], u3 = u[3][cc];
+    const T t1 = (u1 - u0) + (u3 - u2), t2 = (u2 + u3) - (u0 + u1), t3 = (u3 - u2) - (u1 - u0);
+    A[cc][0] = ix * t1; A[cc][1] = ix * t3;
+    B[cc][0] = iy * t2; B[cc][1] = iy * t3;
+  }
+  T mxx0 = 0, mxxs = 0, myy0 = 0, myyr = 0, txy0 = 0, txyr = 0, txys = 0;
+  if (form == 0) {
+    T M[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) M[k] = Mom[k * E + e];
+    mxx0 = 2 * (A[0][0] * M[0] + A[0][1] * M[3]);
+    mxxs = 2 * (A[0][0] * M[3] + A[0][1] * M[6]);
+    myy0 = 2 * (B[1][0] * M[0] + B[1][1] * M[1]);
+    myyr = 2 * (B[1][0] * M[1] + B[1][1] * M[2]);
+    txy0 = B[0][0] * M[0] + B[0][1] * M[1] + A[1][0] * M[0] + A[1][1] * M[3];
+    txyr = B[0][0] * M[1] + B[0][1] * M[2] + A[1][0] * M[1] + A[1][1] * M[4];
+    txys = B[0][0] * M[3] + B[0][1] * M[4] + A[1][0] * M[3] + A[1][1] * M[6];
+  } else {
+    for (int p = pb; p < pe; p++) {
+      const typename V4<T>::type q = pk[p];
+      const T r = q.x, s = q.y, w = q.w;
+      const T uxx = A[0][0] + A[0][1] * s, uxy = B[0][0] + B[0][1] * r;
+      const T uyx = A[1][0] + A[1][1] * s, uyy = B[1][0] + B[1][1] * r;
+      const T w2 = w + w;
+      const T sxx = w2 * uxx, syy = w2 * uyy, sxy = w * (uxy + uyx);
+      mxx0 += sxx; mxxs += sxx * s;
+      myy0 += syy; myyr += syy * r;
+      txy0 += sxy; txyr += sxy * r; txys += sxy * s;
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1;
+    ye[a][0] = ix * ax * (mxx0 + ay * mxxs) + iy * ay * (txy0 + ax * txyr);
+    ye[a][1] = iy * ay * (myy0 + ax * myyr) + ix * ax * (txy0 + ay * txys);
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// 3D tile kernel
+// ---------------------------------------------------------------------------
+template <typename T, int FORM>
+__global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg, const T* __restrict__ x,
+                                                              T* __restrict__ y, T* __restrict__ halo,
+                                                              const T* __restrict__ Mom,
+                                                              const typename V4<T>::type* __restrict__ pk,
+                                                              const int* __restrict__ off,
+                                                              const uint8_t* __restrict__ bcflag) {
+  constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
+  constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
+  __shared__ T sx[NB * 3];
+  __shared__ T sy[NB * 3];
+  const long long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(tile % tg.n[0]), ty = (int)((tile / tg.n[0]) % tg.n[1]), tz = (int)(tile / ((long long)tg.n[0] * tg.n[1]));
+  const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
+  const int nbx = min(TX, g.res[0] - x0) + 1, nby = min(TY, g.res[1] - y0) + 1, nbz = min(TZ, g.res[2] - z0) + 1;
+  const long long sy_ = g.nn[0], sz_ = g.nn[0] * g.nn[1];
+  for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
+    const int c = q % 3, nd = q / 3;
+    const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
+    T v = 0;
+    if (lx < nbx && ly < nby && lz < nbz) v = x[((x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz)) * 3 + c];
+    sx[q] = v;
+    sy[q] = 0;
+  }
+  __syncthreads();
+  const int ex = threadIdx.x % TX, ey = (threadIdx.x / TX) % TY, ez = threadIdx.x / (TX * TY);
+  const bool active = (x0 + ex < g.res[0]) && (y0 + ey < g.res[1]) && (z0 + ez < g.res[2]);
+  T ye[8][3];
+  if (active) {
+    const long long gi = x0 + ex, gj = y0 + ey, gk = z0 + ez;
+    const T ix = (T)(0.25 / (vcoord_t(g, 0, gi + 1) - vcoord_t(g, 0, gi)));
+    const T iy = (T)(0.25 / (vcoord_t(g, 1, gj + 1) - vcoord_t(g, 1, gj)));
+    const T iz = (T)(0.25 / (vcoord_t(g, 2, gk + 1) - vcoord_t(g, 2, gk)));
+    T u[8][3];
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
+#pragma unroll
+      for (int c = 0; c < 3; c++) u[a][c] = sx[ln * 3 + c];
+    }
+    T A[3][4], B[3][4], C[3][4];
+    grad_coeffs3(u, ix, iy, iz, A, B, C);
+    const long long e = gi + (long long)g.res[0] * (gj + (long long)g.res[1] * gk);
+    T m[30];
+    if (FORM == 0)
+      element_moments3(A, B, C, Mom, g.E, e, m);
+    else
+      element_particles3(A, B, C, pk, off[e], off[e + 1], m);
+    epilogue3(m, ix, iy, iz, ye);
+  }
+  const int par = (ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2);
+#pragma unroll 1
+  for (int ph = 0; ph < 8; ph++) {
+    if (active && par == ph) {
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
+#pragma unroll
+        for (int c = 0; c < 3; c++) sy[ln * 3 + c] += ye[a][c];
+      }
+    }
+    __syncthreads();
+  }
+  const bool lastx = tx == tg.n[0] - 1, lasty = ty == tg.n[1] - 1, lastz = tz == tg.n[2] - 1;
+  for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
+    const int c = q % 3, nd = q / 3;
+    const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
+    if (lx >= nbx || ly >= nby || lz >= nbz) continue;
+    const bool hal = (lx == nbx - 1 && !lastx) || (ly == nby - 1 && !lasty) || (lz == nbz - 1 && !lastz);
+    if (hal) {
+      halo[(tile * NB + nd) * 3 + c] = sy[q];
+    } else {
+      const long long gn = (x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz);
+      const unsigned bf = bcflag ? bcflag[gn] : 0u;
+      y[gn * 3 + c] = ((bf >> c) & 1u) ? (T)0 : sy[q];
+    }
+  }
+}
+
+// Adds the halo partial sums of up to seven lower neighbour tiles.
+template <typename T>
+__global__ void k_tile_halo3(Geo g, TileGrid tg, T* __restrict__ y, const T* __restrict__ halo,
+                             const uint8_t* __restrict__ bcflag) {
+  constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
+  constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const int i = (int)(n % g.nn[0]), j = (int)((n / g.nn[0]) % g.nn[1]), k = (int)(n / (g.nn[0] * g.nn[1]));
+  const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1), tz = min(k / TZ, tg.n[2] - 1);
+  const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0), onz = (k == tz * TZ && tz > 0);
+  if (!(onx | ony | onz)) return;
+  T acc[3] = {0, 0, 0};
+  for (int az = 0; az <= onz; az++)
+    for (int ay = 0; ay <= ony; ay++)
+      for (int ax = 0; ax <= onx; ax++) {
+        if (!(ax | ay | az)) continue;
+        const int ux = tx - ax, uy = ty - ay, uz = tz - az;
+        const long long t2 = ux + (long long)tg.n[0] * (uy + (long long)tg.n[1] * uz);
+        const int ln = (i - ux * TX) + NBX * ((j - uy * TY) + NBY * (k - uz * TZ));
+        const T* h = halo + (t2 * NB + ln) * 3;
+        acc[0] += h[0];
+        acc[1] += h[1];
+        acc[2] += h[2];
+      }
+  const unsigned bf = bcflag ? bcflag[n] : 0u;
+  for (int c = 0; c < 3; c++)
+    if (!((bf >> c) & 1u)) y[n * 3 + c] += acc[c];
+}
+
+// ---------------------------------------------------------------------------
+// 2D tile kernel
+// ---------------------------------------------------------------------------
+template <typename T, int FORM>
+__global__ __launch_bounds__(kTileThreads) void k_tile_apply2(Geo g, TileGrid tg, const T* __restrict__ x,
+                                                              T* __restrict__ y, T* __restrict__ halo,
+                                                              const T* __restrict__ Mom,
+                                                              const typename V4<T>::type* __restrict__ pk,
+                                                              const int* __restrict__ off,
+                                                              const uint8_t* __restrict__ bcflag) {
+  constexpr int TX = kTile2X, TY = kTile2Y;
+  constexpr int NBX = TX + 1, NBY = TY + 1, NB = NBX * NBY;
+  __shared__ T sx[NB * 2];
+  __shared__ T sy[NB * 2];
+  const long long tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int tx = (int)(tile % tg.n[0]), ty = (int)(tile / tg.n[0]);
+  const int x0 = tx * TX, y0 = ty * TY;
+  const int nbx = min(TX, g.res[0] - x0) + 1, nby = min(TY, g.res[1] - y0) + 1;
+  const long long sy_ = g.nn[0];
+  for (int q = threadIdx.x; q < NB * 2; q += kTileThreads) {
+    const int c = q % 2, nd = q / 2;
+    const int lx = nd % NBX, ly = nd / NBX;
+    T v = 0;
+    if (lx < nbx && ly < nby) v = x[((x0 + lx) + sy_ * (y0 + ly)) * 2 + c];
+    sx[q] = v;
+    sy[q] = 0;
+  }
+  __syncthreads();
+  const int ex = threadIdx.x % TX, ey = threadIdx.x / TX;
+  const bool active = (x0 + ex < g.res[0]) && (y0 + ey < g.res[1]);
+  T ye[4][2];
+  if (active) {
+    const long long gi = x0 + ex, gj = y0 + ey;
+    const T ix = (T)(0.5 / (vcoord_t(g, 0, gi + 1) - vcoord_t(g, 0, gi)));
+    const T iy = (T)(0.5 / (vcoord_t(g, 1, gj + 1) - vcoord_t(g, 1, gj)));
+    T u[4][2];
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+      const int ln = (ex + (a & 1)) + NBX * (ey + ((a >> 1) & 1));
+      u[a][0] = sx[ln * 2];
+      u[a][1] = sx[ln * 2 + 1];
+    }
+    const long long e = gi + (long long)g.res[0] * gj;
+    element2(u, ix, iy, FORM, Mom, g.E, e, pk, FORM ? off[e] : 0, FORM ? off[e + 1] : 0, ye);
+  }
+  const int par = (ex & 1) | ((ey & 1) << 1);
+#pragma unroll 1
+  for (int ph = 0; ph < 4; ph++) {
+    if (active && par == ph) {
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        const int ln = (ex + (a & 1)) + NBX * (ey + ((a >> 1) & 1));
+        sy[ln * 2] += ye[a][0];
+        sy[ln * 2 + 1] += ye[a][1];
+      }
+    }
+    __syncthreads();
+  }
+  const bool lastx = tx == tg.n[0] - 1, lasty = ty == tg.n[1] - 1;
+  for (int q = threadIdx.x; q < NB * 2; q += kTileThreads) {
+    const int c = q % 2, nd = q / 2;
+    const int lx = nd % NBX, ly = nd / NBX;
+    if (lx >= nbx || ly >= nby) continue;
+    const bool hal = (lx == nbx - 1 && !lastx) || (ly == nby - 1 && !lasty);
+    if (hal) {
+      halo[(tile * NB + nd) * 2 + c] = sy[q];
+    } else {
+      const long long gn = (x0 + lx) + sy_ * (y0 + ly);
+      const unsigned bf = bcflag ? bcflag[gn] : 0u;
+      y[gn * 2 + c] = ((bf >> c) & 1u) ? (T)0 : sy[q];
+    }
+  }
+}
+
+template <typename T>
+__global__ void k_tile_halo2(Geo g, TileGrid tg, T* __restrict__ y, const T* __restrict__ halo,
+                             const uint8_t* __restrict__ bcflag) {
+  constexpr int TX = kTile2X, TY = kTile2Y;
+  constexpr int NBX = TX + 1, NBY = TY + 1, NB = NBX * NBY;
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= g.N) return;
+  const int i = (int)(n % g.nn[0]), j = (int)(n / g.nn[0]);
+  const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1);
+  const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0);
+  if (!(onx | ony)) return;
+  T acc[2] = {0, 0};
+  for (int ay = 0; ay <= ony; ay++)
+    for (int ax = 0; ax <= onx; ax++) {
+      if (!(ax | ay)) continue;
+      const int ux = tx - ax, uy = ty - ay;
+      const long long t2 = ux + (long long)tg.n[0] * uy;
+      const int ln = (i - ux * TX) + NBX * (j - uy * TY);
+      acc[0] += halo[(t2 * NB + ln) * 2];
+      acc[1] += halo[(t2 * NB + ln) * 2 + 1];
+    }
+  const unsigned bf = bcflag ? bcflag[n] : 0u;
+  for (int c = 0; c < 2; c++)
+    if (!((bf >> c) & 1u)) y[n * 2 + c] += acc[c];
+}
+
+// ---------------------------------------------------------------------------
+TileGrid tile_grid(const Geo& g) {
+  TileGrid tg{};
+  if (g.dim == 3) {
+    tg.n[0] = (g.res[0] + kTile3X - 1) / kTile3X;
+    tg.n[1] = (g.res[1] + kTile3Y - 1) / kTile3Y;
+    tg.n[2] = (g.res[2] + kTile3Z - 1) / kTile3Z;
+    tg.nb = (kTile3X + 1) * (kTile3Y + 1) * (kTile3Z + 1);
+  } else {
+    tg.n[0] = (g.res[0] + kTile2X - 1) / kTile2X;
+    tg.n[1] = (g.res[1] + kTile2Y - 1) / kTile2Y;
+    tg.n[2] = 1;
+    tg.nb = (kTile2X + 1) * (kTile2Y + 1);
+  }
+  tg.ntiles = (long long)tg.n[0] * tg.n[1] * tg.n[2];
+  return tg;
+}
+
+long long tile_halo_size(const Geo& g) {
+  const TileGrid tg = tile_grid(g);
+  return tg.ntiles * tg.nb * g.dim;
+}
+
+template <typename T, int FORM>
+static void tile_apply(const Geo& g, const T* x, T* y, T* halo, const T* Mom, const typename V4<T>::type* pk,
+                       const int* off, const uint8_t* bcflag, hipStream_t st) {
+  const TileGrid tg = tile_grid(g);
+  const unsigned nbn = (unsigned)((g.N + kBlock - 1) / kBlock);
+  if (g.dim == 3) {
+    hipLaunchKernelGGL((k_tile_apply3<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x, y,
+                       halo, Mom, pk, off, bcflag);
+    hipLaunchKernelGGL(k_tile_halo3<T>, dim3(nbn), dim3(kBlock), 0, st, g, tg, y, halo, bcflag);
+  } else {
+    hipLaunchKernelGGL((k_tile_apply2<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x, y,
+                       halo, Mom, pk, off, bcflag);
+    hipLaunchKernelGGL(k_tile_halo2<T>, dim3(nbn), dim3(kBlock), 0, st, g, tg, y, halo, bcflag);
+  }
+}
+
+void tile_apply_particles(const Geo& g, const double* x, double* y, double* halo, const double4* pk, const int* off,
+                          const uint8_t* bcflag, hipStream_t st) {
+  tile_apply<double, 1>(g, x, y, halo, nullptr, pk, off, bcflag, st);
+}
+void tile_apply_particles32(const Geo& g, const float* x, float* y, float* halo, const float4* pk, const int* off,
+                            const uint8_t* bcflag, hipStream_t st) {
+  tile_apply<float, 1>(g, x, y, halo, nullptr, pk, off, bcflag, st);
+}
+void tile_apply_moments(const Geo& g, const double* x, double* y, double* halo, const double* Mom,
+                        const uint8_t* bcflag, hipStream_t st) {
+  tile_apply<double, 0>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, st);
+}
+void tile_apply_moments32(const Geo& g, const float* x, float* y, float* halo, const float* Mom,
+                          const uint8_t* bcflag, hipStream_t st) {
+  tile_apply<float, 0>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, st);
+}
+
+}  // namespace gs
