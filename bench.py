@@ -56,6 +56,32 @@ def cpu_baseline(res, threads):
                 inner_iterations=int(rep.inner_iterations), E=cfg.E, P=cfg.P)
 
 
+def profile_apply(args):
+    """Calibration copy of known size (torch elementwise copy, 256 MiB read +
+    256 MiB written per call) followed by the four K apply forms, so that
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to
+    bytes.  torch is imported before the library so both share one HIP runtime."""
+    import torch
+    n = 32 * 1024 * 1024
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    import gale_amd as G
+    from gale_amd import configs
+    cfg = configs.random_viscosity_3d(res=args.res, ppc=args.ppc)
+    pb = G.Problem(**cfg.problem_kwargs())
+    s = G.Stokes_SLE_UzawaSolver(pb, G.uzawa_params(tolerance=1e-6))
+    s.setup()
+    for form in (G.APPLY_MOMENTS_FP64, G.APPLY_MOMENTS_FP32, G.APPLY_PARTICLES_FP64, G.APPLY_PARTICLES_FP32):
+        ms = s.time_apply_K(form, 3)
+        print(f"# form {form}: {ms:.4f} ms", file=sys.stderr, flush=True)
+    print(json.dumps({"copy_bytes_each_way": n * 8, "res": args.res, "ppc": args.ppc,
+                      "moments_bytes": k_moments_bytes(cfg.dim, cfg.N, cfg.E),
+                      "particles_bytes": k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,11 +94,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-outer", type=int, default=20000)
     ap.add_argument("--mg-smooth", type=int, default=2)
+    ap.add_argument("--profile-apply", action="store_true",
+                    help="only run a calibration copy and the K applies (for rocprofv3 --pmc traffic counts)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is a design in DESIGN.md, not built).", file=sys.stderr)
         sys.exit(2)
+
+    if args.profile_apply:
+        profile_apply(args)
+        return
 
     import gale_amd as G
     from gale_amd import configs
@@ -174,11 +206,11 @@ def main():
                    "k_moments_fp32_ms": ms_m32, "k_moments_fp32_gbs": bytes_m32 / (ms_m32 * 1e-3) / 1e9},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gs::k_mom_apply3<double> (moment-form K apply: 8 parity-colour launches + zero fill)",
+                     "kernel": "gs::k_tile_apply3<double,0> + gs::k_tile_halo3<double> (moment-form K apply, fine level)",
                      "ms_per_apply": ms_m, "algorithmic_bytes": bytes_m},
         "roofline_particle_apply": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                    "kernel": "gs::k_apply_K3<double> (particle-form K apply)",
+                                    "kernel": "gs::k_tile_apply3<double,1> + gs::k_tile_halo3<double> (particle-form K apply)",
                                     "ms_per_apply": ms, "algorithmic_bytes": nbytes,
                                     "fp32_ms_per_apply": ms32, "fp32_algorithmic_bytes": nbytes32},
         "cpu_baseline": cpu,
