@@ -20,7 +20,7 @@ LIB_PATH = os.path.join(HERE, "libgale_stokes.so")
 
 OK, E_NAN, E_MAXIT, E_ZERO_FORCE, E_STAGNATION, E_HIP, E_INVALID, E_ESCAPED, E_INNER_MAXIT, E_PICARD, E_NO_DEVICE = range(11)
 YIELD_NONE, YIELD_DRUCKER_PRAGER, YIELD_VON_MISES = 0, 1, 2
-PC_NONE, PC_DIAGONAL = 0, 1
+PC_NONE, PC_DIAGONAL, PC_LEARNED = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 INNER_JACOBI, INNER_MULTIGRID = 0, 1
 APPLY_PARTICLES_FP64, APPLY_PARTICLES_FP32, APPLY_MOMENTS_FP64, APPLY_MOMENTS_FP32 = 0, 1, 2, 3
@@ -35,6 +35,10 @@ EXPORTS = (
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
+    "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
+    "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
+    "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
+    "gs_solver_set_learned", "gs_lpc_selftest",
 )
 
 
@@ -85,7 +89,7 @@ class Report(C.Structure):
         ("status", C.c_int32), ("uzawa_iterations", C.c_int32), ("inner_iterations", C.c_int64),
         ("rel_residual", C.c_double), ("abs_residual", C.c_double), ("div_u", C.c_double),
         ("momentum_residual", C.c_double), ("solve_seconds", C.c_double), ("setup_seconds", C.c_double),
-        ("picard_iterations", C.c_int32), ("pad_", C.c_int32), ("picard_residual", C.c_double),
+        ("picard_iterations", C.c_int32), ("learned_fallback", C.c_int32), ("picard_residual", C.c_double),
         ("k_applies", C.c_int64),
     ]
 
@@ -146,6 +150,23 @@ def lib():
             "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
             "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
             "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
+            "gs_lpc_create": (i32, [C.c_uint64, vp]),
+            "gs_lpc_destroy": (None, [vp]),
+            "gs_lpc_num_params": (i64, []),
+            "gs_lpc_get_params": (i32, [vp, vp]),
+            "gs_lpc_set_params": (i32, [vp, vp]),
+            "gs_lpc_forward": (i32, [vp, i32, i32, i32, i32, vp, vp]),
+            "gs_lpc_backward": (i32, [vp, i32, i32, i32, i32, vp, vp, vp]),
+            "gs_lpc_loss": (i32, [vp, i32, i32, i32, i32, vp, vp, vp, vp, vp, vp]),
+            "gs_lpc_dataset_create": (i32, [i32, i32, i32, i32, vp]),
+            "gs_lpc_dataset_destroy": (None, [vp]),
+            "gs_lpc_dataset_size": (i32, [vp]),
+            "gs_lpc_dataset_get": (i32, [vp, i32, vp, vp, vp, vp]),
+            "gs_solver_record_pairs": (i32, [vp, vp, i32]),
+            "gs_lpc_train": (i32, [vp, vp, vp, vp]),
+            "gs_lpc_eval": (i32, [vp, vp, i32, i32, vp]),
+            "gs_solver_set_learned": (i32, [vp, vp]),
+            "gs_lpc_selftest": (i32, [vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -400,11 +421,14 @@ class Problem:
 def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
                  Preconditioner=True, precision=PREC_FP64, inner_max_iterations=0, inner_rtol_factor=0.1,
                  inner_rtol_override=0.0, monitor=False, check_every=4, inner_preconditioner=INNER_MULTIGRID,
-                 mg_smooth=2) -> UzawaParams:
+                 mg_smooth=2, learned=False) -> UzawaParams:
     """Stokes_SLE_UzawaSolver parameters under their XML names
-    (Stokes_SLE_UzawaSolver.cxx:251-257 defaults) plus the inner-solve controls."""
+    (Stokes_SLE_UzawaSolver.cxx:251-257 defaults) plus the inner-solve controls.
+    ``learned=True`` selects the learned Schur preconditioner (attach a network
+    with ``Stokes_SLE_UzawaSolver.set_learned``)."""
+    pc = PC_LEARNED if learned else (PC_DIAGONAL if Preconditioner else PC_NONE)
     return UzawaParams(tolerance, maxIterations, minIterations, int(bool(useAbsoluteTolerance)),
-                       PC_DIAGONAL if Preconditioner else PC_NONE, precision, inner_max_iterations,
+                       pc, precision, inner_max_iterations,
                        inner_rtol_factor, inner_rtol_override, int(monitor), check_every,
                        inner_preconditioner, mg_smooth)
 
@@ -491,6 +515,17 @@ class Stokes_SLE_UzawaSolver:
         _check(st, "advect_rk2", allow=(E_ESCAPED,))
         return st
 
+    def set_learned(self, lpc: "LearnedPreconditioner | None"):
+        """Attach the learned Schur preconditioner (used with uzawa_params(learned=True))."""
+        self._lpc = lpc
+        _check(lib().gs_solver_set_learned(self._h, lpc._h if lpc is not None else None), "set_learned")
+
+    def record_pairs(self, dataset: "PairDataset | None", krylov_every=0):
+        """Record training pairs of the following solves into ``dataset``."""
+        self._ds = dataset
+        _check(lib().gs_solver_record_pairs(self._h, dataset._h if dataset is not None else None,
+                                            int(krylov_every)), "record_pairs")
+
     def setup_arrays(self):
         P = self.problem.P
         dg, Q, f, h, eta = np.zeros(self.n), np.zeros(self.np), np.zeros(self.n), np.zeros(self.np), np.zeros(P)
@@ -521,3 +556,129 @@ class Stokes_SLE_UzawaSolver:
         ms = C.c_double()
         _check(lib().gs_time_apply_K(self._h, form, n_rep, C.byref(ms)), "time_apply_K")
         return ms.value
+
+
+# ---------------------------------------------------------------------------
+# learned Schur-complement preconditioner (include/gale_learned.h)
+# ---------------------------------------------------------------------------
+class TrainParams(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int32), ("batch", C.c_int32), ("lr", C.c_float), ("lr_final", C.c_float),
+        ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float), ("clip", C.c_float),
+        ("warmup", C.c_int32), ("first", C.c_int32), ("count", C.c_int32), ("seed", C.c_uint32),
+    ]
+
+
+def _dptr(t):
+    """Device pointer of a torch CUDA tensor (float32, contiguous)."""
+    if t is None:
+        return None
+    assert t.is_cuda and t.is_contiguous(), "expected a contiguous device tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+class PairDataset:
+    """Training pairs (S e = r) recorded on the device by plain Uzawa solves."""
+
+    def __init__(self, res, capacity):
+        nx, ny, nz = (res, res, res) if np.isscalar(res) else tuple(res)
+        self.shape = (nx, ny, nz)
+        self.V = nx * ny * nz
+        h = C.c_void_p()
+        _check(lib().gs_lpc_dataset_create(nx, ny, nz, int(capacity), C.byref(h)), "dataset_create")
+        self._h = h
+
+    def __len__(self):
+        return int(lib().gs_lpc_dataset_size(self._h))
+
+    def get(self, idx):
+        inp = np.zeros(2 * self.V, np.float32)
+        tgt = np.zeros(self.V, np.float32)
+        w = C.c_float()
+        pj = C.c_int32()
+        _check(lib().gs_lpc_dataset_get(self._h, int(idx), _ptr(inp), _ptr(tgt), C.byref(w), C.byref(pj)))
+        return inp.reshape(self.V, 2), tgt, w.value, pj.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gs_lpc_dataset_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class LearnedPreconditioner:
+    """BF16 matrix-core conv encoder-decoder preconditioning the Schur residual."""
+
+    def __init__(self, seed=1):
+        h = C.c_void_p()
+        _check(lib().gs_lpc_create(C.c_uint64(seed), C.byref(h)), "lpc_create")
+        self._h = h
+
+    @staticmethod
+    def num_params():
+        return int(lib().gs_lpc_num_params())
+
+    def get_params(self):
+        out = np.zeros(self.num_params(), np.float32)
+        _check(lib().gs_lpc_get_params(self._h, _ptr(out)))
+        return out
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, np.float32)
+        assert params.size == self.num_params()
+        _check(lib().gs_lpc_set_params(self._h, _ptr(params)))
+
+    def forward(self, d_in, d_out):
+        """d_in: float32 device tensor [B, nz, ny, nx, 2]; d_out: [B, nz, ny, nx]."""
+        B, nz, ny, nx, _ = d_in.shape
+        _check(lib().gs_lpc_forward(self._h, nx, ny, nz, B, _dptr(d_in), _dptr(d_out)), "lpc_forward")
+
+    def backward(self, d_in, d_gout):
+        B, nz, ny, nx, _ = d_in.shape
+        g = np.zeros(self.num_params(), np.float32)
+        _check(lib().gs_lpc_backward(self._h, nx, ny, nz, B, _dptr(d_in), _dptr(d_gout), _ptr(g)), "lpc_backward")
+        return g
+
+    def loss(self, d_out, d_target, weights, proj, d_gout):
+        B, nz, ny, nx = d_out.shape
+        w = np.ascontiguousarray(weights, np.float32)
+        pj = np.ascontiguousarray(proj, np.int32)
+        L = C.c_double()
+        _check(lib().gs_lpc_loss(self._h, nx, ny, nz, B, _dptr(d_out), _dptr(d_target), _ptr(w), _ptr(pj),
+                                 _dptr(d_gout), C.byref(L)), "lpc_loss")
+        return L.value
+
+    def train(self, dataset: PairDataset, steps, batch=8, lr=1e-3, lr_final=1e-5, clip=1.0, warmup=50,
+              first=0, count=0, seed=1, beta1=0.9, beta2=0.999, eps=1e-8):
+        p = TrainParams(int(steps), int(batch), lr, lr_final, beta1, beta2, eps, clip, int(warmup), int(first),
+                        int(count), int(seed))
+        hist = np.zeros(max(int(steps), 1), np.float32)
+        _check(lib().gs_lpc_train(self._h, dataset._h, C.byref(p), _ptr(hist)), "lpc_train")
+        return hist[:steps]
+
+    def evaluate(self, dataset: PairDataset, first, count):
+        L = C.c_double()
+        _check(lib().gs_lpc_eval(self._h, dataset._h, int(first), int(count), C.byref(L)), "lpc_eval")
+        return L.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gs_lpc_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lpc_selftest():
+    e = C.c_float()
+    _check(lib().gs_lpc_selftest(C.byref(e)), "lpc_selftest")
+    return e.value

@@ -17,6 +17,7 @@
 
 #include "../../include/gale_stokes.h"
 #include "gs_internal.h"
+#include "gs_learned.h"
 #include "gs_mg.h"
 #include "gs_tile.h"
 
@@ -60,6 +61,7 @@ struct gs_problem {
   uint8_t* bcflag = nullptr;  // N (bit d = dof d is Dirichlet)
   double* ubc = nullptr;      // N*dim, Dirichlet values, zeros elsewhere
   bool has_bc = false;
+  bool closed = false;  // every boundary node has its normal velocity fixed
   long long n_free = 0;
   long long P = 0, escaped = 0;
   double4* pk = nullptr;
@@ -141,6 +143,13 @@ struct gs_solver {
   int has_executed = 0;
   long long k_applies = 0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // learned Schur preconditioner and training-pair recording
+  gs_lpc* lpc = nullptr;
+  gs_lpc_dataset* rec = nullptr;
+  int rec_every = 0;
+  float* logeta = nullptr;
+  double* rec_r0 = nullptr;
+  double* rec_p0 = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -304,6 +313,15 @@ extern "C" int32_t gs_problem_create(const gs_problem_desc* d, gs_problem** out)
     }
   }
   pb->n_free = nfree;
+  {
+    bool closed = true;
+    for (long long nd = 0; nd < g.N && closed; nd++) {
+      long long idx[3] = {nd % g.nn[0], (nd / g.nn[0]) % g.nn[1], nd / (g.nn[0] * g.nn[1])};
+      for (int dd = 0; dd < g.dim; dd++)
+        if ((idx[dd] == 0 || idx[dd] == g.res[dd]) && !(bcflag[nd] & (1u << dd))) closed = false;
+    }
+    pb->closed = closed;
+  }
   const long long P = d->n_particles;
   pb->P = P;
   if (dalloc(&pb->mats_d, d->n_materials) || dalloc(&pb->bcdof, ND) || dalloc(&pb->bcflag, g.N) ||
@@ -461,6 +479,9 @@ extern "C" int32_t gs_solver_set_params(gs_solver* s, const gs_uzawa_params* up)
   if (s->up.mg_smooth <= 0) s->up.mg_smooth = 2;
   if (s->up.inner_preconditioner != GS_INNER_JACOBI && s->up.inner_preconditioner != GS_INNER_MULTIGRID)
     return GS_E_INVALID;
+  if (s->up.preconditioner != GS_PC_NONE && s->up.preconditioner != GS_PC_DIAGONAL &&
+      s->up.preconditioner != GS_PC_LEARNED)
+    return GS_E_INVALID;
   s->setup_done = false;
   if (!(s->up.tolerance > 0)) return GS_E_INVALID;
   return GS_OK;
@@ -511,6 +532,7 @@ extern "C" void gs_solver_destroy(gs_solver* s) {
   dfree(s->cg_z);
   dfree(s->cg_r); dfree(s->cg_p); dfree(s->cg_Ap); dfree(s->tmpN); dfree(s->tmpE); dfree(s->eta); dfree(s->uprev);
   dfree(s->yielded); dfree(s->x32); dfree(s->y32); dfree(s->S); dfree(s->rd.partials); dfree(s->rd.counter);
+  dfree(s->logeta); dfree(s->rec_r0); dfree(s->rec_p0);
   if (s->hS) (void)hipHostFree(s->hS);
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
@@ -870,6 +892,10 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
     launch_fill(s->h, s->np, 0.0, st);
   }
   if (s->up.inner_preconditioner == GS_INNER_MULTIGRID || s->up.precision == GS_PREC_MIXED) CKS(mg_numeric(s));
+  if (s->lpc || s->rec) {
+    if (!s->logeta) CK(dalloc(&s->logeta, s->np));
+    launch_elem_logeta(pb->g, pb->pk, pb->off, s->logeta, st);
+  }
   s->setup_done = true;
   return GS_OK;
 }
@@ -1003,7 +1029,10 @@ static int32_t uzawa(gs_solver* s, gs_report* rep) {
   int32_t err = GS_OK;
   int it = 0;
   double relRes = 0.0, absRes = 0.0;
-  const double* Qinv = s->up.preconditioner == GS_PC_DIAGONAL ? s->Qinv : nullptr;
+  const bool learned = s->up.preconditioner == GS_PC_LEARNED;
+  const double* Qinv = (s->up.preconditioner == GS_PC_DIAGONAL || learned) ? s->Qinv : nullptr;
+  if (learned && (!s->lpc || !lpc_shape_ok(pb->g) || !s->logeta)) return GS_E_INVALID;
+  const bool recording = s->rec && s->logeta && Qinv;
   // :476-493 zero-force firewall
   const double ff = dot_host(s, s->f, s->f, n, &err);
   CKS(err);
@@ -1031,14 +1060,41 @@ static int32_t uzawa(gs_solver* s, gs_report* rep) {
     if (!std::isfinite(std::sqrt(rn) / std::sqrt(un))) return GS_E_NAN;
   }
   launch_axpby(s->r, s->h, -1.0, 1.0, np, st);
-  // STEP 4 (:664-837)
+  if (recording) {
+    CK(hipMemcpyAsync(s->rec_r0, s->r, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    CK(hipMemcpyAsync(s->rec_p0, s->p, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+  }
+  // Closed domains (every boundary normal fixed) leave the pressure defined up
+  // to a constant: training pairs then compare mean-free fields.
+  const int proj = pb->closed ? 1 : 0;
+  // STEP 4 (:664-837); with the learned preconditioner the iteration is the
+  // flexible (Polak-Ribiere) variant, falling back to the diagonal Q when the
+  // residual stagnates or z.r <= 0.
+  bool lp_on = learned;
+  bool restart = false;
+  const int kWindow = 25;
+  std::vector<double> hist;
   launch_fill(s->S + S_ZR_PREV, 1, 1.0, st);
   do {
-    launch_uz_precond(s->r, Qinv, s->z, np, s->S, s->rd, st);
-    launch_uz_sdir(s->s, s->z, np, s->S, it == 0, st);
+    if (lp_on) {
+      CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, it == 0 || restart ? nullptr : s->q, s->z, s->S,
+                      s->rd, st));
+      CKS(read_scalars(s));
+      const double zr = s->hS[S_ZR];
+      if (!(zr > 0.0) || !std::isfinite(zr) || !std::isfinite(s->hS[S_UBETA])) {
+        lp_on = false;
+        rep->learned_fallback = it + 1;
+        restart = true;
+      }
+    }
+    if (!lp_on) launch_uz_precond(s->r, Qinv, s->z, np, s->S, s->rd, st);
+    launch_uz_sdir(s->s, s->z, np, s->S, it == 0 || restart, st);
+    restart = false;
     launch_apply_G(pb->g, s->s, s->fT, pb->bcflag, st);
     CKS(inner_solve(s, s->fT, s->vstar, irt, &inner));
     launch_apply_Gt(pb->g, s->vstar, s->q, st);
+    if (recording && s->rec_every > 0 && it % s->rec_every == 0)
+      CKS(lpc_record(s->rec, pb->g, s->q, s->s, s->Qinv, s->logeta, proj, s->S, s->rd, st));
     launch_dot(s->s, s->q, np, s->S, S_SQ, s->rd, st);
     launch_uz_update(s->p, s->s, s->r, s->q, np, s->S, s->rd, st);
     launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
@@ -1056,12 +1112,27 @@ static int32_t uzawa(gs_solver* s, gs_report* rep) {
     }
     if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
       std::fprintf(stderr, "  %04d uzawa residual norm %12.13e\n", it + 1, relRes);
+    if (lp_on) {
+      hist.push_back(relRes);
+      const size_t k = hist.size();
+      if (k > (size_t)(2 * kWindow) && relRes > 0.95 * hist[k - 1 - kWindow]) {
+        lp_on = false;
+        rep->learned_fallback = it + 1;
+        restart = true;
+      }
+    }
     if (!(it < s->up.max_iterations)) {
       err = GS_E_MAXIT;
       break;
     }
     it++;
   } while (((s->up.use_absolute_tolerance ? absRes : relRes) > tol) || (it < s->up.min_iterations));
+  if (recording && err == GS_OK) {
+    // pair (r0, p* - p0)
+    CK(hipMemcpyAsync(s->tmpE, s->p, sizeof(double) * np, hipMemcpyDeviceToDevice, st));
+    launch_axpby(s->tmpE, s->rec_p0, -1.0, 1.0, np, st);
+    CKS(lpc_record(s->rec, pb->g, s->rec_r0, s->tmpE, s->Qinv, s->logeta, proj, s->S, s->rd, st));
+  }
   rep->uzawa_iterations += it;
   rep->inner_iterations += inner;
   rep->rel_residual = relRes;
@@ -1358,5 +1429,29 @@ extern "C" int32_t gs_solver_setup_arrays(gs_solver* s, double* diagK, double* Q
     CK(hipMemcpy(&offE, pb->off + pb->g.E, sizeof(int), hipMemcpyDeviceToHost));
     for (long long i = 0; i < P; i++) eta[pid[i]] = i < offE ? e[i] : 0.0;
   }
+  return GS_OK;
+}
+
+// ---------------------------------------------------------------------------
+// learned preconditioner attachment (include/gale_learned.h)
+// ---------------------------------------------------------------------------
+extern "C" int32_t gs_solver_set_learned(gs_solver* s, gs_lpc* L) {
+  if (!s) return GS_E_INVALID;
+  if (L && !lpc_shape_ok(s->pb->g)) return GS_E_INVALID;
+  s->lpc = L;
+  if (L) s->setup_done = false;  // the element viscosity field is made at setup
+  return GS_OK;
+}
+
+extern "C" int32_t gs_solver_record_pairs(gs_solver* s, gs_lpc_dataset* D, int32_t krylov_every) {
+  if (!s || krylov_every < 0) return GS_E_INVALID;
+  if (D && !lpc_shape_ok(s->pb->g)) return GS_E_INVALID;
+  if (D && (!s->rec_r0 || !s->rec_p0)) {
+    CK(dalloc(&s->rec_r0, s->np));
+    CK(dalloc(&s->rec_p0, s->np));
+  }
+  if (D && !s->logeta) s->setup_done = false;
+  s->rec = D;
+  s->rec_every = krylov_every;
   return GS_OK;
 }

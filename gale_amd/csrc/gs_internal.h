@@ -43,6 +43,11 @@ enum Slot {
   S_TMP2 = 18,
   S_VMAX = 20,
   S_ESC = 24,    // escaped particle count (stored as double)
+  S_LP_TT = 26,  // learned preconditioner: sum (Q^-1 r)^2
+  S_LP_ZQ = 27,  // learned preconditioner: z . q_prev
+  S_DS_TT = 28,  // training-pair recording: sum (Q^-1 r)^2, sum e, sum e^2
+  S_DS_E = 29,
+  S_DS_EE = 30,
   S_COUNT = 32
 };
 

@@ -116,7 +116,7 @@ typedef struct gs_report {
   double solve_seconds;         /* device time of the Uzawa solve(s) */
   double setup_seconds;         /* device time of viscosity pass + diagonal operators + RHS */
   int32_t picard_iterations;
-  int32_t pad_;
+  int32_t learned_fallback;     /* learned preconditioner: Uzawa iteration of the fallback to Q (0 = none) */
   double picard_residual;
   int64_t k_applies;            /* K applies issued (FP64 + FP32) */
 } gs_report;

@@ -1,0 +1,185 @@
+"""GPU checks of the learned Schur preconditioner: the BF16 matrix-core
+convolutions (forward, data and weight gradients) and the training loss
+against the CPU network oracle, the recorded training pairs against the
+oracle's assembled Schur complement, device training, and the learned
+(flexible) Uzawa solve reaching the plain solve's solution, including the
+fallback to the diagonal Q."""
+import numpy as np
+import pytest
+
+from conftest import rel
+from oracle import lpc_net as N
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _rand_in(B, nz, ny, nx, seed):
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal((B, nz, ny, nx, 2)).astype(np.float32)
+    x[..., 1] = rng.uniform(-1, 1, (B, nz, ny, nx)).astype(np.float32)
+    return x
+
+
+def _net_with_live_output(gale, seed=3, scale=100.0):
+    net = gale.LearnedPreconditioner(seed=seed)
+    P = net.get_params()
+    sl = N.layer_slices()[0]
+    P[sl[13][0]] *= scale  # the output layer starts near zero; make it carry signal
+    P[sl[13][1]] = 0.05
+    net.set_params(P)
+    return net, net.get_params()
+
+
+def test_mfma_operand_layout(gale):
+    assert gale.lpc_selftest() == 0.0
+
+
+def test_forward_matches_oracle(gale):
+    torch = _torch()
+    net, P = _net_with_live_output(gale)
+    B, nz, ny, nx = 2, 8, 16, 24
+    x = _rand_in(B, nz, ny, nx, 11)
+    d_in = torch.tensor(x, device="cuda")
+    d_out = torch.empty((B, nz, ny, nx), dtype=torch.float32, device="cuda")
+    net.forward(d_in, d_out)
+    out = d_out.cpu().numpy()
+    ref, _ = N.forward(P, x, mimic=True)
+    assert np.isfinite(out).all()
+    assert rel(out, ref) < 1e-2, rel(out, ref)
+    exact, _ = N.forward(P, x, mimic=False)
+    assert rel(out, exact) < 5e-2
+
+
+def test_backward_matches_oracle(gale):
+    torch = _torch()
+    net, P = _net_with_live_output(gale, seed=5)
+    B, nz, ny, nx = 2, 16, 8, 8
+    x = _rand_in(B, nz, ny, nx, 12)
+    gout = np.random.default_rng(13).standard_normal((B, nz, ny, nx)).astype(np.float32)
+    g = net.backward(torch.tensor(x, device="cuda"), torch.tensor(gout, device="cuda"))
+    _, cache = N.forward(P, x, mimic=True)
+    gref = N.backward(P, cache, gout.astype(np.float64))
+    sl = N.layer_slices()[0]
+    for l, (a, b) in enumerate(sl):
+        assert rel(g[a], gref[a]) < 3e-2, (l, "W", rel(g[a], gref[a]))
+        assert rel(g[b], gref[b]) < 3e-2, (l, "b", rel(g[b], gref[b]))
+    # and the exact-arithmetic gradient within the BF16 budget
+    _, cex = N.forward(P, x, mimic=False)
+    gex = N.backward(P, cex, gout.astype(np.float64))
+    assert rel(g, gex) < 0.1
+
+
+def test_loss_matches_oracle(gale):
+    torch = _torch()
+    net = gale.LearnedPreconditioner(seed=1)
+    rng = np.random.default_rng(4)
+    B, nz, ny, nx = 3, 8, 8, 16
+    c = rng.standard_normal((B, nz, ny, nx)).astype(np.float32)
+    a = rng.standard_normal((B, nz, ny, nx)).astype(np.float32)
+    w = np.array([0.5, 2.0, 1.0], np.float32)
+    proj = np.array([1, 0, 1], np.int32)
+    d_g = torch.empty((B, nz, ny, nx), dtype=torch.float32, device="cuda")
+    L = net.loss(torch.tensor(c, device="cuda"), torch.tensor(a, device="cuda"), w, proj, d_g)
+    Lr, gr = N.loss(c.astype(np.float64), a.astype(np.float64), w, proj)
+    assert abs(L - Lr) < 1e-9 * abs(Lr)
+    assert rel(d_g.cpu().numpy(), gr) < 1e-6
+
+
+def _rv(res, field_seed, ppc=4, seed=13):
+    from gale_amd import configs
+    return configs.random_viscosity_3d(res=res, ppc=ppc, seed=seed, field_seed=field_seed)
+
+
+def test_recorded_pairs_satisfy_schur_relation(gale, oracle):
+    """Each pair (input t = Q^-1 r / rms, target (e - t) / rms) satisfies
+    S (t + target) = Q t with S = G^T K^-1 G of the oracle's eliminated system."""
+    from scipy.sparse.linalg import splu
+    cfg = _rv(8, 2001)
+    pb = gale.Problem(**cfg.problem_kwargs())
+    s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(tolerance=1e-10, inner_rtol_override=1e-13,
+                                                          maxIterations=5000, inner_max_iterations=100000))
+    ds = gale.PairDataset(8, 32)
+    s.record_pairs(ds, krylov_every=4)
+    _, _, rep = s.Solve()
+    assert rep.status == 0
+    n = len(ds)
+    assert n >= 3
+    Q = s.setup_arrays()["Q"]
+    o = oracle.Problem(**cfg.oracle_kwargs())
+    o.assemble()
+    K, G, _, _, _, _ = o.system()
+    lu = splu(K.tocsc())
+    for i in range(n):
+        inp, tgt, w, pj = ds.get(i)
+        t = inp[:, 0].astype(np.float64)
+        assert abs(np.sqrt(np.mean(t * t)) - 1.0) < 1e-5
+        assert w > 0 and pj == 1
+        e = t + tgt.astype(np.float64)
+        Se = G.T @ lu.solve(G @ e)
+        assert rel(Se, Q * t) < 1e-2, (i, rel(Se, Q * t))
+        # channel 1: log10 of the element mean viscosity / 3 (viscosity clamped to [1e-3, 1e3])
+        assert np.abs(inp[:, 1]).max() <= 1.0 + 1e-6 and np.std(inp[:, 1]) > 0.05
+    # the last pair is (r0, p* - p0): the converged pressure of this solve
+    s.record_pairs(None)
+
+
+@pytest.fixture(scope="module")
+def trained(gale):
+    """A dataset from plain solves of four 16^3 fields and a network trained on it."""
+    ds = gale.PairDataset(16, 256)
+    for fs in (3001, 3002, 3003, 3004):
+        cfg = _rv(16, fs)
+        pb = gale.Problem(**cfg.problem_kwargs())
+        s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(tolerance=1e-6, maxIterations=5000))
+        s.record_pairs(ds, krylov_every=4)
+        _, _, rep = s.Solve()
+        assert rep.status == 0
+        s.record_pairs(None)
+        s.close()
+    n = len(ds)
+    net = gale.LearnedPreconditioner(seed=7)
+    l0 = net.evaluate(ds, 0, n)
+    hist = net.train(ds, steps=300, batch=4, lr=1e-3, lr_final=1e-5, warmup=20)
+    l1 = net.evaluate(ds, 0, n)
+    print(f"training pairs {n}: loss {l0:.4f} -> {l1:.4f} (last batch {hist[-1]:.4f})", flush=True)
+    return dict(ds=ds, net=net, l0=l0, l1=l1, hist=hist, n=n)
+
+
+def test_training_reduces_loss(trained):
+    assert trained["n"] >= 40
+    assert np.isfinite(trained["hist"]).all()
+    assert trained["l1"] < 0.9 * trained["l0"], (trained["l0"], trained["l1"])
+
+
+def _solve(gale, cfg, net=None, tol=1e-8, precision=0, fallback_ok=True):
+    pb = gale.Problem(**cfg.problem_kwargs())
+    prm = gale.uzawa_params(tolerance=tol, maxIterations=20000, precision=precision, learned=net is not None)
+    s = gale.Stokes_SLE_UzawaSolver(pb, prm)
+    if net is not None:
+        s.set_learned(net)
+    u, p, rep = s.Solve()
+    return u, p - p.mean(), rep
+
+
+def test_learned_uzawa_reaches_plain_solution(gale, trained):
+    cfg = _rv(16, 3999)
+    u0, p0, r0 = _solve(gale, cfg)
+    u1, p1, r1 = _solve(gale, cfg, net=trained["net"], precision=gale.PREC_MIXED)
+    print(f"uzawa iterations plain {r0.uzawa_iterations}, learned+mixed {r1.uzawa_iterations} "
+          f"(fallback at {r1.learned_fallback}); inner {r0.inner_iterations} vs {r1.inner_iterations}", flush=True)
+    assert r1.status == 0 and r1.rel_residual <= 1e-8
+    assert rel(u1, u0) < 1e-5 and rel(p1, p0) < 1e-5
+
+
+def test_bad_network_falls_back_to_plain_q(gale):
+    net, _ = _net_with_live_output(gale, seed=9, scale=1e5)
+    cfg = _rv(16, 3998)
+    u0, p0, r0 = _solve(gale, cfg)
+    u1, p1, r1 = _solve(gale, cfg, net=net)
+    assert r1.status == 0 and r1.learned_fallback > 0
+    assert rel(u1, u0) < 1e-5 and rel(p1, p0) < 1e-5
