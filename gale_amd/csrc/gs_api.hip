@@ -150,6 +150,11 @@ struct gs_solver {
   float* logeta = nullptr;
   double* rec_r0 = nullptr;
   double* rec_p0 = nullptr;
+  // truncated flexible CG directions (learned preconditioner)
+  int fl_m = 0;
+  double* fl_s = nullptr;
+  double* fl_q = nullptr;
+  double* fl_aux = nullptr;
 };
 
 // ---------------------------------------------------------------------------
@@ -532,7 +537,7 @@ extern "C" void gs_solver_destroy(gs_solver* s) {
   dfree(s->cg_z);
   dfree(s->cg_r); dfree(s->cg_p); dfree(s->cg_Ap); dfree(s->tmpN); dfree(s->tmpE); dfree(s->eta); dfree(s->uprev);
   dfree(s->yielded); dfree(s->x32); dfree(s->y32); dfree(s->S); dfree(s->rd.partials); dfree(s->rd.counter);
-  dfree(s->logeta); dfree(s->rec_r0); dfree(s->rec_p0);
+  dfree(s->logeta); dfree(s->rec_r0); dfree(s->rec_p0); dfree(s->fl_s); dfree(s->fl_q); dfree(s->fl_aux);
   if (s->hS) (void)hipHostFree(s->hS);
   if (s->e0) (void)hipEventDestroy(s->e0);
   if (s->e1) (void)hipEventDestroy(s->e1);
@@ -1018,6 +1023,111 @@ static double dot_host(gs_solver* s, const double* a, const double* b, long long
   return s->hS[S_TMP1];
 }
 
+// Uzawa with the learned Schur preconditioner: truncated flexible CG.  Each
+// preconditioned residual z = M(r) (network correction of Q^-1 r) is
+// S-orthogonalised against the last m directions; alpha = s.r / s.q.  When
+// z.r <= 0 or the residual falls by less than 5% over the stagnation window,
+// the rest of the solve preconditions with Q^-1 (same loop, same tolerance).
+// Stopping test, residual and firewalls are those of the reference loop.
+static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsScale, double irt, bool recording,
+                                  int proj, int* it_io, long long* inner_io, double* relRes_io, double* absRes_io) {
+  gs_problem* pb = s->pb;
+  hipStream_t st = pb->st;
+  const long long n = s->n, np = s->np;
+  const double tol = s->up.tolerance;
+  const double qRecip = 1.0 / std::sqrt((double)np);
+  int m = s->up.flexible_window > 0 ? s->up.flexible_window : 16;
+  if (m > kMaxWindow) m = kMaxWindow;
+  const int win = s->up.stagnation_window > 0 ? s->up.stagnation_window : 25;
+  if (s->fl_m != m) {
+    dfree(s->fl_s);
+    dfree(s->fl_q);
+    s->fl_m = 0;
+    CK(dalloc(&s->fl_s, (long long)m * np));
+    CK(dalloc(&s->fl_q, (long long)m * np));
+    s->fl_m = m;
+  }
+  if (!s->fl_aux) CK(dalloc(&s->fl_aux, (long long)kMaxReduceBlocks * kMaxWindow + 2 * kMaxWindow));
+  double* partials = s->fl_aux;
+  double* beta = s->fl_aux + (long long)kMaxReduceBlocks * kMaxWindow;
+  double* sqv = beta + kMaxWindow;
+  int it = *it_io;
+  long long inner = *inner_io;
+  double relRes = 0.0, absRes = 0.0;
+  int32_t err = GS_OK;
+  bool lp_on = true;
+  int base = 0;  // first iteration of the current orthogonalisation window (reset on restart / fallback)
+  int restarts_left = s->up.learned_restarts > 0 ? s->up.learned_restarts : 0;
+  const int q_every = s->up.learned_q_every > 0 ? s->up.learned_q_every : 0;
+  std::vector<double> hist;
+  do {
+    const bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
+    if (use_net) {
+      CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, nullptr, s->z, s->S, s->rd, st));
+      CKS(read_scalars(s));
+      const double zr = s->hS[S_ZR];
+      if (!(zr > 0.0) || !std::isfinite(zr)) {
+        lp_on = false;
+        rep->learned_fallback = it + 1;
+        base = it;
+      }
+    }
+    if (!use_net || !lp_on) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
+    const int k = it - base;
+    const int c = k < m ? k : m;
+    const int slot = k % m;
+    double* sk = s->fl_s + (long long)slot * np;
+    double* qk = s->fl_q + (long long)slot * np;
+    launch_orthogonalise(s->z, s->fl_s, s->fl_q, sqv, c, slot, np, partials, beta, st);
+    launch_apply_G(pb->g, sk, s->fT, pb->bcflag, st);
+    CKS(inner_solve(s, s->fT, s->vstar, irt, &inner));
+    launch_apply_Gt(pb->g, s->vstar, qk, st);
+    if (recording && s->rec_every > 0 && it % s->rec_every == 0)
+      CKS(lpc_record(s->rec, pb->g, qk, sk, s->Qinv, s->logeta, proj, s->S, s->rd, st));
+    launch_flex_alpha(sk, s->r, qk, np, s->S, sqv + slot, s->rd, st);
+    launch_uz_update(s->p, sk, s->r, qk, np, s->S, s->rd, st);
+    launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
+    CKS(read_scalars(s));
+    const double sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
+    if (!std::isfinite(sq) || !std::isfinite(alpha)) {
+      err = GS_E_NAN;
+      break;
+    }
+    absRes = std::sqrt(s->hS[S_ORR]) * qRecip;
+    relRes = absRes / uzawaRhsScale;
+    if (!std::isfinite(absRes)) {
+      err = GS_E_NAN;
+      break;
+    }
+    if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
+      std::fprintf(stderr, "  %04d uzawa residual norm %12.13e%s\n", it + 1, relRes, lp_on ? " (learned)" : "");
+    if (lp_on) {
+      hist.push_back(relRes);
+      const size_t h = hist.size();
+      if (h > (size_t)(2 * win) && relRes > 0.95 * hist[h - 1 - win]) {
+        if (restarts_left > 0) {
+          restarts_left--;
+          hist.clear();
+        } else {
+          lp_on = false;
+          rep->learned_fallback = it + 1;
+        }
+        base = it + 1;
+      }
+    }
+    if (!(it < s->up.max_iterations)) {
+      err = GS_E_MAXIT;
+      break;
+    }
+    it++;
+  } while (((s->up.use_absolute_tolerance ? absRes : relRes) > tol) || (it < s->up.min_iterations));
+  *it_io = it;
+  *inner_io = inner;
+  *relRes_io = relRes;
+  *absRes_io = absRes;
+  return err;
+}
+
 // _Stokes_SLE_UzawaSolver_Solve, Stokes_SLE_UzawaSolver.cxx:393-938.
 static int32_t uzawa(gs_solver* s, gs_report* rep) {
   gs_problem* pb = s->pb;
@@ -1067,66 +1177,43 @@ static int32_t uzawa(gs_solver* s, gs_report* rep) {
   // Closed domains (every boundary normal fixed) leave the pressure defined up
   // to a constant: training pairs then compare mean-free fields.
   const int proj = pb->closed ? 1 : 0;
-  // STEP 4 (:664-837); with the learned preconditioner the iteration is the
-  // flexible (Polak-Ribiere) variant, falling back to the diagonal Q when the
-  // residual stagnates or z.r <= 0.
-  bool lp_on = learned;
-  bool restart = false;
-  const int kWindow = 25;
-  std::vector<double> hist;
+  // STEP 4 (:664-837)
   launch_fill(s->S + S_ZR_PREV, 1, 1.0, st);
-  do {
-    if (lp_on) {
-      CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, it == 0 || restart ? nullptr : s->q, s->z, s->S,
-                      s->rd, st));
+  if (learned) {
+    err = uzawa_learned_loop(s, rep, uzawaRhsScale, irt, recording, proj, &it, &inner, &relRes, &absRes);
+  } else {
+    do {
+      launch_uz_precond(s->r, Qinv, s->z, np, s->S, s->rd, st);
+      launch_uz_sdir(s->s, s->z, np, s->S, it == 0, st);
+      launch_apply_G(pb->g, s->s, s->fT, pb->bcflag, st);
+      CKS(inner_solve(s, s->fT, s->vstar, irt, &inner));
+      launch_apply_Gt(pb->g, s->vstar, s->q, st);
+      if (recording && s->rec_every > 0 && it % s->rec_every == 0)
+        CKS(lpc_record(s->rec, pb->g, s->q, s->s, s->Qinv, s->logeta, proj, s->S, s->rd, st));
+      launch_dot(s->s, s->q, np, s->S, S_SQ, s->rd, st);
+      launch_uz_update(s->p, s->s, s->r, s->q, np, s->S, s->rd, st);
+      launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
       CKS(read_scalars(s));
-      const double zr = s->hS[S_ZR];
-      if (!(zr > 0.0) || !std::isfinite(zr) || !std::isfinite(s->hS[S_UBETA])) {
-        lp_on = false;
-        rep->learned_fallback = it + 1;
-        restart = true;
+      const double zr = s->hS[S_ZR], sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
+      if (!std::isfinite(zr) || !std::isfinite(sq) || !std::isfinite(alpha)) {
+        err = GS_E_NAN;
+        break;
       }
-    }
-    if (!lp_on) launch_uz_precond(s->r, Qinv, s->z, np, s->S, s->rd, st);
-    launch_uz_sdir(s->s, s->z, np, s->S, it == 0 || restart, st);
-    restart = false;
-    launch_apply_G(pb->g, s->s, s->fT, pb->bcflag, st);
-    CKS(inner_solve(s, s->fT, s->vstar, irt, &inner));
-    launch_apply_Gt(pb->g, s->vstar, s->q, st);
-    if (recording && s->rec_every > 0 && it % s->rec_every == 0)
-      CKS(lpc_record(s->rec, pb->g, s->q, s->s, s->Qinv, s->logeta, proj, s->S, s->rd, st));
-    launch_dot(s->s, s->q, np, s->S, S_SQ, s->rd, st);
-    launch_uz_update(s->p, s->s, s->r, s->q, np, s->S, s->rd, st);
-    launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
-    CKS(read_scalars(s));
-    const double zr = s->hS[S_ZR], sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
-    if (!std::isfinite(zr) || !std::isfinite(sq) || !std::isfinite(alpha)) {
-      err = GS_E_NAN;
-      break;
-    }
-    absRes = std::sqrt(s->hS[S_ORR]) * qRecip;
-    relRes = absRes / uzawaRhsScale;
-    if (!std::isfinite(absRes)) {
-      err = GS_E_NAN;
-      break;
-    }
-    if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
-      std::fprintf(stderr, "  %04d uzawa residual norm %12.13e\n", it + 1, relRes);
-    if (lp_on) {
-      hist.push_back(relRes);
-      const size_t k = hist.size();
-      if (k > (size_t)(2 * kWindow) && relRes > 0.95 * hist[k - 1 - kWindow]) {
-        lp_on = false;
-        rep->learned_fallback = it + 1;
-        restart = true;
+      absRes = std::sqrt(s->hS[S_ORR]) * qRecip;
+      relRes = absRes / uzawaRhsScale;
+      if (!std::isfinite(absRes)) {
+        err = GS_E_NAN;
+        break;
       }
-    }
-    if (!(it < s->up.max_iterations)) {
-      err = GS_E_MAXIT;
-      break;
-    }
-    it++;
-  } while (((s->up.use_absolute_tolerance ? absRes : relRes) > tol) || (it < s->up.min_iterations));
+      if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
+        std::fprintf(stderr, "  %04d uzawa residual norm %12.13e\n", it + 1, relRes);
+      if (!(it < s->up.max_iterations)) {
+        err = GS_E_MAXIT;
+        break;
+      }
+      it++;
+    } while (((s->up.use_absolute_tolerance ? absRes : relRes) > tol) || (it < s->up.min_iterations));
+  }
   if (recording && err == GS_OK) {
     // pair (r0, p* - p0)
     CK(hipMemcpyAsync(s->tmpE, s->p, sizeof(double) * np, hipMemcpyDeviceToDevice, st));

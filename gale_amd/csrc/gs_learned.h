@@ -21,7 +21,17 @@ int lpc_precond(gs_lpc* L, const Geo& g, const double* r, const double* Qinv, co
 int lpc_record(gs_lpc_dataset* D, const Geo& g, const double* rlike, const double* elike, const double* Qinv,
                const float* logeta, int proj, double* S, Reducer rd, hipStream_t st);
 
-// Shape check: 3D and every element count divisible by 8.
+// Truncated flexible CG on the Schur complement: with the c previous
+// directions s_j (ring sring[j*n]) and q_j = S s_j (qring), sq[j] = s_j . q_j,
+//   s = z - sum_j (z . q_j / sq[j]) s_j        (written into sring slot `out`)
+constexpr int kMaxWindow = 32;
+void launch_orthogonalise(const double* z, double* sring, const double* qring, const double* sq, int c, int out,
+                          long long n, double* partials, double* beta, hipStream_t st);
+// S[S_ZR] = s.r, S[S_SQ] = s.q, sq_out[0] = s.q
+void launch_flex_alpha(const double* s, const double* r, const double* q, long long n, double* S, double* sq_out,
+                       Reducer rd, hipStream_t st);
+
+// Shape check: 3D and every element count divisible by 16 (the network's pooling depth).
 bool lpc_shape_ok(const Geo& g);
 
 }  // namespace gs

@@ -56,8 +56,9 @@ struct gs_lpc_dataset {
 namespace gs {
 
 bool lpc_shape_ok(const Geo& g) {
-  return g.dim == 3 && g.res[0] % 8 == 0 && g.res[1] % 8 == 0 && g.res[2] % 8 == 0 && g.res[0] >= 8 &&
-         g.res[1] >= 8 && g.res[2] >= 8;
+  const int m = net::kGridMultiple;
+  return g.dim == 3 && g.res[0] % m == 0 && g.res[1] % m == 0 && g.res[2] % m == 0 && g.res[0] >= m &&
+         g.res[1] >= m && g.res[2] >= m;
 }
 
 // ---------------------------------------------------------------------------
@@ -150,6 +151,85 @@ __global__ void k_ds_write(const double* __restrict__ r, const double* __restric
     w[0] = (pe > 0.0 && rms > 0.0) ? (float)(rms * rms / pe) : 0.f;
     pj[0] = proj;
   }
+}
+
+// partial[b][j] = sum over this block's elements of z . qring[j]
+__global__ void k_mdot_partial(const double* __restrict__ z, const double* __restrict__ qring, int c, long long n,
+                               double* __restrict__ partial) {
+  __shared__ double sh[kMaxWindow][kBlock / 64];
+  double acc[kMaxWindow];
+#pragma unroll
+  for (int j = 0; j < kMaxWindow; j++) acc[j] = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const double zi = z[i];
+#pragma unroll
+    for (int j = 0; j < kMaxWindow; j++)
+      if (j < c) acc[j] += zi * qring[(long long)j * n + i];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kMaxWindow; j++) {
+    if (j < c) {  // c is uniform across the block
+      const double v = wave_sum(acc[j]);
+      if (lane == 0) sh[j][wid] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < c) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += sh[threadIdx.x][w];
+    partial[(long long)blockIdx.x * kMaxWindow + threadIdx.x] = t;
+  }
+}
+
+// beta[j] = (sum_b partial[b][j]) / sq[j]
+__global__ void k_mdot_final(const double* __restrict__ partial, int nb, int c, const double* __restrict__ sq,
+                             double* __restrict__ beta) {
+  const int j = threadIdx.x;
+  if (j >= c) return;
+  double t = 0.0;
+  for (int b = 0; b < nb; b++) t += partial[(long long)b * kMaxWindow + j];
+  beta[j] = sq[j] != 0.0 ? t / sq[j] : 0.0;
+}
+
+// s_out = z - sum_j beta[j] s_j  (s_out may be one of the ring slots: each
+// element is read and written by the same thread)
+__global__ void k_orth(const double* __restrict__ z, double* sring, const double* __restrict__ beta, int c, int out,
+                       long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double v = z[i];
+    for (int j = 0; j < c; j++) v -= beta[j] * sring[(long long)j * n + i];
+    sring[(long long)out * n + i] = v;
+  }
+}
+
+__global__ void k_flex_alpha(const double* __restrict__ s, const double* __restrict__ r, const double* __restrict__ q,
+                             long long n, double* S, double* sq_out, Reducer rd) {
+  double v[2] = {0.0, 0.0}, tot[2];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    v[0] += s[i] * r[i];
+    v[1] += s[i] * q[i];
+  }
+  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
+    S[S_ZR] = tot[0];
+    S[S_SQ] = tot[1];
+    sq_out[0] = tot[1];
+  }
+}
+
+void launch_orthogonalise(const double* z, double* sring, const double* qring, const double* sq, int c, int out,
+                          long long n, double* partials, double* beta, hipStream_t st) {
+  if (c > 0) {
+    int nb = reduce_grid(n);
+    hipLaunchKernelGGL(k_mdot_partial, dim3(nb), dim3(kBlock), 0, st, z, qring, c, n, partials);
+    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxWindow), 0, st, partials, nb, c, sq, beta);
+  }
+  hipLaunchKernelGGL(k_orth, dim3(grid_for(n)), dim3(kBlock), 0, st, z, sring, beta, c, out, n);
+}
+
+void launch_flex_alpha(const double* s, const double* r, const double* q, long long n, double* S, double* sq_out,
+                       Reducer rd, hipStream_t st) {
+  hipLaunchKernelGGL(k_flex_alpha, dim3(reduce_grid(n)), dim3(kBlock), 0, st, s, r, q, n, S, sq_out, rd);
 }
 
 void launch_elem_logeta(const Geo& g, const double4* pk, const int* off, float* out, hipStream_t st) {
@@ -262,7 +342,8 @@ extern "C" int32_t gs_lpc_set_params(gs_lpc* L, const float* params) {
 }
 
 static int shape_of(int32_t nx, int32_t ny, int32_t nz, int32_t batch, net::Shape* sh) {
-  if (nx < 8 || ny < 8 || nz < 8 || nx % 8 || ny % 8 || nz % 8 || batch < 1) return GS_E_INVALID;
+  const int m = net::kGridMultiple;
+  if (nx < m || ny < m || nz < m || nx % m || ny % m || nz % m || batch < 1) return GS_E_INVALID;
   *sh = net::Shape{nx, ny, nz, batch};
   return GS_OK;
 }

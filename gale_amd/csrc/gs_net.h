@@ -6,18 +6,14 @@
 // viscosity / 3.  Output: one correction channel c; the preconditioned
 // residual is  z = Q^-1 r + rms(Q^-1 r) * c.
 //
-// Layers (3x3x3 convolutions, zero padding, ReLU except the last):
-//   level 0: 2(4)->16, 16->16                       -> skip0
-//   level 1: avgpool, 16->32, 32->32                -> skip1
-//   level 2: avgpool, 32->64, 64->64                -> skip2
-//   level 3: avgpool, 64->128, 128->64
-//   level 2: nearest up + skip2, 64->64, 64->32
-//   level 1: nearest up + skip1, 32->32, 32->16
-//   level 0: nearest up + skip0, 16->16, 16->1
-// Activations and convolution operands are BF16 (round to nearest even),
-// channels-last [batch][voxel][channel]; every convolution accumulates in
-// FP32 on the matrix cores (v_mfma_f32_16x16x16_bf16); master weights,
-// gradients and Adam moments are FP32.
+// U-Net with kLevels grid levels and channels kChan[level] (3x3x3
+// convolutions, zero padding, ReLU except on the output layer):
+//   encoder level l: [avgpool,] conv C(l-1)->C(l), conv C(l)->C(l)   (skip l)
+//   bottom level L-1: avgpool, conv C(L-2)->C(L-1), conv C(L-1)->C(L-2)
+//   decoder level l: nearest up + skip l, conv C(l)->C(l), conv C(l)->C(l-1)
+//   (level 0 ends with conv C(0)->1), the input has 4 stored channels (2 used).
+// With C = {16, 32, 48, 64, 96} this is 18 convolutions, 1.03 M parameters;
+// grids must be divisible by 2^(kLevels-1) = 16 along each axis.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,8 +26,14 @@
 namespace gs {
 namespace net {
 
-constexpr int kLayers = 14;
-constexpr int kLevels = 4;
+constexpr int kLevels = 5;
+constexpr int kLayers = 4 * kLevels - 2;
+constexpr int kChan[kLevels] = {16, 32, 48, 64, 96};
+constexpr int kGridMultiple = 1 << (kLevels - 1);
+inline int enc_a(int l) { return 2 * l; }
+inline int enc_b(int l) { return 2 * l + 1; }
+inline int dec_a(int l) { return 2 * kLevels + 2 * (kLevels - 2 - l); }
+inline int dec_b(int l) { return dec_a(l) + 1; }
 
 struct LayerSpec {
   int cin;    // input channels as stored (multiple of 4)
@@ -41,7 +43,7 @@ struct LayerSpec {
   int cin_real;  // channels that carry data (fan-in for the initialisation)
 };
 
-extern const LayerSpec kSpec[kLayers];
+extern const LayerSpec* const kSpec;  // kLayers entries
 
 // Parameter layout: per layer W[27*cin][cout] (row kk = tap*cin + ci, tap =
 // (dz+1)*9 + (dy+1)*3 + (dx+1)) followed by bias[cout].
@@ -60,11 +62,11 @@ struct Shape {
 // pass), gradients, weight-gradient partial sums.
 struct Work {
   Shape sh{0, 0, 0, 0};
-  long long BV[kLevels] = {0, 0, 0, 0};
+  long long BV[kLevels] = {};
   uint16_t* x4 = nullptr;             // input, 4 channels
   uint16_t* act[kLayers] = {};        // conv outputs (bf16); the last layer writes out32
-  uint16_t* pool[kLevels] = {};       // pooled inputs of levels 1..3
-  uint16_t* upadd[kLevels] = {};      // up + skip sums of levels 0..2
+  uint16_t* pool[kLevels] = {};       // pooled inputs of levels 1..L-1
+  uint16_t* upadd[kLevels] = {};      // up + skip sums of levels 0..L-2
   float* out32 = nullptr;             // network output c [B*V0]
   uint16_t* g4 = nullptr;             // dL/dc packed to 4 channels
   uint16_t* ga = nullptr;             // gradient scratch (3 buffers of the largest tensor)

@@ -28,11 +28,31 @@ namespace net {
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-const LayerSpec kSpec[kLayers] = {
-    {4, 16, 0, 1, 2},    {16, 16, 0, 1, 16},  {16, 32, 1, 1, 16},  {32, 32, 1, 1, 32},  {32, 64, 2, 1, 32},
-    {64, 64, 2, 1, 64},  {64, 128, 3, 1, 64}, {128, 64, 3, 1, 128}, {64, 64, 2, 1, 64},  {64, 32, 2, 1, 64},
-    {32, 32, 1, 1, 32},  {32, 16, 1, 1, 32},  {16, 16, 0, 1, 16},  {16, 1, 0, 0, 16},
-};
+static LayerSpec make_layer(int l) {
+  // encoder / bottom
+  if (l < 2 * kLevels) {
+    const int lev = l / 2;
+    const bool a = (l % 2) == 0;
+    if (a) {
+      const int cin = lev == 0 ? 4 : kChan[lev - 1];
+      return LayerSpec{cin, kChan[lev], lev, 1, lev == 0 ? 2 : cin};
+    }
+    const int cout = lev == kLevels - 1 ? kChan[kLevels - 2] : kChan[lev];
+    return LayerSpec{kChan[lev], cout, lev, 1, kChan[lev]};
+  }
+  // decoder
+  const int k = l - 2 * kLevels;
+  const int lev = kLevels - 2 - k / 2;
+  if ((k % 2) == 0) return LayerSpec{kChan[lev], kChan[lev], lev, 1, kChan[lev]};
+  return lev == 0 ? LayerSpec{kChan[0], 1, 0, 0, kChan[0]} : LayerSpec{kChan[lev], kChan[lev - 1], lev, 1, kChan[lev]};
+}
+
+static const LayerSpec* build_spec() {
+  static LayerSpec t[kLayers];
+  for (int l = 0; l < kLayers; l++) t[l] = make_layer(l);
+  return t;
+}
+const LayerSpec* const kSpec = build_spec();
 
 long long weight_offset(int l) {
   long long o = 0;
@@ -570,7 +590,8 @@ static constexpr int kChunk = 1024;  // voxels per weight-gradient partial
 
 int ensure_work(Net* n, const Shape& sh) {
   if (n->work.sh == sh && n->work.x4) return GS_OK;
-  if (sh.nx % 8 || sh.ny % 8 || sh.nz % 8 || sh.nx < 8 || sh.ny < 8 || sh.nz < 8 || sh.batch < 1)
+  if (sh.nx % kGridMultiple || sh.ny % kGridMultiple || sh.nz % kGridMultiple || sh.nx < kGridMultiple ||
+      sh.ny < kGridMultiple || sh.nz < kGridMultiple || sh.batch < 1)
     return GS_E_INVALID;
   free_work(n->work);
   Work& w = n->work;
@@ -587,15 +608,11 @@ int ensure_work(Net* n, const Shape& sh) {
     const long long cin_cnt = w.BV[kSpec[l].level] * kSpec[l].cin;
     gmax = cin_cnt > gmax ? cin_cnt : gmax;
   }
-  NCK(al(&w.pool[1], w.BV[1] * 16));
-  NCK(al(&w.pool[2], w.BV[2] * 32));
-  NCK(al(&w.pool[3], w.BV[3] * 64));
-  NCK(al(&w.upadd[2], w.BV[2] * 64));
-  NCK(al(&w.upadd[1], w.BV[1] * 32));
-  NCK(al(&w.upadd[0], w.BV[0] * 16));
-  NCK(al(&w.gskip[2], w.BV[2] * 64));
-  NCK(al(&w.gskip[1], w.BV[1] * 32));
-  NCK(al(&w.gskip[0], w.BV[0] * 16));
+  for (int l = 1; l < kLevels; l++) NCK(al(&w.pool[l], w.BV[l] * kChan[l - 1]));
+  for (int l = 0; l < kLevels - 1; l++) {
+    NCK(al(&w.upadd[l], w.BV[l] * kChan[l]));
+    NCK(al(&w.gskip[l], w.BV[l] * kChan[l]));
+  }
   NCK(hipMalloc(&w.out32, sizeof(float) * w.BV[0]));
   NCK(al(&w.g4, w.BV[0] * 4));
   NCK(al(&w.ga, gmax));
@@ -617,7 +634,8 @@ int ensure_work(Net* n, const Shape& sh) {
 }
 
 // dispatch of the templated convolution kernels
-#define CONV_CASES(X) X(4, 1) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 4) X(64, 2) X(64, 4) X(64, 8) X(128, 4)
+#define CONV_CASES(X) \
+  X(4, 1) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3) X(48, 2) X(48, 3) X(48, 4) X(64, 3) X(64, 4) X(64, 6) X(96, 4)
 
 static int launch_conv(int cin, int nt, bool f32, const uint16_t* in, const void* Wp, const float* bias, void* out,
                        int cout_store, int nx, int ny, int nz, long long BV, int relu, hipStream_t st) {
@@ -679,40 +697,31 @@ int forward(Net* n, const Shape& sh, const float* in2, float* out) {
   hipStream_t st = n->st;
   hipLaunchKernelGGL(k_pack_input, dim3(gridn(w.BV[0])), dim3(kBlock), 0, st, in2, w.x4, w.BV[0]);
   int nx, ny, nz;
-  // encoder
-  if ((e = fwd_layer(n, 0, w.x4, w.act[0]))) return e;
-  if ((e = fwd_layer(n, 1, w.act[0], w.act[1]))) return e;
-  lvl_dims(sh, 0, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[1] * 16)), dim3(kBlock), 0, st, w.act[1], w.pool[1], 16, nx, ny, nz,
-                     (long long)sh.batch);
-  if ((e = fwd_layer(n, 2, w.pool[1], w.act[2]))) return e;
-  if ((e = fwd_layer(n, 3, w.act[2], w.act[3]))) return e;
-  lvl_dims(sh, 1, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[2] * 32)), dim3(kBlock), 0, st, w.act[3], w.pool[2], 32, nx, ny, nz,
-                     (long long)sh.batch);
-  if ((e = fwd_layer(n, 4, w.pool[2], w.act[4]))) return e;
-  if ((e = fwd_layer(n, 5, w.act[4], w.act[5]))) return e;
-  lvl_dims(sh, 2, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[3] * 64)), dim3(kBlock), 0, st, w.act[5], w.pool[3], 64, nx, ny, nz,
-                     (long long)sh.batch);
-  if ((e = fwd_layer(n, 6, w.pool[3], w.act[6]))) return e;
-  if ((e = fwd_layer(n, 7, w.act[6], w.act[7]))) return e;
+  // encoder and bottom level
+  for (int l = 0; l < kLevels; l++) {
+    const uint16_t* in = w.x4;
+    if (l > 0) {
+      lvl_dims(sh, l - 1, &nx, &ny, &nz);
+      const int C = kChan[l - 1];
+      hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[l] * C)), dim3(kBlock), 0, st, w.act[enc_b(l - 1)], w.pool[l], C,
+                         nx, ny, nz, (long long)sh.batch);
+      in = w.pool[l];
+    }
+    if ((e = fwd_layer(n, enc_a(l), in, w.act[enc_a(l)]))) return e;
+    if ((e = fwd_layer(n, enc_b(l), w.act[enc_a(l)], w.act[enc_b(l)]))) return e;
+  }
   // decoder
-  lvl_dims(sh, 2, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[2] * 64)), dim3(kBlock), 0, st, w.act[7], w.act[5], w.upadd[2], 64,
-                     nx, ny, nz, w.BV[2]);
-  if ((e = fwd_layer(n, 8, w.upadd[2], w.act[8]))) return e;
-  if ((e = fwd_layer(n, 9, w.act[8], w.act[9]))) return e;
-  lvl_dims(sh, 1, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[1] * 32)), dim3(kBlock), 0, st, w.act[9], w.act[3], w.upadd[1], 32,
-                     nx, ny, nz, w.BV[1]);
-  if ((e = fwd_layer(n, 10, w.upadd[1], w.act[10]))) return e;
-  if ((e = fwd_layer(n, 11, w.act[10], w.act[11]))) return e;
-  lvl_dims(sh, 0, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[0] * 16)), dim3(kBlock), 0, st, w.act[11], w.act[1], w.upadd[0], 16,
-                     nx, ny, nz, w.BV[0]);
-  if ((e = fwd_layer(n, 12, w.upadd[0], w.act[12]))) return e;
-  if ((e = fwd_layer(n, 13, w.act[12], out ? out : w.out32))) return e;
+  int cur = enc_b(kLevels - 1);
+  for (int l = kLevels - 2; l >= 0; l--) {
+    lvl_dims(sh, l, &nx, &ny, &nz);
+    const int C = kChan[l];
+    hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[l] * C)), dim3(kBlock), 0, st, w.act[cur], w.act[enc_b(l)],
+                       w.upadd[l], C, nx, ny, nz, w.BV[l]);
+    if ((e = fwd_layer(n, dec_a(l), w.upadd[l], w.act[dec_a(l)]))) return e;
+    void* o = l > 0 ? (void*)w.act[dec_b(l)] : (void*)(out ? out : w.out32);
+    if ((e = fwd_layer(n, dec_b(l), w.act[dec_a(l)], o))) return e;
+    cur = dec_b(l);
+  }
   return GS_OK;
 }
 
@@ -749,49 +758,38 @@ int backward(Net* n, const Shape& sh, const float* gout) {
   int e, nx, ny, nz;
   uint16_t *A = w.ga, *B = w.gb, *C = w.gc;
   hipLaunchKernelGGL(k_pack_gout, dim3(gridn(w.BV[0])), dim3(kBlock), 0, st, gout, w.g4, w.BV[0]);
-  // level 0 decoder: out = conv13(act12); act12 = relu(conv12(upadd0))
-  if ((e = bwd_layer(n, 13, w.act[12], w.g4, 4, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[0] * 16)), dim3(kBlock), 0, st, A, w.act[12], w.BV[0] * 16);
-  if ((e = bwd_layer(n, 12, w.upadd[0], A, 16, w.gskip[0]))) return e;  // gradient of upadd0 = of skip0
-  lvl_dims(sh, 0, &nx, &ny, &nz);
-  // upadd0 = up(act11) + act1
-  hipLaunchKernelGGL(k_up_bwd, dim3(gridn(w.BV[1] * 16)), dim3(kBlock), 0, st, w.gskip[0], w.act[11], B, 16, nx, ny,
-                     nz, (long long)sh.batch);
-  if ((e = bwd_layer(n, 11, w.act[10], B, 16, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[1] * 32)), dim3(kBlock), 0, st, A, w.act[10], w.BV[1] * 32);
-  if ((e = bwd_layer(n, 10, w.upadd[1], A, 32, w.gskip[1]))) return e;
-  lvl_dims(sh, 1, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_up_bwd, dim3(gridn(w.BV[2] * 32)), dim3(kBlock), 0, st, w.gskip[1], w.act[9], B, 32, nx, ny,
-                     nz, (long long)sh.batch);
-  if ((e = bwd_layer(n, 9, w.act[8], B, 32, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[2] * 64)), dim3(kBlock), 0, st, A, w.act[8], w.BV[2] * 64);
-  if ((e = bwd_layer(n, 8, w.upadd[2], A, 64, w.gskip[2]))) return e;
-  lvl_dims(sh, 2, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_up_bwd, dim3(gridn(w.BV[3] * 64)), dim3(kBlock), 0, st, w.gskip[2], w.act[7], B, 64, nx, ny,
-                     nz, (long long)sh.batch);
-  // bottleneck
-  if ((e = bwd_layer(n, 7, w.act[6], B, 64, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[3] * 128)), dim3(kBlock), 0, st, A, w.act[6], w.BV[3] * 128);
-  if ((e = bwd_layer(n, 6, w.pool[3], A, 128, C))) return e;  // C = gradient of pool3 (64 ch, level 3)
-  // act5 feeds pool3 and skip2
-  lvl_dims(sh, 2, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(gridn(w.BV[2] * 64)), dim3(kBlock), 0, st, C, w.gskip[2], w.act[5], B, 64, nx,
-                     ny, nz, w.BV[2]);
-  if ((e = bwd_layer(n, 5, w.act[4], B, 64, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[2] * 64)), dim3(kBlock), 0, st, A, w.act[4], w.BV[2] * 64);
-  if ((e = bwd_layer(n, 4, w.pool[2], A, 64, C))) return e;  // gradient of pool2 (32 ch, level 2)
-  lvl_dims(sh, 1, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(gridn(w.BV[1] * 32)), dim3(kBlock), 0, st, C, w.gskip[1], w.act[3], B, 32, nx,
-                     ny, nz, w.BV[1]);
-  if ((e = bwd_layer(n, 3, w.act[2], B, 32, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[1] * 32)), dim3(kBlock), 0, st, A, w.act[2], w.BV[1] * 32);
-  if ((e = bwd_layer(n, 2, w.pool[1], A, 32, C))) return e;  // gradient of pool1 (16 ch, level 1)
-  lvl_dims(sh, 0, &nx, &ny, &nz);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(gridn(w.BV[0] * 16)), dim3(kBlock), 0, st, C, w.gskip[0], w.act[1], B, 16, nx,
-                     ny, nz, w.BV[0]);
-  if ((e = bwd_layer(n, 1, w.act[0], B, 16, A))) return e;
-  hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[0] * 16)), dim3(kBlock), 0, st, A, w.act[0], w.BV[0] * 16);
-  if ((e = bwd_layer(n, 0, w.x4, A, 16, nullptr))) return e;
+  // decoder, finest level first; B holds the ReLU-masked gradient of the
+  // level's output (level 0: the packed dL/dc)
+  for (int l = 0; l <= kLevels - 2; l++) {
+    const int a = dec_a(l), b = dec_b(l);
+    const int Cl = kChan[l];
+    if ((e = bwd_layer(n, b, w.act[a], l == 0 ? w.g4 : B, l == 0 ? 4 : kSpec[b].cout, A))) return e;
+    hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[l] * Cl)), dim3(kBlock), 0, st, A, w.act[a], w.BV[l] * Cl);
+    if ((e = bwd_layer(n, a, w.upadd[l], A, Cl, w.gskip[l]))) return e;
+    // upadd[l] = up(src) + skip l
+    const int src = l == kLevels - 2 ? enc_b(kLevels - 1) : dec_b(l + 1);
+    lvl_dims(sh, l, &nx, &ny, &nz);
+    hipLaunchKernelGGL(k_up_bwd, dim3(gridn(w.BV[l + 1] * Cl)), dim3(kBlock), 0, st, w.gskip[l], w.act[src], B, Cl,
+                       nx, ny, nz, (long long)sh.batch);
+  }
+  // bottom level
+  {
+    const int L = kLevels - 1;
+    if ((e = bwd_layer(n, enc_b(L), w.act[enc_a(L)], B, kChan[L - 1], A))) return e;
+    hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[L] * kChan[L])), dim3(kBlock), 0, st, A, w.act[enc_a(L)],
+                       w.BV[L] * kChan[L]);
+    if ((e = bwd_layer(n, enc_a(L), w.pool[L], A, kChan[L], C))) return e;  // C: gradient of pool[L]
+  }
+  // encoder, coarse to fine: act[enc_b(l)] feeds pool[l+1] and skip l
+  for (int l = kLevels - 2; l >= 0; l--) {
+    const int Cl = kChan[l];
+    lvl_dims(sh, l, &nx, &ny, &nz);
+    hipLaunchKernelGGL(k_pool_bwd, dim3(gridn(w.BV[l] * Cl)), dim3(kBlock), 0, st, C, w.gskip[l], w.act[enc_b(l)], B,
+                       Cl, nx, ny, nz, w.BV[l]);
+    if ((e = bwd_layer(n, enc_b(l), w.act[enc_a(l)], B, Cl, A))) return e;
+    hipLaunchKernelGGL(k_relu_mask, dim3(gridn(w.BV[l] * Cl)), dim3(kBlock), 0, st, A, w.act[enc_a(l)], w.BV[l] * Cl);
+    if ((e = bwd_layer(n, enc_a(l), l > 0 ? w.pool[l] : w.x4, A, Cl, l > 0 ? C : nullptr))) return e;
+  }
   return GS_OK;
 }
 

@@ -7,10 +7,11 @@
  * UzawaPreconditionerTerm.cxx:156-239 only.  Here a 3D convolutional
  * encoder-decoder (BF16 matrix-core convolutions, FP32 accumulation) maps the
  * element log10 viscosity and the diagonally preconditioned residual Q^-1 r to
- * a correction: z = Q^-1 r + rms(Q^-1 r) * net(Q^-1 r / rms, log10 eta / 3).
- * The Uzawa loop then runs as flexible (Polak-Ribiere) preconditioned CG; the
- * first preconditioned step is the network's initial pressure guess (scaled
- * by the line search).  When the residual stagnates, or z.r <= 0, the solve
+ * a correction (5-level U-Net, channels 16/32/48/64/96, 1.03 M parameters): z = Q^-1 r + rms(Q^-1 r) * net(Q^-1 r / rms, log10 eta / 3).
+ * The Uzawa loop then runs as truncated flexible CG (each direction
+ * S-orthogonalised against the previous flexible_window ones); the first
+ * preconditioned step is the network's initial pressure guess (scaled by the
+ * line search).  When the residual stagnates, or z.r <= 0, the solve
  * falls back to the plain diagonal Q and continues to the same tolerance.
  *
  * The network is trained on the device (forward and backward kernels of the
@@ -19,7 +20,7 @@
  *   (q_k, s_k):    Krylov pairs q_k = G^T K^-1 G s_k of the Uzawa iterations.
  * Both satisfy S e = r exactly (up to the inner solve tolerance).
  *
- * Requirements: 3D problems with element counts divisible by 8 per axis.
+ * Requirements: 3D problems with element counts divisible by 16 per axis.
  * Device pointers are marked d_; everything else is host memory.
  */
 #ifndef GALE_LEARNED_H

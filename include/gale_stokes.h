@@ -85,7 +85,7 @@ typedef struct gs_uzawa_params {
   int32_t max_iterations;       /* "maxIterations" (1000) */
   int32_t min_iterations;       /* "minIterations" (1) */
   int32_t use_absolute_tolerance; /* "useAbsoluteTolerance" (0) */
-  int32_t preconditioner;       /* "Preconditioner": GS_PC_NONE or GS_PC_DIAGONAL */
+  int32_t preconditioner;       /* "Preconditioner": GS_PC_NONE, GS_PC_DIAGONAL (or GS_PC_LEARNED, gale_learned.h) */
   int32_t precision;            /* GS_PREC_FP64 or GS_PREC_MIXED (FP32 K-apply preconditioning FP64 CG) */
   int32_t inner_max_iterations; /* cap per inner velocity solve (0 = 50000) */
   double inner_rtol_factor;     /* inner rtol = factor * tolerance (reference: 0.1, :591,596) */
@@ -94,6 +94,14 @@ typedef struct gs_uzawa_params {
   int32_t check_every;          /* inner CG convergence check interval (0 = 4) */
   int32_t inner_preconditioner; /* GS_INNER_JACOBI or GS_INNER_MULTIGRID */
   int32_t mg_smooth;            /* Chebyshev-Jacobi steps before and after each coarse correction (0 = 2) */
+  int32_t flexible_window;      /* learned preconditioner: previous directions each new one is
+                                   S-orthogonalised against (0 = 16, max 32) */
+  int32_t stagnation_window;    /* learned preconditioner: stagnation = the residual drops by less than 5%
+                                   over this many iterations (0 = 25) */
+  int32_t learned_restarts;     /* learned preconditioner: on stagnation, restart the flexible iteration
+                                   (keeping the network) up to this many times before falling back to Q */
+  int32_t learned_q_every;      /* learned preconditioner: every k-th iteration preconditions with Q alone
+                                   (0 = never) */
 } gs_uzawa_params;
 
 /* SystemLinearEquations nonlinear keys (SystemLinearEquations.cxx:346-356). */

@@ -3,15 +3,13 @@ only: imported by tests/, never by the product).
 
 A numpy restatement of the 3D convolutional encoder-decoder that the library
 runs on the matrix cores (no counterpart in the reference, whose Uzawa loop
-uses the diagonal Q of UzawaPreconditionerTerm.cxx:156-239 only):
+uses the diagonal Q of UzawaPreconditionerTerm.cxx:156-239 only).  U-Net with
+LEVELS grid levels and channels CHAN[level]:
 
-  level 0: conv 4->16, conv 16->16                      (skip0)
-  level 1: avgpool, conv 16->32, conv 32->32            (skip1)
-  level 2: avgpool, conv 32->64, conv 64->64            (skip2)
-  level 3: avgpool, conv 64->128, conv 128->64
-  level 2: nearest up + skip2, conv 64->64, conv 64->32
-  level 1: nearest up + skip1, conv 32->32, conv 32->16
-  level 0: nearest up + skip0, conv 16->16, conv 16->1 (no ReLU)
+  encoder level l: [avgpool,] conv C(l-1)->C(l), conv C(l)->C(l)    (skip l)
+  bottom level L-1: avgpool, conv C(L-2)->C(L-1), conv C(L-1)->C(L-2)
+  decoder level l: nearest up + skip l, conv C(l)->C(l), conv C(l)->C(l-1)
+  (level 0 ends with conv C(0)->1, no ReLU); the input has 4 stored channels.
 
 3x3x3 convolutions with zero padding, channels last [B, z, y, x, C].
 Parameters: per layer W[27*cin, cout] (row = tap*cin + ci, tap = (dz+1)*9 +
@@ -27,12 +25,43 @@ from __future__ import annotations
 
 import numpy as np
 
-# (cin stored, cout, level, relu, cin carrying data)
-SPEC = [
-    (4, 16, 0, 1, 2), (16, 16, 0, 1, 16), (16, 32, 1, 1, 16), (32, 32, 1, 1, 32), (32, 64, 2, 1, 32),
-    (64, 64, 2, 1, 64), (64, 128, 3, 1, 64), (128, 64, 3, 1, 128), (64, 64, 2, 1, 64), (64, 32, 2, 1, 64),
-    (32, 32, 1, 1, 32), (32, 16, 1, 1, 32), (16, 16, 0, 1, 16), (16, 1, 0, 0, 16),
-]
+LEVELS = 5
+CHAN = [16, 32, 48, 64, 96]
+
+
+def enc_a(l):
+    return 2 * l
+
+
+def enc_b(l):
+    return 2 * l + 1
+
+
+def dec_a(l):
+    return 2 * LEVELS + 2 * (LEVELS - 2 - l)
+
+
+def dec_b(l):
+    return dec_a(l) + 1
+
+
+def _make_spec():
+    """(cin stored, cout, level, relu, cin carrying data) per layer."""
+    spec = []
+    for lev in range(LEVELS):
+        cin = 4 if lev == 0 else CHAN[lev - 1]
+        spec.append((cin, CHAN[lev], lev, 1, 2 if lev == 0 else cin))
+        cout = CHAN[LEVELS - 2] if lev == LEVELS - 1 else CHAN[lev]
+        spec.append((CHAN[lev], cout, lev, 1, CHAN[lev]))
+    for lev in range(LEVELS - 2, -1, -1):
+        spec.append((CHAN[lev], CHAN[lev], lev, 1, CHAN[lev]))
+        spec.append((CHAN[lev], CHAN[lev - 1], lev, 1, CHAN[lev]) if lev else (CHAN[0], 1, 0, 0, CHAN[0]))
+    return spec
+
+
+SPEC = _make_spec()
+NL = len(SPEC)
+GRID_MULTIPLE = 2 ** (LEVELS - 1)
 
 TAPS = [(t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1) for t in range(27)]  # (dx, dy, dz)
 
@@ -127,9 +156,12 @@ def forward(params, in2, mimic=True):
     Ws = [rnd(P[a]).reshape(27 * SPEC[l][0], SPEC[l][1]) for l, (a, _) in enumerate(sl)]
     bs = [P[b] for (_, b) in sl]
     B, nz, ny, nx, _ = in2.shape
+    assert nz % GRID_MULTIPLE == 0 and ny % GRID_MULTIPLE == 0 and nx % GRID_MULTIPLE == 0
     x4 = np.zeros((B, nz, ny, nx, 4))
     x4[..., :2] = rnd(in2)
-    act = [None] * 14
+    act = [None] * NL
+    pools = [None] * LEVELS
+    upadd = [None] * LEVELS
 
     def L(l, x, last=False):
         cin, cout, _, relu, _ = SPEC[l]
@@ -138,27 +170,24 @@ def forward(params, in2, mimic=True):
             y = np.maximum(y, 0.0)
         return y if last else rnd(y)
 
-    act[0] = L(0, x4)
-    act[1] = L(1, act[0])
-    p1 = pool(act[1], rnd)
-    act[2] = L(2, p1)
-    act[3] = L(3, act[2])
-    p2 = pool(act[3], rnd)
-    act[4] = L(4, p2)
-    act[5] = L(5, act[4])
-    p3 = pool(act[5], rnd)
-    act[6] = L(6, p3)
-    act[7] = L(7, act[6])
-    u2 = rnd(up(act[7]) + act[5])
-    act[8] = L(8, u2)
-    act[9] = L(9, act[8])
-    u1 = rnd(up(act[9]) + act[3])
-    act[10] = L(10, u1)
-    act[11] = L(11, act[10])
-    u0 = rnd(up(act[11]) + act[1])
-    act[12] = L(12, u0)
-    c = L(13, act[12], last=True)[..., 0]
-    cache = dict(x4=x4, act=act, p1=p1, p2=p2, p3=p3, u2=u2, u1=u1, u0=u0, Ws=Ws, mimic=mimic)
+    for lev in range(LEVELS):
+        x = x4
+        if lev:
+            pools[lev] = pool(act[enc_b(lev - 1)], rnd)
+            x = pools[lev]
+        act[enc_a(lev)] = L(enc_a(lev), x)
+        act[enc_b(lev)] = L(enc_b(lev), act[enc_a(lev)])
+    cur = enc_b(LEVELS - 1)
+    c = None
+    for lev in range(LEVELS - 2, -1, -1):
+        upadd[lev] = rnd(up(act[cur]) + act[enc_b(lev)])
+        act[dec_a(lev)] = L(dec_a(lev), upadd[lev])
+        if lev:
+            act[dec_b(lev)] = L(dec_b(lev), act[dec_a(lev)])
+        else:
+            c = L(dec_b(0), act[dec_a(0)], last=True)[..., 0]
+        cur = dec_b(lev)
+    cache = dict(x4=x4, act=act, pools=pools, upadd=upadd, Ws=Ws, mimic=mimic)
     return c, cache
 
 
@@ -168,7 +197,7 @@ def backward(params, cache, gout):
     rnd = bf16 if mimic else _id
     sl, n = layer_slices()
     grads = np.zeros(n)
-    Ws, act = cache["Ws"], cache["act"]
+    Ws, act, pools, upadd = cache["Ws"], cache["act"], cache["pools"], cache["upadd"]
     mask = lambda g, a: np.where(a > 0, g, 0.0)  # noqa: E731
 
     def bw(l, x, g, need_dx=True):
@@ -180,26 +209,20 @@ def backward(params, cache, gout):
 
     g4 = np.zeros(gout.shape + (1,))
     g4[..., 0] = rnd(gout)
-    g = mask(bw(13, act[12], g4), act[12])
-    gu0 = bw(12, cache["u0"], g)
-    g = mask(rnd(down_sum(gu0)), act[11])
-    g = mask(bw(11, act[10], g), act[10])
-    gu1 = bw(10, cache["u1"], g)
-    g = mask(rnd(down_sum(gu1)), act[9])
-    g = mask(bw(9, act[8], g), act[8])
-    gu2 = bw(8, cache["u2"], g)
-    g = mask(rnd(down_sum(gu2)), act[7])
-    g = mask(bw(7, act[6], g), act[6])
-    gp3 = bw(6, cache["p3"], g)
-    g = mask(rnd(up(gp3) * 0.125 + gu2), act[5])
-    g = mask(bw(5, act[4], g), act[4])
-    gp2 = bw(4, cache["p2"], g)
-    g = mask(rnd(up(gp2) * 0.125 + gu1), act[3])
-    g = mask(bw(3, act[2], g), act[2])
-    gp1 = bw(2, cache["p1"], g)
-    g = mask(rnd(up(gp1) * 0.125 + gu0), act[1])
-    g = mask(bw(1, act[0], g), act[0])
-    bw(0, cache["x4"], g, need_dx=False)
+    gskip = [None] * LEVELS
+    g = g4
+    for lev in range(LEVELS - 1):
+        ga = mask(bw(dec_b(lev), act[dec_a(lev)], g), act[dec_a(lev)])
+        gskip[lev] = bw(dec_a(lev), upadd[lev], ga)
+        src = enc_b(LEVELS - 1) if lev == LEVELS - 2 else dec_b(lev + 1)
+        g = mask(rnd(down_sum(gskip[lev])), act[src])
+    Lb = LEVELS - 1
+    ga = mask(bw(enc_b(Lb), act[enc_a(Lb)], g), act[enc_a(Lb)])
+    gpool = bw(enc_a(Lb), pools[Lb], ga)
+    for lev in range(LEVELS - 2, -1, -1):
+        g = mask(rnd(up(gpool) * 0.125 + gskip[lev]), act[enc_b(lev)])
+        ga = mask(bw(enc_b(lev), act[enc_a(lev)], g), act[enc_a(lev)])
+        gpool = bw(enc_a(lev), pools[lev] if lev else cache["x4"], ga, need_dx=lev > 0)
     return grads
 
 

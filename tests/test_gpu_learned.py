@@ -29,8 +29,9 @@ def _net_with_live_output(gale, seed=3, scale=100.0):
     net = gale.LearnedPreconditioner(seed=seed)
     P = net.get_params()
     sl = N.layer_slices()[0]
-    P[sl[13][0]] *= scale  # the output layer starts near zero; make it carry signal
-    P[sl[13][1]] = 0.05
+    last = N.NL - 1
+    P[sl[last][0]] *= scale  # the output layer starts near zero; make it carry signal
+    P[sl[last][1]] = 0.05
     net.set_params(P)
     return net, net.get_params()
 
@@ -42,7 +43,7 @@ def test_mfma_operand_layout(gale):
 def test_forward_matches_oracle(gale):
     torch = _torch()
     net, P = _net_with_live_output(gale)
-    B, nz, ny, nx = 2, 8, 16, 24
+    B, nz, ny, nx = 2, 16, 16, 32
     x = _rand_in(B, nz, ny, nx, 11)
     d_in = torch.tensor(x, device="cuda")
     d_out = torch.empty((B, nz, ny, nx), dtype=torch.float32, device="cuda")
@@ -55,30 +56,61 @@ def test_forward_matches_oracle(gale):
     assert rel(out, exact) < 5e-2
 
 
-def test_backward_matches_oracle(gale):
+def _grads(gale, net, P, x, gout):
     torch = _torch()
-    net, P = _net_with_live_output(gale, seed=5)
-    B, nz, ny, nx = 2, 16, 8, 8
-    x = _rand_in(B, nz, ny, nx, 12)
-    gout = np.random.default_rng(13).standard_normal((B, nz, ny, nx)).astype(np.float32)
     g = net.backward(torch.tensor(x, device="cuda"), torch.tensor(gout, device="cuda"))
     _, cache = N.forward(P, x, mimic=True)
-    gref = N.backward(P, cache, gout.astype(np.float64))
+    return g, N.backward(P, cache, gout.astype(np.float64))
+
+
+def test_backward_matches_oracle_affine_regime(gale):
+    """Biases large enough that every ReLU is active: the network is affine,
+    so device and oracle gradients differ only by accumulation-order rounding;
+    any indexing or operand-layout error in the forward, data-gradient or
+    weight-gradient kernels shows as an O(1) difference."""
+    net = gale.LearnedPreconditioner(seed=5)
+    P = net.get_params()
     sl = N.layer_slices()[0]
     for l, (a, b) in enumerate(sl):
-        assert rel(g[a], gref[a]) < 3e-2, (l, "W", rel(g[a], gref[a]))
-        assert rel(g[b], gref[b]) < 3e-2, (l, "b", rel(g[b], gref[b]))
-    # and the exact-arithmetic gradient within the BF16 budget
+        P[a] *= 0.02
+        P[b] = 5.0 if l < N.NL - 1 else 0.05
+    net.set_params(P)
+    P = net.get_params()
+    B, nz, ny, nx = 2, 16, 32, 16
+    x = _rand_in(B, nz, ny, nx, 12)
+    gout = np.random.default_rng(13).standard_normal((B, nz, ny, nx)).astype(np.float32)
+    _, cache = N.forward(P, x, mimic=False)
+    assert all((a is None) or (a.min() > 0) for a in cache["act"][:-1])  # every ReLU active
+    g, gref = _grads(gale, net, P, x, gout)
+    assert rel(g, gref) < 2e-3, rel(g, gref)
+    for l, (a, b) in enumerate(sl):
+        assert rel(g[a], gref[a]) < 1e-2, (l, "W", rel(g[a], gref[a]))
+        assert rel(g[b], gref[b]) < 1e-2, (l, "b", rel(g[b], gref[b]))
+
+
+def test_backward_matches_oracle_relu_regime(gale):
+    """He-initialised network (ReLU masks active): BF16 rounding and mask
+    flips make the gradient sensitive to last-bit differences, so the bound is
+    the oracle's own change under a 1e-6 relative input perturbation."""
+    net, P = _net_with_live_output(gale, seed=5)
+    B, nz, ny, nx = 2, 16, 32, 16
+    x = _rand_in(B, nz, ny, nx, 12)
+    gout = np.random.default_rng(13).standard_normal((B, nz, ny, nx)).astype(np.float32)
+    g, gref = _grads(gale, net, P, x, gout)
+    xp = (x * (1 + 1e-6 * np.random.default_rng(14).standard_normal(x.shape))).astype(np.float32)
+    _, cp = N.forward(P, xp, mimic=True)
+    floor = rel(N.backward(P, cp, gout.astype(np.float64)), gref)
+    assert rel(g, gref) < max(2.0 * floor, 1e-2), (rel(g, gref), floor)
     _, cex = N.forward(P, x, mimic=False)
     gex = N.backward(P, cex, gout.astype(np.float64))
-    assert rel(g, gex) < 0.1
+    assert rel(g, gex) < 0.25
 
 
 def test_loss_matches_oracle(gale):
     torch = _torch()
     net = gale.LearnedPreconditioner(seed=1)
     rng = np.random.default_rng(4)
-    B, nz, ny, nx = 3, 8, 8, 16
+    B, nz, ny, nx = 3, 16, 16, 16
     c = rng.standard_normal((B, nz, ny, nx)).astype(np.float32)
     a = rng.standard_normal((B, nz, ny, nx)).astype(np.float32)
     w = np.array([0.5, 2.0, 1.0], np.float32)
@@ -99,11 +131,11 @@ def test_recorded_pairs_satisfy_schur_relation(gale, oracle):
     """Each pair (input t = Q^-1 r / rms, target (e - t) / rms) satisfies
     S (t + target) = Q t with S = G^T K^-1 G of the oracle's eliminated system."""
     from scipy.sparse.linalg import splu
-    cfg = _rv(8, 2001)
+    cfg = _rv(16, 2001, ppc=2)
     pb = gale.Problem(**cfg.problem_kwargs())
     s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(tolerance=1e-10, inner_rtol_override=1e-13,
                                                           maxIterations=5000, inner_max_iterations=100000))
-    ds = gale.PairDataset(8, 32)
+    ds = gale.PairDataset(16, 32)
     s.record_pairs(ds, krylov_every=4)
     _, _, rep = s.Solve()
     assert rep.status == 0

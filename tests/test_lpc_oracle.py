@@ -57,7 +57,7 @@ def test_linear_blocks_are_adjoint():
     assert abs(np.sum(N.pool(x, N._id) * c) - np.sum(x * N.up(c) * 0.125)) < 1e-5
 
 
-@pytest.mark.parametrize("shape", [(1, 8, 8, 8), (2, 8, 16, 8)])
+@pytest.mark.parametrize("shape", [(1, 16, 16, 16), (2, 16, 16, 32)])
 def test_backward_matches_finite_differences(shape):
     rng = np.random.default_rng(3)
     B, nz, ny, nx = shape
