@@ -422,7 +422,7 @@ class Problem:
 def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,
                  Preconditioner=True, precision=PREC_FP64, inner_max_iterations=0, inner_rtol_factor=0.1,
                  inner_rtol_override=0.0, monitor=False, check_every=4, inner_preconditioner=INNER_MULTIGRID,
-                 mg_smooth=2, learned=False, flexible_window=16, stagnation_window=25, learned_restarts=5,
+                 mg_smooth=2, learned=False, flexible_window=16, stagnation_window=50, learned_restarts=5,
                  learned_q_every=0) -> UzawaParams:
     """Stokes_SLE_UzawaSolver parameters under their XML names
     (Stokes_SLE_UzawaSolver.cxx:251-257 defaults) plus the inner-solve controls.

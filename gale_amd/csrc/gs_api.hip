@@ -1059,20 +1059,26 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   int base = 0;  // first iteration of the current orthogonalisation window (reset on restart / fallback)
   int restarts_left = s->up.learned_restarts > 0 ? s->up.learned_restarts : 0;
   const int q_every = s->up.learned_q_every > 0 ? s->up.learned_q_every : 0;
+  constexpr int kMaxBad = 5;
+  int bad_left = kMaxBad;
   std::vector<double> hist;
   do {
-    const bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
+    bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
     if (use_net) {
       CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, nullptr, s->z, s->S, s->rd, st));
       CKS(read_scalars(s));
       const double zr = s->hS[S_ZR];
       if (!(zr > 0.0) || !std::isfinite(zr)) {
-        lp_on = false;
-        rep->learned_fallback = it + 1;
-        base = it;
+        // not a descent direction: this iteration takes Q^-1 r; after
+        // kMaxBad such iterations the network is dropped for the solve
+        use_net = false;
+        if (--bad_left <= 0) {
+          lp_on = false;
+          rep->learned_fallback = it + 1;
+        }
       }
     }
-    if (!use_net || !lp_on) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
+    if (!use_net) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
     const int k = it - base;
     const int c = k < m ? k : m;
     const int slot = k % m;
