@@ -1025,10 +1025,16 @@ static double dot_host(gs_solver* s, const double* a, const double* b, long long
 
 // Uzawa with the learned Schur preconditioner: truncated flexible CG.  Each
 // preconditioned residual z = M(r) (network correction of Q^-1 r) is
-// S-orthogonalised against the last m directions; alpha = s.r / s.q.  When
-// z.r <= 0 or the residual falls by less than 5% over the stagnation window,
-// the rest of the solve preconditions with Q^-1 (same loop, same tolerance).
-// Stopping test, residual and firewalls are those of the reference loop.
+// S-orthogonalised against the last m directions; alpha = s.r / s.q.  A
+// network output with z.r <= 0 is replaced by Q^-1 r for that iteration; on
+// stagnation (residual down by less than 5% over the window) the iteration
+// restarts, and after learned_restarts restarts the rest of the solve uses Q.
+// With inner_relax_max > 0 the inner solves are relaxed as the outer
+// residual falls (inner rtol = min(inner_relax_max, irt |r_0| / |r_k|), the
+// inexact-Krylov relaxation rule); the recursive residual then drifts from
+// the true one, so convergence is re-checked with an accurate solve of
+// u = K^-1 (f - G p), and the loop resumes from the true residual if needed.
+// Stopping test, residual definition and firewalls are the reference's.
 static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsScale, double irt, bool recording,
                                   int proj, int* it_io, long long* inner_io, double* relRes_io, double* absRes_io) {
   gs_problem* pb = s->pb;
@@ -1039,6 +1045,7 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   int m = s->up.flexible_window > 0 ? s->up.flexible_window : 16;
   if (m > kMaxWindow) m = kMaxWindow;
   const int win = s->up.stagnation_window > 0 ? s->up.stagnation_window : 25;
+  const double relax = s->up.inner_relax_max > 0 ? s->up.inner_relax_max : 0.0;
   if (s->fl_m != m) {
     dfree(s->fl_s);
     dfree(s->fl_q);
@@ -1053,80 +1060,104 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   double* sqv = beta + kMaxWindow;
   int it = *it_io;
   long long inner = *inner_io;
-  double relRes = 0.0, absRes = 0.0;
   int32_t err = GS_OK;
   bool lp_on = true;
-  int base = 0;  // first iteration of the current orthogonalisation window (reset on restart / fallback)
   int restarts_left = s->up.learned_restarts > 0 ? s->up.learned_restarts : 0;
   const int q_every = s->up.learned_q_every > 0 ? s->up.learned_q_every : 0;
   constexpr int kMaxBad = 5;
   int bad_left = kMaxBad;
-  std::vector<double> hist;
-  do {
-    bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
-    if (use_net) {
-      CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, nullptr, s->z, s->S, s->rd, st));
+  int32_t e2 = GS_OK;
+  double relRes = std::sqrt(dot_host(s, s->r, s->r, np, &e2)) * qRecip / uzawaRhsScale;
+  CKS(e2);
+  double absRes = relRes * uzawaRhsScale;
+  constexpr int kMaxRounds = 4;
+  for (int round = 0; round < kMaxRounds; round++) {
+    const double relax_ref = relRes;  // residual the relaxation is measured from
+    int base = it;  // first iteration of the current orthogonalisation window
+    std::vector<double> hist;
+    bool converged = (s->up.use_absolute_tolerance ? absRes : relRes) <= tol && it >= s->up.min_iterations;
+    while (!converged) {
+      bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
+      if (use_net) {
+        CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, nullptr, s->z, s->S, s->rd, st));
+        CKS(read_scalars(s));
+        const double zr = s->hS[S_ZR];
+        if (!(zr > 0.0) || !std::isfinite(zr)) {
+          // not a descent direction: this iteration takes Q^-1 r; after
+          // kMaxBad such iterations the network is dropped for the solve
+          use_net = false;
+          if (--bad_left <= 0) {
+            lp_on = false;
+            rep->learned_fallback = it + 1;
+          }
+        }
+      }
+      if (!use_net) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
+      const int k = it - base;
+      const int c = k < m ? k : m;
+      const int slot = k % m;
+      double* sk = s->fl_s + (long long)slot * np;
+      double* qk = s->fl_q + (long long)slot * np;
+      launch_orthogonalise(s->z, s->fl_s, s->fl_q, sqv, c, slot, np, partials, beta, st);
+      launch_apply_G(pb->g, sk, s->fT, pb->bcflag, st);
+      double irt_k = irt;
+      if (relax > 0 && relRes > 0) irt_k = std::min(relax, std::max(irt, irt * relax_ref / relRes));
+      CKS(inner_solve(s, s->fT, s->vstar, irt_k, &inner));
+      launch_apply_Gt(pb->g, s->vstar, qk, st);
+      if (recording && s->rec_every > 0 && it % s->rec_every == 0)
+        CKS(lpc_record(s->rec, pb->g, qk, sk, s->Qinv, s->logeta, proj, s->S, s->rd, st));
+      launch_flex_alpha(sk, s->r, qk, np, s->S, sqv + slot, s->rd, st);
+      launch_uz_update(s->p, sk, s->r, qk, np, s->S, s->rd, st);
+      launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
       CKS(read_scalars(s));
-      const double zr = s->hS[S_ZR];
-      if (!(zr > 0.0) || !std::isfinite(zr)) {
-        // not a descent direction: this iteration takes Q^-1 r; after
-        // kMaxBad such iterations the network is dropped for the solve
-        use_net = false;
-        if (--bad_left <= 0) {
-          lp_on = false;
-          rep->learned_fallback = it + 1;
+      const double sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
+      if (!std::isfinite(sq) || !std::isfinite(alpha)) {
+        err = GS_E_NAN;
+        break;
+      }
+      absRes = std::sqrt(s->hS[S_ORR]) * qRecip;
+      relRes = absRes / uzawaRhsScale;
+      if (!std::isfinite(absRes)) {
+        err = GS_E_NAN;
+        break;
+      }
+      if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
+        std::fprintf(stderr, "  %04d uzawa residual norm %12.13e%s\n", it + 1, relRes, lp_on ? " (learned)" : "");
+      if (lp_on) {
+        hist.push_back(relRes);
+        const size_t h = hist.size();
+        if (h > (size_t)(2 * win) && relRes > 0.95 * hist[h - 1 - win]) {
+          if (restarts_left > 0) {
+            restarts_left--;
+            hist.clear();
+          } else {
+            lp_on = false;
+            rep->learned_fallback = it + 1;
+          }
+          base = it + 1;
         }
       }
+      if (!(it < s->up.max_iterations)) {
+        err = GS_E_MAXIT;
+        break;
+      }
+      it++;
+      converged = ((s->up.use_absolute_tolerance ? absRes : relRes) <= tol) && (it >= s->up.min_iterations);
     }
-    if (!use_net) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
-    const int k = it - base;
-    const int c = k < m ? k : m;
-    const int slot = k % m;
-    double* sk = s->fl_s + (long long)slot * np;
-    double* qk = s->fl_q + (long long)slot * np;
-    launch_orthogonalise(s->z, s->fl_s, s->fl_q, sqv, c, slot, np, partials, beta, st);
-    launch_apply_G(pb->g, sk, s->fT, pb->bcflag, st);
-    CKS(inner_solve(s, s->fT, s->vstar, irt, &inner));
-    launch_apply_Gt(pb->g, s->vstar, qk, st);
-    if (recording && s->rec_every > 0 && it % s->rec_every == 0)
-      CKS(lpc_record(s->rec, pb->g, qk, sk, s->Qinv, s->logeta, proj, s->S, s->rd, st));
-    launch_flex_alpha(sk, s->r, qk, np, s->S, sqv + slot, s->rd, st);
-    launch_uz_update(s->p, sk, s->r, qk, np, s->S, s->rd, st);
-    launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
-    CKS(read_scalars(s));
-    const double sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
-    if (!std::isfinite(sq) || !std::isfinite(alpha)) {
-      err = GS_E_NAN;
-      break;
-    }
-    absRes = std::sqrt(s->hS[S_ORR]) * qRecip;
+    if (err != GS_OK || relax <= 0) break;
+    // true residual with an accurate inner solve (STEP 3 of the reference)
+    launch_apply_G(pb->g, s->p, s->tmpN, pb->bcflag, st);
+    launch_axpby(s->tmpN, s->f, 1.0, -1.0, n, st);
+    CKS(inner_solve(s, s->tmpN, s->u, irt, &inner));
+    launch_apply_Gt(pb->g, s->u, s->r, st);
+    launch_axpby(s->r, s->h, -1.0, 1.0, np, st);
+    absRes = std::sqrt(dot_host(s, s->r, s->r, np, &e2)) * qRecip;
+    CKS(e2);
     relRes = absRes / uzawaRhsScale;
-    if (!std::isfinite(absRes)) {
-      err = GS_E_NAN;
-      break;
-    }
-    if (s->up.monitor > 0 && (it + 1) % s->up.monitor == 0)
-      std::fprintf(stderr, "  %04d uzawa residual norm %12.13e%s\n", it + 1, relRes, lp_on ? " (learned)" : "");
-    if (lp_on) {
-      hist.push_back(relRes);
-      const size_t h = hist.size();
-      if (h > (size_t)(2 * win) && relRes > 0.95 * hist[h - 1 - win]) {
-        if (restarts_left > 0) {
-          restarts_left--;
-          hist.clear();
-        } else {
-          lp_on = false;
-          rep->learned_fallback = it + 1;
-        }
-        base = it + 1;
-      }
-    }
-    if (!(it < s->up.max_iterations)) {
-      err = GS_E_MAXIT;
-      break;
-    }
-    it++;
-  } while (((s->up.use_absolute_tolerance ? absRes : relRes) > tol) || (it < s->up.min_iterations));
+    rep->verify_rounds = round + 1;
+    if (((s->up.use_absolute_tolerance ? absRes : relRes) <= tol) || !std::isfinite(relRes)) break;
+    if (round == kMaxRounds - 1) err = GS_E_MAXIT;
+  }
   *it_io = it;
   *inner_io = inner;
   *relRes_io = relRes;

@@ -102,6 +102,8 @@ typedef struct gs_uzawa_params {
                                    (keeping the network) up to this many times before falling back to Q */
   int32_t learned_q_every;      /* learned preconditioner: every k-th iteration preconditions with Q alone
                                    (0 = never) */
+  double inner_relax_max;       /* learned preconditioner: > 0 relaxes the inner tolerance as the outer
+                                   residual falls, up to this rtol, and re-checks the true residual */
 } gs_uzawa_params;
 
 /* SystemLinearEquations nonlinear keys (SystemLinearEquations.cxx:346-356). */
@@ -127,6 +129,8 @@ typedef struct gs_report {
   int32_t learned_fallback;     /* learned preconditioner: Uzawa iteration of the fallback to Q (0 = none) */
   double picard_residual;
   int64_t k_applies;            /* K applies issued (FP64 + FP32) */
+  int32_t verify_rounds;        /* learned preconditioner with relaxed inner solves: true-residual checks */
+  int32_t pad2_;
 } gs_report;
 
 typedef struct gs_problem gs_problem;
