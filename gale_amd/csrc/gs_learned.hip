@@ -182,14 +182,16 @@ __global__ void k_mdot_partial(const double* __restrict__ z, const double* __res
   }
 }
 
-// beta[j] = (sum_b partial[b][j]) / sq[j]
+// beta[j] = (sum_b partial[b][j]) / sq[j]; one wavefront per j (fixed order:
+// lane-strided partial sums, then the wavefront tree)
 __global__ void k_mdot_final(const double* __restrict__ partial, int nb, int c, const double* __restrict__ sq,
                              double* __restrict__ beta) {
-  const int j = threadIdx.x;
+  const int j = blockIdx.x, lane = threadIdx.x;
   if (j >= c) return;
   double t = 0.0;
-  for (int b = 0; b < nb; b++) t += partial[(long long)b * kMaxWindow + j];
-  beta[j] = sq[j] != 0.0 ? t / sq[j] : 0.0;
+  for (int b = lane; b < nb; b += 64) t += partial[(long long)b * kMaxWindow + j];
+  t = wave_sum(t);
+  if (lane == 0) beta[j] = sq[j] != 0.0 ? t / sq[j] : 0.0;
 }
 
 // s_out = z - sum_j beta[j] s_j  (s_out may be one of the ring slots: each
@@ -222,7 +224,7 @@ void launch_orthogonalise(const double* z, double* sring, const double* qring, c
   if (c > 0) {
     int nb = reduce_grid(n);
     hipLaunchKernelGGL(k_mdot_partial, dim3(nb), dim3(kBlock), 0, st, z, qring, c, n, partials);
-    hipLaunchKernelGGL(k_mdot_final, dim3(1), dim3(kMaxWindow), 0, st, partials, nb, c, sq, beta);
+    hipLaunchKernelGGL(k_mdot_final, dim3(c), dim3(64), 0, st, partials, nb, c, sq, beta);
   }
   hipLaunchKernelGGL(k_orth, dim3(grid_for(n)), dim3(kBlock), 0, st, z, sring, beta, c, out, n);
 }

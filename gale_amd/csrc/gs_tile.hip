@@ -3,7 +3,7 @@
 // One workgroup owns a tile of TX x TY (x TZ) elements.  It stages the
 // tile's nodal velocities in LDS with coalesced row loads, evaluates each
 // element (one thread per element), accumulates the element forces into an
-// LDS node array in 2^dim parity phases (deterministic, no atomics), and then
+// LDS node array in 2^dim corner phases (deterministic, no atomics), and then
 // stores every node it owns straight to y.  Nodes on a tile's upper faces are
 // owned by the next tile: their partial sums go to a per-tile halo buffer and
 // a second, node-parallel kernel adds them in a fixed order.  So one apply is
@@ -265,16 +265,15 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
       element_particles3(A, B, C, pk, off[e], off[e + 1], m);
     epilogue3(m, ix, iy, iz, ye);
   }
-  const int par = (ex & 1) | ((ey & 1) << 1) | ((ez & 1) << 2);
-#pragma unroll 1
-  for (int ph = 0; ph < 8; ph++) {
-    if (active && par == ph) {
+  // One phase per element corner a: for fixed a, element -> node (e + a) is
+  // injective, so all threads add without conflicts; each node receives its
+  // (up to 8) contributions in corner order, so the sum is reproducible.
 #pragma unroll
-      for (int a = 0; a < 8; a++) {
-        const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
+  for (int a = 0; a < 8; a++) {
+    if (active) {
+      const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
 #pragma unroll
-        for (int c = 0; c < 3; c++) sy[ln * 3 + c] += ye[a][c];
-      }
+      for (int c = 0; c < 3; c++) sy[ln * 3 + c] += ye[a][c];
     }
     __syncthreads();
   }
@@ -369,16 +368,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply2(Geo g, TileGrid tg
     const long long e = gi + (long long)g.res[0] * gj;
     element2(u, ix, iy, FORM, Mom, g.E, e, pk, FORM ? off[e] : 0, FORM ? off[e + 1] : 0, ye);
   }
-  const int par = (ex & 1) | ((ey & 1) << 1);
-#pragma unroll 1
-  for (int ph = 0; ph < 4; ph++) {
-    if (active && par == ph) {
 #pragma unroll
-      for (int a = 0; a < 4; a++) {
-        const int ln = (ex + (a & 1)) + NBX * (ey + ((a >> 1) & 1));
-        sy[ln * 2] += ye[a][0];
-        sy[ln * 2 + 1] += ye[a][1];
-      }
+  for (int a = 0; a < 4; a++) {  // one conflict-free phase per corner (see the 3D kernel)
+    if (active) {
+      const int ln = (ex + (a & 1)) + NBX * (ey + ((a >> 1) & 1));
+      sy[ln * 2] += ye[a][0];
+      sy[ln * 2 + 1] += ye[a][1];
     }
     __syncthreads();
   }

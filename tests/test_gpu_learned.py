@@ -215,3 +215,35 @@ def test_bad_network_falls_back_to_plain_q(gale):
     u1, p1, r1 = _solve(gale, cfg, net=net)
     assert r1.status == 0 and r1.learned_fallback > 0
     assert rel(u1, u0) < 1e-5 and rel(p1, p0) < 1e-5
+
+
+def test_shipped_network_relaxed_solve_accuracy(gale):
+    """The shipped weights (trained on 32^3 fields of the generator) with
+    mixed precision and relaxed inner solves: the true Schur residual reaches
+    the tolerance and the solution is at least as close to a 1e-10 reference
+    as the plain FP64 solve at the same tolerance (stated bounds: velocity
+    within 2x the plain error + 1e-6, pressure within 2x + 1e-3)."""
+    import os
+    from conftest import ROOT
+    net = gale.LearnedPreconditioner(seed=1)
+    net.set_params(np.load(os.path.join(ROOT, "gale_amd", "lpc_weights.npy")))
+    cfg = _rv(32, 4242, ppc=8)
+    pb = gale.Problem(**cfg.problem_kwargs())
+
+    def run(**kw):
+        s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(maxIterations=40000, **kw))
+        if kw.get("learned"):
+            s.set_learned(net)
+        u, p, rep = s.Solve()
+        s.close()
+        return u, p - p.mean(), rep
+
+    ur, pr, _ = run(tolerance=1e-10)
+    u0, p0, r0 = run(tolerance=1e-6)
+    u1, p1, r1 = run(tolerance=1e-6, learned=True, precision=gale.PREC_MIXED, inner_relax_max=0.1)
+    print(f"plain its {r0.uzawa_iterations} err u {rel(u0, ur):.2e} p {rel(p0, pr):.2e}; learned its "
+          f"{r1.uzawa_iterations} inner {r1.inner_iterations} vs {r0.inner_iterations} err u {rel(u1, ur):.2e} "
+          f"p {rel(p1, pr):.2e} rounds {r1.verify_rounds}", flush=True)
+    assert r1.status == 0 and r1.rel_residual <= 1e-6 and r1.verify_rounds >= 1
+    assert rel(u1, ur) <= 2 * rel(u0, ur) + 1e-6
+    assert rel(p1, pr) <= 2 * rel(p0, pr) + 1e-3
