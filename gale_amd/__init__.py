@@ -39,6 +39,8 @@ EXPORTS = (
     "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
     "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
     "gs_solver_set_learned", "gs_lpc_selftest",
+    "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
+    "gs_batch_solution",
 )
 
 
@@ -169,6 +171,12 @@ def lib():
             "gs_lpc_eval": (i32, [vp, vp, i32, i32, vp]),
             "gs_solver_set_learned": (i32, [vp, vp]),
             "gs_lpc_selftest": (i32, [vp]),
+            "gs_batch_create": (i32, [i32, vp, vp]),
+            "gs_batch_destroy": (None, [vp]),
+            "gs_batch_set_tile": (i32, [vp, i32, i32]),
+            "gs_batch_autotune": (i32, [vp, vp, vp, vp, vp]),
+            "gs_batch_solve": (i32, [vp, vp, vp, vp]),
+            "gs_batch_solution": (i32, [vp, i32, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -323,6 +331,54 @@ def material(eta0=1.0, density=0.0, yield_type=YIELD_NONE, cohesion=0.0, frictio
                 min_yield_stress=min_yield_stress, min_viscosity=min_viscosity, max_strain_rate=max_strain_rate)
 
 
+def _make_desc(dim, res, lo, hi, coord, material_index, materials, bc_mask=None, bc_value=None, gravity=1.0,
+               ghat=(0.0, -1.0, 0.0), eta_clamp=(0.0, 0.0), eta0_particle=None):
+    """A gs_problem_desc and the host arrays it points to (keep them alive)."""
+    N = num_nodes(dim, res)
+    d = ProblemDesc()
+    d.dim = dim
+    for i, v in enumerate(_dims(dim, res, 1)):
+        d.res[i] = v
+    for i, v in enumerate(_dims(dim, lo, 0.0)):
+        d.lo[i] = v
+    for i, v in enumerate(_dims(dim, hi, 1.0)):
+        d.hi[i] = v
+    for i in range(3):
+        d.ghat[i] = ghat[i]
+    keep = []
+    if bc_mask is not None:
+        bc_mask = np.ascontiguousarray(bc_mask, np.uint8).reshape(-1)
+        if bc_mask.size != N * dim:
+            raise GaleError(E_INVALID, "bc_mask size")
+        keep.append(bc_mask)
+        d.bc_mask = _ptr(bc_mask)
+    if bc_value is not None:
+        bc_value = _f64(bc_value).reshape(-1)
+        keep.append(bc_value)
+        d.bc_value = _ptr(bc_value)
+    d.gravity = gravity
+    d.eta_clamp_min, d.eta_clamp_max = eta_clamp
+    mats = (Material * len(materials))()
+    for i, m in enumerate(materials):
+        mats[i] = Material(m.get("eta0", 1.0), m.get("density", 0.0), m.get("yield_type", 0), 0,
+                           m.get("cohesion", 0.0), m.get("friction", 0.0), m.get("min_yield_stress", 0.0),
+                           m.get("min_viscosity", 0.0), m.get("max_strain_rate", 0.0))
+    keep.append(mats)
+    d.n_materials = len(materials)
+    d.materials = C.addressof(mats)
+    coord = _f64(coord).reshape(-1, 3)
+    mi = np.ascontiguousarray(material_index, np.int32)
+    keep += [coord, mi]
+    d.n_particles = coord.shape[0]
+    d.coord = _ptr(coord)
+    d.material = _ptr(mi)
+    if eta0_particle is not None:
+        e0 = _f64(eta0_particle)
+        keep.append(e0)
+        d.eta0_particle = _ptr(e0)
+    return d, keep
+
+
 class Problem:
     """FeMesh + MaterialPointsSwarm + IntegrationPointsSwarm on the GPU."""
 
@@ -333,55 +389,16 @@ class Problem:
         self.lo, self.hi = tuple(list(lo)[:dim]), tuple(list(hi)[:dim])
         self.N = num_nodes(dim, res)
         self.E = int(np.prod(self.res))
-        d = ProblemDesc()
-        d.dim = dim
-        for i, v in enumerate(_dims(dim, res, 1)):
-            d.res[i] = v
-        for i, v in enumerate(_dims(dim, lo, 0.0)):
-            d.lo[i] = v
-        for i, v in enumerate(_dims(dim, hi, 1.0)):
-            d.hi[i] = v
-        for i in range(3):
-            d.ghat[i] = ghat[i]
-        self._keep = []
-        if bc_mask is not None:
-            bc_mask = np.ascontiguousarray(bc_mask, np.uint8).reshape(-1)
-            if bc_mask.size != self.N * dim:
-                raise GaleError(E_INVALID, "bc_mask size")
-            self._keep.append(bc_mask)
-            d.bc_mask = _ptr(bc_mask)
-        if bc_value is not None:
-            bc_value = _f64(bc_value).reshape(-1)
-            self._keep.append(bc_value)
-            d.bc_value = _ptr(bc_value)
-        d.gravity = gravity
-        d.eta_clamp_min, d.eta_clamp_max = eta_clamp
-        mats = (Material * len(materials))()
-        for i, m in enumerate(materials):
-            mats[i] = Material(m.get("eta0", 1.0), m.get("density", 0.0), m.get("yield_type", 0), 0,
-                               m.get("cohesion", 0.0), m.get("friction", 0.0), m.get("min_yield_stress", 0.0),
-                               m.get("min_viscosity", 0.0), m.get("max_strain_rate", 0.0))
-        self._keep.append(mats)
-        d.n_materials = len(materials)
-        d.materials = C.addressof(mats)
-        coord = _f64(coord).reshape(-1, 3)
-        mi = np.ascontiguousarray(material_index, np.int32)
-        self._keep += [coord, mi]
-        d.n_particles = coord.shape[0]
-        d.coord = _ptr(coord)
-        d.material = _ptr(mi)
-        if eta0_particle is not None:
-            e0 = _f64(eta0_particle)
-            self._keep.append(e0)
-            d.eta0_particle = _ptr(e0)
-        self.P = coord.shape[0]
+        d, keep = _make_desc(dim, res, lo, hi, coord, material_index, materials, bc_mask, bc_value, gravity, ghat,
+                             eta_clamp, eta0_particle)
+        self.P = int(d.n_particles)
         h = C.c_void_p()
         st = lib().gs_problem_create(C.byref(d), C.byref(h))
         self._h = h
         self.create_status = st
         if not h.value:
             raise GaleError(st, "gs_problem_create")
-        self._keep = []
+        del keep
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
@@ -686,3 +703,87 @@ def lpc_selftest():
     e = C.c_float()
     _check(lib().gs_lpc_selftest(C.byref(e)), "lpc_selftest")
     return e.value
+
+
+# ---------------------------------------------------------------------------
+# batched solve of many small 2D problems (include/gale_batch.h)
+# ---------------------------------------------------------------------------
+class BatchParams(C.Structure):
+    _fields_ = [
+        ("tolerance", C.c_double), ("max_iterations", C.c_int32), ("max_inner", C.c_int32),
+        ("inner_rtol_factor", C.c_double), ("mg_smooth", C.c_int32), ("use_graph", C.c_int32),
+    ]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("uzawa_iterations", C.c_int32), ("inner_iterations", C.c_int64),
+        ("rel_residual", C.c_double),
+    ]
+
+
+TILE_CANDIDATES = [(256, 16), (256, 32), (512, 32), (512, 64), (1024, 32), (1024, 64)]
+
+
+def batch_params(tolerance=1e-6, maxIterations=5000, max_inner=20000, inner_rtol_factor=0.1, mg_smooth=2,
+                 use_graph=False) -> BatchParams:
+    return BatchParams(tolerance, maxIterations, max_inner, inner_rtol_factor, mg_smooth, int(bool(use_graph)))
+
+
+class BatchSolver:
+    """Many independent 2D problems solved in one persistent launch."""
+
+    def __init__(self, problems):
+        """problems: list of Problem keyword dicts (Config.problem_kwargs())."""
+        self.count = len(problems)
+        descs = (ProblemDesc * self.count)()
+        keep = []
+        self.shapes = []
+        for i, kw in enumerate(problems):
+            d, k = _make_desc(**kw)
+            descs[i] = d
+            keep.append(k)
+            res = tuple(kw["res"])[:2]
+            self.shapes.append(res)
+        h = C.c_void_p()
+        _check(lib().gs_batch_create(self.count, descs, C.byref(h)), "gs_batch_create")
+        self._h = h
+        del keep
+
+    def set_tile(self, block, tx):
+        _check(lib().gs_batch_set_tile(self._h, int(block), int(tx)), "gs_batch_set_tile")
+
+    def autotune(self, params=None):
+        params = params or batch_params()
+        b, t = C.c_int32(), C.c_int32()
+        ms = np.zeros(len(TILE_CANDIDATES), np.float32)
+        _check(lib().gs_batch_autotune(self._h, C.byref(params), C.byref(b), C.byref(t), _ptr(ms)), "autotune")
+        return (b.value, t.value), dict(zip(TILE_CANDIDATES, map(float, ms)))
+
+    def solve(self, params=None, raise_on_error=True):
+        params = params or batch_params()
+        res = (BatchResult * self.count)()
+        sec = C.c_double()
+        st = lib().gs_batch_solve(self._h, C.byref(params), res, C.byref(sec))
+        if raise_on_error:
+            _check(st, "gs_batch_solve")
+        return st, [dict(status=r.status, uzawa_iterations=r.uzawa_iterations,
+                         inner_iterations=r.inner_iterations, rel_residual=r.rel_residual) for r in res], sec.value
+
+    def solution(self, i):
+        nx, ny = self.shapes[i]
+        u = np.zeros(2 * (nx + 1) * (ny + 1))
+        p = np.zeros(nx * ny)
+        _check(lib().gs_batch_solution(self._h, int(i), _ptr(u), _ptr(p)), "gs_batch_solution")
+        return u, p
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().gs_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

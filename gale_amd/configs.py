@@ -73,6 +73,41 @@ def sinker_2d(res=32, ppc=20, seed=13, eta_disc=1e3, rho_disc=1.1) -> Config:
     return Config("sinker_2d", dim, r, lo, hi, coord, mat, mats, mask, vals, steps=1)
 
 
+def sinker_instance(rng, ppc=20, res_range=(16, 128), seed=1):
+    """One instance of the batched sinker family: square mesh n x n with n
+    drawn uniformly from the multiples of 8 in res_range (so that every
+    instance has a nested multigrid hierarchy down to <= 16^2), 1-5 discs (radius in [0.05, 0.15], density contrast
+    in [0.05, 0.2] over the background rho 1, log10 viscosity contrast
+    uniform in [-2, 4] over the background eta 1), free-slip unit square,
+    gravity 1 along -y.  rng: numpy Generator drawing the instance parameters;
+    seed: the particle layout seed."""
+    lo8, hi8 = -(-res_range[0] // 8), res_range[1] // 8
+    n = 8 * int(rng.integers(lo8, hi8 + 1)) if hi8 >= lo8 else int(res_range[0])
+    dim, r = 2, (n, n)
+    lo, hi = (0.0, 0.0), (1.0, 1.0)
+    coord = layout_random(dim, r, lo, hi, ppc, seed)
+    k = int(rng.integers(1, 6))
+    mats = [material(eta0=1.0, density=1.0)]
+    shapes = [everywhere(0)]
+    for i in range(k):
+        rad = float(rng.uniform(0.05, 0.15))
+        cx, cy = (float(v) for v in rng.uniform(rad, 1.0 - rad, 2))
+        drho = float(rng.uniform(0.05, 0.2))
+        leta = float(rng.uniform(-2.0, 4.0))
+        mats.append(material(eta0=10.0 ** leta, density=1.0 + drho))
+        shapes.append(sphere(i + 1, (cx, cy, 0.0), rad))
+    mat = shape_layout(dim, coord, shapes)
+    mask, vals = free_slip(dim, r)
+    return Config(f"sinker_instance_{n}", dim, r, lo, hi, coord, mat, mats, mask, vals, steps=1)
+
+
+def sinker_batch(count=256, seed=2024, ppc=20, res_range=(16, 128)):
+    """The batched workload: count independent sinker instances of uneven mesh
+    size (see sinker_instance), reproducible from seed."""
+    rng = np.random.default_rng(seed)
+    return [sinker_instance(rng, ppc=ppc, res_range=res_range, seed=seed * 1000 + i) for i in range(count)]
+
+
 def lithosphere_2d(nx=256, ny=64, ppc=40, seed=13) -> Config:
     """2D extension of a plastic lid: 4x1 box, lid y >= 0.75 Drucker-Prager
     (cohesion 1e-2, friction angle 30 deg, eta 1e2, minimum viscosity 1e-3),

@@ -905,6 +905,58 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
   return GS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Export of a set-up solver's data for the batched solver (gs_batch.hip).
+// The solver must use the multigrid inner preconditioner in mixed precision
+// (so FP32 level data exist) and have completed gs_solver_setup.
+// ---------------------------------------------------------------------------
+namespace gs {
+struct BatchLevelInfo {
+  int nx, ny;
+  double lmax;
+};
+int32_t batch_mg_info(gs_solver* s, int* nlev, BatchLevelInfo* lv, int max_levels, int* nfc);
+int32_t batch_copy_fine(gs_solver* s, double* f, double* dinv, double* Qinv, double* M64, hipStream_t st);
+int32_t batch_copy_level(gs_solver* s, int l, float* M32, float* dinv32, uint8_t* bcflag, hipStream_t st);
+int32_t batch_copy_dense(gs_solver* s, float* Kinv32, int* freeidx, hipStream_t st);
+}  // namespace gs
+
+int32_t gs::batch_mg_info(gs_solver* s, int* nlev, BatchLevelInfo* lv, int max_levels, int* nfc) {
+  if (!s || !s->setup_done || !s->mg.structured || s->pb->dim != 2) return GS_E_INVALID;
+  const int n = s->mg.L + 1;
+  if (n > max_levels) return GS_E_INVALID;
+  for (int l = 0; l < n; l++) lv[l] = BatchLevelInfo{s->mg.lv[l].g.res[0], s->mg.lv[l].g.res[1], s->mg.lv[l].lmax};
+  *nlev = n;
+  *nfc = s->mg.nfc;
+  return GS_OK;
+}
+
+int32_t gs::batch_copy_fine(gs_solver* s, double* f, double* dinv, double* Qinv, double* M64, hipStream_t st) {
+  const MGLevel& L = s->mg.lv[0];
+  CK(hipMemcpyAsync(f, s->f, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(dinv, s->dinv, sizeof(double) * s->n, hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(Qinv, s->Qinv, sizeof(double) * s->np, hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(M64, L.M64, sizeof(double) * L.g.E * n_moments(L.g.dim), hipMemcpyDeviceToDevice, st));
+  return GS_OK;
+}
+
+int32_t gs::batch_copy_level(gs_solver* s, int l, float* M32, float* dinv32, uint8_t* bcflag, hipStream_t st) {
+  if (l < 0 || l > s->mg.L) return GS_E_INVALID;
+  const MGLevel& L = s->mg.lv[l];
+  CK(hipMemcpyAsync(M32, L.M32, sizeof(float) * L.g.E * n_moments(L.g.dim), hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(dinv32, L.dinv32, sizeof(float) * L.n, hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(bcflag, L.bcflag, L.g.N, hipMemcpyDeviceToDevice, st));
+  return GS_OK;
+}
+
+int32_t gs::batch_copy_dense(gs_solver* s, float* Kinv32, int* freeidx, hipStream_t st) {
+  const int nf = s->mg.nfc;
+  if (nf <= 0) return GS_OK;
+  CK(hipMemcpyAsync(Kinv32, s->mg.Kinv32, sizeof(float) * nf * nf, hipMemcpyDeviceToDevice, st));
+  CK(hipMemcpyAsync(freeidx, s->mg.freeidx, sizeof(int) * nf, hipMemcpyDeviceToDevice, st));
+  return GS_OK;
+}
+
 extern "C" int32_t gs_solver_setup(gs_solver* s, const double* u_prev, const double* p_prev, int32_t yield_on) {
   if (!s) return GS_E_INVALID;
   gs_problem* pb = s->pb;

@@ -1,0 +1,75 @@
+"""GPU checks of the batched solve (one persistent launch over many 2D sinker
+instances of uneven size): every instance agrees with the single-problem
+solver and the oracle at tight tolerances, the reference-tolerance solve is
+deterministic and identical under HIP-graph replay, and the tile autotuner
+returns one of its candidates."""
+import numpy as np
+import pytest
+
+from conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(gale, count, res_range, ppc=8, seed=77):
+    from gale_amd import configs
+    cfgs = configs.sinker_batch(count=count, seed=seed, ppc=ppc, res_range=res_range)
+    return cfgs, gale.BatchSolver([c.problem_kwargs() for c in cfgs])
+
+
+def test_batch_matches_single_problem_and_oracle(gale, oracle):
+    cfgs, bs = _batch(gale, 6, (16, 40))
+    st, res, sec = bs.solve(gale.batch_params(tolerance=1e-10, inner_rtol_factor=1e-3, maxIterations=20000))
+    assert st == 0 and all(r["status"] == 0 for r in res)
+    for i, cfg in enumerate(cfgs):
+        ub, pb_ = bs.solution(i)
+        pb_ = pb_ - pb_.mean()
+        pr = gale.Problem(**cfg.problem_kwargs())
+        s = gale.Stokes_SLE_UzawaSolver(pr, gale.uzawa_params(tolerance=1e-10, inner_rtol_override=1e-13,
+                                                              maxIterations=20000, inner_max_iterations=200000))
+        us, ps, _ = s.Solve()
+        ps = ps - ps.mean()
+        assert rel(ub, us) < 1e-7 and rel(pb_, ps) < 1e-7, (i, cfg.res, rel(ub, us), rel(pb_, ps))
+        s.close()
+        pr.close()
+        if i < 2:
+            o = oracle.Problem(**cfg.oracle_kwargs())
+            sto, uo, po, _ = o.uzawa(oracle.uzawa_params(tolerance=1e-10, inner_rtol_override=1e-13))
+            assert sto == 0
+            assert rel(ub, uo) < 1e-7 and rel(pb_, po - po.mean()) < 1e-7
+
+
+def test_batch_reference_tolerance_graph_replay(gale):
+    cfgs, bs = _batch(gale, 24, (16, 64), seed=78)
+    p = gale.batch_params(tolerance=1e-6)
+    st, r0, t0 = bs.solve(p)
+    assert st == 0 and all(r["status"] == 0 and r["rel_residual"] <= 1e-6 for r in r0)
+    u0 = [bs.solution(i) for i in range(len(cfgs))]
+    pg = gale.batch_params(tolerance=1e-6, use_graph=True)
+    for rep in range(3):  # capture once, replay; eager work in between
+        st, rg, tg = bs.solve(pg)
+        assert st == 0
+        assert [r["uzawa_iterations"] for r in rg] == [r["uzawa_iterations"] for r in r0]
+        for i in range(len(cfgs)):
+            ug, pgv = bs.solution(i)
+            assert np.array_equal(ug, u0[i][0]) and np.array_equal(pgv, u0[i][1])
+        st, _, _ = bs.solve(p)
+        assert st == 0
+
+
+def test_batch_autotune_and_tiles(gale):
+    cfgs, bs = _batch(gale, 12, (16, 96), seed=79)
+    (block, tx), ms = bs.autotune(gale.batch_params(tolerance=1e-6))
+    assert (block, tx) in gale.TILE_CANDIDATES
+    assert all(v > 0 for v in ms.values())
+    ref = None
+    for b, t in gale.TILE_CANDIDATES:  # every tile shape solves the same problems
+        bs.set_tile(b, t)
+        st, r, _ = bs.solve(gale.batch_params(tolerance=1e-8, inner_rtol_factor=1e-2))
+        assert st == 0
+        sol = [bs.solution(i) for i in range(3)]
+        if ref is None:
+            ref = sol
+        else:
+            for (u, p), (ur, pr) in zip(sol, ref):
+                assert rel(u, ur) < 1e-5 and rel(p - p.mean(), pr - pr.mean()) < 1e-5
