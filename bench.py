@@ -56,6 +56,49 @@ def cpu_baseline(res, threads):
                 inner_iterations=int(rep.inner_iterations), E=cfg.E, P=cfg.P)
 
 
+def bench_batch(G, configs, count):
+    """Batched workload: count independent 2D sinkers (meshes 16^2..128^2 in
+    steps of 8, 1-5 discs with log10 viscosity contrast in [-2, 4], 20 ppc),
+    FP32 multigrid applies inside FP64 inner CG and FP64 Uzawa, each to 1e-6,
+    in one persistent launch (eager and HIP-graph replay).  For scale, every
+    16th instance is also solved alone with the single-problem solver."""
+    t = time.perf_counter()
+    cfgs = configs.sinker_batch(count=count)
+    t_gen = time.perf_counter() - t
+    t = time.perf_counter()
+    bs = G.BatchSolver([c.problem_kwargs() for c in cfgs])
+    t_setup = time.perf_counter() - t
+    p = G.batch_params(tolerance=1e-6)
+    best, ms = bs.autotune(p)
+    st, res, sec = bs.solve(p)
+    stg, _, _ = bs.solve(G.batch_params(tolerance=1e-6, use_graph=True))  # capture
+    stg, resg, secg = bs.solve(G.batch_params(tolerance=1e-6, use_graph=True))  # replay
+    outer = [r["uzawa_iterations"] for r in res]
+    inner = [r["inner_iterations"] for r in res]
+    step = max(1, count // 16)
+    seq = 0.0
+    for i in range(0, count, step):
+        pr = G.Problem(**cfgs[i].problem_kwargs())
+        sv = G.Stokes_SLE_UzawaSolver(pr, G.uzawa_params(tolerance=1e-6, maxIterations=20000))
+        _, _, rep = sv.Solve(raise_on_error=False)
+        seq += rep.solve_seconds
+        sv.close()
+        pr.close()
+    bs.close()
+    print(f"# batch {count}: {sec:.3f}s eager, {secg:.3f}s graph, tile {best}", file=sys.stderr, flush=True)
+    return {"workload": f"{count} independent 2D sinkers, meshes 16^2..128^2 (multiples of 8), 1-5 discs, "
+                        f"log10 eta contrast [-2,4], 20 ppc, free slip, Uzawa to 1e-6",
+            "instances": count, "particles": int(sum(c.P for c in cfgs)),
+            "elements": int(sum(c.E for c in cfgs)), "generate_s": t_gen, "setup_s": t_setup,
+            "tile_autotuned": list(best), "tile_ms": {f"{b}x{t}": v for (b, t), v in ms.items()},
+            "solve_s": sec, "graph_solve_s": secg, "status": int(st), "graph_status": int(stg),
+            "all_converged": all(r["status"] == 0 and r["rel_residual"] <= 1e-6 for r in res),
+            "uzawa_iterations_mean": float(np.mean(outer)), "uzawa_iterations_max": int(max(outer)),
+            "inner_iterations_mean": float(np.mean(inner)),
+            "single_problem_sequential_estimate_s": seq * step,
+            "single_problem_sample": f"every {step}th instance solved alone, x{step}"}
+
+
 def profile_apply(args):
     """Calibration copy of known size (torch elementwise copy, 256 MiB read +
     256 MiB written per call) followed by the four K apply forms, so that
@@ -96,6 +139,8 @@ def main():
     ap.add_argument("--mg-smooth", type=int, default=2)
     ap.add_argument("--weights", default=os.path.join(ROOT, "gale_amd", "lpc_weights.npy"))
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--inner-relax", type=float, default=0.1,
                     help="learned solve: cap of the relaxed inner tolerance (0 = reference 0.1 x tol throughout)")
     ap.add_argument("--profile-apply", action="store_true",
@@ -205,6 +250,10 @@ def main():
                    "speedup_vs_plain": solve_s / float(np.mean(ltimes)), "training": meta}
         sl.close()
 
+    batch = None
+    if not args.no_batch:
+        batch = bench_batch(G, configs, args.batch_count)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -259,6 +308,7 @@ def main():
                                     "ms_per_apply": ms, "algorithmic_bytes": nbytes,
                                     "fp32_ms_per_apply": ms32, "fp32_algorithmic_bytes": nbytes32},
         "cpu_baseline": cpu,
+        "batch": batch,
     }
     print(json.dumps(line), flush=True)
 
