@@ -105,14 +105,15 @@ struct MGLevel {
   double* dinv64 = nullptr;
   float* dinv32 = nullptr;
   double lmax = 2.0;
-  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *Ax = nullptr;
-  float *bf = nullptr, *xf = nullptr, *rf = nullptr, *df = nullptr, *Axf = nullptr;
+  double *b = nullptr, *x = nullptr, *r = nullptr, *d = nullptr, *Ax = nullptr, *x2 = nullptr;
+  float *bf = nullptr, *xf = nullptr, *rf = nullptr, *df = nullptr, *Axf = nullptr, *xf2 = nullptr;
   double* h64 = nullptr;  // tiled-apply halo scratch
   float* h32 = nullptr;
 };
 
 struct MGData {
   bool structured = false;
+  bool bottom = true;  // single-workgroup bottom of the cycle (3D) where levels fit
   int L = 0;  // index of the coarsest level
   std::vector<MGLevel> lv;
   int nfc = 0;
@@ -185,8 +186,6 @@ extern "C" int32_t gs_device_count(void) {
 
 static int32_t alloc_reducer(Reducer* rd) {
   CK(dalloc(&rd->partials, (long long)kMaxReduceBlocks * 4));
-  CK(dalloc(&rd->counter, 1));
-  CK(hipMemset(rd->counter, 0, sizeof(unsigned)));
   return GS_OK;
 }
 
@@ -363,7 +362,7 @@ extern "C" void gs_problem_destroy(gs_problem* pb) {
   dfree(pb->mats_d); dfree(pb->bcdof); dfree(pb->bcflag); dfree(pb->ubc); dfree(pb->pk); dfree(pb->pk32);
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->off); dfree(pb->keys); dfree(pb->keys2); dfree(pb->vals);
-  dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->rd.counter); dfree(pb->halo64);
+  dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->halo64);
   dfree(pb->halo32);
   if (pb->sort_tmp) (void)hipFree(pb->sort_tmp);
   if (pb->hS) (void)hipHostFree(pb->hS);
@@ -536,7 +535,7 @@ extern "C" void gs_solver_destroy(gs_solver* s) {
   mg_free(s->mg);
   dfree(s->cg_z);
   dfree(s->cg_r); dfree(s->cg_p); dfree(s->cg_Ap); dfree(s->tmpN); dfree(s->tmpE); dfree(s->eta); dfree(s->uprev);
-  dfree(s->yielded); dfree(s->x32); dfree(s->y32); dfree(s->S); dfree(s->rd.partials); dfree(s->rd.counter);
+  dfree(s->yielded); dfree(s->x32); dfree(s->y32); dfree(s->S); dfree(s->rd.partials);
   dfree(s->logeta); dfree(s->rec_r0); dfree(s->rec_p0); dfree(s->fl_s); dfree(s->fl_q); dfree(s->fl_aux);
   if (s->hS) (void)hipHostFree(s->hS);
   if (s->e0) (void)hipEventDestroy(s->e0);
@@ -564,6 +563,7 @@ static void mg_free(MGData& mg) {
     dfree(L.M64); dfree(L.M32); dfree(L.diag); dfree(L.dinv64); dfree(L.dinv32);
     dfree(L.b); dfree(L.x); dfree(L.r); dfree(L.d); dfree(L.Ax);
     dfree(L.bf); dfree(L.xf); dfree(L.rf); dfree(L.df); dfree(L.Axf); dfree(L.h64); dfree(L.h32);
+    dfree(L.x2); dfree(L.xf2);
     if (L.own_bc) {
       dfree(L.bcflag);
       dfree(L.bcdof);
@@ -583,7 +583,7 @@ static int32_t mg_alloc_level(MGLevel& L) {
   CK(dalloc(&L.diag, L.n)); CK(dalloc(&L.dinv64, L.n)); CK(dalloc(&L.dinv32, L.n));
   CK(dalloc(&L.b, L.n)); CK(dalloc(&L.x, L.n)); CK(dalloc(&L.r, L.n)); CK(dalloc(&L.d, L.n)); CK(dalloc(&L.Ax, L.n));
   CK(dalloc(&L.bf, L.n)); CK(dalloc(&L.xf, L.n)); CK(dalloc(&L.rf, L.n)); CK(dalloc(&L.df, L.n));
-  CK(dalloc(&L.Axf, L.n));
+  CK(dalloc(&L.Axf, L.n)); CK(dalloc(&L.x2, L.n)); CK(dalloc(&L.xf2, L.n));
   CK(dalloc(&L.h64, tile_halo_size(L.g))); CK(dalloc(&L.h32, tile_halo_size(L.g)));
   return GS_OK;
 }
@@ -668,6 +668,10 @@ static int32_t mg_structure(gs_solver* s) {
     CK(dalloc(&mg.Kinv32, (long long)mg.nfc * mg.nfc));
   } else {
     mg.nfc = -1;  // no dense coarse solve: Chebyshev smoothing on the coarsest level
+  }
+  {
+    const char* ev = std::getenv("GALE_MG_BOTTOM");  // 0: multi-launch bottom (A/B timing)
+    mg.bottom = !(ev && std::atoi(ev) == 0);
   }
   mg.structured = true;
   return GS_OK;
@@ -800,57 +804,97 @@ static int32_t mg_numeric(gs_solver* s) {
 
 template <typename T>
 struct LvPtr {
-  T *b, *x, *r, *d, *Ax;
+  T *b, *x, *x2, *r, *d, *Ax;
   const T *M, *dinv;
   T* h;
 };
-static LvPtr<double> lvp(MGLevel& L, double*) { return {L.b, L.x, L.r, L.d, L.Ax, L.M64, L.dinv64, L.h64}; }
-static LvPtr<float> lvp(MGLevel& L, float*) { return {L.bf, L.xf, L.rf, L.df, L.Axf, L.M32, L.dinv32, L.h32}; }
+static LvPtr<double> lvp(MGLevel& L, double*) {
+  return {L.b, L.x, L.x2, L.r, L.d, L.Ax, L.M64, L.dinv64, L.h64};
+}
+static LvPtr<float> lvp(MGLevel& L, float*) {
+  return {L.bf, L.xf, L.xf2, L.rf, L.df, L.Axf, L.M32, L.dinv32, L.h32};
+}
 
+// Chebyshev-Jacobi smoothing of K x = b on [0.1, 1.1] lambda_max(D^-1 K).
+// Each step after the first is one K apply whose node completion performs
+// the Chebyshev update (TileEpi kind 1), writing x + d into the other of the
+// level's two x buffers.  x == nullptr starts from zero (first step without
+// an apply).  Returns the buffer that holds the smoothed x.
 template <typename T>
-static void cheb_smooth(MGLevel& L, int nu, bool zero, hipStream_t st) {
+static T* cheb_smooth(MGLevel& L, int nu, T* x, hipStream_t st) {
   LvPtr<T> P = lvp(L, (T*)nullptr);
   const double emax = 1.1 * L.lmax, emin = 0.1 * L.lmax;
   const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin), sigma = theta / delta;
   double rho_prev = 1.0 / sigma;
-  if (zero) {
+  int k0 = 0;
+  if (!x) {
     launch_cheb<T>(P.b, nullptr, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 1, st);
-  } else {
-    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
-    launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)0, (T)(1.0 / theta), 0, st);
+    x = P.x;
+    k0 = 1;
   }
-  for (int k = 1; k < nu; k++) {
-    const double rho = 1.0 / (2.0 * sigma - rho_prev);
-    launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
-    launch_cheb<T>(P.b, P.Ax, P.dinv, P.d, P.x, L.n, (T)(rho * rho_prev), (T)(2.0 * rho / delta), 0, st);
-    rho_prev = rho;
+  for (int k = k0; k < nu; k++) {
+    T alpha = 0, beta = (T)(1.0 / theta);
+    if (k > 0) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      alpha = (T)(rho * rho_prev);
+      beta = (T)(2.0 * rho / delta);
+      rho_prev = rho;
+    }
+    T* xo = x == P.x ? P.x2 : P.x;
+    TileEpi<T> ep{1, P.b, P.dinv, P.d, xo, nullptr, alpha, beta};
+    tile_apply_moments_epi(L.g, x, P.Ax, P.h, P.M, L.bcflag, ep, st);
+    x = xo;
   }
+  return x;
 }
 
+// Levels l..L in one single-workgroup launch when they fit (gs_tile.hip).
 template <typename T>
-static void vcycle(gs_solver* s, int l) {
+static bool vcycle_bottom(gs_solver* s, int l) {
+  MGData& mg = s->mg;
+  if (!mg.bottom || !bot_fits<T>(mg.lv[l].g) || mg.L - l + 1 > kBotMaxLevels || mg.nfc > kBotMaxDense) return false;
+  BotArgs<T> a{};
+  for (int k = l; k <= mg.L; k++) {
+    MGLevel& Lk = mg.lv[k];
+    LvPtr<T> P = lvp(Lk, (T*)nullptr);
+    a.lv[k - l] = BotLevel<T>{Lk.g, P.M, P.dinv, Lk.bcflag, P.b, P.x, P.x2, P.r, P.d, Lk.lmax};
+  }
+  a.nlev = mg.L - l + 1;
+  a.nu = s->up.mg_smooth > 0 ? s->up.mg_smooth : 2;
+  a.Kinv = mg.nfc > 0 ? (sizeof(T) == 8 ? (const T*)(void*)mg.Kinv64 : (const T*)(void*)mg.Kinv32) : nullptr;
+  a.freeidx = mg.freeidx;
+  a.nfc = mg.nfc > 0 ? mg.nfc : 0;
+  mg_bottom3(a, s->pb->st);
+  return true;
+}
+
+// One V-cycle for K x = b at level l (b in the level's b buffer); returns the
+// buffer holding x.
+template <typename T>
+static T* vcycle(gs_solver* s, int l) {
   MGData& mg = s->mg;
   MGLevel& L = mg.lv[l];
   hipStream_t st = s->pb->st;
   const int nu = s->up.mg_smooth > 0 ? s->up.mg_smooth : 2;
   LvPtr<T> P = lvp(L, (T*)nullptr);
+  if (vcycle_bottom<T>(s, l)) return P.x;
   if (l == mg.L) {
-    if (mg.nfc > 0)
+    if (mg.nfc > 0) {
       launch_dense_solve<T>(sizeof(T) == 8 ? (const T*)(void*)mg.Kinv64 : (const T*)(void*)mg.Kinv32, mg.freeidx,
                             mg.nfc, P.b, P.x, L.n, st);
-    else
-      cheb_smooth<T>(L, 8, true, st);
-    return;
+      return P.x;
+    }
+    return cheb_smooth<T>(L, 8, (T*)nullptr, st);
   }
   MGLevel& C = mg.lv[l + 1];
   LvPtr<T> Q = lvp(C, (T*)nullptr);
-  cheb_smooth<T>(L, nu, true, st);
-  launch_mom_apply<T>(L.g, P.x, P.Ax, P.M, L.bcflag, P.h, st);
-  launch_resid<T>(P.b, P.Ax, P.r, L.n, st);
+  T* x = cheb_smooth<T>(L, nu, (T*)nullptr, st);
+  TileEpi<T> ep{2, P.b, nullptr, nullptr, nullptr, P.r, 0, 0};
+  tile_apply_moments_epi(L.g, x, P.Ax, P.h, P.M, L.bcflag, ep, st);  // r = b - K x
   launch_restrict<T>(C.g, L.g, P.r, Q.b, C.bcflag, st);
-  vcycle<T>(s, l + 1);
-  launch_prolong_add<T>(C.g, L.g, Q.x, P.x, L.bcflag, st);
-  cheb_smooth<T>(L, nu, false, st);
+  T* xc = vcycle<T>(s, l + 1);
+  launch_prolong_add<T>(C.g, L.g, xc, x, L.bcflag, st);
+  return cheb_smooth<T>(L, nu, x, st);
 }
 
 // z = M^-1 r by one V-cycle (FP64, or FP32 in the mixed-precision mode).
@@ -859,12 +903,12 @@ static void mg_precond(gs_solver* s, const double* r, double* z) {
   hipStream_t st = s->pb->st;
   if (s->up.precision == GS_PREC_MIXED) {
     launch_d2f(r, L.bf, L.n, st);
-    vcycle<float>(s, 0);
-    launch_f2d(L.xf, z, L.n, st);
+    const float* x = vcycle<float>(s, 0);
+    launch_f2d(x, z, L.n, st);
   } else {
     (void)hipMemcpyAsync(L.b, r, sizeof(double) * L.n, hipMemcpyDeviceToDevice, st);
-    vcycle<double>(s, 0);
-    (void)hipMemcpyAsync(z, L.x, sizeof(double) * L.n, hipMemcpyDeviceToDevice, st);
+    const double* x = vcycle<double>(s, 0);
+    (void)hipMemcpyAsync(z, x, sizeof(double) * L.n, hipMemcpyDeviceToDevice, st);
   }
 }
 

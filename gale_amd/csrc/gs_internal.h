@@ -52,8 +52,7 @@ enum Slot {
 };
 
 struct Reducer {
-  double* partials;   // >= kMaxReduceBlocks * 4 doubles
-  unsigned* counter;  // one zero-initialised counter per reduction site
+  double* partials;   // >= kMaxReduceBlocks * 4 doubles (block partial sums)
 };
 
 constexpr int kBlock = 256;

@@ -170,27 +170,31 @@ void launch_mul(const double* a, const double* b, double* c, long long n, hipStr
 
 __global__ void k_dot(const double* __restrict__ a, const double* __restrict__ b, long long n, double* S, int slot,
                       Reducer rd) {
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     v[0] += a[i] * b[i];
-  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) S[slot] = tot[0];
+  block_partials<1, false>(v, rd);
 }
 void launch_dot(const double* a, const double* b, long long n, double* S, int slot, Reducer r, hipStream_t st) {
-  hipLaunchKernelGGL(k_dot, dim3(reduce_grid(n)), dim3(kBlock), 0, st, a, b, n, S, slot, r);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_dot, dim3(nb), dim3(kBlock), 0, st, a, b, n, S, slot, r);
+  reduce_finish<1, false>(r, nb, OpStore{S, slot}, st);
 }
 
 __global__ void k_vmax(const double* __restrict__ u, long long nnode, int dim, double* S, int slot, Reducer rd) {
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nnode;
        i += (long long)gridDim.x * blockDim.x) {
     double s = 0.0;
     for (int d = 0; d < dim; d++) s += u[i * dim + d] * u[i * dim + d];
     v[0] = fmax(v[0], sqrt(s));
   }
-  if (reduce_last<1, true>(v, rd, tot) && threadIdx.x == 0) S[slot] = tot[0];
+  block_partials<1, true>(v, rd);
 }
 void launch_vmax(const double* u, long long nnode, int dim, double* S, int slot, Reducer rd, hipStream_t st) {
-  hipLaunchKernelGGL(k_vmax, dim3(reduce_grid(nnode)), dim3(kBlock), 0, st, u, nnode, dim, S, slot, rd);
+  const int nb = reduce_grid(nnode);
+  hipLaunchKernelGGL(k_vmax, dim3(nb), dim3(kBlock), 0, st, u, nnode, dim, S, slot, rd);
+  reduce_finish<1, true>(rd, nb, OpStore{S, slot}, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,7 +202,7 @@ void launch_vmax(const double* u, long long nnode, int dim, double* S, int slot,
 // ---------------------------------------------------------------------------
 __global__ void k_cg_init(const double* __restrict__ b, const double* __restrict__ dinv, double* __restrict__ x,
                           double* __restrict__ r, double* __restrict__ p, long long n, double* S, Reducer rd) {
-  double v[2] = {0.0, 0.0}, tot[2];
+  double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double bi = b[i], zi = dinv[i] * bi;
     x[i] = 0.0;
@@ -207,15 +211,21 @@ __global__ void k_cg_init(const double* __restrict__ b, const double* __restrict
     v[0] += bi * zi;
     v[1] += bi * bi;
   }
-  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_RZ] = tot[0];
-    S[S_BB] = tot[1];
-    S[S_RR] = tot[1];
-  }
+  block_partials<2, false>(v, rd);
 }
+struct OpCgInit {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    S[S_RZ] = t[0];
+    S[S_BB] = t[1];
+    S[S_RR] = t[1];
+  }
+};
 void launch_cg_init(const double* b, const double* dinv, double* x, double* r, double* p, long long n, double* S,
                     Reducer rd, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_init, dim3(reduce_grid(n)), dim3(kBlock), 0, st, b, dinv, x, r, p, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_cg_init, dim3(nb), dim3(kBlock), 0, st, b, dinv, x, r, p, n, S, rd);
+  reduce_finish<2, false>(rd, nb, OpCgInit{S}, st);
 }
 
 __global__ void k_cg_update(double* __restrict__ x, double* __restrict__ r, const double* __restrict__ p,
@@ -223,7 +233,7 @@ __global__ void k_cg_update(double* __restrict__ x, double* __restrict__ r, cons
                             Reducer rd) {
   const double rz = S[S_RZ], pap = S[S_PAP];
   const double alpha = (pap != 0.0 && isfinite(pap)) ? rz / pap : 0.0;
-  double v[2] = {0.0, 0.0}, tot[2];
+  double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * Ap[i];
@@ -231,15 +241,22 @@ __global__ void k_cg_update(double* __restrict__ x, double* __restrict__ r, cons
     v[0] += ri * (dinv[i] * ri);
     v[1] += ri * ri;
   }
-  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_BETA] = rz != 0.0 ? tot[0] / rz : 0.0;
-    S[S_RZ] = tot[0];
-    S[S_RR] = tot[1];
-  }
+  block_partials<2, false>(v, rd);
 }
+struct OpCgUpdate {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    const double rz = S[S_RZ];
+    S[S_BETA] = rz != 0.0 ? t[0] / rz : 0.0;
+    S[S_RZ] = t[0];
+    S[S_RR] = t[1];
+  }
+};
 void launch_cg_update(double* x, double* r, const double* p, const double* Ap, const double* dinv, long long n,
                       double* S, Reducer rd, hipStream_t st) {
-  hipLaunchKernelGGL(k_cg_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, x, r, p, Ap, dinv, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kBlock), 0, st, x, r, p, Ap, dinv, n, S, rd);
+  reduce_finish<2, false>(rd, nb, OpCgUpdate{S}, st);
 }
 
 __global__ void k_cg_pupdate(double* __restrict__ p, const double* __restrict__ r, const double* __restrict__ dinv,
@@ -258,41 +275,54 @@ __global__ void k_fcg_update(double* __restrict__ x, double* __restrict__ r, con
                              const double* __restrict__ Ap, long long n, double* S, Reducer rd) {
   const double rz = S[S_RZ], pap = S[S_PAP];
   const double alpha = (pap != 0.0 && isfinite(pap)) ? rz / pap : 0.0;
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     x[i] += alpha * p[i];
     const double ri = r[i] - alpha * Ap[i];
     r[i] = ri;
     v[0] += ri * ri;
   }
-  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_RR] = tot[0];
-    S[S_ALPHA] = alpha;
-  }
+  block_partials<1, false>(v, rd);
 }
+struct OpFcgUpdate {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    const double rz = S[S_RZ], pap = S[S_PAP];
+    S[S_RR] = t[0];
+    S[S_ALPHA] = (pap != 0.0 && isfinite(pap)) ? rz / pap : 0.0;
+  }
+};
 void launch_fcg_update(double* x, double* r, const double* p, const double* Ap, long long n, double* S, Reducer rd,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_fcg_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, x, r, p, Ap, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_fcg_update, dim3(nb), dim3(kBlock), 0, st, x, r, p, Ap, n, S, rd);
+  reduce_finish<1, false>(rd, nb, OpFcgUpdate{S}, st);
 }
 
 // z.r and z.Ap; beta = (z.(r_new - r_old)) / rz_old = -alpha z.Ap / rz_old (Polak-Ribiere).
 __global__ void k_fcg_dots(const double* __restrict__ z, const double* __restrict__ r, const double* __restrict__ Ap,
                            long long n, double* S, Reducer rd) {
-  double v[2] = {0.0, 0.0}, tot[2];
+  double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double zi = z[i];
     v[0] += zi * r[i];
     v[1] += zi * Ap[i];
   }
-  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
-    const double rz = S[S_RZ];
-    S[S_BETA] = rz != 0.0 ? -S[S_ALPHA] * tot[1] / rz : 0.0;
-    S[S_RZ] = tot[0];
-  }
+  block_partials<2, false>(v, rd);
 }
+struct OpFcgDots {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    const double rz = S[S_RZ];
+    S[S_BETA] = rz != 0.0 ? -S[S_ALPHA] * t[1] / rz : 0.0;
+    S[S_RZ] = t[0];
+  }
+};
 void launch_fcg_dots(const double* z, const double* r, const double* Ap, long long n, double* S, Reducer rd,
                      hipStream_t st) {
-  hipLaunchKernelGGL(k_fcg_dots, dim3(reduce_grid(n)), dim3(kBlock), 0, st, z, r, Ap, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_fcg_dots, dim3(nb), dim3(kBlock), 0, st, z, r, Ap, n, S, rd);
+  reduce_finish<2, false>(rd, nb, OpFcgDots{S}, st);
 }
 
 __global__ void k_fcg_p(double* __restrict__ p, const double* __restrict__ z, long long n, const double* S) {
@@ -309,21 +339,27 @@ void launch_fcg_p(double* p, const double* z, long long n, const double* S, hipS
 // ---------------------------------------------------------------------------
 __global__ void k_uz_precond(const double* __restrict__ r, const double* __restrict__ Qinv, double* __restrict__ z,
                              long long n, double* S, Reducer rd) {
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double zi = Qinv ? r[i] * Qinv[i] : r[i];
     z[i] = zi;
     v[0] += zi * r[i];
   }
-  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
-    const double prev = S[S_ZR_PREV];
-    S[S_UBETA] = prev != 0.0 ? tot[0] / prev : 0.0;
-    S[S_ZR] = tot[0];
-  }
+  block_partials<1, false>(v, rd);
 }
+struct OpUzPrecond {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    const double prev = S[S_ZR_PREV];
+    S[S_UBETA] = prev != 0.0 ? t[0] / prev : 0.0;
+    S[S_ZR] = t[0];
+  }
+};
 void launch_uz_precond(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,
                        hipStream_t st) {
-  hipLaunchKernelGGL(k_uz_precond, dim3(reduce_grid(n)), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_uz_precond, dim3(nb), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+  reduce_finish<1, false>(rd, nb, OpUzPrecond{S}, st);
 }
 
 __global__ void k_uz_sdir(double* __restrict__ s, const double* __restrict__ z, long long n, const double* S,
@@ -339,22 +375,28 @@ void launch_uz_sdir(double* s, const double* z, long long n, const double* S, in
 __global__ void k_uz_update(double* __restrict__ p, const double* __restrict__ s, double* __restrict__ r,
                             const double* __restrict__ q, long long n, double* S, Reducer rd) {
   const double alpha = S[S_ZR] / S[S_SQ];
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     p[i] += alpha * s[i];
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     v[0] += ri * ri;
   }
-  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_ORR] = tot[0];
-    S[S_ZR_PREV] = S[S_ZR];
-    S[S_TMP0] = alpha;
-  }
+  block_partials<1, false>(v, rd);
 }
+struct OpUzUpdate {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    S[S_ORR] = t[0];
+    S[S_TMP0] = S[S_ZR] / S[S_SQ];
+    S[S_ZR_PREV] = S[S_ZR];
+  }
+};
 void launch_uz_update(double* p, const double* s, double* r, const double* q, long long n, double* S, Reducer rd,
                       hipStream_t st) {
-  hipLaunchKernelGGL(k_uz_update, dim3(reduce_grid(n)), dim3(kBlock), 0, st, p, s, r, q, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_uz_update, dim3(nb), dim3(kBlock), 0, st, p, s, r, q, n, S, rd);
+  reduce_finish<1, false>(rd, nb, OpUzUpdate{S}, st);
 }
 
 __global__ void k_uz_uupdate(double* __restrict__ u, const double* __restrict__ v, long long n, const double* S) {

@@ -77,13 +77,13 @@ __global__ void k_elem_logeta(const double4* __restrict__ pk, const int* __restr
 // t = Qinv r (into z); S[S_LP_TT] = sum t^2
 __global__ void k_lp_prep(const double* __restrict__ r, const double* __restrict__ Qinv, double* __restrict__ z,
                           long long n, double* S, Reducer rd) {
-  double v[1] = {0.0}, tot[1];
+  double v[1] = {0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double t = Qinv[i] * r[i];
     z[i] = t;
     v[0] += t * t;
   }
-  if (reduce_last<1, false>(v, rd, tot) && threadIdx.x == 0) S[S_LP_TT] = tot[0];
+  block_partials<1, false>(v, rd);
 }
 
 __global__ void k_lp_input(const double* __restrict__ t, const float* __restrict__ logeta, float* __restrict__ in2,
@@ -100,37 +100,46 @@ __global__ void k_lp_input(const double* __restrict__ t, const float* __restrict
 __global__ void k_lp_combine(const double* __restrict__ r, const double* __restrict__ qprev, double* __restrict__ z,
                              const float* __restrict__ c, long long n, double* S, Reducer rd) {
   const double rms = sqrt(S[S_LP_TT] / (double)n);
-  double v[2] = {0.0, 0.0}, tot[2];
+  double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double zi = z[i] + rms * (double)c[i];
     z[i] = zi;
     v[0] += zi * r[i];
     if (qprev) v[1] += zi * qprev[i];
   }
-  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_ZR] = tot[0];
-    S[S_LP_ZQ] = tot[1];
-    const double prev = S[S_ZR_PREV];
-    S[S_UBETA] = (qprev && prev != 0.0) ? -S[S_TMP0] * tot[1] / prev : 0.0;
-  }
+  block_partials<2, false>(v, rd);
 }
+struct OpLpCombine {
+  double* S;
+  int has_prev;
+  __device__ void operator()(const double* t) const {
+    S[S_ZR] = t[0];
+    S[S_LP_ZQ] = t[1];
+    const double prev = S[S_ZR_PREV];
+    S[S_UBETA] = (has_prev && prev != 0.0) ? -S[S_TMP0] * t[1] / prev : 0.0;
+  }
+};
 
 // training pair statistics: t = Qinv r; sums t^2, e, e^2
 __global__ void k_ds_stats(const double* __restrict__ r, const double* __restrict__ Qinv,
                            const double* __restrict__ e, long long n, double* S, Reducer rd) {
-  double v[3] = {0.0, 0.0, 0.0}, tot[3];
+  double v[3] = {0.0, 0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const double t = Qinv[i] * r[i];
     v[0] += t * t;
     v[1] += e[i];
     v[2] += e[i] * e[i];
   }
-  if (reduce_last<3, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_DS_TT] = tot[0];
-    S[S_DS_E] = tot[1];
-    S[S_DS_EE] = tot[2];
-  }
+  block_partials<3, false>(v, rd);
 }
+struct OpDsStats {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    S[S_DS_TT] = t[0];
+    S[S_DS_E] = t[1];
+    S[S_DS_EE] = t[2];
+  }
+};
 
 // in2 = {t / rms, logeta}; target = (e - t) / rms; weight = rms^2 / |P e|^2
 __global__ void k_ds_write(const double* __restrict__ r, const double* __restrict__ Qinv,
@@ -207,17 +216,22 @@ __global__ void k_orth(const double* __restrict__ z, double* sring, const double
 
 __global__ void k_flex_alpha(const double* __restrict__ s, const double* __restrict__ r, const double* __restrict__ q,
                              long long n, double* S, double* sq_out, Reducer rd) {
-  double v[2] = {0.0, 0.0}, tot[2];
+  double v[2] = {0.0, 0.0};
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     v[0] += s[i] * r[i];
     v[1] += s[i] * q[i];
   }
-  if (reduce_last<2, false>(v, rd, tot) && threadIdx.x == 0) {
-    S[S_ZR] = tot[0];
-    S[S_SQ] = tot[1];
-    sq_out[0] = tot[1];
-  }
+  block_partials<2, false>(v, rd);
 }
+struct OpFlexAlpha {
+  double* S;
+  double* sq_out;
+  __device__ void operator()(const double* t) const {
+    S[S_ZR] = t[0];
+    S[S_SQ] = t[1];
+    sq_out[0] = t[1];
+  }
+};
 
 void launch_orthogonalise(const double* z, double* sring, const double* qring, const double* sq, int c, int out,
                           long long n, double* partials, double* beta, hipStream_t st) {
@@ -231,7 +245,9 @@ void launch_orthogonalise(const double* z, double* sring, const double* qring, c
 
 void launch_flex_alpha(const double* s, const double* r, const double* q, long long n, double* S, double* sq_out,
                        Reducer rd, hipStream_t st) {
-  hipLaunchKernelGGL(k_flex_alpha, dim3(reduce_grid(n)), dim3(kBlock), 0, st, s, r, q, n, S, sq_out, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_flex_alpha, dim3(nb), dim3(kBlock), 0, st, s, r, q, n, S, sq_out, rd);
+  reduce_finish<2, false>(rd, nb, OpFlexAlpha{S, sq_out}, st);
 }
 
 void launch_elem_logeta(const Geo& g, const double4* pk, const int* off, float* out, hipStream_t st) {
@@ -258,12 +274,15 @@ int lpc_precond(gs_lpc* L, const Geo& g, const double* r, const double* Qinv, co
   const long long n = g.E;
   int e = ensure_infer(L, n);
   if (e) return e;
-  hipLaunchKernelGGL(k_lp_prep, dim3(reduce_grid(n)), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_lp_prep, dim3(nb), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+  reduce_finish<1, false>(rd, nb, OpStore{S, S_LP_TT}, st);
   hipLaunchKernelGGL(k_lp_input, dim3(grid_for(n)), dim3(kBlock), 0, st, z, logeta, L->in2, n, S);
   L->net.st = st;
   const net::Shape sh{g.res[0], g.res[1], g.res[2], 1};
   if ((e = net::forward(&L->net, sh, L->in2, L->out))) return e;
-  hipLaunchKernelGGL(k_lp_combine, dim3(reduce_grid(n)), dim3(kBlock), 0, st, r, qprev, z, L->out, n, S, rd);
+  hipLaunchKernelGGL(k_lp_combine, dim3(nb), dim3(kBlock), 0, st, r, qprev, z, L->out, n, S, rd);
+  reduce_finish<2, false>(rd, nb, OpLpCombine{S, qprev ? 1 : 0}, st);
   return GS_OK;
 }
 
@@ -274,7 +293,9 @@ int lpc_record(gs_lpc_dataset* D, const Geo& g, const double* rlike, const doubl
   if (D->size >= D->cap) return GS_OK;
   const long long n = g.E;
   const long long idx = D->size;
-  hipLaunchKernelGGL(k_ds_stats, dim3(reduce_grid(n)), dim3(kBlock), 0, st, rlike, Qinv, elike, n, S, rd);
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_ds_stats, dim3(nb), dim3(kBlock), 0, st, rlike, Qinv, elike, n, S, rd);
+  reduce_finish<3, false>(rd, nb, OpDsStats{S}, st);
   hipLaunchKernelGGL(k_ds_write, dim3(grid_for(n)), dim3(kBlock), 0, st, rlike, Qinv, elike, logeta, n, S, proj,
                      D->in2 + idx * 2 * n, D->tgt + idx * n, D->w + idx, D->proj + idx);
   D->size++;

@@ -76,7 +76,6 @@ struct Work {
   float* partial = nullptr;           // weight-gradient partial sums
   long long partial_cap = 0;
   double* dpartial = nullptr;         // reductions
-  unsigned* counter = nullptr;
 };
 
 struct Net {

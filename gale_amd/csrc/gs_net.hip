@@ -122,21 +122,35 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, c
   for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
   constexpr int KT = 27 * CIN;
   constexpr int KC = (KT + 15) / 16;
-  for (int kc = 0; kc < KC; ++kc) {
-    const int kk = kc * 16 + kg * 4;
-    s4 a = s4{0, 0, 0, 0};
-    if (valid && kk < KT) {
-      const int tap = kk / CIN;
-      const int ci = kk - tap * CIN;
-      const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
-      if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
-        const long long nb = bbase + xx + (long long)nx * (yy + (long long)ny * zz);
-        a = *reinterpret_cast<const s4*>(in + nb * CIN + ci);
+  // K steps in groups of U: all operand loads of a group are issued before
+  // its MFMAs, so a wave keeps U x (1 + NT) loads in flight instead of
+  // waiting on one load per step.
+  constexpr int U = KC < 9 ? KC : 9;
+  for (int kc0 = 0; kc0 < KC; kc0 += U) {
+    s4 a[U];
+    s4 w[U][NT];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kc = kc0 + u;
+      const int kk = kc * 16 + kg * 4;
+      a[u] = s4{0, 0, 0, 0};
+      if (valid && kc < KC && kk < KT) {
+        const int tap = kk / CIN;
+        const int ci = kk - tap * CIN;
+        const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
+        if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+          const long long nb = bbase + xx + (long long)nx * (yy + (long long)ny * zz);
+          a[u] = *reinterpret_cast<const s4*>(in + nb * CIN + ci);
+        }
       }
+#pragma unroll
+      for (int t = 0; t < NT; t++) w[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : s4{0, 0, 0, 0};
     }
 #pragma unroll
-    for (int t = 0; t < NT; t++)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, Wp[(kc * NT + t) * 64 + lane], acc[t], 0, 0, 0);
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], w[u][t], acc[t], 0, 0, 0);
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) {
@@ -566,7 +580,6 @@ static void free_work(Work& w) {
   f(w.gc);
   f(w.partial);
   f(w.dpartial);
-  f(w.counter);
   w = Work{};
 }
 
@@ -628,8 +641,6 @@ int ensure_work(Net* n, const Shape& sh) {
   w.partial_cap = pc;
   NCK(hipMalloc(&w.partial, sizeof(float) * pc));
   NCK(hipMalloc(&w.dpartial, sizeof(double) * 65536));
-  NCK(hipMalloc(&w.counter, sizeof(unsigned)));
-  NCK(hipMemset(w.counter, 0, sizeof(unsigned)));
   return GS_OK;
 }
 

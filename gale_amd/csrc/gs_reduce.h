@@ -17,13 +17,15 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Block reduction of NV values followed by a deterministic cross-block pass
-// in the last block to finish (partials are summed in block order).  Returns
-// true in the last block, with the totals in tot[] for every thread.
+// Deterministic two-launch grid reduction.  Each block stores its NV block
+// sums (or maxima) to partials[block * NV + k] with plain stores; a one-block
+// finishing launch adds them in block order and hands the totals to Op on
+// thread 0.  The launch boundary orders the two, so no cross-block fence or
+// arrival counter is needed: on a chip whose XCDs each have their own L2, an
+// agent-scope fence per block costs more than the reduction itself.
 template <int NV, bool MAX>
-__device__ bool reduce_last(double (&v)[NV], Reducer rd, double (&tot)[NV]) {
+__device__ void block_partials(double (&v)[NV], Reducer rd) {
   __shared__ double sh[NV][kBlock / 64];
-  __shared__ int last;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) v[k] = MAX ? wave_max(v[k]) : wave_sum(v[k]);
@@ -31,43 +33,53 @@ __device__ bool reduce_last(double (&v)[NV], Reducer rd, double (&tot)[NV]) {
 #pragma unroll
     for (int k = 0; k < NV; k++) sh[k][wid] = v[k];
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
 #pragma unroll
     for (int k = 0; k < NV; k++) {
       double s = sh[k][0];
       for (int w = 1; w < nw; w++) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
-      __hip_atomic_store(&rd.partials[blockIdx.x * NV + k], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      rd.partials[blockIdx.x * NV + k] = s;
     }
-    __threadfence();
-    const unsigned prev = atomicAdd(rd.counter, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return false;
-  __threadfence();
+}
+
+template <int NV, bool MAX, typename Op>
+__global__ void k_reduce_finish(Reducer rd, int nb, Op op) {
+  __shared__ double sh[NV][kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; k++) {
     double s = MAX ? -INFINITY : 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
-      const double pv = __hip_atomic_load(&rd.partials[b * NV + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const double pv = rd.partials[b * NV + k];
       s = MAX ? fmax(s, pv) : s + pv;
     }
-    v[k] = MAX ? wave_max(s) : wave_sum(s);
+    s = MAX ? wave_max(s) : wave_sum(s);
+    if (lane == 0) sh[k][wid] = s;
   }
   __syncthreads();
-  if (lane == 0)
+  if (threadIdx.x == 0) {
+    double tot[NV];
 #pragma unroll
-    for (int k = 0; k < NV; k++) sh[k][wid] = v[k];
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < NV; k++) {
-    double s = sh[k][0];
-    for (int w = 1; w < nw; w++) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
-    tot[k] = s;
+    for (int k = 0; k < NV; k++) {
+      double s = sh[k][0];
+      for (int w = 1; w < nw; w++) s = MAX ? fmax(s, sh[k][w]) : s + sh[k][w];
+      tot[k] = s;
+    }
+    op(tot);
   }
-  if (threadIdx.x == 0) __hip_atomic_store(rd.counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return true;
 }
+
+template <int NV, bool MAX, typename Op>
+inline void reduce_finish(Reducer rd, int nb, const Op& op, hipStream_t st) {
+  hipLaunchKernelGGL((k_reduce_finish<NV, MAX, Op>), dim3(1), dim3(kBlock), 0, st, rd, nb, op);
+}
+
+// S[slot] = total
+struct OpStore {
+  double* S;
+  int slot;
+  __device__ void operator()(const double* t) const { S[slot] = t[0]; }
+};
 
 inline int reduce_grid(long long n) {
   long long b = (n + kBlock - 1) / kBlock;

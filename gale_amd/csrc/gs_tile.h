@@ -15,6 +15,59 @@ struct TileGrid {
   long long ntiles;
 };
 
+// Node update fused into a K apply where the apply completes a node, so a
+// multigrid smoothing step or residual needs no separate vector pass.  With
+// ax = (K x)_i (zero on Dirichlet dofs):
+//   kind 0: y_i = ax
+//   kind 1: Chebyshev step  d_i = alpha d_i + beta dinv_i (b_i - ax),  xout_i = x_i + d_i
+//   kind 2: residual  r_i = b_i - ax
+// The update runs in the node-parallel launch that completes tile-face
+// nodes; y holds the apply's partial sums.  xout must not alias x.
+template <typename T>
+struct TileEpi {
+  int kind;
+  const T* b;
+  const T* dinv;
+  T* d;
+  T* xout;
+  T* r;
+  T alpha, beta;
+};
+
+// The bottom of a 3D multigrid V-cycle (the smallest levels and the coarsest
+// solve) in one workgroup: each level's K applies accumulate into one LDS
+// node array, so the whole bottom is a single launch instead of a dozen
+// small ones per level.  Same smoothing, transfers and coarsest solve as the
+// multi-launch cycle; sums are taken in a different (fixed) order.
+template <typename T>
+struct BotLevel {
+  Geo g;
+  const T* M;            // moments, SoA
+  const T* dinv;         // masked inverse diagonal
+  const uint8_t* bc;     // Dirichlet node flags (bit c: dof c)
+  T *b, *x, *x2, *r, *d; // work vectors (x, x2: ping-pong)
+  double lmax;           // Chebyshev bound of D^-1 K
+};
+constexpr int kBotMaxLevels = 6;
+constexpr int kBotMaxDense = 1024;
+template <typename T>
+struct BotArgs {
+  BotLevel<T> lv[kBotMaxLevels];
+  int nlev;              // lv[nlev - 1] is the coarsest level
+  int nu;                // Chebyshev steps per smoothing
+  const T* Kinv;         // dense coarsest inverse (nfc x nfc), or nullptr
+  const int* freeidx;
+  int nfc;               // 0: eight Chebyshev steps on the coarsest level
+};
+// A level fits the bottom kernel when its node accumulator fits in 64 KiB of LDS.
+template <typename T>
+inline bool bot_fits(const Geo& g) {
+  return g.dim == 3 && g.N * 3 * (long long)sizeof(T) <= 65536;
+}
+// Solves lv[0] (b given) by the V-cycle below it; the result is left in lv[0].x.
+void mg_bottom3(const BotArgs<float>& a, hipStream_t st);
+void mg_bottom3(const BotArgs<double>& a, hipStream_t st);
+
 TileGrid tile_grid(const Geo& g);
 // Halo buffer length (values) for one apply on this mesh.
 long long tile_halo_size(const Geo& g);
@@ -27,5 +80,10 @@ void tile_apply_moments(const Geo& g, const double* x, double* y, double* halo, 
                         const uint8_t* bcflag, hipStream_t st);
 void tile_apply_moments32(const Geo& g, const float* x, float* y, float* halo, const float* Mom,
                           const uint8_t* bcflag, hipStream_t st);
+// Moment-form apply with a fused node update (kind 0 is tile_apply_moments).
+void tile_apply_moments_epi(const Geo& g, const double* x, double* y, double* halo, const double* Mom,
+                            const uint8_t* bcflag, const TileEpi<double>& ep, hipStream_t st);
+void tile_apply_moments_epi(const Geo& g, const float* x, float* y, float* halo, const float* Mom,
+                            const uint8_t* bcflag, const TileEpi<float>& ep, hipStream_t st);
 
 }  // namespace gs

@@ -44,6 +44,26 @@ struct V4<float> {
 
 #define MI(a, b, c) ((a) + 3 * (b) + 9 * (c))
 
+__device__ __forceinline__ double wave_sum_bot(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// Completes dof i with ax = (K x)_i (see TileEpi); xv = x_i.
+template <typename T, int EPI>
+__device__ __forceinline__ void epi_store(const TileEpi<T>& ep, T* __restrict__ y, long long i, T ax, T xv) {
+  if (EPI == 0) {
+    y[i] = ax;
+  } else if (EPI == 1) {
+    const T rr = ep.b[i] - ax;
+    const T di = ep.alpha * ep.d[i] + ep.beta * ep.dinv[i] * rr;
+    ep.d[i] = di;
+    ep.xout[i] = xv + di;
+  } else {
+    ep.r[i] = ep.b[i] - ax;
+  }
+}
+
 // ---- 3D element kernels: gradient coefficients from the 8 nodal vectors ----
 template <typename T>
 __device__ __forceinline__ void grad_coeffs3(const T (&u)[8][3], T ix, T iy, T iz, T (&A)[3][4], T (&B)[3][4],
@@ -293,10 +313,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
   }
 }
 
-// Adds the halo partial sums of up to seven lower neighbour tiles.
-template <typename T>
-__global__ void k_tile_halo3(Geo g, TileGrid tg, T* __restrict__ y, const T* __restrict__ halo,
-                             const uint8_t* __restrict__ bcflag) {
+// Adds the halo partial sums of up to seven lower neighbour tiles.  With a
+// fused node update (EPI != 0) every node passes through here, so the update
+// streams all vectors once after the apply.
+template <typename T, int EPI>
+__global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __restrict__ y,
+                             const T* __restrict__ halo, const uint8_t* __restrict__ bcflag, TileEpi<T> ep) {
   constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
   constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -304,7 +326,7 @@ __global__ void k_tile_halo3(Geo g, TileGrid tg, T* __restrict__ y, const T* __r
   const int i = (int)(n % g.nn[0]), j = (int)((n / g.nn[0]) % g.nn[1]), k = (int)(n / (g.nn[0] * g.nn[1]));
   const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1), tz = min(k / TZ, tg.n[2] - 1);
   const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0), onz = (k == tz * TZ && tz > 0);
-  if (!(onx | ony | onz)) return;
+  if (EPI == 0 && !(onx | ony | onz)) return;
   T acc[3] = {0, 0, 0};
   for (int az = 0; az <= onz; az++)
     for (int ay = 0; ay <= ony; ay++)
@@ -319,8 +341,15 @@ __global__ void k_tile_halo3(Geo g, TileGrid tg, T* __restrict__ y, const T* __r
         acc[2] += h[2];
       }
   const unsigned bf = bcflag ? bcflag[n] : 0u;
-  for (int c = 0; c < 3; c++)
-    if (!((bf >> c) & 1u)) y[n * 3 + c] += acc[c];
+  if (EPI == 0) {
+    for (int c = 0; c < 3; c++)
+      if (!((bf >> c) & 1u)) y[n * 3 + c] += acc[c];
+  } else {
+    for (int c = 0; c < 3; c++) {
+      const T ax = ((bf >> c) & 1u) ? (T)0 : y[n * 3 + c] + acc[c];
+      epi_store<T, EPI>(ep, y, n * 3 + c, ax, x[n * 3 + c]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -393,9 +422,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply2(Geo g, TileGrid tg
   }
 }
 
-template <typename T>
-__global__ void k_tile_halo2(Geo g, TileGrid tg, T* __restrict__ y, const T* __restrict__ halo,
-                             const uint8_t* __restrict__ bcflag) {
+template <typename T, int EPI>
+__global__ void k_tile_halo2(Geo g, TileGrid tg, const T* __restrict__ x, T* __restrict__ y,
+                             const T* __restrict__ halo, const uint8_t* __restrict__ bcflag, TileEpi<T> ep) {
   constexpr int TX = kTile2X, TY = kTile2Y;
   constexpr int NBX = TX + 1, NBY = TY + 1, NB = NBX * NBY;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -403,7 +432,7 @@ __global__ void k_tile_halo2(Geo g, TileGrid tg, T* __restrict__ y, const T* __r
   const int i = (int)(n % g.nn[0]), j = (int)(n / g.nn[0]);
   const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1);
   const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0);
-  if (!(onx | ony)) return;
+  if (EPI == 0 && !(onx | ony)) return;
   T acc[2] = {0, 0};
   for (int ay = 0; ay <= ony; ay++)
     for (int ax = 0; ax <= onx; ax++) {
@@ -415,8 +444,15 @@ __global__ void k_tile_halo2(Geo g, TileGrid tg, T* __restrict__ y, const T* __r
       acc[1] += halo[(t2 * NB + ln) * 2 + 1];
     }
   const unsigned bf = bcflag ? bcflag[n] : 0u;
-  for (int c = 0; c < 2; c++)
-    if (!((bf >> c) & 1u)) y[n * 2 + c] += acc[c];
+  if (EPI == 0) {
+    for (int c = 0; c < 2; c++)
+      if (!((bf >> c) & 1u)) y[n * 2 + c] += acc[c];
+  } else {
+    for (int c = 0; c < 2; c++) {
+      const T ax = ((bf >> c) & 1u) ? (T)0 : y[n * 2 + c] + acc[c];
+      epi_store<T, EPI>(ep, y, n * 2 + c, ax, x[n * 2 + c]);
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -442,19 +478,35 @@ long long tile_halo_size(const Geo& g) {
   return tg.ntiles * tg.nb * g.dim;
 }
 
-template <typename T, int FORM>
-static void tile_apply(const Geo& g, const T* x, T* y, T* halo, const T* Mom, const typename V4<T>::type* pk,
-                       const int* off, const uint8_t* bcflag, hipStream_t st) {
+template <typename T, int FORM, int EPI>
+static void tile_launch(const Geo& g, const T* x, T* y, T* halo, const T* Mom, const typename V4<T>::type* pk,
+                        const int* off, const uint8_t* bcflag, const TileEpi<T>& ep, hipStream_t st) {
   const TileGrid tg = tile_grid(g);
   const unsigned nbn = (unsigned)((g.N + kBlock - 1) / kBlock);
   if (g.dim == 3) {
-    hipLaunchKernelGGL((k_tile_apply3<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x, y,
-                       halo, Mom, pk, off, bcflag);
-    hipLaunchKernelGGL(k_tile_halo3<T>, dim3(nbn), dim3(kBlock), 0, st, g, tg, y, halo, bcflag);
+    hipLaunchKernelGGL((k_tile_apply3<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
+                       y, halo, Mom, pk, off, bcflag);
+    hipLaunchKernelGGL((k_tile_halo3<T, EPI>), dim3(nbn), dim3(kBlock), 0, st, g, tg, x, y, halo, bcflag, ep);
   } else {
-    hipLaunchKernelGGL((k_tile_apply2<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x, y,
-                       halo, Mom, pk, off, bcflag);
-    hipLaunchKernelGGL(k_tile_halo2<T>, dim3(nbn), dim3(kBlock), 0, st, g, tg, y, halo, bcflag);
+    hipLaunchKernelGGL((k_tile_apply2<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
+                       y, halo, Mom, pk, off, bcflag);
+    hipLaunchKernelGGL((k_tile_halo2<T, EPI>), dim3(nbn), dim3(kBlock), 0, st, g, tg, x, y, halo, bcflag, ep);
+  }
+}
+
+template <typename T, int FORM>
+static void tile_apply(const Geo& g, const T* x, T* y, T* halo, const T* Mom, const typename V4<T>::type* pk,
+                       const int* off, const uint8_t* bcflag, hipStream_t st) {
+  tile_launch<T, FORM, 0>(g, x, y, halo, Mom, pk, off, bcflag, TileEpi<T>{}, st);
+}
+
+template <typename T>
+static void tile_apply_epi(const Geo& g, const T* x, T* y, T* halo, const T* Mom, const uint8_t* bcflag,
+                           const TileEpi<T>& ep, hipStream_t st) {
+  switch (ep.kind) {
+    case 1: tile_launch<T, 0, 1>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, ep, st); break;
+    case 2: tile_launch<T, 0, 2>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, ep, st); break;
+    default: tile_launch<T, 0, 0>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, ep, st); break;
   }
 }
 
@@ -474,5 +526,226 @@ void tile_apply_moments32(const Geo& g, const float* x, float* y, float* halo, c
                           const uint8_t* bcflag, hipStream_t st) {
   tile_apply<float, 0>(g, x, y, halo, Mom, nullptr, nullptr, bcflag, st);
 }
+
+void tile_apply_moments_epi(const Geo& g, const double* x, double* y, double* halo, const double* Mom,
+                            const uint8_t* bcflag, const TileEpi<double>& ep, hipStream_t st) {
+  tile_apply_epi<double>(g, x, y, halo, Mom, bcflag, ep, st);
+}
+void tile_apply_moments_epi(const Geo& g, const float* x, float* y, float* halo, const float* Mom,
+                            const uint8_t* bcflag, const TileEpi<float>& ep, hipStream_t st) {
+  tile_apply_epi<float>(g, x, y, halo, Mom, bcflag, ep, st);
+}
+
+// ---------------------------------------------------------------------------
+// Single-workgroup bottom of the 3D V-cycle
+// ---------------------------------------------------------------------------
+namespace {
+
+// sy[0, 3N) += K x (unmasked), elements in chunks of one per thread, each
+// chunk's forces added in eight conflict-free corner phases.
+template <typename T, int BS>
+__device__ void bot_kx(const BotLevel<T>& L, const T* __restrict__ x, T* sy) {
+  const Geo& g = L.g;
+  const long long E = g.E, sy_ = g.nn[0], sz_ = g.nn[0] * g.nn[1];
+  for (long long e0 = 0; e0 < E; e0 += BS) {
+    const long long e = e0 + threadIdx.x;
+    const bool act = e < E;
+    T ye[8][3];
+    long long n0 = 0;
+    if (act) {
+      const long long gi = e % g.res[0], gj = (e / g.res[0]) % g.res[1], gk = e / ((long long)g.res[0] * g.res[1]);
+      n0 = gi + sy_ * gj + sz_ * gk;
+      const T ix = (T)(0.25 / (vcoord_t(g, 0, gi + 1) - vcoord_t(g, 0, gi)));
+      const T iy = (T)(0.25 / (vcoord_t(g, 1, gj + 1) - vcoord_t(g, 1, gj)));
+      const T iz = (T)(0.25 / (vcoord_t(g, 2, gk + 1) - vcoord_t(g, 2, gk)));
+      T u[8][3];
+#pragma unroll
+      for (int a = 0; a < 8; a++) {
+        const long long nd = n0 + (a & 1) + sy_ * ((a >> 1) & 1) + sz_ * ((a >> 2) & 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) u[a][c] = x[nd * 3 + c];
+      }
+      T A[3][4], B[3][4], C[3][4];
+      grad_coeffs3(u, ix, iy, iz, A, B, C);
+      T m[30];
+      element_moments3(A, B, C, L.M, E, e, m);
+      epilogue3(m, ix, iy, iz, ye);
+    }
+#pragma unroll
+    for (int a = 0; a < 8; a++) {
+      if (act) {
+        const long long nd = n0 + (a & 1) + sy_ * ((a >> 1) & 1) + sz_ * ((a >> 2) & 1);
+#pragma unroll
+        for (int c = 0; c < 3; c++) sy[nd * 3 + c] += ye[a][c];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// (K x)_i with the Dirichlet mask; clears the accumulator entry.
+template <typename T>
+__device__ __forceinline__ T bot_take(const BotLevel<T>& L, T* sy, long long i) {
+  const unsigned bf = L.bc ? L.bc[i / 3] : 0u;
+  const T v = sy[i];
+  sy[i] = 0;
+  return ((bf >> (i % 3)) & 1u) ? (T)0 : v;
+}
+
+// Chebyshev-Jacobi smoothing (the multi-launch cheb_smooth, step for step);
+// x == nullptr starts from zero.  Returns the buffer holding x.
+template <typename T, int BS>
+__device__ T* bot_cheb(const BotLevel<T>& L, int nu, T* x, T* sy) {
+  const long long n = L.g.N * 3;
+  const double emax = 1.1 * L.lmax, emin = 0.1 * L.lmax;
+  const double theta = 0.5 * (emax + emin), delta = 0.5 * (emax - emin), sigma = theta / delta;
+  double rho_prev = 1.0 / sigma;
+  int k0 = 0;
+  if (!x) {
+    const T beta = (T)(1.0 / theta);
+    for (long long i = threadIdx.x; i < n; i += BS) {
+      const T di = beta * L.dinv[i] * L.b[i];
+      L.d[i] = di;
+      L.x[i] = di;
+    }
+    __syncthreads();
+    x = L.x;
+    k0 = 1;
+  }
+  for (int k = k0; k < nu; k++) {
+    T alpha = 0, beta = (T)(1.0 / theta);
+    if (k > 0) {
+      const double rho = 1.0 / (2.0 * sigma - rho_prev);
+      alpha = (T)(rho * rho_prev);
+      beta = (T)(2.0 * rho / delta);
+      rho_prev = rho;
+    }
+    bot_kx<T, BS>(L, x, sy);
+    T* xo = x == L.x ? L.x2 : L.x;
+    for (long long i = threadIdx.x; i < n; i += BS) {
+      const T rr = L.b[i] - bot_take(L, sy, i);
+      const T di = alpha * L.d[i] + beta * L.dinv[i] * rr;
+      L.d[i] = di;
+      xo[i] = x[i] + di;
+    }
+    __syncthreads();
+    x = xo;
+  }
+  return x;
+}
+
+// C.b = R r_F (full weighting, the multi-launch k_restrict).
+template <typename T, int BS>
+__device__ void bot_restrict(const BotLevel<T>& C, const BotLevel<T>& F) {
+  const Geo &gc = C.g, &gf = F.g;
+  for (long long n = threadIdx.x; n < gc.N; n += BS) {
+    const long long I = n % gc.nn[0], J = (n / gc.nn[0]) % gc.nn[1], K = n / (gc.nn[0] * gc.nn[1]);
+    T acc[3] = {0, 0, 0};
+    for (int dz = -1; dz <= 1; dz++) {
+      const long long k = 2 * K + dz;
+      if (k < 0 || k >= gf.nn[2]) continue;
+      const T wz = dz == 0 ? (T)1 : (T)0.5;
+      for (int dy = -1; dy <= 1; dy++) {
+        const long long j = 2 * J + dy;
+        if (j < 0 || j >= gf.nn[1]) continue;
+        const T wy = dy == 0 ? (T)1 : (T)0.5;
+        for (int dx = -1; dx <= 1; dx++) {
+          const long long i = 2 * I + dx;
+          if (i < 0 || i >= gf.nn[0]) continue;
+          const T w = wz * wy * (dx == 0 ? (T)1 : (T)0.5);
+          const long long f = i + gf.nn[0] * (j + gf.nn[1] * k);
+          for (int d = 0; d < 3; d++) acc[d] += w * F.r[f * 3 + d];
+        }
+      }
+    }
+    const unsigned bf = C.bc ? C.bc[n] : 0u;
+    for (int d = 0; d < 3; d++) C.b[n * 3 + d] = ((bf >> d) & 1u) ? (T)0 : acc[d];
+  }
+  __syncthreads();
+}
+
+// xf += P xc (Q1 interpolation, the multi-launch k_prolong_add).
+template <typename T, int BS>
+__device__ void bot_prolong(const BotLevel<T>& C, const T* __restrict__ xc, const BotLevel<T>& F, T* xf) {
+  const Geo &gc = C.g, &gf = F.g;
+  for (long long n = threadIdx.x; n < gf.N; n += BS) {
+    const long long i = n % gf.nn[0], j = (n / gf.nn[0]) % gf.nn[1], k = n / (gf.nn[0] * gf.nn[1]);
+    T acc[3] = {0, 0, 0};
+    const int nz = (k & 1) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
+    const T wz = nz == 2 ? (T)0.5 : (T)1, wy = ny == 2 ? (T)0.5 : (T)1, wx = nx == 2 ? (T)0.5 : (T)1;
+    for (int a = 0; a < nz; a++)
+      for (int b = 0; b < ny; b++)
+        for (int cc = 0; cc < nx; cc++) {
+          const long long cn = ((i >> 1) + cc) + gc.nn[0] * (((j >> 1) + b) + gc.nn[1] * ((k >> 1) + a));
+          for (int d = 0; d < 3; d++) acc[d] += wx * wy * wz * xc[cn * 3 + d];
+        }
+    const unsigned bf = F.bc ? F.bc[n] : 0u;
+    for (int d = 0; d < 3; d++)
+      if (!((bf >> d) & 1u)) xf[n * 3 + d] += acc[d];
+  }
+  __syncthreads();
+}
+
+template <typename T, int BS>
+__global__ __launch_bounds__(BS) void k_mg_bottom3(BotArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char bot_smem[];
+  T* sy = reinterpret_cast<T*>(bot_smem);
+  for (long long i = threadIdx.x; i < a.lv[0].g.N * 3; i += BS) sy[i] = 0;
+  __syncthreads();
+  const int Lc = a.nlev - 1;
+  unsigned in2 = 0;  // bit l: level l's x is in x2
+  for (int l = 0; l < Lc; l++) {
+    const BotLevel<T>& L = a.lv[l];
+    T* x = bot_cheb<T, BS>(L, a.nu, (T*)nullptr, sy);
+    in2 |= (x == L.x2 ? 1u : 0u) << l;
+    bot_kx<T, BS>(L, x, sy);
+    for (long long i = threadIdx.x; i < L.g.N * 3; i += BS) L.r[i] = L.b[i] - bot_take(L, sy, i);
+    __syncthreads();
+    bot_restrict<T, BS>(a.lv[l + 1], L);
+  }
+  {
+    const BotLevel<T>& L = a.lv[Lc];
+    if (a.nfc > 0) {
+      for (long long i = threadIdx.x; i < L.g.N * 3; i += BS) L.x[i] = 0;
+      __syncthreads();
+      const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      for (int row = w; row < a.nfc; row += BS / 64) {
+        double s = 0;
+        for (int j = lane; j < a.nfc; j += 64) s += (double)(a.Kinv[(long long)row * a.nfc + j] * L.b[a.freeidx[j]]);
+        s = wave_sum_bot(s);
+        if (lane == 0) L.x[a.freeidx[row]] = (T)s;
+      }
+      __syncthreads();
+    } else {
+      T* x = bot_cheb<T, BS>(L, 8, (T*)nullptr, sy);
+      in2 |= (x == L.x2 ? 1u : 0u) << Lc;
+    }
+  }
+  for (int l = Lc - 1; l >= 0; l--) {
+    const BotLevel<T>& L = a.lv[l];
+    const BotLevel<T>& C = a.lv[l + 1];
+    T* x = ((in2 >> l) & 1u) ? L.x2 : L.x;
+    bot_prolong<T, BS>(C, ((in2 >> (l + 1)) & 1u) ? C.x2 : C.x, L, x);
+    x = bot_cheb<T, BS>(L, a.nu, x, sy);
+    in2 = (in2 & ~(1u << l)) | ((x == L.x2 ? 1u : 0u) << l);
+  }
+  if (in2 & 1u) {
+    const BotLevel<T>& L = a.lv[0];
+    for (long long i = threadIdx.x; i < L.g.N * 3; i += BS) L.x[i] = L.x2[i];
+  }
+}
+
+// FP32: 1024 threads (the element evaluation fits 128 VGPRs); FP64: 512.
+template <typename T>
+void bottom_launch(const BotArgs<T>& a, hipStream_t st) {
+  constexpr int BS = sizeof(T) == 4 ? 1024 : 512;
+  const size_t lds = (size_t)a.lv[0].g.N * 3 * sizeof(T);
+  hipLaunchKernelGGL((k_mg_bottom3<T, BS>), dim3(1), dim3(BS), lds, st, a);
+}
+
+}  // namespace
+
+void mg_bottom3(const BotArgs<float>& a, hipStream_t st) { bottom_launch<float>(a, st); }
+void mg_bottom3(const BotArgs<double>& a, hipStream_t st) { bottom_launch<double>(a, st); }
 
 }  // namespace gs
