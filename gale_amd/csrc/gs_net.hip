@@ -174,6 +174,167 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, c
 // Weight gradient: P[chunk][kk][co] = sum_{vox in chunk} X[nbr(vox,tap)][ci] * G[vox][co]
 // A = X (rows kk, K = voxels), B = G (K = voxels, columns co).  One wavefront
 // per 16-row kk tile and voxel chunk; grid (ceil(KC/4), nchunks) x 256.
+// Split-K form of k_conv for the small levels, where a wave per 16 voxels
+// leaves most of the chip idle and each wave would walk all 27 x CIN / 16 K
+// steps in sequence: S waves share one 16-voxel tile, wave w taking K steps
+// in groups of four interleaved with the others, and their partial sums are
+// added in wave order through LDS.
+template <int CIN, int NT, bool OUT_F32, int S>
+__global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restrict__ in, const s4* __restrict__ Wp,
+                                                        const float* __restrict__ bias, void* __restrict__ out,
+                                                        int cout_store, int nx, int ny, int nz, long long BV,
+                                                        int relu) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const long long v0 = (long long)blockIdx.x * 16;
+  const int row = lane & 15, kg = lane >> 4;
+  const long long V = (long long)nx * ny * nz;
+  const long long vox = v0 + row;
+  const bool valid = vox < BV;
+  long long b = 0;
+  int x = 0, y = 0, z = 0;
+  if (valid) vox_xyz(vox, V, nx, ny, &b, &x, &y, &z);
+  const long long bbase = b * V;
+  f4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  constexpr int KT = 27 * CIN;
+  constexpr int KC = (KT + 15) / 16;
+  constexpr int U = 4;
+  for (int kc0 = w * U; kc0 < KC; kc0 += S * U) {
+    s4 a[U];
+    s4 wv[U][NT];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kc = kc0 + u;
+      const int kk = kc * 16 + kg * 4;
+      a[u] = s4{0, 0, 0, 0};
+      if (valid && kc < KC && kk < KT) {
+        const int tap = kk / CIN;
+        const int ci = kk - tap * CIN;
+        const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
+        if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
+          const long long nb = bbase + xx + (long long)nx * (yy + (long long)ny * zz);
+          a[u] = *reinterpret_cast<const s4*>(in + nb * CIN + ci);
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < NT; t++) wv[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : s4{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int t = 0; t < NT; t++)
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], wv[u][t], acc[t], 0, 0, 0);
+  }
+  __shared__ f4 red[S][NT][64];
+#pragma unroll
+  for (int t = 0; t < NT; t++) red[w][t][lane] = acc[t];
+  __syncthreads();
+  if (w != 0) return;
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    f4 sum = red[0][t][lane];
+    for (int ww = 1; ww < S; ww++) sum += red[ww][t][lane];
+    const int co = t * 16 + row;
+    if (co >= cout_store) continue;
+    const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const long long vo = v0 + kg * 4 + i;
+      if (vo >= BV) continue;
+      float val = sum[i] + bv;
+      if (relu) val = fmaxf(val, 0.f);
+      if (OUT_F32)
+        reinterpret_cast<float*>(out)[vo * cout_store + co] = val;
+      else
+        reinterpret_cast<uint16_t*>(out)[vo * cout_store + co] = f2bf(val);
+    }
+  }
+}
+
+// LDS-tiled form of k_conv for the large levels (CIN a multiple of 16; nx,
+// ny, nz multiples of 16, 4, 4).  A block computes a 16 x 4 x 4 voxel tile:
+// it stages the 18 x 6 x 6 input box (zero outside the grid) in LDS once, and
+// each wave takes the four x-rows of one z-slice, so every input voxel is
+// read from memory once per tile instead of once per tap.  The K steps and
+// their order per output are those of k_conv (identical results).
+template <int CIN, int NT, bool OUT_F32>
+__global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__ in, const s4* __restrict__ Wp,
+                                                    const float* __restrict__ bias, void* __restrict__ out,
+                                                    int cout_store, int nx, int ny, int nz, int relu) {
+  constexpr int BX = 18, BY = 6, BZ = 6, NBOX = BX * BY * BZ;
+  constexpr int CG = CIN / 16;   // K steps per tap
+  constexpr int KC = 27 * CG;
+  constexpr int Q = CIN / 4;     // s4 chunks per voxel
+  __shared__ s4 box[NBOX * Q];
+  const int tx = nx / 16, ty = ny / 4, tz = nz / 4;
+  long long t = blockIdx.x;
+  const int bx = (int)(t % tx);
+  t /= tx;
+  const int by = (int)(t % ty);
+  t /= ty;
+  const int bz = (int)(t % tz);
+  const long long b = t / tz;
+  const int x0 = bx * 16, y0 = by * 4, z0 = bz * 4;
+  const long long bbase = b * (long long)nx * ny * nz;
+  const s4* in4 = reinterpret_cast<const s4*>(in);
+  for (int q = threadIdx.x; q < NBOX * Q; q += 256) {
+    const int c4 = q % Q, vx = q / Q;
+    const int xx = vx % BX, yy = (vx / BX) % BY, zz = vx / (BX * BY);
+    const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+    s4 v = s4{0, 0, 0, 0};
+    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+      v = in4[(bbase + gx + (long long)nx * (gy + (long long)ny * gz)) * Q + c4];
+    box[q] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int row = lane & 15, kg = lane >> 4;
+  f4 acc[4][NT];
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int u = 0; u < NT; u++) acc[j][u] = f4{0.f, 0.f, 0.f, 0.f};
+  constexpr int U = 9;
+  for (int kc0 = 0; kc0 < KC; kc0 += U) {
+    s4 wv[U][NT];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+#pragma unroll
+      for (int v = 0; v < NT; v++) wv[u][v] = Wp[((kc0 + u) * NT + v) * 64 + lane];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int kc = kc0 + u;
+      const int tap = kc / CG, cg = kc - tap * CG;
+      const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const s4 a = box[(((w + dz) * BY + (j + dy)) * BX + (row + dx)) * Q + cg * 4 + kg];
+#pragma unroll
+        for (int v = 0; v < NT; v++) acc[j][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wv[u][v], acc[j][v], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int v = 0; v < NT; v++) {
+      const int co = v * 16 + row;
+      if (co >= cout_store) continue;
+      const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const long long vo = bbase + (x0 + kg * 4 + i) + (long long)nx * ((y0 + j) + (long long)ny * (z0 + w));
+        float val = acc[j][v][i] + bv;
+        if (relu) val = fmaxf(val, 0.f);
+        if (OUT_F32)
+          reinterpret_cast<float*>(out)[vo * cout_store + co] = val;
+        else
+          reinterpret_cast<uint16_t*>(out)[vo * cout_store + co] = f2bf(val);
+      }
+    }
+}
+
 template <int CIN, int NT>
 __global__ __launch_bounds__(256) void k_conv_wgrad(const uint16_t* __restrict__ X, const uint16_t* __restrict__ G,
                                                     int cg, float* __restrict__ partial, int nx, int ny, int nz,
@@ -313,24 +474,33 @@ __global__ void k_pack_gout(const float* __restrict__ g, uint16_t* __restrict__ 
 }
 
 // 2x2x2 average pool: fine (nx,ny,nz) -> coarse (nx/2, ny/2, nz/2).
+// 2x2x2 average pool; four channels (one 8-byte word) per thread, C % 4 == 0.
 __global__ void k_pool(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int C, int nx, int ny, int nz,
                        long long Bc) {
   const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
   const long long Vc = (long long)cx * cy * cz, Vf = (long long)nx * ny * nz;
-  const long long n = Bc * Vc * C;
+  const int C4 = C / 4;
+  const long long n = Bc * Vc * C4;
+  const s4* in4 = reinterpret_cast<const s4*>(in);
+  s4* out4 = reinterpret_cast<s4*>(out);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long vv = i / C;
+    const int c = (int)(i % C4);
+    const long long vv = i / C4;
     long long b;
     int x, y, z;
     vox_xyz(vv, Vc, cx, cy, &b, &x, &y, &z);
-    float s = 0.f;
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
     for (int a = 0; a < 8; a++) {
       const long long f = b * Vf + (2 * x + (a & 1)) + (long long)nx * ((2 * y + ((a >> 1) & 1)) +
                                                                        (long long)ny * (2 * z + ((a >> 2) & 1)));
-      s += bf2f(in[f * C + c]);
+      const s4 v = in4[f * C4 + c];
+#pragma unroll
+      for (int k = 0; k < 4; k++) s[k] += bf2f((uint16_t)v[k]);
     }
-    out[i] = f2bf(s * 0.125f);
+    s4 o;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (short)f2bf(s[k] * 0.125f);
+    out4[i] = o;
   }
 }
 
@@ -340,16 +510,24 @@ __global__ void k_up_add(const uint16_t* __restrict__ coarse, const uint16_t* __
   const long long Vf = (long long)nx * ny * nz;
   const int cx = nx / 2, cy = ny / 2, cz = nz / 2;
   const long long Vc = (long long)cx * cy * cz;
-  const long long n = BVf * C;
+  const int C4 = C / 4;  // four channels (one 8-byte word) per thread, C % 4 == 0
+  const long long n = BVf * C4;
+  const s4* c4p = reinterpret_cast<const s4*>(coarse);
+  const s4* s4p = reinterpret_cast<const s4*>(skip);
+  s4* o4p = reinterpret_cast<s4*>(out);
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const int c = (int)(i % C);
-    const long long vv = i / C;
+    const int c = (int)(i % C4);
+    const long long vv = i / C4;
     long long b;
     int x, y, z;
     vox_xyz(vv, Vf, nx, ny, &b, &x, &y, &z);
     const long long p = b * Vc + (x / 2) + (long long)cx * ((y / 2) + (long long)cy * (z / 2));
     (void)cz;
-    out[i] = f2bf(bf2f(coarse[p * C + c]) + bf2f(skip[i]));
+    const s4 cv = c4p[p * C4 + c], sv = s4p[i];
+    s4 o;
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = (short)f2bf(bf2f((uint16_t)cv[k]) + bf2f((uint16_t)sv[k]));
+    o4p[i] = o;
   }
 }
 
@@ -648,8 +826,51 @@ int ensure_work(Net* n, const Shape& sh) {
 #define CONV_CASES(X) \
   X(4, 1) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3) X(48, 2) X(48, 3) X(48, 4) X(64, 3) X(64, 4) X(64, 6) X(96, 4)
 
+#define CONV_TILED_CASES(X) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3)
+
 static int launch_conv(int cin, int nt, bool f32, const uint16_t* in, const void* Wp, const float* bias, void* out,
                        int cout_store, int nx, int ny, int nz, long long BV, int relu, hipStream_t st) {
+  if (nx % 16 == 0 && ny % 4 == 0 && nz % 4 == 0) {
+    const long long V = (long long)nx * ny * nz;
+    const dim3 tgrid((unsigned)(BV / V * (V / 256))), tblock(256);
+#define X(C, N)                                                                                                   \
+  if (cin == C && nt == N) {                                                                                      \
+    if (f32)                                                                                                      \
+      hipLaunchKernelGGL((k_conv_tiled<C, N, true>), tgrid, tblock, 0, st, in, (const s4*)Wp, bias, out,          \
+                         cout_store, nx, ny, nz, relu);                                                           \
+    else                                                                                                          \
+      hipLaunchKernelGGL((k_conv_tiled<C, N, false>), tgrid, tblock, 0, st, in, (const s4*)Wp, bias, out,         \
+                         cout_store, nx, ny, nz, relu);                                                           \
+    return GS_OK;                                                                                                 \
+  }
+    CONV_TILED_CASES(X)
+#undef X
+  }
+  const long long tiles16 = (BV + 15) / 16;
+  if (tiles16 < 4096) {  // small level: split K over 4 (or 8) waves per 16-voxel tile
+    const dim3 sgrid((unsigned)tiles16);
+#define X(C, N)                                                                                                   \
+  if (cin == C && nt == N) {                                                                                      \
+    if (tiles16 < 512) {                                                                                          \
+      if (f32)                                                                                                    \
+        hipLaunchKernelGGL((k_conv_splitk<C, N, true, 8>), sgrid, dim3(512), 0, st, in, (const s4*)Wp, bias, out, \
+                           cout_store, nx, ny, nz, BV, relu);                                                     \
+      else                                                                                                        \
+        hipLaunchKernelGGL((k_conv_splitk<C, N, false, 8>), sgrid, dim3(512), 0, st, in, (const s4*)Wp, bias,     \
+                           out, cout_store, nx, ny, nz, BV, relu);                                                \
+    } else {                                                                                                      \
+      if (f32)                                                                                                    \
+        hipLaunchKernelGGL((k_conv_splitk<C, N, true, 4>), sgrid, dim3(256), 0, st, in, (const s4*)Wp, bias, out, \
+                           cout_store, nx, ny, nz, BV, relu);                                                     \
+      else                                                                                                        \
+        hipLaunchKernelGGL((k_conv_splitk<C, N, false, 4>), sgrid, dim3(256), 0, st, in, (const s4*)Wp, bias,     \
+                           out, cout_store, nx, ny, nz, BV, relu);                                                \
+    }                                                                                                             \
+    return GS_OK;                                                                                                 \
+  }
+    CONV_CASES(X)
+#undef X
+  }
   const dim3 grid((unsigned)((BV + 63) / 64)), block(256);
 #define X(C, N)                                                                                                  \
   if (cin == C && nt == N) {                                                                                     \
@@ -714,7 +935,7 @@ int forward(Net* n, const Shape& sh, const float* in2, float* out) {
     if (l > 0) {
       lvl_dims(sh, l - 1, &nx, &ny, &nz);
       const int C = kChan[l - 1];
-      hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[l] * C)), dim3(kBlock), 0, st, w.act[enc_b(l - 1)], w.pool[l], C,
+      hipLaunchKernelGGL(k_pool, dim3(gridn(w.BV[l] * C / 4)), dim3(kBlock), 0, st, w.act[enc_b(l - 1)], w.pool[l], C,
                          nx, ny, nz, (long long)sh.batch);
       in = w.pool[l];
     }
@@ -726,7 +947,7 @@ int forward(Net* n, const Shape& sh, const float* in2, float* out) {
   for (int l = kLevels - 2; l >= 0; l--) {
     lvl_dims(sh, l, &nx, &ny, &nz);
     const int C = kChan[l];
-    hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[l] * C)), dim3(kBlock), 0, st, w.act[cur], w.act[enc_b(l)],
+    hipLaunchKernelGGL(k_up_add, dim3(gridn(w.BV[l] * C / 4)), dim3(kBlock), 0, st, w.act[cur], w.act[enc_b(l)],
                        w.upadd[l], C, nx, ny, nz, w.BV[l]);
     if ((e = fwd_layer(n, dec_a(l), w.upadd[l], w.act[dec_a(l)]))) return e;
     void* o = l > 0 ? (void*)w.act[dec_b(l)] : (void*)(out ? out : w.out32);
