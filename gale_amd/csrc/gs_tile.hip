@@ -90,21 +90,51 @@ __device__ __forceinline__ void grad_coeffs3(const T (&u)[8][3], T ix, T iy, T i
   }
 }
 
-// Nodal forces from the 30 stress moments (see gs_kernels.hip for the layout).
+// Nodal forces from the 30 stress moments (see gs_kernels.hip for the layout):
+//   ye[a][0] = ix ax P(ay,az) + iy ay Q(ax,az) + iz az R(ax,ay), ...
+// where every P, Q, R is a two-sign combination p0 + s p1 + t p2 + s t p3,
+// so each is evaluated for its four sign pairs with one 2x2 Hadamard step.
+template <typename T>
+__device__ __forceinline__ void had4(T p0, T p1, T p2, T p3, T sc, T (&h)[2][2]) {
+  // h[(s+1)/2][(t+1)/2] = sc (p0 + s p1 + t p2 + s t p3)
+  p0 *= sc; p1 *= sc; p2 *= sc; p3 *= sc;
+  const T am = p0 - p1, ap = p0 + p1, bm = p2 - p3, bp = p2 + p3;
+  h[0][0] = am - bm; h[0][1] = am + bm;
+  h[1][0] = ap - bp; h[1][1] = ap + bp;
+}
+
 template <typename T>
 __device__ __forceinline__ void epilogue3(const T (&m)[30], T ix, T iy, T iz, T (&ye)[8][3]) {
+  T P[2][2], Q[2][2], R[2][2];
+  // component x: P(ay,az) from m0..3, Q(ax,az) from m12,13,15,16, R(ax,ay) from m18,19,20,22
+  had4(m[0], m[1], m[2], m[3], ix, P);
+  had4(m[12], m[13], m[15], m[16], iy, Q);
+  had4(m[18], m[19], m[20], m[22], iz, R);
 #pragma unroll
   for (int a = 0; a < 8; a++) {
-    const T ax = (a & 1) ? (T)1 : (T)-1, ay = (a & 2) ? (T)1 : (T)-1, az = (a & 4) ? (T)1 : (T)-1;
-    ye[a][0] = ix * ax * (m[0] + ay * m[1] + az * m[2] + ay * az * m[3]) +
-               iy * ay * (m[12] + ax * m[13] + az * m[15] + ax * az * m[16]) +
-               iz * az * (m[18] + ax * m[19] + ay * m[20] + ax * ay * m[22]);
-    ye[a][1] = iy * ay * (m[4] + ax * m[5] + az * m[6] + ax * az * m[7]) +
-               ix * ax * (m[12] + ay * m[14] + az * m[15] + ay * az * m[17]) +
-               iz * az * (m[24] + ax * m[25] + ay * m[26] + ax * ay * m[28]);
-    ye[a][2] = iz * az * (m[8] + ax * m[9] + ay * m[10] + ax * ay * m[11]) +
-               ix * ax * (m[18] + ay * m[20] + az * m[21] + ay * az * m[23]) +
-               iy * ay * (m[24] + ax * m[25] + az * m[27] + ax * az * m[29]);
+    const int sx = a & 1, sy = (a >> 1) & 1, sz = (a >> 2) & 1;
+    const T p = sx ? P[sy][sz] : -P[sy][sz], q = sy ? Q[sx][sz] : -Q[sx][sz], r = sz ? R[sx][sy] : -R[sx][sy];
+    ye[a][0] = p + q + r;
+  }
+  // component y: P(ax,az) from m4..7 (iy), Q(ay,az) from m12,14,15,17 (ix), R(ax,ay) from m24,25,26,28 (iz)
+  had4(m[4], m[5], m[6], m[7], iy, P);
+  had4(m[12], m[14], m[15], m[17], ix, Q);
+  had4(m[24], m[25], m[26], m[28], iz, R);
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int sx = a & 1, sy = (a >> 1) & 1, sz = (a >> 2) & 1;
+    const T p = sy ? P[sx][sz] : -P[sx][sz], q = sx ? Q[sy][sz] : -Q[sy][sz], r = sz ? R[sx][sy] : -R[sx][sy];
+    ye[a][1] = p + q + r;
+  }
+  // component z: P(ax,ay) from m8..11 (iz), Q(ay,az) from m18,20,21,23 (ix), R(ax,az) from m24,25,27,29 (iy)
+  had4(m[8], m[9], m[10], m[11], iz, P);
+  had4(m[18], m[20], m[21], m[23], ix, Q);
+  had4(m[24], m[25], m[27], m[29], iy, R);
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int sx = a & 1, sy = (a >> 1) & 1, sz = (a >> 2) & 1;
+    const T p = sz ? P[sx][sy] : -P[sx][sy], q = sx ? Q[sy][sz] : -Q[sy][sz], r = sy ? R[sx][sz] : -R[sx][sz];
+    ye[a][2] = p + q + r;
   }
 }
 
@@ -142,11 +172,14 @@ __device__ __forceinline__ void element_particles3(const T (&A)[3][4], const T (
 // Stress moments as contractions of the gradient coefficients with the 27
 // particle moments (moments stored SoA: M[k * E + e]).
 template <typename T>
-__device__ __forceinline__ void element_moments3(const T (&A)[3][4], const T (&B)[3][4], const T (&C)[3][4],
-                                                 const T* __restrict__ Mom, long long E, long long e, T (&m)[30]) {
-  T M[27];
+__device__ __forceinline__ void load_moments3(const T* __restrict__ Mom, long long E, long long e, T (&M)[27]) {
 #pragma unroll
   for (int k = 0; k < 27; k++) M[k] = Mom[k * E + e];
+}
+
+template <typename T>
+__device__ __forceinline__ void element_moments3(const T (&A)[3][4], const T (&B)[3][4], const T (&C)[3][4],
+                                                 const T (&M)[27], T (&m)[30]) {
   constexpr int XE[4][3] = {{0, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}};
   constexpr int YE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 0, 1}, {1, 0, 1}};
   constexpr int ZE[4][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {1, 1, 0}};
@@ -244,43 +277,53 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
                                                               const uint8_t* __restrict__ bcflag) {
   constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
   constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
-  __shared__ T sx[NB * 3];
-  __shared__ T sy[NB * 3];
+  // node box, one array per component (consecutive elements hit consecutive
+  // LDS words in the corner phases)
+  __shared__ T sx[3][NB];
+  __shared__ T sy[3][NB];
+  __shared__ T sih[TX + TY + TZ];  // 1 / (4 h) of the tile's columns, rows, layers
   const long long tile = xcd_remap(blockIdx.x, gridDim.x);
   const int tx = (int)(tile % tg.n[0]), ty = (int)((tile / tg.n[0]) % tg.n[1]), tz = (int)(tile / ((long long)tg.n[0] * tg.n[1]));
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int nbx = min(TX, g.res[0] - x0) + 1, nby = min(TY, g.res[1] - y0) + 1, nbz = min(TZ, g.res[2] - z0) + 1;
   const long long sy_ = g.nn[0], sz_ = g.nn[0] * g.nn[1];
+  const int ex = threadIdx.x % TX, ey = (threadIdx.x / TX) % TY, ez = threadIdx.x / (TX * TY);
+  const bool active = (x0 + ex < g.res[0]) && (y0 + ey < g.res[1]) && (z0 + ez < g.res[2]);
+  const long long e = (x0 + ex) + (long long)g.res[0] * ((y0 + ey) + (long long)g.res[1] * (z0 + ez));
+  // the element's moments are requested before the node box, so both
+  // memory round trips overlap
+  T M[27];
+  if (FORM == 0 && active) load_moments3(Mom, g.E, e, M);
   for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
     const int c = q % 3, nd = q / 3;
     const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
     T v = 0;
     if (lx < nbx && ly < nby && lz < nbz) v = x[((x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz)) * 3 + c];
-    sx[q] = v;
-    sy[q] = 0;
+    sx[c][nd] = v;
+    sy[c][nd] = 0;
+  }
+  if (threadIdx.x < TX + TY + TZ) {
+    const int t = threadIdx.x;
+    const int d = t < TX ? 0 : (t < TX + TY ? 1 : 2);
+    const long long i = d == 0 ? x0 + t : (d == 1 ? y0 + t - TX : z0 + t - TX - TY);
+    sih[t] = i < g.res[d] ? (T)(0.25 / (vcoord_t(g, d, i + 1) - vcoord_t(g, d, i))) : (T)0;
   }
   __syncthreads();
-  const int ex = threadIdx.x % TX, ey = (threadIdx.x / TX) % TY, ez = threadIdx.x / (TX * TY);
-  const bool active = (x0 + ex < g.res[0]) && (y0 + ey < g.res[1]) && (z0 + ez < g.res[2]);
   T ye[8][3];
   if (active) {
-    const long long gi = x0 + ex, gj = y0 + ey, gk = z0 + ez;
-    const T ix = (T)(0.25 / (vcoord_t(g, 0, gi + 1) - vcoord_t(g, 0, gi)));
-    const T iy = (T)(0.25 / (vcoord_t(g, 1, gj + 1) - vcoord_t(g, 1, gj)));
-    const T iz = (T)(0.25 / (vcoord_t(g, 2, gk + 1) - vcoord_t(g, 2, gk)));
+    const T ix = sih[ex], iy = sih[TX + ey], iz = sih[TX + TY + ez];
     T u[8][3];
 #pragma unroll
     for (int a = 0; a < 8; a++) {
       const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
 #pragma unroll
-      for (int c = 0; c < 3; c++) u[a][c] = sx[ln * 3 + c];
+      for (int c = 0; c < 3; c++) u[a][c] = sx[c][ln];
     }
     T A[3][4], B[3][4], C[3][4];
     grad_coeffs3(u, ix, iy, iz, A, B, C);
-    const long long e = gi + (long long)g.res[0] * (gj + (long long)g.res[1] * gk);
     T m[30];
     if (FORM == 0)
-      element_moments3(A, B, C, Mom, g.E, e, m);
+      element_moments3(A, B, C, M, m);
     else
       element_particles3(A, B, C, pk, off[e], off[e + 1], m);
     epilogue3(m, ix, iy, iz, ye);
@@ -293,7 +336,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
     if (active) {
       const int ln = (ex + (a & 1)) + NBX * ((ey + ((a >> 1) & 1)) + NBY * (ez + ((a >> 2) & 1)));
 #pragma unroll
-      for (int c = 0; c < 3; c++) sy[ln * 3 + c] += ye[a][c];
+      for (int c = 0; c < 3; c++) sy[c][ln] += ye[a][c];
     }
     __syncthreads();
   }
@@ -304,11 +347,11 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
     if (lx >= nbx || ly >= nby || lz >= nbz) continue;
     const bool hal = (lx == nbx - 1 && !lastx) || (ly == nby - 1 && !lasty) || (lz == nbz - 1 && !lastz);
     if (hal) {
-      halo[(tile * NB + nd) * 3 + c] = sy[q];
+      halo[(tile * NB + nd) * 3 + c] = sy[c][nd];
     } else {
       const long long gn = (x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz);
       const unsigned bf = bcflag ? bcflag[gn] : 0u;
-      y[gn * 3 + c] = ((bf >> c) & 1u) ? (T)0 : sy[q];
+      y[gn * 3 + c] = ((bf >> c) & 1u) ? (T)0 : sy[c][nd];
     }
   }
 }
@@ -567,8 +610,9 @@ __device__ void bot_kx(const BotLevel<T>& L, const T* __restrict__ x, T* sy) {
       }
       T A[3][4], B[3][4], C[3][4];
       grad_coeffs3(u, ix, iy, iz, A, B, C);
-      T m[30];
-      element_moments3(A, B, C, L.M, E, e, m);
+      T M[27], m[30];
+      load_moments3(L.M, E, e, M);
+      element_moments3(A, B, C, M, m);
       epilogue3(m, ix, iy, iz, ye);
     }
 #pragma unroll
