@@ -27,7 +27,10 @@ APPLY_PARTICLES_FP64, APPLY_PARTICLES_FP32, APPLY_MOMENTS_FP64, APPLY_MOMENTS_FP
 SHAPE_EVERYWHERE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_BELOW_SINES = 0, 1, 2, 3
 WALLS = {"left": 0, "right": 1, "bottom": 2, "top": 3, "back": 4, "front": 5}
 
-# Exported entry points (must match include/gale_stokes.h and include/gale_generate.h).
+# Integration weights of the material points (include/gale_weights.h).
+WEIGHTS_CONSTANT, WEIGHTS_DVC = 0, 1
+
+# Exported entry points (must match the declarations of include/*.h).
 EXPORTS = (
     "gs_strerror", "gs_version", "gs_device_count", "gs_problem_create", "gs_problem_destroy", "gs_problem_info",
     "gs_problem_particles", "gs_solver_create", "gs_solver_set_params", "gs_solver_destroy", "gs_solver_setup",
@@ -41,6 +44,7 @@ EXPORTS = (
     "gs_solver_set_learned", "gs_lpc_selftest",
     "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
     "gs_batch_solution",
+    "gs_problem_set_weights", "gs_dvc_cells", "gs_dvc_max_particles",
 )
 
 
@@ -140,6 +144,9 @@ def lib():
             "gs_advect_rk2": (i32, [vp, d]),
             "gs_relocate": (i32, [vp, vp]),
             "gs_problem_set_coords": (i32, [vp, vp]),
+            "gs_problem_set_weights": (i32, [vp, i32, vp]),
+            "gs_dvc_cells": (i32, [i32, vp, i64, vp, vp, vp, vp]),
+            "gs_dvc_max_particles": (i64, [i32, vp]),
             "gs_apply_K": (i32, [vp, vp, vp]),
             "gs_apply_K_fp32": (i32, [vp, vp, vp]),
             "gs_apply_K_moments": (i32, [vp, vp, vp]),
@@ -435,6 +442,34 @@ class Problem:
         st = lib().gs_problem_set_coords(self._h, _ptr(c))
         _check(st, "set_coords", allow=(E_ESCAPED,))
         return st
+
+    def set_weights(self, kind=WEIGHTS_DVC, resolution=10):
+        """The integration swarm's WeightsCalculator: WEIGHTS_CONSTANT
+        (ConstantWeights) or WEIGHTS_DVC (DVCWeights, "resolution" or
+        resolutionX/Y/Z as a tuple).  The particles are re-mapped at once."""
+        r = np.array(_dims(3, [resolution] * 3 if np.isscalar(resolution) else resolution, 10), np.int32)
+        st = lib().gs_problem_set_weights(self._h, int(kind), _ptr(r))
+        _check(st, "set_weights", allow=(E_ESCAPED,))
+        return st
+
+
+def dvc_max_particles(dim, resolution=10) -> int:
+    r = np.array(_dims(3, [resolution] * 3 if np.isscalar(resolution) else resolution, 10), np.int32)
+    return int(lib().gs_dvc_max_particles(int(dim), _ptr(r)))
+
+
+def dvc_cells(dim, resolution, off, xi):
+    """Discrete Voronoi centroids and weights of independent FE cells on the
+    device (_DVCWeights_CreateVoronoi + _DVCWeights_GetCentroids per cell):
+    cell e holds rows off[e]..off[e+1] of xi (local coordinates)."""
+    r = np.array(_dims(3, [resolution] * 3 if np.isscalar(resolution) else resolution, 10), np.int32)
+    off = np.ascontiguousarray(off, dtype=np.int32)
+    xi = _f64(xi).reshape(-1, dim)
+    cent = np.zeros_like(xi)
+    w = np.zeros(xi.shape[0])
+    _check(lib().gs_dvc_cells(int(dim), _ptr(r), int(len(off) - 1), _ptr(off), _ptr(xi), _ptr(cent), _ptr(w)),
+           "gs_dvc_cells")
+    return cent, w
 
 
 def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsoluteTolerance=False,

@@ -20,6 +20,8 @@
 #include "gs_learned.h"
 #include "gs_mg.h"
 #include "gs_tile.h"
+#include "gs_weights.h"
+#include "../../include/gale_weights.h"
 
 using namespace gs;
 
@@ -88,6 +90,9 @@ struct gs_problem {
   bool min_one_particle = true;
   double* halo64 = nullptr;  // tiled K apply scratch (fine level)
   float* halo32 = nullptr;
+  int wkind = GS_WEIGHTS_CONSTANT;  // integration weights of the material points
+  int wres[3] = {10, 10, 10};       // DVC sub-grid
+  double* wp = nullptr;             // DVC weight of each (sorted) particle
 };
 
 
@@ -249,6 +254,15 @@ static int32_t relocate(gs_problem* pb) {
       pb->min_one_particle = false;
       break;
     }
+  if (pb->wkind == GS_WEIGHTS_DVC && P > 0) {
+    // DVCWeights: integration point at the centroid of its discrete Voronoi
+    // region, weight = the region's volume (gs_weights.hip)
+    int maxp = 0;
+    for (long long e = 0; e < pb->g.E; e++) maxp = std::max(maxp, off[e + 1] - off[e]);
+    if (maxp > dvc_max_particles(pb->dim, pb->wres)) return GS_E_INVALID;
+    const int32_t r = dvc_weights(pb->dim, pb->wres, pb->g.E, pb->off, pb->pid, pb->pk, pb->wp, maxp, st);
+    if (r != GS_OK) return r;
+  }
   return esc ? GS_E_ESCAPED : GS_OK;
 }
 
@@ -362,6 +376,7 @@ extern "C" void gs_problem_destroy(gs_problem* pb) {
   dfree(pb->mats_d); dfree(pb->bcdof); dfree(pb->bcflag); dfree(pb->ubc); dfree(pb->pk); dfree(pb->pk32);
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->off); dfree(pb->keys); dfree(pb->keys2); dfree(pb->vals);
+  dfree(pb->wp);
   dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->halo64);
   dfree(pb->halo32);
   if (pb->sort_tmp) (void)hipFree(pb->sort_tmp);
@@ -402,6 +417,11 @@ extern "C" int32_t gs_problem_particles(gs_problem* pb, double* coord, int32_t* 
     std::vector<int> off(E + 1);
     CK(hipMemcpy(pk.data(), pb->pk, sizeof(double4) * P, hipMemcpyDeviceToHost));
     CK(hipMemcpy(off.data(), pb->off, sizeof(int) * (E + 1), hipMemcpyDeviceToHost));
+    std::vector<double> wv;
+    if (pb->wkind == GS_WEIGHTS_DVC && P > 0) {
+      wv.resize(P);
+      CK(hipMemcpy(wv.data(), pb->wp, sizeof(double) * P, hipMemcpyDeviceToHost));
+    }
     for (long long e = 0; e < E; e++)
       for (int i = off[e]; i < off[e + 1]; i++) {
         const long long o = pid[i];
@@ -410,7 +430,7 @@ extern "C" int32_t gs_problem_particles(gs_problem* pb, double* coord, int32_t* 
           xi[o * dim + 1] = pk[i].y;
           if (dim == 3) xi[o * dim + 2] = pk[i].z;
         }
-        if (weight) weight[o] = (dim == 2 ? 4.0 : 8.0) / (double)(off[e + 1] - off[e]);
+        if (weight) weight[o] = wv.empty() ? (dim == 2 ? 4.0 : 8.0) / (double)(off[e + 1] - off[e]) : wv[i];
       }
     for (long long i = off[E]; i < P; i++) {
       const long long o = pid[i];
@@ -428,6 +448,20 @@ extern "C" int32_t gs_relocate(gs_problem* pb, int32_t* owner_out) {
   if (st == GS_E_HIP) return st;
   if (owner_out) CKS(gs_problem_particles(pb, nullptr, owner_out, nullptr, nullptr));
   return st;
+}
+
+extern "C" int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const int32_t* res) {
+  if (!pb || (kind != GS_WEIGHTS_CONSTANT && kind != GS_WEIGHTS_DVC)) return GS_E_INVALID;
+  if (kind == GS_WEIGHTS_DVC) {
+    if (!res) return GS_E_INVALID;
+    int r[3] = {res[0], res[1], pb->dim == 3 ? res[2] : 1};
+    if (dvc_max_particles(pb->dim, r) <= 0) return GS_E_INVALID;
+    if (!pb->wp && pb->P > 0) CK(dalloc(&pb->wp, pb->P));
+    for (int d = 0; d < 3; d++) pb->wres[d] = r[d];
+  }
+  pb->wkind = kind;
+  CK(hipStreamSynchronize(pb->st));
+  return relocate(pb);
 }
 
 extern "C" int32_t gs_problem_set_coords(gs_problem* pb, const double* coord) {
@@ -924,6 +958,7 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
   a.mats = pb->mats_d;
   a.mat = pb->mat;
   a.eta0p = pb->eta0p;
+  a.wp = pb->wkind == GS_WEIGHTS_DVC ? pb->wp : nullptr;
   a.clamp_min = pb->clamp_min;
   a.clamp_max = pb->clamp_max;
   for (int d = 0; d < 3; d++) a.g[d] = pb->ghat[d] * pb->gravity;

@@ -66,6 +66,7 @@ struct SetupArgs {
   const MatDev* mats;
   const int* mat;         // P (sorted)
   const double* eta0p;    // P (sorted) or null
+  const double* wp;       // P (sorted) integration weights, or null: 2^dim / count
   double clamp_min, clamp_max;
   double g[3];            // gravity vector
 };

@@ -620,8 +620,9 @@ __global__ __launch_bounds__(kBlock) void k_setup(Geo g, int c, long long mcount
   for (int q = 0; q < NEN; q++)
     for (int d = 0; d < 3; d++) dg[q][d] = fe[q][d] = 0.0;
   const int pb = off[e], pe = off[e + 1];
-  const double w = (DIM == 2 ? 4.0 : 8.0) / (double)(pe - pb);
+  const double wc = (DIM == 2 ? 4.0 : 8.0) / (double)(pe - pb);
   for (int p = pb; p < pe; p++) {
+    const double w = a.wp ? a.wp[p] : wc;
     double4 q4 = pk[p];
     const double xi[3] = {q4.x, q4.y, q4.z};
     const int mi = a.mat[p];

@@ -1,0 +1,318 @@
+// gs_weights.hip -- discrete Voronoi cell (DVC) integration weights of the
+// material points (include/gale_weights.h).
+//
+// One wavefront per FE cell grows the discrete Voronoi regions of the cell's
+// particles on a fixed sub-grid of the local box [-1,1]^dim exactly as
+// DVCWeights.cxx:693-786 does, then moves every integration point to the
+// centroid of its region and gives it the region's volume as weight
+// (DVCWeights.cxx:875-951).  The growth is sequential over particles by
+// construction (particle i contests what particles 0..i-1 just took), but
+// within one particle's step every boundary sub-cell is decided on its own,
+// so the 64 lanes decide a step's sub-cells in parallel: the boundary of each
+// particle is a bit set over the sub-cells, the owner of every sub-cell an
+// LDS array.  The result is the reference's, sub-cell for sub-cell.
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "../../include/gale_weights.h"
+#include "gs_internal.h"
+#include "gs_weights.h"
+
+namespace gs {
+
+namespace {
+
+constexpr int kDvcLanes = 64;
+
+struct DvcGrid {
+  int dim;
+  int r[3];
+  int nc, nwords;
+  double h[3];
+  double vol;
+};
+
+// Closer-claimant test of ClaimCells (DVCWeights.cxx:629-634): positive when
+// q is strictly closer to the sub-cell centre c than the owner o.
+__device__ __forceinline__ double dvc_contest(int dim, const double* c, const double* o, const double* q) {
+#pragma clang fp contract(off)
+  double s = 0.0;
+  for (int d = 0; d < dim; d++) s += (o[d] + q[d] - c[d] - c[d]) * (o[d] - q[d]);
+  return s;
+}
+
+__device__ __forceinline__ int wave_sum_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Adds to particle i's boundary the face neighbours of sub-cell c that it
+// does not own (UpdateBchain, DVCWeights.cxx:420-523).
+__device__ __forceinline__ void dvc_rebound(const DvcGrid& G, const short* owner, unsigned* bi, int i, int c) {
+  const int cx = c % G.r[0], cy = (c / G.r[0]) % G.r[1], cz = c / (G.r[0] * G.r[1]);
+  int nb[6], nn = 0;
+  if (cy > 0) nb[nn++] = c - G.r[0];
+  if (cy < G.r[1] - 1) nb[nn++] = c + G.r[0];
+  if (cx < G.r[0] - 1) nb[nn++] = c + 1;
+  if (cx > 0) nb[nn++] = c - 1;
+  if (G.dim == 3 && cz < G.r[2] - 1) nb[nn++] = c + G.r[0] * G.r[1];
+  if (G.dim == 3 && cz > 0) nb[nn++] = c - G.r[0] * G.r[1];
+  for (int k = 0; k < nn; k++) {
+    const int m = nb[k];
+    if (owner[m] != i) atomicOr(&bi[m >> 5], 1u << (m & 31));
+  }
+}
+
+// LDS layout of one cell (bytes): centres, particle coordinates, order,
+// owners, the current step's claims, and one boundary bit set per particle.
+__host__ __device__ inline size_t dvc_lds_bytes(const DvcGrid& G, int maxp) {
+  size_t b = sizeof(double) * (G.r[0] + G.r[1] + G.r[2]);
+  b += sizeof(double) * 3 * maxp;
+  b += sizeof(int) * maxp;
+  b += (sizeof(short) * G.nc + 7) / 8 * 8;
+  b += sizeof(unsigned) * G.nwords;
+  b += sizeof(unsigned) * (size_t)G.nwords * maxp;
+  return b;
+}
+
+// One FE cell per workgroup.  Particles of cell e are pk[off[e] .. off[e+1])
+// and take part in the growth in ascending pid order (the order in which the
+// reference's cell table lists them).  pk.xyz becomes the centroid, w the
+// weight.
+__global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restrict__ off,
+                                                   const int* __restrict__ pid, double4* __restrict__ pk,
+                                                   double* __restrict__ w, int maxp) {
+  extern __shared__ __align__(16) unsigned char dvc_smem[];
+  const long long e = blockIdx.x;
+  const int b = off[e], n = off[e + 1] - b;
+  if (n <= 0 || n > maxp) return;
+  const int lane = threadIdx.x;
+  double* ctr = reinterpret_cast<double*>(dvc_smem);
+  double* xs = ctr + (G.r[0] + G.r[1] + G.r[2]);
+  int* ord = reinterpret_cast<int*>(xs + 3 * maxp);
+  short* owner = reinterpret_cast<short*>(ord + maxp);
+  unsigned* took = reinterpret_cast<unsigned*>(reinterpret_cast<unsigned char*>(owner) +
+                                               (sizeof(short) * G.nc + 7) / 8 * 8);
+  unsigned* bound = took + G.nwords;
+  // sub-cell centres, accumulated one step at a time (DVCWeights.cxx:525-619)
+  if (lane < G.dim) {
+    const int d = lane;
+    double* cd = ctr + (d == 0 ? 0 : (d == 1 ? G.r[0] : G.r[0] + G.r[1]));
+    double x = -1.0 - G.h[d] / 2.0;
+    for (int i = 0; i < G.r[d]; i++) {
+      x = x + G.h[d];
+      cd[i] = x;
+    }
+  }
+  // rank of each particle by pid
+  for (int j = lane; j < n; j += kDvcLanes) {
+    const int pj = pid[b + j];
+    int rk = 0;
+    for (int k = 0; k < n; k++) rk += pid[b + k] < pj ? 1 : 0;
+    ord[rk] = j;
+  }
+  for (int c = lane; c < G.nc; c += kDvcLanes) owner[c] = -1;
+  for (int q = lane; q < n * G.nwords; q += kDvcLanes) bound[q] = 0;
+  __syncthreads();
+  for (int i = lane; i < n; i += kDvcLanes) {
+    const double4 q = pk[b + ord[i]];
+    xs[3 * i] = q.x;
+    xs[3 * i + 1] = q.y;
+    xs[3 * i + 2] = G.dim == 3 ? q.z : 0.0;
+  }
+  __syncthreads();
+  // seeds, in order (a later particle in the same sub-cell takes it over)
+  if (lane == 0) {
+    for (int i = 0; i < n; i++) {
+      int idx[3] = {0, 0, 0};
+      for (int d = 0; d < G.dim; d++) {
+        int k = (int)((xs[3 * i + d] - (-1.0)) / G.h[d]);
+        if (k == G.r[d]) k--;
+        idx[d] = k;
+      }
+      const int c = idx[0] + G.r[0] * (idx[1] + G.r[1] * idx[2]);
+      owner[c] = (short)i;
+      dvc_rebound(G, owner, bound + (size_t)i * G.nwords, i, c);
+    }
+  }
+  __syncthreads();
+  const double* cx = ctr;
+  const double* cy = ctr + G.r[0];
+  const double* cz = ctr + G.r[0] + G.r[1];
+  // rounds of claims until one claims nothing
+  for (;;) {
+    int total = 0;
+    for (int i = 0; i < n; i++) {
+      unsigned* bi = bound + (size_t)i * G.nwords;
+      int cnt = 0;
+      for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
+        unsigned bits = bi[wd], tk = 0;
+        while (bits) {
+          const int t = __ffs(bits) - 1;
+          bits &= bits - 1u;
+          const int c = wd * 32 + t;
+          const int o = owner[c];
+          bool take = o < 0;
+          if (!take && o != i) {
+            const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
+            const double c0[3] = {cx[ix], cy[iy], G.dim == 3 ? cz[iz] : 0.0};
+            take = dvc_contest(G.dim, c0, xs + 3 * o, xs + 3 * i) > 0.0;
+          }
+          if (take) {
+            owner[c] = (short)i;
+            tk |= 1u << t;
+            cnt++;
+          }
+        }
+        took[wd] = tk;
+        bi[wd] = 0u;
+      }
+      __syncthreads();
+      cnt = wave_sum_i(cnt);
+      total += cnt;
+      if (cnt > 0)
+        for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
+          unsigned tk = took[wd];
+          while (tk) {
+            const int t = __ffs(tk) - 1;
+            tk &= tk - 1u;
+            dvc_rebound(G, owner, bi, i, wd * 32 + t);
+          }
+        }
+      __syncthreads();
+    }
+    if (total == 0) break;
+  }
+  // centroids and volumes (DVCWeights.cxx:202-258), sub-cells in index order
+  for (int i = lane; i < n; i += kDvcLanes) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int cnt = 0;
+    for (int c = 0; c < G.nc; c++)
+      if (owner[c] == i) {
+        const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
+        s0 += cx[ix];
+        s1 += cy[iy];
+        if (G.dim == 3) s2 += cz[iz];
+        cnt++;
+      }
+    const int j = b + ord[i];
+    double4 q = pk[j];
+    q.x = cnt ? s0 / cnt : 0.0;
+    q.y = cnt ? s1 / cnt : 0.0;
+    if (G.dim == 3) q.z = cnt ? s2 / cnt : 0.0;
+    pk[j] = q;
+    w[j] = cnt * G.vol;
+  }
+}
+
+bool make_grid(int dim, const int* res, DvcGrid* G) {
+  if (dim != 2 && dim != 3) return false;
+  G->dim = dim;
+  G->nc = 1;
+  for (int d = 0; d < 3; d++) {
+    G->r[d] = d < dim ? res[d] : 1;
+    if (G->r[d] < 1 || G->r[d] > kDvcMaxRes) return false;
+    G->h[d] = d < dim ? 2.0 / G->r[d] : 0.0;
+    G->nc *= G->r[d];
+  }
+  G->nwords = (G->nc + 31) / 32;
+  G->vol = dim == 2 ? G->h[0] * G->h[1] : G->h[0] * G->h[1] * G->h[2];
+  return true;
+}
+
+}  // namespace
+
+int dvc_weights(int dim, const int* res, long long ncell, const int* off, const int* pid, double4* pk, double* w,
+                int maxp, hipStream_t st) {
+  DvcGrid G;
+  if (!make_grid(dim, res, &G)) return GS_E_INVALID;
+  if (ncell <= 0 || maxp <= 0) return GS_OK;
+  if (maxp > 32767) return GS_E_INVALID;
+  const size_t lds = dvc_lds_bytes(G, maxp);
+  if (lds > kDvcMaxLds) return GS_E_INVALID;
+  hipLaunchKernelGGL(k_dvc, dim3((unsigned)ncell), dim3(kDvcLanes), lds, st, G, off, pid, pk, w, maxp);
+  return hipGetLastError() == hipSuccess ? GS_OK : GS_E_HIP;
+}
+
+long long dvc_max_particles(int dim, const int* res) {
+  DvcGrid G;
+  if (!res) return 0;
+  if (!make_grid(dim, res, &G)) return 0;
+  long long lo = 0, hi = 32767;
+  while (lo < hi) {
+    const long long mid = (lo + hi + 1) / 2;
+    if (dvc_lds_bytes(G, (int)mid) <= kDvcMaxLds)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+}  // namespace gs
+
+using namespace gs;
+
+#define WCK(x)                                \
+  do {                                        \
+    if ((x) != hipSuccess) return GS_E_HIP;   \
+  } while (0)
+
+extern "C" int32_t gs_dvc_cells(int32_t dim, const int32_t* res, int64_t ncell, const int32_t* off,
+                                const double* xi, double* centroid, double* weight) {
+  if ((dim != 2 && dim != 3) || !res || ncell < 0 || (ncell > 0 && (!off || !xi || !centroid || !weight)))
+    return GS_E_INVALID;
+  if (ncell == 0) return GS_OK;
+  if (ncell > 0x7fffffff) return GS_E_INVALID;
+  const long long P = off[ncell];
+  int maxp = 0;
+  for (long long e = 0; e < ncell; e++) {
+    const int c = off[e + 1] - off[e];
+    if (c < 0 || off[e] < 0) return GS_E_INVALID;
+    maxp = c > maxp ? c : maxp;
+  }
+  if (maxp > dvc_max_particles(dim, res)) return GS_E_INVALID;
+  std::vector<double4> pk(P > 0 ? P : 1);
+  std::vector<int> pid(P > 0 ? P : 1);
+  for (long long p = 0; p < P; p++) {
+    pk[p] = make_double4(xi[p * dim], xi[p * dim + 1], dim == 3 ? xi[p * dim + 2] : 0.0, 0.0);
+    pid[p] = (int)(p - off[0]);
+  }
+  int *d_off = nullptr, *d_pid = nullptr;
+  double4* d_pk = nullptr;
+  double* d_w = nullptr;
+  int32_t st = GS_OK;
+  hipStream_t s = nullptr;
+  if (hipMalloc(&d_off, sizeof(int) * (ncell + 1)) != hipSuccess || hipMalloc(&d_pid, sizeof(int) * pid.size()) ||
+      hipMalloc(&d_pk, sizeof(double4) * pk.size()) || hipMalloc(&d_w, sizeof(double) * pk.size())) {
+    st = GS_E_HIP;
+  } else if (hipMemcpy(d_off, off, sizeof(int) * (ncell + 1), hipMemcpyHostToDevice) ||
+             hipMemcpy(d_pid, pid.data(), sizeof(int) * pid.size(), hipMemcpyHostToDevice) ||
+             hipMemcpy(d_pk, pk.data(), sizeof(double4) * pk.size(), hipMemcpyHostToDevice)) {
+    st = GS_E_HIP;
+  } else {
+    st = dvc_weights(dim, res, ncell, d_off, d_pid, d_pk, d_w, maxp, s);
+    if (st == GS_OK) {
+      std::vector<double> wv(pk.size());
+      if (hipMemcpy(pk.data(), d_pk, sizeof(double4) * pk.size(), hipMemcpyDeviceToHost) ||
+          hipMemcpy(wv.data(), d_w, sizeof(double) * wv.size(), hipMemcpyDeviceToHost)) {
+        st = GS_E_HIP;
+      } else {
+        for (long long p = off[0]; p < P; p++) {
+          centroid[p * dim] = pk[p].x;
+          centroid[p * dim + 1] = pk[p].y;
+          if (dim == 3) centroid[p * dim + 2] = pk[p].z;
+          weight[p] = wv[p];
+        }
+      }
+    }
+  }
+  if (d_off) (void)hipFree(d_off);
+  if (d_pid) (void)hipFree(d_pid);
+  if (d_pk) (void)hipFree(d_pk);
+  if (d_w) (void)hipFree(d_w);
+  return st;
+}
+
+extern "C" int64_t gs_dvc_max_particles(int32_t dim, const int32_t* res) { return dvc_max_particles(dim, res); }

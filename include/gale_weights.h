@@ -1,0 +1,59 @@
+/* gale_weights.h -- integration weights of the material points
+ * (libgale_stokes.so).
+ *
+ * The reference picks the weights with the integration swarm's
+ * "WeightsCalculator" component.  Two are provided:
+ *   GS_WEIGHTS_CONSTANT  ConstantWeights (ConstantWeights.cxx:178-186):
+ *                        w = 2^dim / particles-in-cell, xi = the particle's
+ *                        local coordinates (the default).
+ *   GS_WEIGHTS_DVC       DVCWeights (DVCWeights.cxx:693-786, 875-951): the
+ *                        particles of a cell grow discrete Voronoi regions on
+ *                        a fixed sub-grid of the local box [-1,1]^dim; each
+ *                        integration point moves to the centroid of its region
+ *                        and weighs the region's volume.  This is also the
+ *                        weight PCDVC computes when it neither splits nor
+ *                        deletes particles (PCDVC.cxx:1092-1512 with
+ *                        maxSplits = maxDeletions = 0 and Inflow off).
+ * Particles of a cell take part in the growth in the order of the caller's
+ * particle indices (the order of the reference's cell particle table after
+ * the initial layout).  Limits: sub-grid resolution 1..32 per axis; a cell
+ * may hold at most gs_dvc_max_particles(dim, res) particles (GS_E_INVALID
+ * otherwise).
+ */
+#ifndef GALE_WEIGHTS_H
+#define GALE_WEIGHTS_H
+
+#include <stdint.h>
+
+#include "gale_stokes.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { GS_WEIGHTS_CONSTANT = 0, GS_WEIGHTS_DVC = 1 };
+
+/* Selects the weights of pb's material points and re-maps them at once; they
+ * are recomputed on every later relocation (gs_relocate, gs_advect_rk2,
+ * gs_problem_set_coords).  res: "resolutionX/Y/Z" of DVCWeights (reference
+ * default 10), ignored for GS_WEIGHTS_CONSTANT.  Replaces
+ * IntegrationPointsSwarm's WeightsCalculator (IntegrationPointsSwarm.cxx:391-449). */
+int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const int32_t* res);
+
+/* Discrete Voronoi centroids and weights of ncell independent FE cells on the
+ * device (host buffers in and out): cell e holds particles off[e]..off[e+1]
+ * of xi (dim values each, local coordinates in [-1,1]); centroid gets dim
+ * values per particle, weight one.  _DVCWeights_CreateVoronoi(2D) +
+ * _DVCWeights_GetCentroids(2D) (DVCWeights.cxx:202-258, 693-786) per cell. */
+int32_t gs_dvc_cells(int32_t dim, const int32_t* res, int64_t ncell, const int32_t* off, const double* xi,
+                     double* centroid, double* weight);
+
+/* Largest particle count per cell that gs_dvc_cells and GS_WEIGHTS_DVC accept
+ * for this sub-grid (0 for an invalid resolution). */
+int64_t gs_dvc_max_particles(int32_t dim, const int32_t* res);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

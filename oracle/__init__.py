@@ -113,6 +113,8 @@ def lib():
             "orc_compute_dt": (d, [vp, vp, d]),
             "orc_interpolate_velocity": (i32, [vp, vp, vp, vp]),
             "orc_advect_rk2": (i32, [vp, vp, d, vp]),
+            "orc_dvc_cell": (i32, [i32, vp, i32, vp, vp, vp]),
+            "orc_set_weights": (i32, [vp, i32, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -182,6 +184,21 @@ def layout_random(dim, res, lo, hi, ppc, seed):
     return coord
 
 
+def dvc_cell(dim, res, xi):
+    """Discrete Voronoi centroids and weights of one cell's particles
+    (DVCWeights.cxx:693-786 growth, :202-258 centroids); xi: (n, dim) local
+    coordinates in cell-table order, res: sub-grid per axis."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, dim)
+    n = xi.shape[0]
+    r = np.array(list(res) + [1] * (3 - len(res)), dtype=np.int32)
+    cent = np.zeros((n, dim))
+    w = np.zeros(n)
+    st = lib().orc_dvc_cell(dim, r.ctypes.data, n, xi.ctypes.data, cent.ctypes.data, w.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_dvc_cell status {st}")
+    return cent, w
+
+
 def uzawa_params(tolerance=1e-5, max_iterations=1000, min_iterations=1, use_absolute_tolerance=0,
                  use_preconditioner=1, inner=0, inner_max_iterations=0, inner_rtol_factor=0.1,
                  inner_rtol_override=0.0, nthreads=1):
@@ -245,6 +262,13 @@ class Problem:
         if h:
             lib().orc_problem_free(h)
             self._h = None
+
+    def set_weights(self, kind, res=(10, 10, 10)):
+        """0: ConstantWeights; 1: DVCWeights with sub-grid res."""
+        r = np.array(list(res) + [10] * (3 - len(res)), dtype=np.int32)
+        st = lib().orc_set_weights(self._h, kind, r.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"orc_set_weights status {st}")
 
     def particles(self):
         owner = np.zeros(self.P, dtype=np.int32)

@@ -23,6 +23,9 @@
  *     +-1 ulp offsets.
  *   - RK2: rigid rotation closed form and the linear-field exactness of the
  *     midpoint rule.
+ *   - DVC weights: the reference's DVCWeightsSuite fixtures (centroids and
+ *     weights of 10 hard-coded particles, 2D and 3D) and its PCDVC element
+ *     integration statistics (testPCDVC_*.expected).
  * Functions without a check of their own are listed as "parity unpinned" in
  * DESIGN.md.
  */
@@ -155,6 +158,8 @@ struct orc_problem {
   double* eta;
   uint8_t* yielded;
   int has_visc;
+  int wkind;          /* 0: ConstantWeights, 1: discrete Voronoi cell weights */
+  int32_t wres[3];    /* DVC sub-grid resolution */
   /* reduced system */
   int32_t* eq;
   int64_t n_u;
@@ -328,6 +333,151 @@ static void global_to_local(const orc_problem* pb, int64_t e, const double* x, d
   }
 }
 
+/* ------------------------------------------------------------------------ */
+/* Discrete Voronoi cell weights                                             */
+/*   PICellerator/Weights/src/DVCWeights.cxx:202-786, 875-951                */
+/* ------------------------------------------------------------------------ */
+/* One FE cell: a fixed sub-grid of res[0] x res[1] (x res[2]) sub-cells over
+ * the local box [-1,1]^dim, sub-cell k = i + res0 (j + res1 l).  Sub-cell
+ * centres accumulate one step at a time from (-1 - h/2) as
+ * _DVCWeights_ConstructGrid(2D) does (:525-619).  Regions grow as
+ * _DVCWeights_CreateVoronoi(2D) (:693-786): every particle, in order, seeds
+ * the sub-cell it lies in (a later particle in the same sub-cell takes it
+ * over) and its boundary becomes the face neighbours of that sub-cell it does
+ * not own (UpdateBchain, :420-523).  Then, round after round until a round
+ * claims nothing, every particle in order contests its boundary (ClaimCells,
+ * :260-397): an unowned sub-cell is taken, an owned one is taken when the
+ * contest (x1 + x2 - 2 x0).(x1 - x2) > 0, i.e. the claimant is strictly
+ * closer to the centre x0 than the owner; the new boundary is the face
+ * neighbours of the sub-cells it just took that it does not own.  Centroid
+ * and weight as _DVCWeights_GetCentroids(2D) (:202-258): mean of the owned
+ * centres in sub-cell order, weight = count x sub-cell volume (a particle
+ * left with no sub-cell keeps centroid 0 and weight 0). */
+static double dvc_contest(int dim, const double* c, const double* o, const double* q) {
+  double s = 0.0;
+  for (int d = 0; d < dim; d++) s += (o[d] + q[d] - c[d] - c[d]) * (o[d] - q[d]);
+  return s;
+}
+
+int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w) {
+  if (dim != 2 && dim != 3) return ORC_E_INVALID;
+  int r[3] = {res[0], res[1], dim == 3 ? res[2] : 1};
+  for (int d = 0; d < dim; d++)
+    if (r[d] < 1) return ORC_E_INVALID;
+  const int nc = r[0] * r[1] * r[2];
+  double h[3], *ctr[3];
+  for (int d = 0; d < 3; d++) {
+    h[d] = d < dim ? 2.0 / r[d] : 0.0;
+    ctr[d] = (double*)malloc(sizeof(double) * r[d]);
+    double x = -1.0 - h[d] / 2.0;
+    for (int i = 0; i < r[d]; i++) {
+      x = x + h[d];
+      ctr[d][i] = x;
+    }
+  }
+  const double vol = dim == 2 ? h[0] * h[1] : h[0] * h[1] * h[2];
+  int* owner = (int*)malloc(sizeof(int) * nc);
+  unsigned char* mark = (unsigned char*)calloc(nc, 1);
+  int** bound = (int**)malloc(sizeof(int*) * (nump > 0 ? nump : 1));
+  int* nbound = (int*)calloc(nump > 0 ? nump : 1, sizeof(int));
+  int* took = (int*)malloc(sizeof(int) * nc);
+  for (int c = 0; c < nc; c++) owner[c] = -1;
+  for (int i = 0; i < nump; i++) bound[i] = (int*)malloc(sizeof(int) * nc);
+#define DVC_CENTRE(cc, out)                                         \
+  do {                                                              \
+    const int ci_ = (cc) % r[0], cj_ = ((cc) / r[0]) % r[1];        \
+    const int ck_ = (cc) / (r[0] * r[1]);                           \
+    (out)[0] = ctr[0][ci_];                                         \
+    (out)[1] = ctr[1][cj_];                                         \
+    (out)[2] = dim == 3 ? ctr[2][ck_] : 0.0;                        \
+  } while (0)
+  /* boundary of particle i from the sub-cells it just took */
+#define DVC_REBOUND(i, list, nlist)                                                        \
+  do {                                                                                     \
+    int nb_ = 0;                                                                           \
+    for (int t_ = 0; t_ < (nlist); t_++) {                                                 \
+      const int c_ = (list)[t_];                                                           \
+      const int ci_ = c_ % r[0], cj_ = (c_ / r[0]) % r[1], ck_ = c_ / (r[0] * r[1]);       \
+      int nbr_[6], nn_ = 0;                                                                \
+      if (cj_ > 0) nbr_[nn_++] = c_ - r[0];                                                \
+      if (cj_ < r[1] - 1) nbr_[nn_++] = c_ + r[0];                                         \
+      if (ci_ < r[0] - 1) nbr_[nn_++] = c_ + 1;                                            \
+      if (ci_ > 0) nbr_[nn_++] = c_ - 1;                                                   \
+      if (dim == 3 && ck_ < r[2] - 1) nbr_[nn_++] = c_ + r[0] * r[1];                      \
+      if (dim == 3 && ck_ > 0) nbr_[nn_++] = c_ - r[0] * r[1];                             \
+      for (int k_ = 0; k_ < nn_; k_++) {                                                   \
+        const int m_ = nbr_[k_];                                                           \
+        if (owner[m_] != (i) && !mark[m_]) {                                               \
+          mark[m_] = 1;                                                                    \
+          bound[i][nb_++] = m_;                                                            \
+        }                                                                                  \
+      }                                                                                    \
+    }                                                                                      \
+    for (int t_ = 0; t_ < nb_; t_++) mark[bound[i][t_]] = 0;                               \
+    nbound[i] = nb_;                                                                       \
+  } while (0)
+  for (int i = 0; i < nump; i++) {
+    int idx[3] = {0, 0, 0};
+    for (int d = 0; d < dim; d++) {
+      int k = (int)((xi[i * dim + d] - (-1.0)) / h[d]);
+      if (k == r[d]) k--;
+      idx[d] = k;
+    }
+    const int c = idx[0] + r[0] * (idx[1] + r[1] * idx[2]);
+    owner[c] = i;
+    took[0] = c;
+    DVC_REBOUND(i, took, 1);
+  }
+  int claimed = 1;
+  while (claimed) {
+    claimed = 0;
+    for (int i = 0; i < nump; i++) {
+      int nt = 0;
+      for (int t = 0; t < nbound[i]; t++) {
+        const int c = bound[i][t];
+        const int o = owner[c];
+        int take = 0;
+        if (o == -1) {
+          take = 1;
+        } else if (o != i) {
+          double x0[3], x1[3], x2[3];
+          DVC_CENTRE(c, x0);
+          for (int d = 0; d < 3; d++) {
+            x1[d] = d < dim ? xi[o * dim + d] : 0.0;
+            x2[d] = d < dim ? xi[i * dim + d] : 0.0;
+          }
+          take = dvc_contest(dim, x0, x1, x2) > 0.0;
+        }
+        if (take) {
+          owner[c] = i;
+          took[nt++] = c;
+        }
+      }
+      claimed += nt;
+      DVC_REBOUND(i, took, nt);
+    }
+  }
+  for (int i = 0; i < nump; i++) {
+    double sc[3] = {0.0, 0.0, 0.0};
+    int cnt = 0;
+    for (int c = 0; c < nc; c++)
+      if (owner[c] == i) {
+        double x0[3];
+        DVC_CENTRE(c, x0);
+        for (int d = 0; d < dim; d++) sc[d] += x0[d];
+        cnt++;
+      }
+    w[i] = cnt * vol;
+    for (int d = 0; d < dim; d++) cent[i * dim + d] = cnt ? sc[d] / cnt : 0.0;
+  }
+#undef DVC_CENTRE
+#undef DVC_REBOUND
+  for (int i = 0; i < nump; i++) free(bound[i]);
+  free(bound); free(nbound); free(took); free(owner); free(mark);
+  for (int d = 0; d < 3; d++) free(ctr[d]);
+  return ORC_OK;
+}
+
 /* Stable counting sort of particles by owning element: the cell tables that
  * CoincidentMapper builds by visiting material points in index order
  * (CoincidentMapper.cxx:207-240), weights from ConstantWeights.cxx:178-186
@@ -347,16 +497,44 @@ static int relocate_all(orc_problem* pb) {
   for (int64_t p = 0; p < pb->P; p++) pb->cell_list[fill[pb->owner[p]]++] = p;
   free(fill);
   const double vol = dim == 2 ? 4.0 : 8.0;
+  int64_t maxc = 0;
   for (int64_t e = 0; e < pb->E; e++) {
     const int64_t cnt = pb->cell_off[e + 1] - pb->cell_off[e];
+    if (cnt > maxc) maxc = cnt;
     for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
       const int64_t p = pb->cell_list[q];
       pb->weight[p] = vol / (double)cnt;
       global_to_local(pb, e, pb->coord + 3 * p, pb->xi + dim * p);
     }
   }
+  if (pb->wkind == 1) {
+    /* DVCWeights: integration point at the Voronoi centroid, weight = its
+     * volume (DVCWeights.cxx:875-951), particles in cell-table order */
+    double* xs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
+    double* cs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
+    double* ws = (double*)malloc(sizeof(double) * (maxc > 0 ? maxc : 1));
+    for (int64_t e = 0; e < pb->E; e++) {
+      const int64_t b = pb->cell_off[e], n = pb->cell_off[e + 1] - b;
+      for (int64_t q = 0; q < n; q++)
+        for (int d = 0; d < dim; d++) xs[q * dim + d] = pb->xi[dim * pb->cell_list[b + q] + d];
+      orc_dvc_cell(dim, pb->wres, (int32_t)n, xs, cs, ws);
+      for (int64_t q = 0; q < n; q++) {
+        const int64_t p = pb->cell_list[b + q];
+        for (int d = 0; d < dim; d++) pb->xi[dim * p + d] = cs[q * dim + d];
+        pb->weight[p] = ws[q];
+      }
+    }
+    free(xs); free(cs); free(ws);
+  }
   pb->assembled = 0;
   return ORC_OK;
+}
+
+int32_t orc_set_weights(orc_problem* pb, int32_t kind, const int32_t* res) {
+  if (kind != 0 && kind != 1) return ORC_E_INVALID;
+  pb->wkind = kind;
+  for (int d = 0; d < 3; d++) pb->wres[d] = res ? res[d] : 10;
+  return relocate_all(pb);
 }
 
 orc_problem* orc_problem_new(const orc_problem_desc* d) {

@@ -121,8 +121,17 @@ int32_t orc_search_element(int32_t dim, const int32_t* res, const double* lo, co
 int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi,
                           int32_t ppc, long seed, double* coord);
 
+/* --- discrete Voronoi cell weights (DVCWeights.cxx:202-786) --- */
+/* One FE cell: sub-grid res (x, y[, z]) over [-1,1]^dim; xi: nump x dim local
+ * coordinates in cell-table order; cent (nump x dim) and w (nump) out. */
+int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w);
+
 /* --- problem --- */
 orc_problem* orc_problem_new(const orc_problem_desc* d);
+/* Weights of the material points: 0 ConstantWeights (ConstantWeights.cxx
+ * :178-186), 1 DVCWeights with sub-grid res (integration points move to the
+ * Voronoi centroids, DVCWeights.cxx:875-951).  Re-maps the particles. */
+int32_t orc_set_weights(orc_problem* pb, int32_t kind, const int32_t* res);
 void orc_problem_free(orc_problem* pb);
 int64_t orc_num_nodes(const orc_problem* pb);
 int64_t orc_num_elements(const orc_problem* pb);

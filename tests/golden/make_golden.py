@@ -6,6 +6,11 @@ available; the outputs are committed and the tests never read the reference).
   k_material_viscosity_10x10.npz
                               Underworld/Rheology/tests/expected/testMaterialViscosity2D.matrix
   constitutive_2d.json        Underworld/Rheology/tests/expected/testConstitutiveMatrix_Cartesian2D.expected
+  dvc_weights.json            the ten particle positions of PICellerator/Weights/tests/DVCWeightsSuite.cxx
+                              (2D and 3D) with the centroids and weights of
+                              PICellerator/Weights/tests/expected/DVCWeightsSuite_testGetCentroids{,2D}.txt
+  pcdvc_expected.json         PICellerator/Utils/tests/expected/testPCDVC_*.expected (tolerance and value of
+                              the mean and standard deviation of the relative integration error)
 
 The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
 1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
@@ -55,6 +60,34 @@ def main():
     json.dump(dict(viscosity=visc, D=D, strain_rate=strain, stress=stress),
               open(os.path.join(OUT, "constitutive_2d.json"), "w"), indent=1)
     print("constitutive_2d.json", visc, D, strain, stress)
+    dvc_fixtures()
+
+
+def dvc_fixtures():
+    src = open(os.path.join(REF, "PICellerator/Weights/tests/DVCWeightsSuite.cxx")).read()
+    out = {}
+    for dim, lst, fn in ((3, "pList", "DVCWeightsSuite_testGetCentroids.txt"),
+                         (2, "pList2D", "DVCWeightsSuite_testGetCentroids2D.txt")):
+        xs = []
+        for ax in "xyz"[:dim]:
+            v = [float(t) for t in re.findall(r"data->%s\[i\]\.%s = ([-+\d.]+)" % (lst, ax), src)]
+            assert len(v) == 10, (lst, ax, len(v))
+            xs.append(v)
+        xi = [[xs[d][i] for d in range(dim)] for i in range(10)]
+        exp = open(os.path.join(REF, "PICellerator/Weights/tests/expected", fn)).read()
+        cent = [[float(t) for t in m] for m in re.findall(r"centroids: \([^)]*\) = \(" + ",".join([r"\s*([-\d.]+)"] * dim) + r"\)", exp)]
+        w = [float(t) for t in re.findall(r"weight = ([-\d.]+)", exp)]
+        assert len(cent) == 10 and len(w) == 10
+        out["%dd" % dim] = dict(res=[10] * dim, xi=xi, centroids=cent, weights=w, tol_sq=1e-6)
+    json.dump(out, open(os.path.join(OUT, "dvc_weights.json"), "w"), indent=1)
+    print("dvc_weights.json", {k: len(v["xi"]) for k, v in out.items()})
+    pc = {}
+    for name in ("CircleInterface", "ExponentialInterface", "PolynomialFunction"):
+        lines = open(os.path.join(REF, "PICellerator/Utils/tests/expected/testPCDVC_%s.expected" % name)).read().split("\n")
+        (mt, mv), (st, sv) = [[float(t) for t in l.split()] for l in lines[:2]]
+        pc[name] = dict(mean_tol=mt, mean=mv, std_tol=st, std=sv)
+    json.dump(pc, open(os.path.join(OUT, "pcdvc_expected.json"), "w"), indent=1)
+    print("pcdvc_expected.json", pc)
 
 
 if __name__ == "__main__":

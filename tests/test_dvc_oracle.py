@@ -1,0 +1,140 @@
+"""CPU pins of the oracle's discrete Voronoi cell (DVC) weights
+(DVCWeights.cxx:202-786): the reference's own DVCWeightsSuite fixtures (ten
+hard-coded particles in 2D and 3D with their expected centroids and weights)
+and its PCDVC element-integration statistics (testPCDVC_*.expected: 1000
+random layouts of 20 particles per cell on a 2 x 2 mesh over [0,4]^2,
+PCDVC weights at resolution 10 with no splits or deletions, relative error of
+the element-0 integral of three functions of the local coordinates)."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+
+def _fixtures():
+    return json.load(open(os.path.join(GOLDEN, "dvc_weights.json")))
+
+
+@pytest.mark.parametrize("key", ["2d", "3d"])
+def test_oracle_dvc_reference_fixture(oracle, key):
+    f = _fixtures()[key]
+    dim = len(f["res"])
+    cent, w = oracle.dvc_cell(dim, f["res"], np.array(f["xi"]))
+    # the reference's own check: squared differences below 1e-6
+    assert np.all((cent - np.array(f["centroids"])) ** 2 < f["tol_sq"])
+    assert np.all((w - np.array(f["weights"])) ** 2 < f["tol_sq"])
+    # weights are whole sub-cells: count x (2/10)^dim, summing to the cell volume
+    vol = (2.0 / 10) ** dim
+    assert np.allclose(w / vol, np.round(w / vol), atol=1e-9)
+    assert abs(w.sum() - 2.0 ** dim) < 1e-12
+
+
+def test_oracle_dvc_symmetric_quadrants(oracle):
+    """Four particles at the quadrant centres own a quadrant each: weight 1,
+    centroid at the particle (what the mathematics fixes for a symmetric set)."""
+    xi = np.array([[-0.5, -0.5], [0.5, -0.5], [-0.5, 0.5], [0.5, 0.5]])
+    cent, w = oracle.dvc_cell(2, (10, 10), xi)
+    assert np.allclose(w, 1.0, atol=1e-14)
+    assert np.allclose(cent, xi, atol=1e-14)
+    xi3 = np.array([[sx, sy, sz] for sz in (-0.5, 0.5) for sy in (-0.5, 0.5) for sx in (-0.5, 0.5)])
+    cent3, w3 = oracle.dvc_cell(3, (8, 8, 8), xi3)
+    assert np.allclose(w3, 1.0, atol=1e-14) and np.allclose(cent3, xi3, atol=1e-14)
+
+
+def test_oracle_dvc_matches_nearest_particle_when_regions_are_convex(oracle):
+    """With few well-spread particles every discrete region is reached by the
+    growth, so ownership equals the nearest particle to each sub-cell centre
+    (brute force), and the weights are the counts of that assignment."""
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        xi = rng.uniform(-0.95, 0.95, (3, 2))
+        res = 12
+        cent, w = oracle.dvc_cell(2, (res, res), xi)
+        h = 2.0 / res
+        c = -1.0 + h / 2 + h * np.arange(res)
+        X, Y = np.meshgrid(c, c)
+        d2 = (X[..., None] - xi[:, 0]) ** 2 + (Y[..., None] - xi[:, 1]) ** 2
+        near = np.argmin(d2, axis=-1)
+        srt = np.sort(d2, axis=-1)
+        if np.any(srt[..., 1] - srt[..., 0] < 1e-12):
+            continue  # a tie: the reference keeps the first claimant
+        counts = np.bincount(near.ravel(), minlength=3)
+        assert np.allclose(w, counts * h * h, atol=1e-12)
+
+
+def _integrand(name, x, y):
+    if name == "CircleInterface":
+        return (x * x + y * y <= 1.0).astype(np.float64)
+    if name == "ExponentialInterface":
+        return (y <= 0.1 * np.exp(2 * x)).astype(np.float64)
+    return 11 + 2 * x * x + 3 * x * x * x * y + y + x * x * x + 2 * y * y
+
+
+_ANALYTIC = {"CircleInterface": math.pi, "ExponentialInterface": 0.05 * (math.exp(2) - math.exp(-2)) + 2.0,
+             "PolynomialFunction": 148.0 / 3.0}
+
+
+def pcdvc_samples(layout, count=1000):
+    """Element 0's particles of the PCDVC test layouts, in local coordinates
+    (the element spans [0,2]^2: xi = x - 1, detJ = 1)."""
+    out = []
+    for seed in range(count):
+        coord = layout(2, (2, 2), (0.0, 0.0), (4.0, 4.0), 20, seed)
+        out.append(coord[:20, :2] - 1.0)
+    return out
+
+
+def pcdvc_statistics(cents, ws):
+    stats = {}
+    for name, exact in _ANALYTIC.items():
+        err = np.array([abs(np.sum(w * _integrand(name, c[:, 0], c[:, 1])) - exact) / abs(exact)
+                        for c, w in zip(cents, ws)])
+        stats[name] = (err.mean(), math.sqrt((err * err).mean() - err.mean() ** 2))
+    return stats
+
+
+def test_oracle_pcdvc_integration_statistics(oracle):
+    """PCDVCSuite (PICellerator/Utils/tests/PCDVCSuite.cxx) with
+    testPCDVC.xml: the mean and standard deviation of the relative error match
+    the reference's expected values within the reference's tolerances."""
+    exp = json.load(open(os.path.join(GOLDEN, "pcdvc_expected.json")))
+    cents, ws = [], []
+    for xi in pcdvc_samples(oracle.layout_random):
+        c, w = oracle.dvc_cell(2, (10, 10), xi)
+        cents.append(c)
+        ws.append(w)
+    stats = pcdvc_statistics(cents, ws)
+    for name, (mean, std) in stats.items():
+        e = exp[name]
+        assert abs(mean - e["mean"]) < e["mean_tol"], (name, mean, e)
+        assert abs(std - e["std"]) < e["std_tol"], (name, std, e)
+
+
+def test_oracle_problem_dvc_weights(oracle):
+    """Problem-level DVC weights: each element's weights sum to 2^dim and the
+    integration points are the centroids of the per-element growth."""
+    from gale_amd import configs
+    cfg = configs.sinker_2d(res=6, ppc=12, seed=3)
+    o = oracle.Problem(**cfg.oracle_kwargs())
+    o.set_weights(1, (10, 10, 10))
+    owner, xi, w, coord = o.particles()
+    for e in range(int(np.prod(cfg.res))):
+        sel = np.nonzero(owner == e)[0]
+        assert abs(w[sel].sum() - 4.0) < 1e-12
+        c, ww = oracle.dvc_cell(2, (10, 10), _local(cfg, e, coord[sel]))
+        assert np.array_equal(ww, w[sel]) and np.array_equal(c, xi[sel])
+
+
+def _local(cfg, e, coord):
+    res, lo, hi = cfg.res, cfg.lo, cfg.hi
+    i, j = e % res[0], e // res[0]
+    out = np.zeros((coord.shape[0], 2))
+    for d, k in ((0, i), (1, j)):
+        a = lo[d] + (k / res[d]) * (hi[d] - lo[d]) if k > 0 else lo[d]
+        b = lo[d] + ((k + 1) / res[d]) * (hi[d] - lo[d]) if k + 1 < res[d] else hi[d]
+        out[:, d] = (coord[:, d] - 0.5 * (a + b)) * (2.0 / (b - a))
+    return out

@@ -1,0 +1,96 @@
+"""GPU checks of the discrete Voronoi cell (DVC) integration weights
+(gs_dvc_cells, gs_problem_set_weights) against the oracle: sub-cell
+ownership is integer work, so centroids and weights must equal the oracle's
+bit for bit; the reference's fixtures and PCDVC statistics follow."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, rel
+from test_dvc_oracle import pcdvc_samples, pcdvc_statistics
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("key", ["2d", "3d"])
+def test_dvc_reference_fixture(gale, oracle, key):
+    f = json.load(open(os.path.join(GOLDEN, "dvc_weights.json")))[key]
+    dim = len(f["res"])
+    xi = np.array(f["xi"])
+    cent, w = gale.dvc_cells(dim, f["res"], [0, len(xi)], xi)
+    oc, ow = oracle.dvc_cell(dim, f["res"], xi)
+    assert np.array_equal(cent, oc) and np.array_equal(w, ow)
+    assert np.all((cent - np.array(f["centroids"])) ** 2 < f["tol_sq"])
+    assert np.all((w - np.array(f["weights"])) ** 2 < f["tol_sq"])
+
+
+def test_dvc_random_cells_bit_exact(gale, oracle):
+    """Many cells of uneven particle counts (1..60), 2D and 3D, several
+    sub-grids: every centroid and weight equals the oracle's."""
+    rng = np.random.default_rng(11)
+    for dim, res in ((2, (10, 10)), (2, (7, 13)), (3, (10, 10, 10)), (3, (6, 9, 12))):
+        counts = rng.integers(1, 61, size=300)
+        counts[:3] = (1, 2, 60)
+        off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+        xi = rng.uniform(-1.0, 1.0, (off[-1], dim))
+        xi[:5] = np.clip(np.round(xi[:5] * 5) / 5, -1, 1)  # on sub-cell faces and the box boundary
+        print(f"dim {dim} res {res}: {len(counts)} cells, {off[-1]} particles", flush=True)
+        cent, w = gale.dvc_cells(dim, res, off, xi)
+        for e in range(len(counts)):
+            sl = slice(off[e], off[e + 1])
+            oc, ow = oracle.dvc_cell(dim, res, xi[sl])
+            assert np.array_equal(cent[sl], oc) and np.array_equal(w[sl], ow), (dim, res, e)
+        vol = 2.0 ** dim
+        sums = np.add.reduceat(w, off[:-1])
+        assert np.allclose(sums, vol, atol=1e-12)
+
+
+def test_pcdvc_statistics_on_gpu(gale, oracle):
+    """The 1000 PCDVC test layouts in one call: bit-exact against the oracle
+    and the reference's expected error statistics."""
+    exp = json.load(open(os.path.join(GOLDEN, "pcdvc_expected.json")))
+    samples = pcdvc_samples(gale.layout_random)
+    ref = pcdvc_samples(oracle.layout_random)
+    assert all(np.array_equal(a, b) for a, b in zip(samples, ref))
+    xi = np.concatenate(samples)
+    off = np.arange(0, 20 * len(samples) + 1, 20, dtype=np.int32)
+    cent, w = gale.dvc_cells(2, (10, 10), off, xi)
+    cents = [cent[off[i]:off[i + 1]] for i in range(len(samples))]
+    ws = [w[off[i]:off[i + 1]] for i in range(len(samples))]
+    for i in (0, 1, 499, 999):
+        oc, ow = oracle.dvc_cell(2, (10, 10), samples[i])
+        assert np.array_equal(cents[i], oc) and np.array_equal(ws[i], ow)
+    for name, (mean, std) in pcdvc_statistics(cents, ws).items():
+        e = exp[name]
+        assert abs(mean - e["mean"]) < e["mean_tol"] and abs(std - e["std"]) < e["std_tol"], (name, mean, std)
+
+
+@pytest.mark.parametrize("dim,res", [(2, 12), (3, 6)])
+def test_problem_dvc_weights_solve_parity(gale, oracle, dim, res):
+    """A problem switched to DVC weights: the integration points and weights
+    equal the oracle's, the Stokes solve matches it at tight tolerance, and
+    after one RK2 step both recompute the same weights."""
+    from test_gpu_parity import otight, pair, small_sinker, tight
+    cfg = small_sinker(dim, res, ppc=10)
+    g, o = pair(gale, oracle, cfg)
+    g.set_weights(gale.WEIGHTS_DVC, 10)
+    o.set_weights(1, (10, 10, 10))
+    _, gown, gxi, gw = g.particles()
+    oown, oxi, ow, _ = o.particles()
+    assert np.array_equal(gown, oown)
+    assert np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale))
+    u, p, rep = s.Solve()
+    st, ou, op, _ = o.uzawa(otight(oracle))
+    assert st == 0 and rep.status == 0
+    assert rel(u, ou) < 1e-8 and rel(p - p.mean(), op - op.mean()) < 1e-8
+    dt = s.compute_dt(1.0)
+    assert s.advect_rk2(dt) == 0
+    coord, gown, gxi, gw = g.particles()
+    o.set_coords(coord)
+    oown, oxi, ow, _ = o.particles()
+    assert np.array_equal(gown, oown) and np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
+    s.close()
+    g.close()
