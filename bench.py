@@ -56,6 +56,40 @@ def cpu_baseline(res, threads):
                 inner_iterations=int(rep.inner_iterations), E=cfg.E, P=cfg.P)
 
 
+def bench_dvc(G, pb, cfg, sample_cells=2000, res=10):
+    """Discrete Voronoi cell weights (DVCWeights, resolution 10) of every
+    element of the workload: GPU time of a relocation with DVC weights minus
+    one with constant weights (the DVC growth kernel), and the oracle's
+    restatement on a sample of the same elements on one host core."""
+    import oracle as O
+    _, owner, xi, _ = pb.particles()
+    rng = np.random.default_rng(3)
+    cells = rng.choice(cfg.E, size=min(sample_cells, cfg.E), replace=False)
+    order = np.argsort(owner, kind="stable")
+    starts = np.searchsorted(owner[order], np.arange(cfg.E + 1))
+    t = time.perf_counter()
+    for e in cells:
+        O.dvc_cell(cfg.dim, (res,) * cfg.dim, xi[order[starts[e]:starts[e + 1]]])
+    cpu_s = time.perf_counter() - t
+    times = {}
+    for rep in range(3):
+        for kind in (G.WEIGHTS_CONSTANT, G.WEIGHTS_DVC):
+            pb.set_weights(G.WEIGHTS_CONSTANT)  # same starting order for both
+            t = time.perf_counter()
+            pb.set_weights(kind, res)
+            times.setdefault(kind, []).append(time.perf_counter() - t)
+    pb.set_weights(G.WEIGHTS_CONSTANT)
+    gpu_ms = 1e3 * (min(times[G.WEIGHTS_DVC]) - min(times[G.WEIGHTS_CONSTANT]))
+    cpu_ms_all = 1e3 * cpu_s / len(cells) * cfg.E
+    out = {"elements": cfg.E, "particles": cfg.P, "resolution": res, "gpu_ms": gpu_ms,
+           "gpu_elements_per_s": cfg.E / (gpu_ms * 1e-3) if gpu_ms > 0 else None,
+           "relocate_constant_ms": 1e3 * min(times[G.WEIGHTS_CONSTANT]),
+           "cpu_oracle_ms_projected": cpu_ms_all, "cpu_sample_elements": int(len(cells)), "cpu_cores": 1}
+    print(f"# dvc weights: GPU {gpu_ms:.2f} ms for {cfg.E} elements, oracle (1 core) {cpu_ms_all:.0f} ms projected",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def bench_batch(G, configs, count):
     """Batched workload: count independent 2D sinkers (meshes 16^2..128^2 in
     steps of 8, 1-5 discs with log10 viscosity contrast in [-2, 4], 20 ppc),
@@ -141,6 +175,8 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--no-dvc", action="store_true", help="skip the DVC weights measurement")
+    ap.add_argument("--only-dvc", action="store_true", help="measure only the DVC weights of the workload")
     ap.add_argument("--inner-relax", type=float, default=0.1,
                     help="learned solve: cap of the relaxed inner tolerance (0 = reference 0.1 x tol throughout)")
     ap.add_argument("--profile-apply", action="store_true",
@@ -163,6 +199,9 @@ def main():
     t_gen = time.perf_counter() - t
     print(f"# generated {cfg.name} res={args.res} E={cfg.E} P={cfg.P} in {t_gen:.1f}s", file=sys.stderr, flush=True)
     pb = G.Problem(**cfg.problem_kwargs())
+    if args.only_dvc:
+        print(json.dumps({"dvc_weights": bench_dvc(G, pb, cfg)}), flush=True)
+        return
     params = G.uzawa_params(tolerance=1e-6, maxIterations=args.max_outer, inner_max_iterations=20000,
                             mg_smooth=args.mg_smooth, monitor=100)
     s = G.Stokes_SLE_UzawaSolver(pb, params)
@@ -262,6 +301,7 @@ def main():
     batch = None
     if not args.no_batch:
         batch = bench_batch(G, configs, args.batch_count)
+    dvc = None if args.no_dvc else bench_dvc(G, pb, cfg)
 
     cpu = None
     if not args.no_cpu:
@@ -318,6 +358,7 @@ def main():
                                     "fp32_ms_per_apply": ms32, "fp32_algorithmic_bytes": nbytes32},
         "cpu_baseline": cpu,
         "batch": batch,
+        "dvc_weights": dvc,
     }
     print(json.dumps(line), flush=True)
 

@@ -262,6 +262,7 @@ static int32_t relocate(gs_problem* pb) {
     if (maxp > dvc_max_particles(pb->dim, pb->wres)) return GS_E_INVALID;
     const int32_t r = dvc_weights(pb->dim, pb->wres, pb->g.E, pb->off, pb->pid, pb->pk, pb->wp, maxp, st);
     if (r != GS_OK) return r;
+    CK(hipStreamSynchronize(st));  // relocation returns with the weights in place
   }
   return esc ? GS_E_ESCAPED : GS_OK;
 }

@@ -47,10 +47,19 @@ __device__ __forceinline__ int wave_sum_i(int v) {
   return v;
 }
 
+// Sub-cell coordinates packed as ix | iy << 10 | iz << 20 (LDS table, so
+// the hot loops need no integer division).
+__device__ __forceinline__ int dvc_ix(unsigned v) { return (int)(v & 1023u); }
+__device__ __forceinline__ int dvc_iy(unsigned v) { return (int)((v >> 10) & 1023u); }
+__device__ __forceinline__ int dvc_iz(unsigned v) { return (int)(v >> 20); }
+
 // Adds to particle i's boundary the face neighbours of sub-cell c that it
-// does not own (UpdateBchain, DVCWeights.cxx:420-523).
-__device__ __forceinline__ void dvc_rebound(const DvcGrid& G, const short* owner, unsigned* bi, int i, int c) {
-  const int cx = c % G.r[0], cy = (c / G.r[0]) % G.r[1], cz = c / (G.r[0] * G.r[1]);
+// does not own (UpdateBchain, DVCWeights.cxx:420-523); returns how many bits
+// it newly set.
+__device__ __forceinline__ int dvc_rebound(const DvcGrid& G, const short* owner, const unsigned* cxyz, unsigned* bi,
+                                           int i, int c) {
+  const unsigned v = cxyz[c];
+  const int cx = dvc_ix(v), cy = dvc_iy(v), cz = dvc_iz(v);
   int nb[6], nn = 0;
   if (cy > 0) nb[nn++] = c - G.r[0];
   if (cy < G.r[1] - 1) nb[nn++] = c + G.r[0];
@@ -58,20 +67,27 @@ __device__ __forceinline__ void dvc_rebound(const DvcGrid& G, const short* owner
   if (cx > 0) nb[nn++] = c - 1;
   if (G.dim == 3 && cz < G.r[2] - 1) nb[nn++] = c + G.r[0] * G.r[1];
   if (G.dim == 3 && cz > 0) nb[nn++] = c - G.r[0] * G.r[1];
+  int added = 0;
   for (int k = 0; k < nn; k++) {
     const int m = nb[k];
-    if (owner[m] != i) atomicOr(&bi[m >> 5], 1u << (m & 31));
+    if (owner[m] != i) {
+      const unsigned bit = 1u << (m & 31);
+      if (!(atomicOr(&bi[m >> 5], bit) & bit)) added++;
+    }
   }
+  return added;
 }
 
 // LDS layout of one cell (bytes): centres, particle coordinates, order,
-// owners, the current step's claims, and one boundary bit set per particle.
+// boundary sizes, sub-cell coordinates, owners, the current step's claims,
+// and one boundary bit set per particle.
 __host__ __device__ inline size_t dvc_lds_bytes(const DvcGrid& G, int maxp) {
   size_t b = sizeof(double) * (G.r[0] + G.r[1] + G.r[2]);
   b += sizeof(double) * 3 * maxp;
-  b += sizeof(int) * maxp;
+  b += sizeof(int) * 2 * maxp;
+  b += sizeof(unsigned) * G.nc;
   b += (sizeof(short) * G.nc + 7) / 8 * 8;
-  b += sizeof(unsigned) * G.nwords;
+  b += sizeof(unsigned) * (G.nwords + 1);
   b += sizeof(unsigned) * (size_t)G.nwords * maxp;
   return b;
 }
@@ -91,10 +107,12 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
   double* ctr = reinterpret_cast<double*>(dvc_smem);
   double* xs = ctr + (G.r[0] + G.r[1] + G.r[2]);
   int* ord = reinterpret_cast<int*>(xs + 3 * maxp);
-  short* owner = reinterpret_cast<short*>(ord + maxp);
+  int* bsize = ord + maxp;  // boundary size of each particle
+  unsigned* cxyz = reinterpret_cast<unsigned*>(bsize + maxp);
+  short* owner = reinterpret_cast<short*>(cxyz + G.nc);
   unsigned* took = reinterpret_cast<unsigned*>(reinterpret_cast<unsigned char*>(owner) +
                                                (sizeof(short) * G.nc + 7) / 8 * 8);
-  unsigned* bound = took + G.nwords;
+  unsigned* bound = took + G.nwords + 1;
   // sub-cell centres, accumulated one step at a time (DVCWeights.cxx:525-619)
   if (lane < G.dim) {
     const int d = lane;
@@ -112,7 +130,11 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
     for (int k = 0; k < n; k++) rk += pid[b + k] < pj ? 1 : 0;
     ord[rk] = j;
   }
-  for (int c = lane; c < G.nc; c += kDvcLanes) owner[c] = -1;
+  for (int c = lane; c < G.nc; c += kDvcLanes) {
+    owner[c] = -1;
+    const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
+    cxyz[c] = (unsigned)ix | ((unsigned)iy << 10) | ((unsigned)iz << 20);
+  }
   for (int q = lane; q < n * G.nwords; q += kDvcLanes) bound[q] = 0;
   __syncthreads();
   for (int i = lane; i < n; i += kDvcLanes) {
@@ -133,18 +155,22 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
       }
       const int c = idx[0] + G.r[0] * (idx[1] + G.r[1] * idx[2]);
       owner[c] = (short)i;
-      dvc_rebound(G, owner, bound + (size_t)i * G.nwords, i, c);
+      bsize[i] = dvc_rebound(G, owner, cxyz, bound + (size_t)i * G.nwords, i, c);
     }
   }
   __syncthreads();
   const double* cx = ctr;
   const double* cy = ctr + G.r[0];
   const double* cz = ctr + G.r[0] + G.r[1];
-  // rounds of claims until one claims nothing
+  // Rounds of claims until one claims nothing.  Lanes take the boundary's
+  // 32-bit words; a particle with an empty boundary (it claimed nothing last
+  // time) is skipped, as ClaimCells over an empty list claims nothing.
   for (;;) {
     int total = 0;
     for (int i = 0; i < n; i++) {
+      if (bsize[i] == 0) continue;  // uniform across the wave (LDS value)
       unsigned* bi = bound + (size_t)i * G.nwords;
+      const double* xi_ = xs + 3 * i;
       int cnt = 0;
       for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
         unsigned bits = bi[wd], tk = 0;
@@ -155,9 +181,9 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
           const int o = owner[c];
           bool take = o < 0;
           if (!take && o != i) {
-            const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
-            const double c0[3] = {cx[ix], cy[iy], G.dim == 3 ? cz[iz] : 0.0};
-            take = dvc_contest(G.dim, c0, xs + 3 * o, xs + 3 * i) > 0.0;
+            const unsigned v = cxyz[c];
+            const double c3[3] = {cx[dvc_ix(v)], cy[dvc_iy(v)], G.dim == 3 ? cz[dvc_iz(v)] : 0.0};
+            take = dvc_contest(G.dim, c3, xs + 3 * o, xi_) > 0.0;
           }
           if (take) {
             owner[c] = (short)i;
@@ -171,31 +197,48 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
       __syncthreads();
       cnt = wave_sum_i(cnt);
       total += cnt;
+      int added = 0;
       if (cnt > 0)
         for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
           unsigned tk = took[wd];
           while (tk) {
             const int t = __ffs(tk) - 1;
             tk &= tk - 1u;
-            dvc_rebound(G, owner, bi, i, wd * 32 + t);
+            added += dvc_rebound(G, owner, cxyz, bi, i, wd * 32 + t);
           }
         }
+      added = wave_sum_i(added);
+      if (lane == 0) bsize[i] = added;
       __syncthreads();
     }
     if (total == 0) break;
   }
-  // centroids and volumes (DVCWeights.cxx:202-258), sub-cells in index order
+  // centroids and volumes (DVCWeights.cxx:202-258): each particle sums the
+  // centres of its sub-cells in index order, walking a bit set of the
+  // sub-cells it owns (the boundary storage, no longer needed)
+  for (int q = lane; q < n * G.nwords; q += kDvcLanes) bound[q] = 0;
+  __syncthreads();
+  for (int c = lane; c < G.nc; c += kDvcLanes) {
+    const int o = owner[c];
+    if (o >= 0) atomicOr(&bound[(size_t)o * G.nwords + (c >> 5)], 1u << (c & 31));
+  }
+  __syncthreads();
   for (int i = lane; i < n; i += kDvcLanes) {
     double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     int cnt = 0;
-    for (int c = 0; c < G.nc; c++)
-      if (owner[c] == i) {
-        const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
-        s0 += cx[ix];
-        s1 += cy[iy];
-        if (G.dim == 3) s2 += cz[iz];
+    const unsigned* own = bound + (size_t)i * G.nwords;
+    for (int wd = 0; wd < G.nwords; wd++) {
+      unsigned bits = own[wd];
+      while (bits) {
+        const int t = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        const unsigned v = cxyz[wd * 32 + t];
+        s0 += cx[dvc_ix(v)];
+        s1 += cy[dvc_iy(v)];
+        if (G.dim == 3) s2 += cz[dvc_iz(v)];
         cnt++;
       }
+    }
     const int j = b + ord[i];
     double4 q = pk[j];
     q.x = cnt ? s0 / cnt : 0.0;
@@ -205,6 +248,10 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
     w[j] = cnt * G.vol;
   }
 }
+
+}  // namespace
+
+namespace {
 
 bool make_grid(int dim, const int* res, DvcGrid* G) {
   if (dim != 2 && dim != 3) return false;
