@@ -170,6 +170,8 @@ def main():
     ap.add_argument("--cpu-res", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-outer", type=int, default=20000)
+    ap.add_argument("--plain-max-outer", type=int, default=0,
+                    help="cap of the plain solve only (profiling the learned solve); 0 = --max-outer")
     ap.add_argument("--mg-smooth", type=int, default=2)
     ap.add_argument("--weights", default=os.path.join(ROOT, "gale_amd", "lpc_weights.npy"))
     ap.add_argument("--no-learned", action="store_true")
@@ -202,7 +204,7 @@ def main():
     if args.only_dvc:
         print(json.dumps({"dvc_weights": bench_dvc(G, pb, cfg)}), flush=True)
         return
-    params = G.uzawa_params(tolerance=1e-6, maxIterations=args.max_outer, inner_max_iterations=20000,
+    params = G.uzawa_params(tolerance=1e-6, maxIterations=args.plain_max_outer or args.max_outer, inner_max_iterations=20000,
                             mg_smooth=args.mg_smooth, monitor=100)
     s = G.Stokes_SLE_UzawaSolver(pb, params)
     s.setup()  # first linear solve of the Picard sequence: no yielding yet
