@@ -44,7 +44,7 @@ EXPORTS = (
     "gs_solver_set_learned", "gs_lpc_selftest",
     "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
     "gs_batch_solution",
-    "gs_problem_set_weights", "gs_dvc_cells", "gs_dvc_max_particles",
+    "gs_problem_set_weights", "gs_dvc_cells", "gs_dvc_max_particles", "gs_solver_particle_fields",
 )
 
 
@@ -147,6 +147,7 @@ def lib():
             "gs_problem_set_weights": (i32, [vp, i32, vp]),
             "gs_dvc_cells": (i32, [i32, vp, i64, vp, vp, vp, vp]),
             "gs_dvc_max_particles": (i64, [i32, vp]),
+            "gs_solver_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
             "gs_apply_K": (i32, [vp, vp, vp]),
             "gs_apply_K_fp32": (i32, [vp, vp, vp]),
             "gs_apply_K_moments": (i32, [vp, vp, vp]),
@@ -581,6 +582,18 @@ class Stokes_SLE_UzawaSolver:
         self._ds = dataset
         _check(lib().gs_solver_record_pairs(self._h, dataset._h if dataset is not None else None,
                                             int(krylov_every)), "record_pairs")
+
+    def particle_fields(self, u=None):
+        """StoreStress / StoreViscosity / strain-rate fields of the material
+        points (caller order) for the full velocity u (default: the last
+        solution)."""
+        P, dim = self.problem.P, self.problem.dim
+        nst = 3 if dim == 2 else 6
+        sr, inv, eta, st = np.zeros((P, nst)), np.zeros(P), np.zeros(P), np.zeros((P, nst))
+        uu = None if u is None else _f64(u)
+        _check(lib().gs_solver_particle_fields(self._h, _ptr(uu), _ptr(sr), _ptr(inv), _ptr(eta), _ptr(st)),
+               "gs_solver_particle_fields")
+        return dict(strain_rate=sr, strain_rate_inv=inv, viscosity=eta, stress=st)
 
     def setup_arrays(self):
         P = self.problem.P

@@ -1688,6 +1688,63 @@ extern "C" int32_t gs_solver_setup_arrays(gs_solver* s, double* diagK, double* Q
   return GS_OK;
 }
 
+// Output fields of the material points in the caller's order (StoreViscosity
+// / StoreStress / the strain-rate fields).  u: full nodal velocity (N*dim,
+// host), or NULL for the solver's last solution.
+extern "C" int32_t gs_solver_particle_fields(gs_solver* s, const double* u, double* strain_rate,
+                                             double* strain_rate_inv, double* viscosity, double* stress) {
+  if (!s) return GS_E_INVALID;
+  gs_problem* pb = s->pb;
+  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  const long long P = pb->P;
+  const int nst = pb->dim == 2 ? 3 : 6;
+  hipStream_t st = pb->st;
+  if (P <= 0) return GS_OK;
+  double *d_sr = nullptr, *d_inv = nullptr, *d_st = nullptr;
+  const double* ud = s->ufull;
+  if (u) {
+    CK(hipMemcpyAsync(s->tmpN, u, sizeof(double) * s->n, hipMemcpyHostToDevice, st));
+    ud = s->tmpN;
+  }
+  int32_t rc = GS_OK;
+  if (dalloc(&d_sr, P * nst) != hipSuccess || dalloc(&d_inv, P) != hipSuccess || dalloc(&d_st, P * nst) != hipSuccess) {
+    rc = GS_E_HIP;
+  } else {
+    launch_particle_fields(pb->g, pb->pk, pb->keys, P, ud, s->eta, d_sr, d_inv, d_st, st);
+    std::vector<double> sr(P * nst), inv(P), sv(P * nst);
+    std::vector<int> pid(P);
+    if (hipStreamSynchronize(st) != hipSuccess ||
+        hipMemcpy(sr.data(), d_sr, sizeof(double) * P * nst, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(inv.data(), d_inv, sizeof(double) * P, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(sv.data(), d_st, sizeof(double) * P * nst, hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost) != hipSuccess) {
+      rc = GS_E_HIP;
+    } else {
+      std::vector<double> eta;
+      if (viscosity) {
+        eta.resize(P);
+        if (hipMemcpy(eta.data(), s->eta, sizeof(double) * P, hipMemcpyDeviceToHost) != hipSuccess) rc = GS_E_HIP;
+      }
+      int offE = 0;
+      if (hipMemcpy(&offE, pb->off + pb->g.E, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) rc = GS_E_HIP;
+      if (rc == GS_OK)
+        for (long long i = 0; i < P; i++) {
+          const long long o = pid[i];
+          for (int k = 0; k < nst; k++) {
+            if (strain_rate) strain_rate[o * nst + k] = sr[i * nst + k];
+            if (stress) stress[o * nst + k] = sv[i * nst + k];
+          }
+          if (strain_rate_inv) strain_rate_inv[o] = inv[i];
+          if (viscosity) viscosity[o] = i < offE ? eta[i] : 0.0;
+        }
+    }
+  }
+  dfree(d_sr);
+  dfree(d_inv);
+  dfree(d_st);
+  return rc;
+}
+
 // ---------------------------------------------------------------------------
 // learned preconditioner attachment (include/gale_learned.h)
 // ---------------------------------------------------------------------------

@@ -85,6 +85,10 @@ void launch_apply_Gt(const Geo& g, const double* x, double* q, hipStream_t st);
 void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f,
                   double* Q, double* eta_out, uint8_t* yielded, hipStream_t st);
 void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st);
+// Strain rate (P x nst), its second invariant (P) and stress (P x nst) of the
+// material points (sorted order) for velocity u and viscosity eta.
+void launch_particle_fields(const Geo& g, const double4* pk, const unsigned* keys, long long P, const double* u,
+                            const double* eta, double* sr, double* inv, double* st, hipStream_t st_);
 
 // Reductions / vector kernels.  Results land in S[slot].
 void launch_dot(const double* a, const double* b, long long n, double* S, int slot, Reducer r, hipStream_t st);

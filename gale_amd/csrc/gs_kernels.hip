@@ -695,6 +695,85 @@ __global__ __launch_bounds__(kBlock) void k_setup(Geo g, int c, long long mcount
     }
 }
 
+// Output fields of the material points after a solve: strain rate of the
+// velocity u at the integration point (the symmetric part of the velocity
+// gradient, StrainRateField), its second invariant (TensorMath.cxx:431-459),
+// and the stress of the isotropic constitutive matrix with the point's
+// viscosity from the last constitutive pass (StoreStress.cxx / StoreViscosity
+// .cxx: ConstitutiveMatrix_CalculateStress, ConstitutiveMatrixCartesian
+// .cxx:727-775).  Sorted particle order; escaped particles get zeros.
+template <int DIM>
+__global__ void k_particle_fields(Geo g, const double4* __restrict__ pk, const unsigned* __restrict__ keys,
+                                  long long P, const double* __restrict__ u, const double* __restrict__ eta,
+                                  double* __restrict__ sr_out, double* __restrict__ inv_out,
+                                  double* __restrict__ st_out) {
+  constexpr int NEN = DIM == 2 ? 4 : 8, NST = DIM == 2 ? 3 : 6;
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const long long e = keys[p];
+  double sr[6] = {0, 0, 0, 0, 0, 0}, st[6] = {0, 0, 0, 0, 0, 0}, inv = 0.0;
+  if (e < g.E) {
+    const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
+                              DIM == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
+    double gi[3] = {1, 1, 1};
+    for (int d = 0; d < DIM; d++) gi[d] = 2.0 / (vcoord(g, d, ijk[d] + 1) - vcoord(g, d, ijk[d]));
+    const double4 q4 = pk[p];
+    const double xi[3] = {q4.x, q4.y, q4.z};
+    const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
+    const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
+    double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int q = 0; q < NEN; q++) {
+      double fct[3], sg[3];
+      for (int d = 0; d < DIM; d++) {
+        sg[d] = ((q >> d) & 1) ? 1.0 : -1.0;
+        fct[d] = 1.0 + sg[d] * xi[d];
+      }
+      const double cN = DIM == 2 ? 0.25 : 0.125;
+      double dN[3];
+      for (int d = 0; d < DIM; d++) {
+        double v = cN * sg[d] * gi[d];
+        for (int d2 = 0; d2 < DIM; d2++)
+          if (d2 != d) v *= fct[d2];
+        dN[d] = v;
+      }
+      const long long nd = n0 + (q & 1) + ((q >> 1) & 1) * sy + ((q >> 2) & 1) * sz;
+      for (int i = 0; i < DIM; i++)
+        for (int j = 0; j < DIM; j++) L[i][j] += dN[j] * u[nd * DIM + i];
+    }
+    if (DIM == 2) {
+      sr[0] = L[0][0];
+      sr[1] = L[1][1];
+      sr[2] = 0.5 * (L[0][1] + L[1][0]);
+    } else {
+      sr[0] = L[0][0];
+      sr[1] = L[1][1];
+      sr[2] = L[2][2];
+      sr[3] = 0.5 * (L[0][1] + L[1][0]);
+      sr[4] = 0.5 * (L[0][2] + L[2][0]);
+      sr[5] = 0.5 * (L[1][2] + L[2][1]);
+    }
+    inv = sym_inv(DIM, sr);
+    const double et = eta[p];
+    for (int k = 0; k < NST; k++) st[k] = (k < DIM ? 2.0 * et : et * 2.0) * sr[k];
+  }
+  for (int k = 0; k < NST; k++) {
+    sr_out[p * NST + k] = sr[k];
+    st_out[p * NST + k] = st[k];
+  }
+  inv_out[p] = inv;
+}
+
+void launch_particle_fields(const Geo& g, const double4* pk, const unsigned* keys, long long P, const double* u,
+                            const double* eta, double* sr, double* inv, double* st, hipStream_t st_) {
+  if (P <= 0) return;
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_particle_fields<3>, dim3(grid_for(P)), dim3(kBlock), 0, st_, g, pk, keys, P, u, eta, sr, inv,
+                       st);
+  else
+    hipLaunchKernelGGL(k_particle_fields<2>, dim3(grid_for(P)), dim3(kBlock), 0, st_, g, pk, keys, P, u, eta, sr, inv,
+                       st);
+}
+
 void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f, double* Q,
                   double* eta_out, uint8_t* yielded, hipStream_t st) {
   launch_fill(diag, g.N * g.dim, 0.0, st);

@@ -220,6 +220,17 @@ int32_t gs_apply_K_moments(gs_solver* s, const double* x_dev, double* y_dev);
  * Q (E), f after lifting (N*dim, zero on Dirichlet dofs), h (E),
  * per-particle effective viscosity (P, caller order).  NULL skips. */
 int32_t gs_solver_setup_arrays(gs_solver* s, double* diagK, double* Q, double* f, double* h, double* eta);
+/* Output fields of the material points, caller order, any pointer may be NULL:
+ * strain rate of the velocity u at each integration point (nst = 3 in 2D:
+ * xx, yy, xy; 6 in 3D: xx, yy, zz, xy, xz, yz; the symmetric part of the
+ * velocity gradient, the reference's StrainRateField), its second invariant
+ * (SymmetricTensor_2ndInvariant, TensorMath.cxx:431-459), the viscosity of
+ * the last constitutive pass (StoreVisc, StoreViscosity.cxx:156-180) and the
+ * stress of the isotropic constitutive matrix with that viscosity
+ * (StoreStress, StoreStress.cxx:190-217 via ConstitutiveMatrix_CalculateStress).
+ * u: full nodal velocity (N*dim), or NULL for the solver's last solution. */
+int32_t gs_solver_particle_fields(gs_solver* s, const double* u, double* strain_rate, double* strain_rate_inv,
+                                  double* viscosity, double* stress);
 
 #ifdef __cplusplus
 }

@@ -115,6 +115,7 @@ def lib():
             "orc_advect_rk2": (i32, [vp, vp, d, vp]),
             "orc_dvc_cell": (i32, [i32, vp, i32, vp, vp, vp]),
             "orc_set_weights": (i32, [vp, i32, vp]),
+            "orc_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -262,6 +263,21 @@ class Problem:
         if h:
             lib().orc_problem_free(h)
             self._h = None
+
+    def particle_fields(self, u_full):
+        """Strain rate, its invariant, viscosity and stress per material point
+        (caller order) for the full velocity u_full; the viscosity is the one
+        of the last constitutive pass (uzawa / assemble)."""
+        nst = 3 if self.dim == 2 else 6
+        u = _f64(u_full)
+        sr = np.zeros((self.P, nst))
+        inv = np.zeros(self.P)
+        eta = np.zeros(self.P)
+        st = np.zeros((self.P, nst))
+        code = lib().orc_particle_fields(self._h, _p(u), _p(sr), _p(inv), _p(eta), _p(st))
+        if code != 0:
+            raise RuntimeError(f"orc_particle_fields status {code}")
+        return dict(strain_rate=sr, strain_rate_inv=inv, viscosity=eta, stress=st)
 
     def set_weights(self, kind, res=(10, 10, 10)):
         """0: ConstantWeights; 1: DVCWeights with sub-grid res."""

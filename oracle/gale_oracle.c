@@ -747,6 +747,50 @@ static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, co
   return eta;
 }
 
+/* Output fields of the material points (StoreStress.cxx:190-217,
+ * StoreViscosity.cxx:156-180, the StrainRateField): strain rate of u_full at
+ * each integration point, its second invariant (TensorMath.cxx:431-459), the
+ * viscosity of the last constitutive pass and the isotropic stress
+ * (ConstitutiveMatrixCartesian CalculateStress).  Any output may be NULL. */
+int32_t orc_particle_fields(orc_problem* pb, const double* u_full, double* sr_out, double* inv_out,
+                            double* eta_out, double* st_out) {
+  if (!pb->has_visc) return ORC_E_INVALID;
+  const int dim = pb->dim, nen = pb->nen, nst = dim == 2 ? 3 : 6;
+  for (int64_t e = 0; e < pb->E; e++)
+    for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+      const int64_t p = pb->cell_list[q];
+      double GNx[3][MAXNEN];
+      int64_t nodes[MAXNEN];
+      global_derivs(pb, e, pb->xi + dim * p, GNx);
+      elem_nodes(pb, e, nodes);
+      double L[3][3] = {{0}};
+      for (int a = 0; a < nen; a++)
+        for (int i = 0; i < dim; i++)
+          for (int j = 0; j < dim; j++) L[i][j] += GNx[j][a] * u_full[nodes[a] * dim + i];
+      double sr[6], st[6];
+      if (dim == 2) {
+        sr[0] = L[0][0];
+        sr[1] = L[1][1];
+        sr[2] = 0.5 * (L[0][1] + L[1][0]);
+      } else {
+        sr[0] = L[0][0];
+        sr[1] = L[1][1];
+        sr[2] = L[2][2];
+        sr[3] = 0.5 * (L[0][1] + L[1][0]);
+        sr[4] = 0.5 * (L[0][2] + L[2][0]);
+        sr[5] = 0.5 * (L[1][2] + L[2][1]);
+      }
+      orc_isotropic_stress(dim, pb->eta[p], sr, st);
+      for (int k = 0; k < nst; k++) {
+        if (sr_out) sr_out[p * nst + k] = sr[k];
+        if (st_out) st_out[p * nst + k] = st[k];
+      }
+      if (inv_out) inv_out[p] = sym_invariant(dim, sr);
+      if (eta_out) eta_out[p] = pb->eta[p];
+    }
+  return ORC_OK;
+}
+
 int32_t orc_particle_viscosity(orc_problem* pb, const double* u_prev_full, const double* p_prev,
                                int32_t yield_on, double* eta, uint8_t* yielded) {
   for (int64_t e = 0; e < pb->E; e++)

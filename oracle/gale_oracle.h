@@ -121,6 +121,10 @@ int32_t orc_search_element(int32_t dim, const int32_t* res, const double* lo, co
 int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi,
                           int32_t ppc, long seed, double* coord);
 
+/* --- output fields of the material points (StoreStress / StoreViscosity) --- */
+int32_t orc_particle_fields(orc_problem* pb, const double* u_full, double* sr, double* inv, double* eta,
+                            double* stress);
+
 /* --- discrete Voronoi cell weights (DVCWeights.cxx:202-786) --- */
 /* One FE cell: sub-grid res (x, y[, z]) over [-1,1]^dim; xi: nump x dim local
  * coordinates in cell-table order; cent (nump x dim) and w (nump) out. */

@@ -291,3 +291,34 @@ def test_uzawa_firewalls(oracle):
     pb = _sinker(oracle, 2, (6, 6))
     st, u, p, rep = pb.uzawa(tolerance=1e-14, max_iterations=1)
     assert st == oracle.E_MAXIT
+
+
+# ---------------------------------------------------------------------------
+# Output fields of the material points (StoreStress / StoreViscosity / the
+# strain-rate field): a linear velocity field has the constant strain rate
+# sym(A) everywhere, whose invariant and isotropic stress are closed forms.
+@pytest.mark.parametrize("dim", [2, 3])
+def test_particle_fields_linear_velocity_closed_form(oracle, dim):
+    from gale_amd import configs
+    cfg = configs.sinker_2d(res=6, ppc=6, seed=4) if dim == 2 else configs.rayleigh_taylor_3d(res=4, ppc=4, seed=4)
+    o = oracle.Problem(**cfg.oracle_kwargs())
+    o.viscosity()
+    rng = np.random.default_rng(dim)
+    A = rng.normal(size=(dim, dim))
+    b = rng.normal(size=dim)
+    res, lo, hi = list(cfg.res), list(cfg.lo), list(cfg.hi)
+    axes = [np.array([oracle.vertex_coord(res[d], lo[d], hi[d], i) for i in range(res[d] + 1)]) for d in range(dim)]
+    grids = np.meshgrid(*axes, indexing="ij")
+    X = np.stack([g.transpose().ravel() for g in grids], axis=1)  # x fastest
+    u = (X @ A.T + b).ravel()
+    f = o.particle_fields(u)
+    S = 0.5 * (A + A.T)
+    if dim == 2:
+        want = np.array([S[0, 0], S[1, 1], S[0, 1]])
+    else:
+        want = np.array([S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[0, 2], S[1, 2]])
+    assert np.allclose(f["strain_rate"], want, rtol=1e-12, atol=1e-12)
+    inv = np.sqrt(0.5 * np.sum(S * S))
+    assert np.allclose(f["strain_rate_inv"], inv, rtol=1e-12)
+    eta = f["viscosity"][:, None]
+    assert np.allclose(f["stress"], 2.0 * eta * want, rtol=1e-12, atol=1e-12)
