@@ -45,6 +45,7 @@ EXPORTS = (
     "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
     "gs_batch_solution",
     "gs_problem_set_weights", "gs_dvc_cells", "gs_dvc_max_particles", "gs_solver_particle_fields",
+    "gs_problem_population_control",
 )
 
 
@@ -148,6 +149,7 @@ def lib():
             "gs_dvc_cells": (i32, [i32, vp, i64, vp, vp, vp, vp]),
             "gs_dvc_max_particles": (i64, [i32, vp]),
             "gs_solver_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
+            "gs_problem_population_control": (i32, [vp, vp, vp]),
             "gs_apply_K": (i32, [vp, vp, vp]),
             "gs_apply_K_fp32": (i32, [vp, vp, vp]),
             "gs_apply_K_moments": (i32, [vp, vp, vp]),
@@ -444,6 +446,17 @@ class Problem:
         _check(st, "set_coords", allow=(E_ESCAPED,))
         return st
 
+    def population_control(self, params=None):
+        """PCDVC splits and deletions, then DVC weights; returns (added,
+        deleted).  The particle count (self.P) changes: caller order is the
+        survivors in their previous order, then the new particles."""
+        p = params if params is not None else pcdvc_params()
+        counts = np.zeros(2, np.int64)
+        st = lib().gs_problem_population_control(self._h, C.byref(p), _ptr(counts))
+        _check(st, "population_control", allow=(E_ESCAPED,))
+        self.P = self.info()["P"]
+        return int(counts[0]), int(counts[1])
+
     def set_weights(self, kind=WEIGHTS_DVC, resolution=10):
         """The integration swarm's WeightsCalculator: WEIGHTS_CONSTANT
         (ConstantWeights) or WEIGHTS_DVC (DVCWeights, "resolution" or
@@ -452,6 +465,20 @@ class Problem:
         st = lib().gs_problem_set_weights(self._h, int(kind), _ptr(r))
         _check(st, "set_weights", allow=(E_ESCAPED,))
         return st
+
+
+class PcdvcParams(C.Structure):
+    _fields_ = [("upper_t", C.c_double), ("lower_t", C.c_double), ("max_splits", C.c_int32),
+                ("max_deletions", C.c_int32), ("split_in_interface_cells", C.c_int32),
+                ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("pad_", C.c_int32)]
+
+
+def pcdvc_params(upperT=25.0, lowerT=0.6, maxSplits=3, maxDeletions=2, splitInInterfaceCells=False,
+                 deleteInInterfaceCells=False, resolution=10):
+    """PCDVC component parameters with the reference's XML names and defaults."""
+    r = _dims(3, [resolution] * 3 if np.isscalar(resolution) else resolution, 10)
+    return PcdvcParams(upperT, lowerT, maxSplits, maxDeletions, int(bool(splitInInterfaceCells)),
+                       int(bool(deleteInInterfaceCells)), (C.c_int32 * 3)(*r), 0)
 
 
 def dvc_max_particles(dim, resolution=10) -> int:

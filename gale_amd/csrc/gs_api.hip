@@ -93,6 +93,7 @@ struct gs_problem {
   int wkind = GS_WEIGHTS_CONSTANT;  // integration weights of the material points
   int wres[3] = {10, 10, 10};       // DVC sub-grid
   double* wp = nullptr;             // DVC weight of each (sorted) particle
+  unsigned long long gen = 0;       // bumped by every relocation (solvers re-run their setup)
 };
 
 
@@ -143,8 +144,10 @@ struct gs_solver {
   double* hS = nullptr;
   Reducer rd{};
   double* cg_z = nullptr;
+  long long P_alloc = 0;  // particle count of eta / yielded
   MGData mg;
   bool setup_done = false;
+  unsigned long long setup_gen = 0;  // problem generation the setup was made for
   bool has_solution = false;
   int has_executed = 0;
   long long k_applies = 0;
@@ -162,6 +165,9 @@ struct gs_solver {
   double* fl_q = nullptr;
   double* fl_aux = nullptr;
 };
+// The solver's arrays describe the problem's particles as of its last setup.
+static bool needs_setup(const gs_solver* s) { return !s->setup_done || s->setup_gen != s->pb->gen; }
+
 
 // ---------------------------------------------------------------------------
 extern "C" const char* gs_strerror(int32_t s) {
@@ -214,6 +220,7 @@ __global__ void k_gather_particles(const int* __restrict__ perm, long long P, co
 // sort: within an element particles keep their previous relative order),
 // carry all particle arrays along, rebuild the element offsets and remap xi.
 static int32_t relocate(gs_problem* pb) {
+  pb->gen++;
   const long long P = pb->P;
   hipStream_t st = pb->st;
   launch_fill(pb->S + S_ESC, 1, 0.0, st);
@@ -465,6 +472,123 @@ extern "C" int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const in
   return relocate(pb);
 }
 
+extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_params* pp, int64_t* counts) {
+  if (!pb || !pp || pp->max_splits < 0 || pp->max_deletions < 0) return GS_E_INVALID;
+  int r[3] = {pp->res[0], pp->res[1], pb->dim == 3 ? pp->res[2] : 1};
+  if (dvc_max_particles(pb->dim, r) <= 0) return GS_E_INVALID;
+  double upT = pp->upper_t, lowT = pp->lower_t;
+  if (upT < lowT) {  // PCDVC.cxx:265-270 resets inconsistent thresholds
+    lowT = 0.6;
+    upT = 25.0;
+  }
+  hipStream_t st = pb->st;
+  const long long P = pb->P, E = pb->g.E;
+  CK(hipStreamSynchronize(st));
+  if (!pb->wp && P > 0) CK(dalloc(&pb->wp, P));
+  std::vector<int> off(E + 1);
+  CK(hipMemcpy(off.data(), pb->off, sizeof(int) * (E + 1), hipMemcpyDeviceToHost));
+  int maxp = 0;
+  for (long long e = 0; e < E; e++) maxp = std::max(maxp, off[e + 1] - off[e]);
+  if (maxp > dvc_max_particles(pb->dim, r)) return GS_E_INVALID;
+  // DVC of the particles' own local coordinates
+  launch_remap(pb->g, pb->coord, pb->off, pb->pk, E, st);
+  CKS(dvc_weights(pb->dim, r, E, pb->off, pb->pid, pb->pk, pb->wp, maxp, st));
+  const int slots = std::max(1, pp->max_splits);
+  unsigned char* del = nullptr;
+  int *nsplit = nullptr, *sparent = nullptr, *dmap = nullptr, *dbase = nullptr;
+  double* scoord = nullptr;
+  double *coordN = nullptr, *eta0N = nullptr;
+  int* matN = nullptr;
+  int32_t rc = GS_OK;
+  auto cleanup = [&]() {
+    dfree(del); dfree(nsplit); dfree(sparent); dfree(dmap); dfree(dbase); dfree(scoord);
+    dfree(coordN); dfree(eta0N); dfree(matN);
+  };
+  if (dalloc(&del, P > 0 ? P : 1) || dalloc(&nsplit, E) || dalloc(&sparent, E * slots) ||
+      dalloc(&scoord, E * slots * 3)) {
+    cleanup();
+    return GS_E_HIP;
+  }
+  const double V = pb->dim == 2 ? 4.0 : 8.0;
+  PcdvcArgs a{upT * V / 100.0, lowT * V / 100.0, pp->max_splits, pp->max_deletions,
+              pp->split_in_interface_cells, pp->delete_in_interface_cells, slots};
+  pcdvc_decide(pb->g, pb->off, pb->pid, pb->mat, pb->pk, pb->wp, a, del, nsplit, scoord, sparent, st);
+  // new caller order: survivors in their previous order, then the new particles
+  std::vector<unsigned char> hdel(P > 0 ? P : 1);
+  std::vector<int> pid(P > 0 ? P : 1), ns(E), base(E), map(P > 0 ? P : 1);
+  if (hipStreamSynchronize(st) || hipMemcpy(hdel.data(), del, P, hipMemcpyDeviceToHost) ||
+      hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost) ||
+      hipMemcpy(ns.data(), nsplit, sizeof(int) * E, hipMemcpyDeviceToHost)) {
+    cleanup();
+    return GS_E_HIP;
+  }
+  std::vector<int> delc(P > 0 ? P : 1, 0), newid(P > 0 ? P : 1, -1);
+  for (long long i = 0; i < P; i++) delc[pid[i]] = hdel[i];
+  long long nsurv = 0;
+  for (long long c = 0; c < P; c++)
+    if (!delc[c]) newid[c] = (int)nsurv++;
+  for (long long i = 0; i < P; i++) map[i] = newid[pid[i]];
+  long long nnew = 0;
+  for (long long e = 0; e < E; e++) {
+    base[e] = (int)nnew;
+    nnew += ns[e];
+  }
+  const long long P2 = nsurv + nnew;
+  if (P2 > 0x7fffffffLL) {
+    cleanup();
+    return GS_E_INVALID;
+  }
+  if (dalloc(&dmap, P > 0 ? P : 1) || dalloc(&dbase, E) || dalloc(&coordN, 3 * (P2 > 0 ? P2 : 1)) ||
+      dalloc(&matN, P2 > 0 ? P2 : 1) || (pb->eta0p && dalloc(&eta0N, P2 > 0 ? P2 : 1)) ||
+      hipMemcpy(dmap, map.data(), sizeof(int) * P, hipMemcpyHostToDevice) ||
+      hipMemcpy(dbase, base.data(), sizeof(int) * E, hipMemcpyHostToDevice)) {
+    cleanup();
+    return GS_E_HIP;
+  }
+  pcdvc_rebuild(P, E, dmap, pb->coord, pb->mat, pb->eta0p, nsplit, dbase, slots, scoord, sparent, nsurv, coordN, matN,
+                eta0N, st);
+  if (hipStreamSynchronize(st)) {
+    cleanup();
+    return GS_E_HIP;
+  }
+  // resize the particle arrays of the problem
+  dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
+  dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->pk); dfree(pb->pk32); dfree(pb->keys); dfree(pb->keys2);
+  dfree(pb->vals); dfree(pb->vals2); dfree(pb->wp);
+  pb->pk32_valid = false;
+  pb->coord = coordN;
+  pb->mat = matN;
+  pb->eta0p = eta0N;
+  coordN = nullptr;
+  matN = nullptr;
+  eta0N = nullptr;
+  cleanup();
+  pb->P = P2;
+  const long long PA = P2 > 0 ? P2 : 1;
+  CK(dalloc(&pb->coord2, 3 * PA));
+  CK(dalloc(&pb->mat2, PA));
+  CK(dalloc(&pb->pid, PA));
+  CK(dalloc(&pb->pid2, PA));
+  CK(dalloc(&pb->pk, PA));
+  CK(dalloc(&pb->keys, PA));
+  CK(dalloc(&pb->keys2, PA));
+  CK(dalloc(&pb->vals, PA));
+  CK(dalloc(&pb->vals2, PA));
+  CK(dalloc(&pb->wp, PA));
+  if (pb->eta0p) CK(dalloc(&pb->eta0p2, PA));
+  std::vector<int> ids(PA);
+  for (long long i = 0; i < PA; i++) ids[i] = (int)i;
+  CK(hipMemcpy(pb->pid, ids.data(), sizeof(int) * PA, hipMemcpyHostToDevice));
+  pb->wkind = GS_WEIGHTS_DVC;
+  for (int d = 0; d < 3; d++) pb->wres[d] = r[d];
+  if (counts) {
+    counts[0] = nnew;
+    counts[1] = P - nsurv;
+  }
+  rc = relocate(pb);
+  return rc;
+}
+
 extern "C" int32_t gs_problem_set_coords(gs_problem* pb, const double* coord) {
   if (!pb || (!coord && pb->P > 0)) return GS_E_INVALID;
   const long long P = pb->P;
@@ -549,6 +673,7 @@ extern "C" int32_t gs_solver_create(gs_problem* pb, const gs_uzawa_params* up, g
       dalloc(&s->tmpE, np) || dalloc(&s->eta, P) || dalloc(&s->uprev, n) || dalloc(&s->yielded, P) ||
       dalloc(&s->S, S_COUNT))
     return fail(GS_E_HIP);
+  s->P_alloc = P;
   if (hipHostMalloc(reinterpret_cast<void**>(&s->hS), sizeof(double) * S_COUNT) != hipSuccess) return fail(GS_E_HIP);
   if (alloc_reducer(&s->rd)) return fail(GS_E_HIP);
   if (hipEventCreate(&s->e0) != hipSuccess || hipEventCreate(&s->e1) != hipSuccess) return fail(GS_E_HIP);
@@ -951,6 +1076,14 @@ static void mg_precond(gs_solver* s, const double* r, double* z) {
 static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_prev, int yield_on) {
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
+  if (s->P_alloc != pb->P) {  // the particle count changed (population control)
+    dfree(s->eta);
+    dfree(s->yielded);
+    const long long PA = pb->P > 0 ? pb->P : 1;
+    CK(dalloc(&s->eta, PA));
+    CK(dalloc(&s->yielded, PA));
+    s->P_alloc = pb->P;
+  }
   if (!pb->min_one_particle) return GS_E_INVALID;
   SetupArgs a{};
   a.u_prev = u_prev;
@@ -982,6 +1115,7 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
     launch_elem_logeta(pb->g, pb->pk, pb->off, s->logeta, st);
   }
   s->setup_done = true;
+  s->setup_gen = pb->gen;
   return GS_OK;
 }
 
@@ -1421,7 +1555,7 @@ extern "C" int32_t gs_stokes_solve(gs_solver* s, double* u, double* p, gs_report
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
   const long long k0 = s->k_applies;
-  if (!s->setup_done) {
+  if (needs_setup(s)) {
     CK(hipEventRecord(s->e0, st));
     CKS(setup_dev(s, nullptr, nullptr, 0));
     CK(hipEventRecord(s->e1, st));
@@ -1513,7 +1647,7 @@ extern "C" int32_t gs_picard_solve(gs_solver* s, const gs_picard_params* pp, dou
 
 extern "C" int32_t gs_get_rhs(gs_solver* s, double* fhat) {
   if (!s) return GS_E_INVALID;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   gs_problem* pb = s->pb;
   long long its = 0;
   const double rt = s->up.inner_rtol_override > 0 ? s->up.inner_rtol_override : s->up.tolerance;
@@ -1578,7 +1712,7 @@ extern "C" int32_t gs_advect_rk2(gs_solver* s, double dt) {
 
 extern "C" int32_t gs_apply_K(gs_solver* s, const double* x, double* y) {
   if (!s || !x || !y) return GS_E_INVALID;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   launch_apply_K(s->pb->g, x, y, s->pb->pk, s->pb->off, nullptr, s->pb->halo64, s->pb->st);
   s->k_applies++;
   return GS_OK;
@@ -1586,7 +1720,7 @@ extern "C" int32_t gs_apply_K(gs_solver* s, const double* x, double* y) {
 
 static int32_t ensure_pk32(gs_solver* s) {
   gs_problem* pb = s->pb;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   if (!pb->pk32) CK(dalloc(&pb->pk32, pb->P));
   if (!pb->pk32_valid) {
     launch_pack32(pb->pk, pb->pk32, pb->P, pb->st);
@@ -1623,7 +1757,7 @@ extern "C" int32_t gs_solver_sync(gs_solver* s) {
 
 extern "C" int32_t gs_apply_K_moments(gs_solver* s, const double* x, double* y) {
   if (!s || !x || !y) return GS_E_INVALID;
-  if (!s->setup_done || !s->mg.structured) {
+  if (needs_setup(s) || !s->mg.structured) {
     CKS(setup_dev(s, nullptr, nullptr, 0));
     if (!s->mg.structured) CKS(mg_numeric(s));
   }
@@ -1636,7 +1770,7 @@ extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, do
   if (!s || n_rep <= 0 || !ms_per_apply || form < 0 || form > 3) return GS_E_INVALID;
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   if (form >= 2 && !s->mg.structured) CKS(mg_numeric(s));
   if (form == 1 || form == 3) {
     if (form == 1) CKS(ensure_pk32(s));
@@ -1669,7 +1803,7 @@ extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, do
 extern "C" int32_t gs_solver_setup_arrays(gs_solver* s, double* diagK, double* Q, double* f, double* h, double* eta) {
   if (!s) return GS_E_INVALID;
   gs_problem* pb = s->pb;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   CK(hipStreamSynchronize(pb->st));
   if (diagK) CK(hipMemcpy(diagK, s->diag, sizeof(double) * s->n, hipMemcpyDeviceToHost));
   if (Q) CK(hipMemcpy(Q, s->Q, sizeof(double) * s->np, hipMemcpyDeviceToHost));
@@ -1695,7 +1829,7 @@ extern "C" int32_t gs_solver_particle_fields(gs_solver* s, const double* u, doub
                                              double* strain_rate_inv, double* viscosity, double* stress) {
   if (!s) return GS_E_INVALID;
   gs_problem* pb = s->pb;
-  if (!s->setup_done) CKS(setup_dev(s, nullptr, nullptr, 0));
+  if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   const long long P = pb->P;
   const int nst = pb->dim == 2 ? 3 : 6;
   hipStream_t st = pb->st;

@@ -16,4 +16,19 @@ int dvc_weights(int dim, const int* res, long long ncell, const int* off, const 
 // Largest per-cell particle count whose LDS fits kDvcMaxLds.
 long long dvc_max_particles(int dim, const int* res);
 
+// PCDVC decisions per cell (one thread per element, particles in pid order):
+// del[i] = 1 for a deleted (sorted) particle, nsplit[e] new particles at
+// scoord[(e * slots + k) * 3 ..] with parent (sorted index) sparent[e * slots + k].
+struct PcdvcArgs {
+  double max_w, min_w;
+  int max_splits, max_deletions, split_if, delete_if, slots;
+};
+void pcdvc_decide(const Geo& g, const int* off, const int* pid, const int* mat, const double4* pk, const double* w,
+                  const PcdvcArgs& a, unsigned char* del, int* nsplit, double* scoord, int* sparent, hipStream_t st);
+// Rebuild the particle arrays in the new caller order: survivors to map[i]
+// (-1 = deleted), new particles after n_surv in cell order.
+void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, const double* eta0,
+                   const int* nsplit, const int* base, int slots, const double* scoord, const int* sparent,
+                   long long n_surv, double* coordN, int* matN, double* eta0N, hipStream_t st);
+
 }  // namespace gs

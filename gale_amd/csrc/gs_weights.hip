@@ -25,6 +25,15 @@ namespace {
 
 constexpr int kDvcLanes = 64;
 
+// Vertex coordinate of a regular mesh (CartesianGenerator): lo + (i / n)(hi - lo),
+// the end vertices exactly lo and hi.
+__device__ __forceinline__ double vcoord_w(const Geo& g, int d, long long idx) {
+#pragma clang fp contract(off)
+  if (idx <= 0) return g.lo[d];
+  if (idx >= g.res[d]) return g.hi[d];
+  return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
+}
+
 struct DvcGrid {
   int dim;
   int r[3];
@@ -295,6 +304,118 @@ long long dvc_max_particles(int dim, const int* res) {
       hi = mid - 1;
   }
   return lo;
+}
+
+}  // namespace gs
+
+namespace gs {
+
+namespace {
+
+__global__ void k_pcdvc_decide(Geo g, const int* __restrict__ off, const int* __restrict__ pid,
+                               const int* __restrict__ mat, const double4* __restrict__ pk,
+                               const double* __restrict__ w, PcdvcArgs a, unsigned char* __restrict__ del,
+                               int* __restrict__ nsplit, double* __restrict__ scoord, int* __restrict__ sparent) {
+#pragma clang fp contract(off)
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= g.E) return;
+  const int b = off[e], n = off[e + 1] - b;
+  for (int j = 0; j < n; j++) del[b + j] = 0;
+  nsplit[e] = 0;
+  if (n == 0) return;
+  // first particle in pid order and whether the cell holds several materials
+  int first = 0;
+  for (int j = 1; j < n; j++)
+    if (pid[b + j] < pid[b + first]) first = j;
+  bool interface = false;
+  for (int j = 0; j < n; j++) interface = interface || mat[b + j] != mat[b + first];
+  int mS = a.max_splits, mD = a.max_deletions;
+  if (interface) {
+    if (!a.delete_if) mD = 0;
+    if (!a.split_if) mS = 0;
+  }
+  if (mD > n - 4) mD = n / 2;
+  const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
+                            g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
+  int ns = 0, nd = 0, last = -1;
+  for (int t = 0; t < n; t++) {
+    // next particle in pid order
+    int j = -1;
+    for (int k = 0; k < n; k++) {
+      const int pk_ = pid[b + k];
+      if (pk_ > last && (j < 0 || pk_ < pid[b + j])) j = k;
+    }
+    last = pid[b + j];
+    const double wj = w[b + j];
+    if (wj > a.max_w && ns < mS) {
+      const double4 c = pk[b + j];
+      const double xi[3] = {c.x, c.y, c.z};
+      double* out = scoord + ((long long)e * a.slots + ns) * 3;
+      for (int d = 0; d < 3; d++) {
+        double x = 0.0;
+        if (d < g.dim) {
+          const double lo = vcoord_w(g, d, ijk[d]), hi = vcoord_w(g, d, ijk[d] + 1);
+          x = 0.5 * ((1.0 - xi[d]) * lo + (1.0 + xi[d]) * hi);
+        }
+        out[d] = x;
+      }
+      sparent[(long long)e * a.slots + ns] = b + j;
+      ns++;
+    }
+    if (wj < a.min_w && nd < mD) {
+      del[b + j] = 1;
+      nd++;
+    }
+  }
+  nsplit[e] = ns;
+}
+
+__global__ void k_pcdvc_compact(long long P, const int* __restrict__ map, const double* __restrict__ coord,
+                                const int* __restrict__ mat, const double* __restrict__ eta0,
+                                double* __restrict__ coordN, int* __restrict__ matN, double* __restrict__ eta0N) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= P) return;
+  const int m = map[i];
+  if (m < 0) return;
+  for (int d = 0; d < 3; d++) coordN[3LL * m + d] = coord[3 * i + d];
+  matN[m] = mat[i];
+  if (eta0N) eta0N[m] = eta0[i];
+}
+
+__global__ void k_pcdvc_append(long long E, const int* __restrict__ nsplit, const int* __restrict__ base, int slots,
+                               const double* __restrict__ scoord, const int* __restrict__ sparent,
+                               const int* __restrict__ mat, const double* __restrict__ eta0, long long n_surv,
+                               double* __restrict__ coordN, int* __restrict__ matN, double* __restrict__ eta0N) {
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  for (int k = 0; k < nsplit[e]; k++) {
+    const long long np = n_surv + base[e] + k;
+    const long long s = e * slots + k;
+    for (int d = 0; d < 3; d++) coordN[3 * np + d] = scoord[3 * s + d];
+    const int par = sparent[s];
+    matN[np] = mat[par];
+    if (eta0N) eta0N[np] = eta0[par];
+  }
+}
+
+}  // namespace
+
+void pcdvc_decide(const Geo& g, const int* off, const int* pid, const int* mat, const double4* pk, const double* w,
+                  const PcdvcArgs& a, unsigned char* del, int* nsplit, double* scoord, int* sparent, hipStream_t st) {
+  if (g.E <= 0) return;
+  hipLaunchKernelGGL(k_pcdvc_decide, dim3((unsigned)((g.E + 255) / 256)), dim3(256), 0, st, g, off, pid, mat, pk, w,
+                     a, del, nsplit, scoord, sparent);
+}
+
+void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, const double* eta0,
+                   const int* nsplit, const int* base, int slots, const double* scoord, const int* sparent,
+                   long long n_surv, double* coordN, int* matN, double* eta0N, hipStream_t st) {
+  if (P > 0)
+    hipLaunchKernelGGL(k_pcdvc_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, map, coord, mat, eta0,
+                       coordN, matN, eta0N);
+  if (E > 0)
+    hipLaunchKernelGGL(k_pcdvc_append, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, E, nsplit, base, slots,
+                       scoord, sparent, mat, eta0, n_surv, coordN, matN, eta0N);
 }
 
 }  // namespace gs

@@ -48,6 +48,32 @@ int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const int32_t* res)
 int32_t gs_dvc_cells(int32_t dim, const int32_t* res, int64_t ncell, const int32_t* off, const double* xi,
                      double* centroid, double* weight);
 
+/* PCDVC population control (PCDVC.cxx:656-1090 / 1092-1512 with Inflow off)
+ * followed by DVC weights.  Per cell, on the discrete Voronoi regions of its
+ * particles: the first max_splits particles (in cell order) whose region
+ * exceeds upper_t percent of the cell volume are split -- a copy of the
+ * material point (material, eta0) is added at the centroid of the parent's
+ * region -- and the first max_deletions whose region is below lower_t
+ * percent are deleted; a cell holding more than one material splits and
+ * deletes nothing unless split_in_interface_cells / delete_in_interface_cells;
+ * max_deletions is cut to nump / 2 when it exceeds nump - 4.  The particle
+ * count changes: afterwards the caller's particle order is the surviving
+ * particles in their previous order, then the new ones (cell order, then
+ * split order).  The problem is left with DVC weights of resolution res.
+ * counts (may be NULL) = {particles added, particles deleted}.  Reference
+ * defaults: upperT 25, lowerT 0.6, maxSplits 3, maxDeletions 2, resolution 10. */
+typedef struct gs_pcdvc_params {
+  double upper_t;
+  double lower_t;
+  int32_t max_splits;
+  int32_t max_deletions;
+  int32_t split_in_interface_cells;
+  int32_t delete_in_interface_cells;
+  int32_t res[3];
+  int32_t pad_;
+} gs_pcdvc_params;
+int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_params* p, int64_t* counts);
+
 /* Largest particle count per cell that gs_dvc_cells and GS_WEIGHTS_DVC accept
  * for this sub-grid (0 for an invalid resolution). */
 int64_t gs_dvc_max_particles(int32_t dim, const int32_t* res);

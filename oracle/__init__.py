@@ -116,6 +116,7 @@ def lib():
             "orc_dvc_cell": (i32, [i32, vp, i32, vp, vp, vp]),
             "orc_set_weights": (i32, [vp, i32, vp]),
             "orc_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
+            "orc_population_control": (i32, [vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -263,6 +264,22 @@ class Problem:
         if h:
             lib().orc_problem_free(h)
             self._h = None
+
+    def population_control(self, upperT=25.0, lowerT=0.6, maxSplits=3, maxDeletions=2,
+                           splitInInterfaceCells=False, deleteInInterfaceCells=False, resolution=(10, 10, 10)):
+        class _P(C.Structure):
+            _fields_ = [("upper_t", C.c_double), ("lower_t", C.c_double), ("max_splits", C.c_int32),
+                        ("max_deletions", C.c_int32), ("split_in_interface_cells", C.c_int32),
+                        ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("pad_", C.c_int32)]
+        r = list(resolution) + [10] * (3 - len(resolution))
+        p = _P(upperT, lowerT, maxSplits, maxDeletions, int(splitInInterfaceCells), int(deleteInInterfaceCells),
+               (C.c_int32 * 3)(*r), 0)
+        counts = np.zeros(2, np.int64)
+        st = lib().orc_population_control(self._h, C.byref(p), _p(counts))
+        if st != 0:
+            raise RuntimeError(f"orc_population_control status {st}")
+        self.P = int(lib().orc_num_particles(self._h))
+        return int(counts[0]), int(counts[1])
 
     def particle_fields(self, u_full):
         """Strain rate, its invariant, viscosity and stress per material point

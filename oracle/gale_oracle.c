@@ -530,6 +530,119 @@ static int relocate_all(orc_problem* pb) {
   return ORC_OK;
 }
 
+/* PCDVC population control (PCDVC.cxx:656-1090 / 1092-1512, Inflow off),
+ * then DVC weights.  Per cell, on the DVC of its particles' local
+ * coordinates: maxW = upperT% and minW = lowerT% of the cell volume; in a
+ * cell holding more than one material no particle is deleted (split) unless
+ * deleteInInterfaceCells (splitInInterfaceCells); maxDeletions is cut to
+ * nump/2 when it exceeds nump - 4; the first maxSplits particles (in cell
+ * order) with weight above maxW are split -- a copy of the material point is
+ * put at the centroid of the parent's region, at the global position
+ * 0.5 ((1 - xi) a + (1 + xi) b) per axis -- and the first maxDeletions with
+ * weight below minW are deleted.  The particle order afterwards: the
+ * surviving particles in their previous order, then the new ones in cell
+ * order and split order.  counts = {added, deleted}. */
+int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int64_t* counts) {
+  const int dim = pb->dim;
+  const double V = dim == 2 ? 4.0 : 8.0;
+  const double maxW = pp->upper_t * V / 100.0, minW = pp->lower_t * V / 100.0;
+  int32_t res[3] = {pp->res[0], pp->res[1], dim == 3 ? pp->res[2] : 1};
+  int64_t maxc = 0;
+  for (int64_t e = 0; e < pb->E; e++)
+    if (pb->cell_off[e + 1] - pb->cell_off[e] > maxc) maxc = pb->cell_off[e + 1] - pb->cell_off[e];
+  uint8_t* del = (uint8_t*)calloc(pb->P > 0 ? pb->P : 1, 1);
+  int64_t cap = 64, nnew = 0;
+  double* ncoord = (double*)malloc(sizeof(double) * 3 * cap);
+  int64_t* nparent = (int64_t*)malloc(sizeof(int64_t) * cap);
+  double* xs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
+  double* cs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
+  double* ws = (double*)malloc(sizeof(double) * (maxc > 0 ? maxc : 1));
+  int64_t ndel = 0;
+  for (int64_t e = 0; e < pb->E; e++) {
+    const int64_t b = pb->cell_off[e], n = pb->cell_off[e + 1] - b;
+    if (n == 0) continue;
+    for (int64_t q = 0; q < n; q++) global_to_local(pb, e, pb->coord + 3 * pb->cell_list[b + q], xs + dim * q);
+    orc_dvc_cell(dim, res, (int32_t)n, xs, cs, ws);
+    int interface = 0;
+    for (int64_t q = 1; q < n; q++)
+      if (pb->material[pb->cell_list[b + q]] != pb->material[pb->cell_list[b]]) interface = 1;
+    int64_t mS = pp->max_splits, mD = pp->max_deletions;
+    if (interface) {
+      if (!pp->delete_in_interface_cells) mD = 0;
+      if (!pp->split_in_interface_cells) mS = 0;
+    }
+    if (mD > n - 4) mD = n / 2;
+    int64_t ns = 0, nd = 0;
+    int32_t ijk[3] = {(int32_t)(e % pb->res[0]), (int32_t)((e / pb->res[0]) % pb->res[1]),
+                      dim == 3 ? (int32_t)(e / ((int64_t)pb->res[0] * pb->res[1])) : 0};
+    for (int64_t q = 0; q < n; q++) {
+      const int64_t p = pb->cell_list[b + q];
+      if (ws[q] > maxW && ns < mS) {
+        if (nnew == cap) {
+          cap *= 2;
+          ncoord = (double*)realloc(ncoord, sizeof(double) * 3 * cap);
+          nparent = (int64_t*)realloc(nparent, sizeof(int64_t) * cap);
+        }
+        for (int d = 0; d < 3; d++) {
+          double x = 0.0;
+          if (d < dim) {
+            const double a = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d]);
+            const double bb = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d] + 1);
+            const double xi = cs[q * dim + d];
+            x = 0.5 * ((1.0 - xi) * a + (1.0 + xi) * bb);
+          }
+          ncoord[nnew * 3 + d] = x;
+        }
+        nparent[nnew++] = p;
+        ns++;
+      }
+      if (ws[q] < minW && nd < mD) {
+        del[p] = 1;
+        nd++;
+      }
+    }
+    ndel += nd;
+  }
+  const int64_t P2 = pb->P - ndel + nnew;
+  double* coord2 = (double*)malloc(sizeof(double) * 3 * (P2 > 0 ? P2 : 1));
+  int32_t* mat2 = (int32_t*)malloc(sizeof(int32_t) * (P2 > 0 ? P2 : 1));
+  double* e02 = pb->eta0p ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
+  int64_t k = 0;
+  for (int64_t p = 0; p < pb->P; p++) {
+    if (del[p]) continue;
+    memcpy(coord2 + 3 * k, pb->coord + 3 * p, sizeof(double) * 3);
+    mat2[k] = pb->material[p];
+    if (e02) e02[k] = pb->eta0p[p];
+    k++;
+  }
+  for (int64_t j = 0; j < nnew; j++, k++) {
+    memcpy(coord2 + 3 * k, ncoord + 3 * j, sizeof(double) * 3);
+    mat2[k] = pb->material[nparent[j]];
+    if (e02) e02[k] = pb->eta0p[nparent[j]];
+  }
+  free(pb->coord); free(pb->material); free(pb->eta0p);
+  pb->coord = coord2;
+  pb->material = mat2;
+  pb->eta0p = e02;
+  pb->P = P2;
+  pb->owner = (int32_t*)realloc(pb->owner, sizeof(int32_t) * (P2 > 0 ? P2 : 1));
+  pb->xi = (double*)realloc(pb->xi, sizeof(double) * dim * (P2 > 0 ? P2 : 1));
+  pb->weight = (double*)realloc(pb->weight, sizeof(double) * (P2 > 0 ? P2 : 1));
+  pb->cell_list = (int64_t*)realloc(pb->cell_list, sizeof(int64_t) * (P2 > 0 ? P2 : 1));
+  free(pb->eta); free(pb->yielded);
+  pb->eta = (double*)calloc(P2 > 0 ? P2 : 1, sizeof(double));
+  pb->yielded = (uint8_t*)calloc(P2 > 0 ? P2 : 1, 1);
+  pb->has_visc = 0;
+  free(del); free(ncoord); free(nparent); free(xs); free(cs); free(ws);
+  if (counts) {
+    counts[0] = nnew;
+    counts[1] = ndel;
+  }
+  pb->wkind = 1;
+  for (int d = 0; d < 3; d++) pb->wres[d] = res[d];
+  return relocate_all(pb);
+}
+
 int32_t orc_set_weights(orc_problem* pb, int32_t kind, const int32_t* res) {
   if (kind != 0 && kind != 1) return ORC_E_INVALID;
   pb->wkind = kind;

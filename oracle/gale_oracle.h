@@ -136,6 +136,14 @@ orc_problem* orc_problem_new(const orc_problem_desc* d);
  * :178-186), 1 DVCWeights with sub-grid res (integration points move to the
  * Voronoi centroids, DVCWeights.cxx:875-951).  Re-maps the particles. */
 int32_t orc_set_weights(orc_problem* pb, int32_t kind, const int32_t* res);
+/* PCDVC population control (Inflow off) followed by DVC weights. */
+typedef struct orc_pcdvc_params {
+  double upper_t, lower_t;            /* "upperT", "lowerT" (percent of the cell volume) */
+  int32_t max_splits, max_deletions;  /* "maxSplits", "maxDeletions" */
+  int32_t split_in_interface_cells, delete_in_interface_cells;
+  int32_t res[3], pad_;
+} orc_pcdvc_params;
+int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* p, int64_t* counts);
 void orc_problem_free(orc_problem* pb);
 int64_t orc_num_nodes(const orc_problem* pb);
 int64_t orc_num_elements(const orc_problem* pb);

@@ -138,3 +138,87 @@ def _local(cfg, e, coord):
         b = lo[d] + ((k + 1) / res[d]) * (hi[d] - lo[d]) if k + 1 < res[d] else hi[d]
         out[:, d] = (coord[:, d] - 0.5 * (a + b)) * (2.0 / (b - a))
     return out
+
+
+# ---------------------------------------------------------------------------
+# PCDVC population control (PCDVC.cxx:1092-1512, Inflow off)
+def two_by_two_kwargs(coord, material):
+    """A 2 x 2 mesh of unit cells, two isoviscous materials, no boundary conditions."""
+    return dict(dim=2, res=(2, 2), lo=(0.0, 0.0), hi=(2.0, 2.0), coord=coord, material=material,
+                materials=[dict(eta0=1.0, density=1.0), dict(eta0=2.0, density=2.0)])
+
+
+def cells_layout(rng, clustered):
+    """16 particles in each of the four unit cells; in cell 0 fifteen sit in a
+    small corner cluster and one far away, so the far one owns a large region
+    (split) and several clustered ones own none (deleted)."""
+    pts = []
+    for e in range(4):
+        ox, oy = e % 2, e // 2
+        if e == 0 and clustered:
+            c = np.c_[rng.uniform(0.02, 0.2, 15), rng.uniform(0.02, 0.2, 15)]
+            c = np.vstack([c, [[0.8, 0.8]]])
+        else:
+            c = rng.uniform(0.0, 1.0, (16, 2))
+        pts.append(c + [ox, oy])
+    xy = np.vstack(pts)
+    return np.c_[xy, np.zeros(len(xy))]
+
+
+def expected_population_control(coord, mat, maxS=3, maxD=2, upT=25.0, lowT=0.6):
+    """The decisions of PCDVC per cell of the 2 x 2 unit-cell mesh, from the
+    DVC of each cell's particles (cell order = particle index order)."""
+    import oracle as O
+    keep, new_xy, new_mat = np.ones(len(coord), bool), [], []
+    for e in range(4):
+        ox, oy = e % 2, e // 2
+        idx = [p for p in range(len(coord)) if int(coord[p, 0] >= 1.0) == ox and int(coord[p, 1] >= 1.0) == oy]
+        xi = 2.0 * (coord[idx, :2] - [ox, oy]) - 1.0
+        cent, w = O.dvc_cell(2, (10, 10), xi)
+        n = len(idx)
+        iface = len(set(mat[idx])) > 1
+        mS, mD = (0, 0) if iface else (maxS, maxD)
+        if mD > n - 4:
+            mD = n // 2
+        s_ = [q for q in range(n) if w[q] > upT * 4 / 100][:mS]
+        d_ = [q for q in range(n) if w[q] < lowT * 4 / 100][:mD]
+        for q in d_:
+            keep[idx[q]] = False
+        for q in s_:
+            lo = np.array([ox, oy], float)
+            new_xy.append(0.5 * ((1.0 - cent[q]) * lo + (1.0 + cent[q]) * (lo + 1.0)))
+            new_mat.append(mat[idx[q]])
+    return keep, np.array(new_xy).reshape(-1, 2), np.array(new_mat, np.int32)
+
+
+def test_oracle_population_control_designed_cell(oracle):
+    rng = np.random.default_rng(2)
+    coord = cells_layout(rng, True)
+    mat = np.zeros(len(coord), np.int32)
+    o = oracle.Problem(**two_by_two_kwargs(coord, mat))
+    # cell 0 by construction: its far particle splits, two clustered ones go
+    cent, w = oracle.dvc_cell(2, (10, 10), 2.0 * coord[:16, :2] - 1.0)
+    assert [i for i in range(16) if w[i] > 1.0][:3] == [15]
+    assert len([i for i in range(16) if w[i] < 0.024]) >= 2
+    keep, new_xy, new_mat = expected_population_control(coord, mat)
+    added, deleted = o.population_control()
+    owner, xi, wt, coord2 = o.particles()
+    assert (added, deleted) == (len(new_xy), int((~keep).sum()))
+    assert o.P == len(coord) + added - deleted
+    ns = int(keep.sum())
+    assert np.array_equal(coord2[:ns], coord[keep])
+    assert np.array_equal(coord2[ns:, :2], new_xy)
+    # weights are the DVC of the new particle sets: every cell sums to 4
+    for e in range(4):
+        assert abs(wt[owner == e].sum() - 4.0) < 1e-12
+
+
+def test_oracle_population_control_interface_cell_untouched(oracle):
+    rng = np.random.default_rng(2)
+    coord = cells_layout(rng, True)
+    mat = np.zeros(len(coord), np.int32)
+    mat[3] = 1  # cell 0 now holds two materials
+    o = oracle.Problem(**two_by_two_kwargs(coord, mat))
+    added, deleted = o.population_control()
+    owner, _, _, _ = o.particles()
+    assert np.sum(owner == 0) == 16  # no split or deletion in the interface cell

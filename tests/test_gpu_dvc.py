@@ -94,3 +94,61 @@ def test_problem_dvc_weights_solve_parity(gale, oracle, dim, res):
     assert np.array_equal(gown, oown) and np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
     s.close()
     g.close()
+
+
+def _compare_particles(g, o):
+    coord, gown, gxi, gw = g.particles()
+    oown, oxi, ow, ocoord = o.particles()
+    assert g.P == o.P
+    assert np.array_equal(coord, ocoord)
+    assert np.array_equal(gown, oown)
+    assert np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
+
+
+def test_population_control_designed_cells(gale, oracle):
+    from test_dvc_oracle import cells_layout, two_by_two_kwargs
+    rng = np.random.default_rng(2)
+    coord = cells_layout(rng, True)
+    mat = np.zeros(len(coord), np.int32)
+    kw = two_by_two_kwargs(coord, mat)
+    g = gale.Problem(dim=2, res=kw["res"], lo=kw["lo"], hi=kw["hi"], coord=coord, material_index=mat,
+                     materials=[gale.material(eta0=1.0, density=1.0), gale.material(eta0=2.0, density=2.0)])
+    o = oracle.Problem(**kw)
+    ga = g.population_control()
+    oa = o.population_control()
+    assert ga == oa and ga[0] >= 1 and ga[1] >= 2
+    _compare_particles(g, o)
+
+
+@pytest.mark.parametrize("dim,res", [(2, 12), (3, 5)])
+def test_population_control_matches_oracle_and_solves(gale, oracle, dim, res):
+    """Uneven particle density (coordinates squeezed towards the origin) and
+    two materials: splits, deletions and interface cells all occur; the new
+    particle sets, weights and a tight Stokes solve equal the oracle's, and a
+    second pass agrees too."""
+    from test_gpu_parity import otight, small_sinker, tight
+    cfg = small_sinker(dim, res, ppc=12)
+    kw, okw = cfg.problem_kwargs(), cfg.oracle_kwargs()
+    c = np.array(kw["coord"], dtype=np.float64)
+    lo, hi = np.array(cfg.lo, float), np.array(cfg.hi, float)
+    for d in range(dim):
+        t = (c[:, d] - lo[d]) / (hi[d] - lo[d])
+        c[:, d] = lo[d] + (hi[d] - lo[d]) * t ** 1.7
+    kw["coord"] = c
+    okw["coord"] = c
+    g = gale.Problem(**kw)
+    o = oracle.Problem(**okw)
+    for rep in range(2):
+        ga = g.population_control()
+        oa = o.population_control()
+        print(f"pass {rep}: added {ga[0]} deleted {ga[1]}", flush=True)
+        assert ga == oa
+        _compare_particles(g, o)
+    assert ga[0] + ga[1] > 0
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale))
+    u, p, rep_ = s.Solve()
+    st, ou, op, _ = o.uzawa(otight(oracle))
+    assert st == 0 and rep_.status == 0
+    assert rel(u, ou) < 1e-8 and rel(p - p.mean(), op - op.mean()) < 1e-8
+    s.close()
+    g.close()
