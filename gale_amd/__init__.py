@@ -37,7 +37,7 @@ EXPORTS = (
     "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
-    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
+    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
     "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
     "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
     "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
@@ -161,6 +161,8 @@ def lib():
             "gs_rng_stream": (d, [i64, i64, vp]),
             "gs_layout_random": (i64, [i32, vp, vp, vp, i32, i64, vp]),
             "gs_layout_gauss": (i64, [i32, vp, vp, vp, vp]),
+            "gs_sobol_sequence": (i32, [i32, i64, i64, vp]),
+            "gs_layout_spacefiller": (i64, [i32, vp, vp, vp, i64, d, i32, vp]),
             "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
             "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
             "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
@@ -237,7 +239,7 @@ def rng_stream(seed: int, count: int) -> np.ndarray:
 
 def layout_random(dim, res, lo, hi, ppc, seed=13) -> np.ndarray:
     r = np.asarray(_dims(dim, res, 1), np.int32)
-    l, h = np.asarray(_dims(dim, lo, 0.0)), np.asarray(_dims(dim, hi, 1.0))
+    l, h = np.asarray(_dims(dim, lo, 0.0), np.float64), np.asarray(_dims(dim, hi, 1.0), np.float64)
     E = int(np.prod(r[:dim]))
     coord = np.zeros((E * ppc, 3))
     n = lib().gs_layout_random(dim, _ptr(r), _ptr(l), _ptr(h), ppc, seed, _ptr(coord))
@@ -248,10 +250,35 @@ def layout_random(dim, res, lo, hi, ppc, seed=13) -> np.ndarray:
 
 def layout_gauss(dim, res, lo, hi) -> np.ndarray:
     r = np.asarray(_dims(dim, res, 1), np.int32)
-    l, h = np.asarray(_dims(dim, lo, 0.0)), np.asarray(_dims(dim, hi, 1.0))
+    l, h = np.asarray(_dims(dim, lo, 0.0), np.float64), np.asarray(_dims(dim, hi, 1.0), np.float64)
     E = int(np.prod(r[:dim]))
     coord = np.zeros((E * 2 ** dim, 3))
     lib().gs_layout_gauss(dim, _ptr(r), _ptr(l), _ptr(h), _ptr(coord))
+    return coord
+
+
+def sobol_sequence(generator: int, count: int, first: int = 0) -> np.ndarray:
+    """Numbers first .. first+count-1 of tabulated Sobol generator `generator`
+    (SobolGenerator_GetNumberByIndex, StgDomain/Utils/src/SobolGenerator.cxx:271-294)."""
+    out = np.zeros(count)
+    if lib().gs_sobol_sequence(generator, first, count, _ptr(out)) != 0:
+        raise GaleError(E_INVALID, "sobol_sequence")
+    return out
+
+
+def layout_spacefiller(dim, res, lo, hi, total=0, avg_per_cell=0.0, first_generator=0) -> np.ndarray:
+    """SpaceFillerParticleLayout (StgDomain/Swarm/src/SpaceFillerParticleLayout.cxx:211-240):
+    quasi-random particles over the whole box, totalInitialParticles = total or
+    averageInitialParticlesPerCell = avg_per_cell (exactly one of the two)."""
+    r = np.asarray(_dims(dim, res, 1), np.int32)
+    l, h = np.asarray(_dims(dim, lo, 0.0), np.float64), np.asarray(_dims(dim, hi, 1.0), np.float64)
+    n = lib().gs_layout_spacefiller(dim, _ptr(r), _ptr(l), _ptr(h), int(total), float(avg_per_cell),
+                                    first_generator, None)
+    if n < 0:
+        raise GaleError(E_INVALID, "layout_spacefiller")
+    coord = np.zeros((n, 3))
+    lib().gs_layout_spacefiller(dim, _ptr(r), _ptr(l), _ptr(h), int(total), float(avg_per_cell), first_generator,
+                                _ptr(coord))
     return coord
 
 

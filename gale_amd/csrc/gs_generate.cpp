@@ -4,6 +4,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/gale_generate.h"
@@ -93,6 +95,225 @@ extern "C" int64_t gs_layout_gauss(int32_t dim, const int32_t* res, const double
 
 // WallVC (StgDomain/Utils/src/WallVC.cxx): every node on the named wall,
 // corners and edges included.
+namespace {
+
+// Sobol generators: the primitive polynomials and initial direction numbers
+// published by Joe & Kuo (ACM TOMS 29(1), 2003, remark on Bratley & Fox's
+// Algorithm 659), the first 100 rows in the order of the reference's table
+// (StgDomain/Utils/src/SobolGenerator.cxx:307-420).  The layouts draw from
+// them in this order, so the rows must match the reference's one for one.
+// `coefficient` holds the polynomial's inner coefficients a_1..a_{s-1} as
+// bits s-2..0.
+struct SobolRow {
+  uint8_t degree, coefficient;
+  uint16_t m[9];
+};
+constexpr int kSobolGenerators = 100;
+constexpr int kSobolBits = 30;   // numbers are 30-bit binary fractions
+const SobolRow kSobolTable[kSobolGenerators] = {
+    // 0
+    {1, 0, {1}},
+    {2, 1, {1, 1}},
+    {3, 1, {1, 3, 7}},
+    {3, 2, {1, 1, 5}},
+    {4, 1, {1, 3, 1, 1}},
+    {4, 4, {1, 1, 3, 7}},
+    {5, 2, {1, 3, 3, 9, 9}},
+    {5, 13, {1, 3, 7, 13, 3}},
+    {5, 7, {1, 1, 5, 11, 27}},
+    {5, 14, {1, 3, 5, 1, 15}},
+    // 10
+    {5, 11, {1, 1, 7, 3, 29}},
+    {5, 4, {1, 3, 7, 7, 21}},
+    {6, 1, {1, 1, 1, 9, 23, 37}},
+    {6, 16, {1, 3, 3, 5, 19, 33}},
+    {6, 13, {1, 1, 3, 13, 11, 7}},
+    {6, 22, {1, 1, 7, 13, 25, 5}},
+    {6, 19, {1, 3, 5, 11, 7, 11}},
+    {6, 25, {1, 1, 1, 3, 13, 39}},
+    {7, 1, {1, 3, 1, 15, 17, 63, 13}},
+    {7, 32, {1, 1, 5, 5, 1, 59, 33}},
+    // 20
+    {7, 4, {1, 3, 3, 3, 25, 17, 115}},
+    {7, 8, {1, 1, 7, 15, 29, 15, 41}},
+    {7, 7, {1, 3, 1, 7, 3, 23, 79}},
+    {7, 56, {1, 3, 7, 9, 31, 29, 17}},
+    {7, 14, {1, 1, 5, 13, 11, 3, 29}},
+    {7, 28, {1, 1, 1, 9, 5, 21, 119}},
+    {7, 19, {1, 1, 3, 1, 23, 13, 75}},
+    {7, 50, {1, 3, 7, 11, 27, 31, 73}},
+    {7, 21, {1, 1, 7, 7, 19, 25, 105}},
+    {7, 42, {1, 3, 1, 5, 21, 9, 7}},
+    // 30
+    {7, 31, {1, 1, 1, 15, 5, 49, 59}},
+    {7, 62, {1, 3, 1, 1, 1, 33, 65}},
+    {7, 37, {1, 3, 5, 15, 17, 19, 21}},
+    {7, 41, {1, 1, 7, 11, 13, 29, 3}},
+    {7, 55, {1, 3, 7, 5, 7, 11, 113}},
+    {7, 59, {1, 1, 5, 11, 15, 19, 61}},
+    {8, 14, {1, 1, 1, 1, 9, 27, 89, 7}},
+    {8, 56, {1, 1, 3, 7, 31, 15, 45, 23}},
+    {8, 21, {1, 3, 3, 9, 25, 25, 107, 39}},
+    {8, 22, {1, 1, 7, 7, 3, 63, 21, 217}},
+    // 40
+    {8, 38, {1, 3, 5, 7, 5, 55, 71, 141}},
+    {8, 47, {1, 1, 5, 1, 23, 17, 79, 27}},
+    {8, 49, {1, 1, 5, 15, 7, 63, 19, 53}},
+    {8, 50, {1, 1, 3, 15, 3, 49, 71, 181}},
+    {8, 52, {1, 3, 3, 15, 17, 19, 61, 169}},
+    {8, 67, {1, 3, 3, 13, 23, 41, 41, 35}},
+    {8, 70, {1, 1, 1, 3, 3, 59, 57, 15}},
+    {8, 84, {1, 3, 1, 3, 3, 3, 121, 207}},
+    {8, 97, {1, 3, 5, 15, 21, 57, 87, 45}},
+    {8, 103, {1, 1, 1, 5, 25, 33, 119, 247}},
+    // 50
+    {8, 115, {1, 1, 1, 9, 25, 49, 55, 185}},
+    {8, 122, {1, 3, 5, 7, 23, 53, 85, 117}},
+    {9, 8, {1, 3, 3, 13, 11, 57, 121, 41, 235}},
+    {9, 13, {1, 1, 3, 3, 19, 57, 119, 81, 307}},
+    {9, 16, {1, 3, 3, 7, 3, 39, 11, 223, 495}},
+    {9, 22, {1, 3, 3, 5, 11, 21, 23, 151, 417}},
+    {9, 25, {1, 3, 1, 11, 31, 7, 61, 81, 57}},
+    {9, 44, {1, 1, 3, 9, 7, 53, 11, 189, 151}},
+    {9, 47, {1, 3, 7, 1, 9, 9, 35, 61, 19}},
+    {9, 52, {1, 1, 5, 9, 5, 55, 33, 95, 119}},
+    // 60
+    {9, 55, {1, 3, 7, 1, 17, 15, 43, 185, 375}},
+    {9, 59, {1, 1, 3, 5, 23, 59, 107, 23, 451}},
+    {9, 62, {1, 1, 7, 7, 17, 19, 113, 73, 55}},
+    {9, 67, {1, 3, 1, 13, 17, 49, 101, 113, 449}},
+    {9, 74, {1, 3, 3, 9, 25, 31, 29, 239, 501}},
+    {9, 81, {1, 1, 3, 9, 13, 3, 87, 85, 53}},
+    {9, 82, {1, 1, 5, 1, 11, 39, 119, 9, 185}},
+    {9, 87, {1, 1, 1, 7, 31, 5, 97, 201, 317}},
+    {9, 91, {1, 1, 3, 3, 27, 5, 29, 83, 17}},
+    {9, 94, {1, 3, 5, 5, 19, 41, 17, 53, 21}},
+    // 70
+    {9, 103, {1, 1, 5, 1, 17, 9, 89, 183, 487}},
+    {9, 104, {1, 1, 7, 11, 23, 19, 5, 203, 13}},
+    {9, 109, {1, 3, 7, 11, 7, 9, 127, 91, 347}},
+    {9, 122, {1, 1, 7, 13, 5, 57, 89, 149, 393}},
+    {9, 124, {1, 1, 1, 7, 11, 25, 119, 101, 15}},
+    {9, 137, {1, 1, 1, 7, 19, 1, 117, 13, 391}},
+    {9, 138, {1, 3, 3, 9, 19, 15, 103, 111, 307}},
+    {9, 143, {1, 3, 3, 9, 7, 51, 105, 239, 189}},
+    {9, 145, {1, 1, 1, 1, 13, 11, 41, 3, 381}},
+    {9, 152, {1, 3, 1, 1, 21, 19, 83, 205, 71}},
+    // 80
+    {9, 157, {1, 3, 5, 3, 21, 61, 25, 253, 163}},
+    {9, 167, {1, 1, 1, 9, 7, 53, 41, 247, 99}},
+    {9, 173, {1, 3, 5, 15, 9, 29, 55, 121, 467}},
+    {9, 176, {1, 3, 7, 1, 11, 19, 69, 189, 167}},
+    {9, 181, {1, 3, 5, 5, 1, 11, 117, 169, 433}},
+    {9, 182, {1, 1, 1, 13, 5, 9, 49, 179, 337}},
+    {9, 185, {1, 3, 7, 1, 21, 21, 127, 197, 257}},
+    {9, 191, {1, 3, 5, 9, 11, 19, 29, 175, 179}},
+    {9, 194, {1, 3, 3, 9, 13, 43, 1, 217, 47}},
+    {9, 199, {1, 1, 3, 9, 25, 13, 99, 249, 385}},
+    // 90
+    {9, 218, {1, 3, 1, 9, 9, 13, 53, 195, 23}},
+    {9, 220, {1, 3, 5, 9, 7, 41, 83, 95, 117}},
+    {9, 227, {1, 1, 7, 13, 7, 25, 15, 63, 369}},
+    {9, 229, {1, 3, 1, 11, 27, 31, 31, 19, 425}},
+    {9, 230, {1, 3, 7, 3, 15, 9, 73, 7, 207}},
+    {9, 234, {1, 3, 5, 5, 25, 11, 115, 5, 433}},
+    {9, 236, {1, 1, 1, 11, 15, 19, 35, 75, 301}},
+    {9, 241, {1, 3, 7, 11, 21, 5, 21, 217, 147}},
+    {9, 244, {1, 1, 3, 13, 17, 53, 89, 245, 333}},
+    {9, 253, {1, 3, 1, 5, 19, 37, 5, 111, 85}},
+};
+
+// Direction numbers of one generator as 30-bit fractions, V[b] = v_b << (29 - b),
+// from v_i = 2 a_1 v_{i-1} ^ 4 a_2 v_{i-2} ^ ... ^ 2^s v_{i-s} ^ v_{i-s}
+// (SobolGenerator.cxx:198-244 evaluates the same recurrence recursively).
+void sobol_directions(int gen, uint32_t V[kSobolBits]) {
+  const SobolRow& r = kSobolTable[gen];
+  const int s = r.degree;
+  uint32_t v[kSobolBits];
+  for (int i = 0; i < kSobolBits; i++) {
+    if (i < s) {
+      v[i] = r.m[i];
+    } else {
+      uint32_t x = v[i - s] ^ (v[i - s] << s);
+      for (int j = 1; j < s; j++)
+        if ((r.coefficient >> (s - 1 - j)) & 1u) x ^= v[i - j] << j;
+      v[i] = x;
+    }
+    V[i] = v[i] << (kSobolBits - 1 - i);
+  }
+}
+
+// The n-th number of the sequence (n >= 1; SobolGenerator.cxx:271-294 with
+// n = index + 1): XOR of V[b] over the set bits of the Gray code of n.
+uint32_t sobol_at(const uint32_t V[kSobolBits], uint64_t n) {
+  const uint64_t g = n ^ (n >> 1);
+  uint32_t x = 0;
+  for (int b = 0; b < kSobolBits; b++)
+    if ((g >> b) & 1u) x ^= V[b];
+  return x;
+}
+
+// Numbers n0+1 .. n0+count of one generator written with stride `stride` as
+// (hi - lo) * u + lo.  Successive Gray codes differ in the lowest zero bit of
+// n, so x_{n+1} = x_n ^ V[ctz(~n)] (SobolGenerator.cxx:246-269).
+void sobol_fill(const uint32_t V[kSobolBits], uint64_t n0, int64_t count, double lo, double hi, double* out,
+                int64_t stride) {
+#pragma clang fp contract(off)
+  uint32_t x = sobol_at(V, n0);
+  const double span = hi - lo;
+  for (int64_t k = 0; k < count; k++) {
+    const uint64_t n = n0 + (uint64_t)k;
+    x ^= V[__builtin_ctzll(~n)];
+    const double u = (double)x / (double)(1u << kSobolBits);
+    const double t = span * u;
+    out[k * stride] = t + lo;
+  }
+}
+
+}  // namespace
+
+extern "C" int32_t gs_sobol_sequence(int32_t generator, int64_t first, int64_t count, double* out) {
+  if (generator < 0 || generator >= kSobolGenerators || first < 0 || count < 0 || !out ||
+      first + count >= (1LL << kSobolBits))
+    return -1;
+  uint32_t V[kSobolBits];
+  sobol_directions(generator, V);
+  sobol_fill(V, (uint64_t)first, count, 0.0, 1.0, out, 1);
+  return 0;
+}
+
+extern "C" int64_t gs_layout_spacefiller(int32_t dim, const int32_t* res, const double* lo, const double* hi,
+                                         int64_t total, double avg_per_cell, int32_t first_generator,
+                                         double* coord) {
+  if ((dim != 2 && dim != 3) || !res || !lo || !hi) return -1;
+  if ((total > 0) == (avg_per_cell > 0.0) || total < 0 || avg_per_cell < 0.0) return -1;
+  if (first_generator < 0 || first_generator + dim > kSobolGenerators) return -1;
+  const long long E = (long long)res[0] * res[1] * (dim == 3 ? res[2] : 1);
+  // GlobalParticleLayout.cxx:_SetInitialCounts stores cells * average in an unsigned int
+  const double want = total > 0 ? (double)total : (double)E * avg_per_cell;
+  if (want >= (double)((1LL << kSobolBits) - 1)) return -1;
+  const int64_t P = total > 0 ? total : (int64_t)(uint32_t)want;
+  if (!coord) return P;
+  uint32_t V[3][kSobolBits];
+  for (int d = 0; d < dim; d++) sobol_directions(first_generator + d, V[d]);
+  const int64_t kChunk = 1 << 20;
+  const int64_t chunks = (P + kChunk - 1) / kChunk;
+  const int nthreads = (int)std::min<int64_t>(chunks, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  auto work = [&](int t) {
+    for (int64_t c = t; c < chunks; c += nthreads) {
+      const int64_t p0 = c * kChunk, n = std::min(kChunk, P - p0);
+      for (int d = 0; d < dim; d++) sobol_fill(V[d], (uint64_t)p0, n, lo[d], hi[d], coord + 3 * p0 + d, 3);
+      if (dim == 2)
+        for (int64_t k = 0; k < n; k++) coord[3 * (p0 + k) + 2] = 0.0;
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthreads; t++) pool.emplace_back(work, t);
+  work(0);
+  for (auto& th : pool) th.join();
+  return P;
+}
+
 extern "C" int32_t gs_wall_bc(int32_t dim, const int32_t* res, int32_t wall, int32_t comp, double value,
                               uint8_t* mask, double* values) {
   if ((dim != 2 && dim != 3) || comp < 0 || comp >= dim || wall < 0 || wall > 5 || !mask) return 6;

@@ -28,6 +28,25 @@ int64_t gs_layout_random(int32_t dim, const int32_t* res, const double* lo, cons
  * placed as global coordinates; with ConstantWeights they carry weight 1. */
 int64_t gs_layout_gauss(int32_t dim, const int32_t* res, const double* lo, const double* hi, double* coord);
 
+/* Sobol sequence of tabulated generator `generator` (0..99, the rows of
+ * StgDomain/Utils/src/SobolGenerator.cxx:307-420): out[k] is the number the
+ * reference's SobolGenerator_GetNumberByIndex(first + k) returns, i.e. the
+ * (first+k+1)-th SobolGenerator_GetNextNumber (SobolGenerator.cxx:246-294).
+ * Needs first + count < 2^30.  Returns 0, or -1 on bad arguments. */
+int32_t gs_sobol_sequence(int32_t generator, int64_t first, int64_t count, double* out);
+
+/* SpaceFillerParticleLayout (StgDomain/Swarm/src/SpaceFillerParticleLayout.cxx:88-113,
+ * :211-240) through GlobalParticleLayout (GlobalParticleLayout.cxx
+ * _SetInitialCounts / _InitialiseParticles): exactly one of total > 0 or
+ * avg_per_cell > 0 (then P = (unsigned)(cells * avg_per_cell)); particle p's
+ * coordinate d is (hi[d] - lo[d]) * u + lo[d], u the (p+1)-th number of
+ * generator first_generator + d (the reference creates its x, y[, z]
+ * generators from the table in construction order: 0, 1, 2 for the first
+ * layout of a run).  Coordinates P*3 in generation order, the third 0 in 2D;
+ * coord NULL returns P only.  Returns P, or -1 on bad arguments. */
+int64_t gs_layout_spacefiller(int32_t dim, const int32_t* res, const double* lo, const double* hi, int64_t total,
+                              double avg_per_cell, int32_t first_generator, double* coord);
+
 /* WallVC (StgDomain/Utils/src/WallVC.cxx): wall 0 left, 1 right, 2 bottom,
  * 3 top, 4 back, 5 front; marks component comp of every node on the wall in
  * mask (N*dim) and writes value into values (N*dim, may be NULL). */

@@ -89,6 +89,8 @@ def lib():
             "orc_isotropic_stress": (None, [i32, d, vp, vp]),
             "orc_search_element": (i32, [i32, vp, vp, vp, vp, vp]),
             "orc_layout_random": (i64, [i32, vp, vp, vp, i32, C.c_long, vp]),
+            "orc_sobol_sequence": (i32, [C.c_uint, C.c_uint, vp, i64, vp, vp]),
+            "orc_layout_spacefiller": (i64, [i32, vp, vp, vp, i64, d, vp, vp, vp, vp]),
             "orc_problem_new": (vp, [vp]),
             "orc_problem_free": (None, [vp]),
             "orc_num_nodes": (i64, [vp]),
@@ -183,6 +185,39 @@ def layout_random(dim, res, lo, hi, ppc, seed):
     E = int(np.prod(r[:dim]))
     coord = np.zeros((E * ppc, 3), dtype=np.float64)
     lib().orc_layout_random(dim, _p(r), _p(l), _p(h), ppc, seed, _p(coord))
+    return coord
+
+
+def sobol_sequence(degree, coefficient, m, count):
+    """count numbers of one Sobol generator (SobolGenerator.cxx:198-294):
+    (by GetNextNumber, by GetNumberByIndex(k))."""
+    mm = np.zeros(32, dtype=np.uint32)
+    mm[:len(m)] = m
+    nxt = np.zeros(count, dtype=np.float64)
+    idx = np.zeros(count, dtype=np.float64)
+    if lib().orc_sobol_sequence(degree, coefficient, _p(mm), count, _p(nxt), _p(idx)) != 0:
+        raise ValueError("sobol_sequence")
+    return nxt, idx
+
+
+def layout_spacefiller(dim, res, lo, hi, generators, total=0, avg_per_cell=0.0):
+    """SpaceFillerParticleLayout.cxx:211-240: particles at the Sobol points of
+    `generators` (one (degree, coefficient, m) per axis) scaled to [lo, hi]."""
+    r = np.asarray(list(res) + [1] * (3 - len(res)), dtype=np.int32)
+    l = np.asarray(list(lo) + [0.0] * (3 - len(lo)), dtype=np.float64)
+    h = np.asarray(list(hi) + [1.0] * (3 - len(hi)), dtype=np.float64)
+    deg = np.zeros(3, dtype=np.uint32)
+    coef = np.zeros(3, dtype=np.uint32)
+    m = np.zeros((3, 9), dtype=np.uint32)
+    for d, (dg, cf, mv) in enumerate(generators[:dim]):
+        deg[d], coef[d] = dg, cf
+        m[d, :len(mv)] = mv
+    E = int(np.prod(r[:dim]))
+    P = int(total) if total > 0 else int(np.uint32(E * avg_per_cell))
+    coord = np.zeros((P, 3), dtype=np.float64)
+    n = lib().orc_layout_spacefiller(dim, _p(r), _p(l), _p(h), int(total), float(avg_per_cell), _p(deg), _p(coef),
+                                     _p(m), _p(coord))
+    assert n == P
     return coord
 
 

@@ -26,6 +26,11 @@
  *   - DVC weights: the reference's DVCWeightsSuite fixtures (centroids and
  *     weights of 10 hard-coded particles, 2D and 3D) and its PCDVC element
  *     integration statistics (testPCDVC_*.expected).
+ *   - Sobol sequence: the reference's testSobolGeneratorOutput.*.expected
+ *     fixtures for all 100 tabulated generators, its own GetNextNumber /
+ *     GetNumberByIndex consistency check, and the (0,1)-sequence property
+ *     (every dyadic interval of length 2^-k holds one point of each run of
+ *     2^k); SpaceFiller layout: testSpaceFillerParticleLayoutOutput.expected.
  * Functions without a check of their own are listed as "parity unpinned" in
  * DESIGN.md.
  */
@@ -130,6 +135,96 @@ int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, con
     }
   }
   return p;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sobol sequence and the SpaceFiller layout                                 */
+/* ------------------------------------------------------------------------ */
+/* SobolGenerator.cxx:87-112: the first `degree` direction numbers are given;
+ * the running number starts at 0 (the reference assigns 0.5 to an unsigned). */
+void orc_sobol_init(orc_sobol* s, unsigned degree, unsigned coefficient, const unsigned* m) {
+  memset(s, 0, sizeof(*s));
+  s->degree = degree;
+  s->coefficient = coefficient;
+  for (unsigned i = 0; i < degree; i++) {
+    s->v[i] = m[i];
+    s->have[i] = 1;
+  }
+}
+
+/* SobolGenerator.cxx:198-244, the direction-number recurrence
+ * v_i = 2 a_1 v_{i-1} ^ 4 a_2 v_{i-2} ^ ... ^ 2^s v_{i-s} ^ v_{i-s},
+ * a_k the bit degree-1-k of the coefficient, evaluated recursively. */
+unsigned orc_sobol_direction(orc_sobol* s, unsigned i) {
+  if (s->have[i]) return s->v[i];
+  unsigned r = 0, last = 0;
+  for (unsigned k = 0; k < s->degree; k++) {
+    const int on = k == s->degree - 1 || ((s->coefficient >> (s->degree - 2 - k)) & 1u);
+    if (on) {
+      last = orc_sobol_direction(s, i - 1 - k);
+      r ^= (2u << k) * last;
+    }
+  }
+  r ^= last;
+  s->have[i] = 1;
+  return s->v[i] = r;
+}
+
+/* SobolGenerator.cxx:184-196 and :246-269: the Gray codes of n and n+1 differ
+ * in the lowest zero bit c of n, so x_{n+1} = x_n ^ (v_c << (29 - c)). */
+double orc_sobol_next(orc_sobol* s) {
+  unsigned c = 0, n = s->count;
+  while (n & 1u) {
+    n >>= 1;
+    c++;
+  }
+  s->count++;
+  s->x ^= orc_sobol_direction(s, c) << (30 - c - 1);
+  return (double)s->x / (double)(1L << 30);
+}
+
+/* SobolGenerator.cxx:271-294: x(index) = XOR over the set bits b of the Gray
+ * code of index + 1 of v_b << (29 - b). */
+double orc_sobol_by_index(orc_sobol* s, unsigned index) {
+  const unsigned n = index + 1, g = n ^ (n >> 1);
+  unsigned r = 0;
+  for (unsigned b = 0; b < 30; b++)
+    if (g & (1u << b)) r ^= orc_sobol_direction(s, b) << (30 - b - 1);
+  return (double)r / (double)(1L << 30);
+}
+
+int32_t orc_sobol_sequence(unsigned degree, unsigned coefficient, const unsigned* m, int64_t count, double* next,
+                           double* by_index) {
+  if (degree < 1 || degree > 30 || count < 0 || count >= (1L << 30)) return -1;
+  orc_sobol s;
+  orc_sobol_init(&s, degree, coefficient, m);
+  for (int64_t k = 0; k < count; k++) {
+    const double a = orc_sobol_next(&s);
+    if (next) next[k] = a;
+    if (by_index) by_index[k] = orc_sobol_by_index(&s, (unsigned)k);
+  }
+  return 0;
+}
+
+/* SpaceFillerParticleLayout.cxx:211-240 through GlobalParticleLayout.cxx
+ * (_SetInitialCounts: total = cells * average, truncated to an unsigned;
+ * _InitialiseParticles: particles in generation order): coordinate d of
+ * particle p is (max - min) * next(generator d) + min over the mesh's box. */
+int64_t orc_layout_spacefiller(int32_t dim, const int32_t* res, const double* lo, const double* hi, int64_t total,
+                               double avg_per_cell, const unsigned* deg, const unsigned* coef, const unsigned* m,
+                               double* coord) {
+  const int64_t E = (int64_t)res[0] * res[1] * (dim == 3 ? res[2] : 1);
+  const int64_t P = total > 0 ? total : (int64_t)(unsigned)(E * avg_per_cell);
+  orc_sobol s[3];
+  for (int d = 0; d < dim; d++) orc_sobol_init(&s[d], deg[d], coef[d], m + 9 * d);
+  for (int64_t p = 0; p < P; p++) {
+    coord[3 * p + 2] = 0.0;
+    for (int d = 0; d < dim; d++) {
+      const double u = orc_sobol_next(&s[d]);
+      coord[3 * p + d] = (hi[d] - lo[d]) * u + lo[d];
+    }
+  }
+  return P;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -121,6 +121,32 @@ int32_t orc_search_element(int32_t dim, const int32_t* res, const double* lo, co
 int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi,
                           int32_t ppc, long seed, double* coord);
 
+/* --- Sobol sequence (StgDomain/Utils/src/SobolGenerator.cxx) --- */
+/* One generator: primitive polynomial of degree `degree` whose inner
+ * coefficients are the bits of `coefficient`, initial direction numbers m[0..degree). */
+typedef struct orc_sobol {
+  unsigned degree, coefficient;
+  unsigned v[32];
+  int have[32];
+  unsigned count;   /* numbers drawn so far */
+  unsigned x;       /* last number, as a 30-bit integer */
+} orc_sobol;
+void orc_sobol_init(orc_sobol* s, unsigned degree, unsigned coefficient, const unsigned* m);
+unsigned orc_sobol_direction(orc_sobol* s, unsigned i);      /* GetDirectionalNumber */
+double orc_sobol_next(orc_sobol* s);                         /* GetNextNumber */
+double orc_sobol_by_index(orc_sobol* s, unsigned index);     /* GetNumberByIndex */
+/* count numbers of one generator: next[k] by GetNextNumber, by_index[k] by
+ * GetNumberByIndex(k) (either may be NULL). */
+int32_t orc_sobol_sequence(unsigned degree, unsigned coefficient, const unsigned* m, int64_t count, double* next,
+                           double* by_index);
+
+/* SpaceFillerParticleLayout: total particles (or E * avg_per_cell when total
+ * is 0) at the Sobol points of generators (deg[d], coef[d], m + 9 d) scaled to
+ * the box [lo, hi]; P*3 coordinates, the unused third one 0 in 2D.  Returns P. */
+int64_t orc_layout_spacefiller(int32_t dim, const int32_t* res, const double* lo, const double* hi, int64_t total,
+                               double avg_per_cell, const unsigned* deg, const unsigned* coef, const unsigned* m,
+                               double* coord);
+
 /* --- output fields of the material points (StoreStress / StoreViscosity) --- */
 int32_t orc_particle_fields(orc_problem* pb, const double* u_full, double* sr, double* inv, double* eta,
                             double* stress);

@@ -11,6 +11,12 @@ available; the outputs are committed and the tests never read the reference).
                               PICellerator/Weights/tests/expected/DVCWeightsSuite_testGetCentroids{,2D}.txt
   pcdvc_expected.json         PICellerator/Utils/tests/expected/testPCDVC_*.expected (tolerance and value of
                               the mean and standard deviation of the relative integration error)
+  sobol.json                  the 100 (degree, coefficient, initial direction numbers) rows of the published
+                              Joe-Kuo table as tabulated in StgDomain/Utils/src/SobolGenerator.cxx, and the
+                              sampled outputs (index % 773 == 3 of 200000, printed %.4g) of
+                              StgDomain/Utils/tests/expected/testSobolGeneratorOutput.<degree>-<coeff>.expected
+  spacefiller.json            StgDomain/Swarm/tests/expected/testSpaceFillerParticleLayoutOutput.expected
+                              (cell, in-cell index, global index, coordinates as printed %7.5g)
 
 The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
 1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
@@ -88,6 +94,47 @@ def dvc_fixtures():
         pc[name] = dict(mean_tol=mt, mean=mv, std_tol=st, std=sv)
     json.dump(pc, open(os.path.join(OUT, "pcdvc_expected.json"), "w"), indent=1)
     print("pcdvc_expected.json", pc)
+    sobol_fixtures()
+
+
+def c_array(src, name):
+    body = re.search(name + r"\[\]\s*(?:\[\d+\])?\s*=\s*\{(.*?)\};", src, re.S).group(1)
+    return re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+
+
+def sobol_fixtures():
+    src = open(os.path.join(REF, "StgDomain/Utils/src/SobolGenerator.cxx")).read()
+    deg = [int(t) for t in re.findall(r"\d+", c_array(src, "SobolGenerator_PolynomialDegree"))]
+    coef = [int(t) for t in re.findall(r"\d+", c_array(src, "SobolGenerator_PolynomialCoefficient"))]
+    m = [[int(t) for t in re.findall(r"\d+", row)]
+         for row in re.findall(r"\{([^{}]*)\}", c_array(src, "SobolGenerator_InitialDirectionNumbers"))]
+    assert len(deg) == len(coef) == len(m) == 100 and all(len(m[i]) == deg[i] for i in range(100))
+    gens = []
+    for i in range(100):
+        fn = "StgDomain/Utils/tests/expected/testSobolGeneratorOutput.%03d-%03d.expected" % (deg[i], coef[i])
+        lines = [l.strip() for l in open(os.path.join(REF, fn)).read().split("\n")[1:] if l.strip()]
+        assert len(lines) == len(range(3, 200000, 773)), (fn, len(lines))
+        gens.append(dict(degree=deg[i], coefficient=coef[i], m=m[i], sampled=lines))
+    json.dump(dict(stride=773, offset=3, count=200000, generators=gens),
+              open(os.path.join(OUT, "sobol.json"), "w"), separators=(",", ":"))
+    print("sobol.json", len(gens))
+    txt = open(os.path.join(REF, "StgDomain/Swarm/tests/expected/testSpaceFillerParticleLayoutOutput.expected")).read()
+    parts = []
+    cell = -1
+    for line in txt.split("\n"):
+        mc = re.match(r"\s*Local Cell\s+(\d+):", line)
+        if mc:
+            cell = int(mc.group(1))
+            continue
+        mp = re.match(r"\s*part\. InCell\s+(\d+) \(local index\s+(\d+)\): \((.*)\)", line)
+        if mp:
+            parts.append(dict(cell=cell, in_cell=int(mp.group(1)), index=int(mp.group(2)),
+                              coord=[t.strip() for t in mp.group(3).split(",")]))
+    assert len(parts) == 160
+    json.dump(dict(dim=3, res=[4, 2, 1], lo=[0.0, 0.0, 0.0], hi=[400.0, 200.0, 100.0], avg_per_cell=20,
+                   format="%7.5g", particles=parts),
+              open(os.path.join(OUT, "spacefiller.json"), "w"), indent=0)
+    print("spacefiller.json", len(parts))
 
 
 if __name__ == "__main__":

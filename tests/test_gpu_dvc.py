@@ -152,3 +152,41 @@ def test_population_control_matches_oracle_and_solves(gale, oracle, dim, res):
     assert rel(u, ou) < 1e-8 and rel(p - p.mean(), op - op.mean()) < 1e-8
     s.close()
     g.close()
+
+
+@pytest.mark.parametrize("dim,res,avg", [(2, 12, 20), (3, 5, 12)])
+def test_spacefiller_pcdvc_problem_parity(gale, oracle, dim, res, avg):
+    """The reference's PIC input pattern: a SpaceFiller (Sobol) layout over
+    the whole box, DVC weights and a population-control pass.  The layout
+    equals the oracle's bit for bit, as do the particles after population
+    control, and the Stokes solve matches the oracle's at tight tolerance."""
+    from gale_amd import configs
+    from test_gpu_parity import otight, small_sinker, tight
+    from test_sobol import GENS
+    cfg = small_sinker(dim, res, ppc=4)
+    kw, okw = cfg.problem_kwargs(), cfg.oracle_kwargs()
+    coord = gale.layout_spacefiller(dim, cfg.res, cfg.lo, cfg.hi, avg_per_cell=avg)
+    assert np.array_equal(coord, oracle.layout_spacefiller(dim, cfg.res, cfg.lo, cfg.hi, GENS[:dim],
+                                                           avg_per_cell=avg))
+    shapes = [configs.everywhere(0), configs.sphere(1, (0.5, 0.6, 0.5), 0.2)]
+    mat = configs.shape_layout(dim, coord, shapes)
+    kw.update(coord=coord, material_index=mat)
+    okw.update(coord=coord, material=mat)
+    kw.pop("eta0_particle", None)
+    okw.pop("eta0_particle", None)
+    g = gale.Problem(**kw)
+    o = oracle.Problem(**okw)
+    g.set_weights(gale.WEIGHTS_DVC, 10)
+    o.set_weights(1, (10, 10, 10))
+    ga = g.population_control()
+    oa = o.population_control()
+    print(f"population control: added {ga[0]} deleted {ga[1]}", flush=True)
+    assert ga == oa
+    _compare_particles(g, o)
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale))
+    u, p, rep = s.Solve()
+    st, ou, op, _ = o.uzawa(otight(oracle))
+    assert st == 0 and rep.status == 0
+    assert rel(u, ou) < 1e-8 and rel(p - p.mean(), op - op.mean()) < 1e-8
+    s.close()
+    g.close()
