@@ -37,7 +37,9 @@ EXPORTS = (
     "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
-    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_wall_bc", "gs_shape_layout", "gs_gaussian_field",
+    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_wall_bc",
+    "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
+    "gs_h5_read_swarm", "gs_shape_layout", "gs_gaussian_field",
     "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
     "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
     "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
@@ -163,6 +165,12 @@ def lib():
             "gs_layout_gauss": (i64, [i32, vp, vp, vp, vp]),
             "gs_sobol_sequence": (i32, [i32, i64, i64, vp]),
             "gs_layout_spacefiller": (i64, [i32, vp, vp, vp, i64, d, i32, vp]),
+            "gs_h5_dataset_info": (i32, [C.c_char_p, C.c_char_p, vp, vp, vp]),
+            "gs_h5_read_f64": (i64, [C.c_char_p, C.c_char_p, vp, i64]),
+            "gs_h5_read_i32": (i64, [C.c_char_p, C.c_char_p, vp, i64]),
+            "gs_h5_attribute_i64": (i32, [C.c_char_p, C.c_char_p, vp, i32]),
+            "gs_h5_read_field": (i64, [C.c_char_p, i32, i32, i64, vp]),
+            "gs_h5_read_swarm": (i64, [C.c_char_p, C.c_char_p, i32, i64, vp, vp]),
             "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
             "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
             "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
@@ -280,6 +288,69 @@ def layout_spacefiller(dim, res, lo, hi, total=0, avg_per_cell=0.0, first_genera
     lib().gs_layout_spacefiller(dim, _ptr(r), _ptr(l), _ptr(h), int(total), float(avg_per_cell), first_generator,
                                 _ptr(coord))
     return coord
+
+
+# ---------------------------------------------------------------------------
+# HDF5 checkpoints (gale_checkpoint.h)
+# ---------------------------------------------------------------------------
+H5_I32, H5_I64, H5_F32, H5_F64 = 1, 2, 3, 4
+_H5_ERRORS = {-1: "cannot read file", -2: "not an HDF5 file or corrupt structure",
+              -3: "outside the supported HDF5 subset", -4: "not found", -5: "shape mismatch",
+              -6: "invalid argument"}
+
+
+def _h5(st, what):
+    if st < 0:
+        raise GaleError(E_INVALID, f"{what}: {_H5_ERRORS.get(int(st), st)}")
+    return st
+
+
+def h5_dataset_info(path, name):
+    """(shape, type) of root-group dataset `name` (type one of H5_*)."""
+    rank = np.zeros(1, np.int32)
+    dims = np.zeros(4, np.int64)
+    typ = np.zeros(1, np.int32)
+    _h5(lib().gs_h5_dataset_info(os.fsencode(path), name.encode(), _ptr(rank), _ptr(dims), _ptr(typ)), name)
+    return tuple(int(v) for v in dims[:rank[0]]), int(typ[0])
+
+
+def h5_read(path, name) -> np.ndarray:
+    """Root-group dataset `name` as float64 (floating types) or int32 (integer types)."""
+    shape, typ = h5_dataset_info(path, name)
+    n = int(np.prod(shape)) if shape else 1
+    if typ in (H5_I32, H5_I64):
+        out = np.zeros(n, np.int32)
+        _h5(lib().gs_h5_read_i32(os.fsencode(path), name.encode(), _ptr(out), n), name)
+    else:
+        out = np.zeros(n, np.float64)
+        _h5(lib().gs_h5_read_f64(os.fsencode(path), name.encode(), _ptr(out), n), name)
+    return out.reshape(shape)
+
+
+def h5_attribute(path, name, capacity=16):
+    """Integer attribute `name` of the root group, or None when absent."""
+    out = np.zeros(capacity, np.int64)
+    n = lib().gs_h5_attribute_i64(os.fsencode(path), name.encode(), _ptr(out), capacity)
+    if n == -4:
+        return None
+    _h5(n, name)
+    return out[:min(n, capacity)].copy()
+
+
+def read_field(path, dim, dofs, n_nodes) -> np.ndarray:
+    """FeVariable_ReadFromFile: (n_nodes, dofs) nodal values of a field checkpoint."""
+    out = np.zeros((n_nodes, dofs))
+    _h5(lib().gs_h5_read_field(os.fsencode(path), dim, dofs, n_nodes, _ptr(out)), "read_field")
+    return out
+
+
+def read_swarm(path, swarm, dim):
+    """FileParticleLayout on a SwarmDump file: (coord (P, 3), material index (P,))."""
+    P = _h5(lib().gs_h5_read_swarm(os.fsencode(path), swarm.encode(), dim, 0, None, None), "read_swarm")
+    coord = np.zeros((P, 3))
+    mat = np.zeros(P, np.int32)
+    _h5(lib().gs_h5_read_swarm(os.fsencode(path), swarm.encode(), dim, P, _ptr(coord), _ptr(mat)), "read_swarm")
+    return coord, mat
 
 
 def num_nodes(dim, res) -> int:

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libgale_stokes.so")
 BUILD_DIR = os.path.join(HERE, "_build")
 
-SOURCES = ["gs_kernels.hip", "gs_tile.hip", "gs_mg.hip", "gs_net.hip", "gs_learned.hip", "gs_batch.hip", "gs_weights.hip", "gs_api.hip", "gs_generate.cpp"]
+SOURCES = ["gs_kernels.hip", "gs_tile.hip", "gs_mg.hip", "gs_net.hip", "gs_learned.hip", "gs_batch.hip", "gs_weights.hip", "gs_api.hip", "gs_generate.cpp", "gs_checkpoint.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,450 @@
+// gs_checkpoint.cpp -- reads the reference's HDF5 checkpoints without an HDF5
+// library: the subset of the file format that its FeVariable_SaveToFile,
+// SwarmDump and timeInfo writers produce (superblock version 0 or 1, version-1
+// object headers, symbol-table groups, contiguous or compact little- or
+// big-endian integer and floating-point datasets, root-group attributes).
+// Anything outside that subset (chunked or filtered storage, version-2 object
+// headers, other datatypes) is refused with GS_H5_UNSUPPORTED, never guessed.
+// Every offset read from the file is bounds-checked against its size.
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gale_checkpoint.h"
+
+namespace {
+
+struct File {
+  std::vector<uint8_t> b;
+  uint64_t root = 0;  // object header address of the root group
+
+  bool in(uint64_t off, uint64_t len) const { return off <= b.size() && len <= b.size() - off; }
+  uint64_t u(uint64_t off, int n) const {  // little-endian unsigned of n bytes (caller checked bounds)
+    uint64_t v = 0;
+    for (int i = n - 1; i >= 0; i--) v = (v << 8) | b[off + i];
+    return v;
+  }
+};
+
+struct Msg {
+  uint16_t type;
+  uint64_t off, size;
+};
+
+struct Dataset {
+  int rank = -1;
+  int64_t dims[GS_H5_MAX_RANK] = {0, 0, 0, 0};
+  int type = 0;          // GS_H5_*
+  int elem = 0;          // bytes per element
+  bool big_endian = false;
+  uint64_t addr = 0;     // first byte of the raw data
+  uint64_t bytes = 0;
+  bool have_layout = false;
+};
+
+int load(const char* path, File& f) {
+  if (!path) return GS_H5_INVALID;
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return GS_H5_IO;
+  std::fseek(fp, 0, SEEK_END);
+  const long n = std::ftell(fp);
+  std::fseek(fp, 0, SEEK_SET);
+  if (n < 0) {
+    std::fclose(fp);
+    return GS_H5_IO;
+  }
+  f.b.resize((size_t)n);
+  const size_t got = n ? std::fread(f.b.data(), 1, (size_t)n, fp) : 0;
+  std::fclose(fp);
+  if (got != (size_t)n) return GS_H5_IO;
+  static const uint8_t sig[8] = {0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n'};
+  if (!f.in(0, 100) || std::memcmp(f.b.data(), sig, 8) != 0) return GS_H5_FORMAT;
+  const int sb = f.b[8];
+  if (sb != 0 && sb != 1) return GS_H5_UNSUPPORTED;
+  if (f.b[13] != 8 || f.b[14] != 8) return GS_H5_UNSUPPORTED;  // offsets and lengths of 8 bytes
+  // v0: 16 bytes of versions/sizes, 4 of group K's, 4 of flags, 4 addresses; v1 adds 4 bytes
+  const uint64_t entry = (sb == 0 ? 24 : 28) + 32;
+  if (!f.in(entry, 40)) return GS_H5_FORMAT;
+  f.root = f.u(entry + 8, 8);
+  return 0;
+}
+
+// Messages of a version-1 object header, following continuation blocks.
+int messages(const File& f, uint64_t addr, std::vector<Msg>& out) {
+  if (!f.in(addr, 16)) return GS_H5_FORMAT;
+  if (f.b[addr] != 1) return GS_H5_UNSUPPORTED;
+  std::vector<std::pair<uint64_t, uint64_t>> blocks{{addr + 16, f.u(addr + 8, 4)}};
+  for (size_t k = 0; k < blocks.size(); k++) {
+    if (blocks.size() > 1024) return GS_H5_FORMAT;
+    uint64_t o = blocks[k].first;
+    const uint64_t end = o + blocks[k].second;
+    if (!f.in(blocks[k].first, blocks[k].second)) return GS_H5_FORMAT;
+    while (o + 8 <= end) {
+      const uint16_t type = (uint16_t)f.u(o, 2);
+      const uint64_t size = f.u(o + 2, 2);
+      const uint64_t data = o + 8;
+      if (data + size > end) return GS_H5_FORMAT;
+      if (type == 0x10) {  // continuation: address and length of the next block
+        if (size < 16) return GS_H5_FORMAT;
+        blocks.push_back({f.u(data, 8), f.u(data + 8, 8)});
+      } else if (type != 0) {
+        out.push_back({type, data, size});
+      }
+      o = data + size;
+    }
+  }
+  return 0;
+}
+
+// Object header address of `name` in the root group (symbol table, B-tree v1
+// of group nodes, symbol-table nodes, names in the local heap).
+int lookup(const File& f, const char* name, uint64_t& addr) {
+  if (!name) return GS_H5_INVALID;
+  while (*name == '/') name++;
+  std::vector<Msg> ms;
+  int st = messages(f, f.root, ms);
+  if (st) return st;
+  const Msg* stab = nullptr;
+  for (const Msg& m : ms)
+    if (m.type == 0x11 && m.size >= 16) stab = &m;
+  if (!stab) return GS_H5_UNSUPPORTED;
+  const uint64_t tree = f.u(stab->off, 8), heap = f.u(stab->off + 8, 8);
+  if (!f.in(heap, 32) || std::memcmp(&f.b[heap], "HEAP", 4) != 0) return GS_H5_FORMAT;
+  const uint64_t seg_size = f.u(heap + 8, 8), seg = f.u(heap + 24, 8);
+  if (!f.in(seg, seg_size)) return GS_H5_FORMAT;
+  std::vector<uint64_t> stack{tree};
+  int visited = 0;
+  while (!stack.empty()) {
+    const uint64_t node = stack.back();
+    stack.pop_back();
+    if (++visited > 100000) return GS_H5_FORMAT;
+    if (!f.in(node, 8)) return GS_H5_FORMAT;
+    if (std::memcmp(&f.b[node], "TREE", 4) == 0) {
+      if (!f.in(node, 24)) return GS_H5_FORMAT;
+      const int level = f.b[node + 5];
+      const uint64_t n = f.u(node + 6, 2);
+      if (f.b[node + 4] != 0 || !f.in(node + 24, 8 + n * 16)) return GS_H5_FORMAT;
+      for (uint64_t i = 0; i < n; i++) stack.push_back(f.u(node + 24 + 8 + i * 16, 8));
+      (void)level;
+    } else if (std::memcmp(&f.b[node], "SNOD", 4) == 0) {
+      const uint64_t n = f.u(node + 6, 2);
+      if (!f.in(node + 8, n * 40)) return GS_H5_FORMAT;
+      for (uint64_t j = 0; j < n; j++) {
+        const uint64_t e = node + 8 + j * 40;
+        const uint64_t noff = f.u(e, 8);
+        if (noff >= seg_size) return GS_H5_FORMAT;
+        const char* s = (const char*)&f.b[seg + noff];
+        const size_t maxlen = (size_t)(seg_size - noff);
+        const size_t len = strnlen(s, maxlen);
+        if (len == maxlen) return GS_H5_FORMAT;
+        if (std::strcmp(s, name) == 0) {
+          addr = f.u(e + 8, 8);
+          return 0;
+        }
+      }
+    } else {
+      return GS_H5_FORMAT;
+    }
+  }
+  return GS_H5_NOT_FOUND;
+}
+
+// Datatype message: fixed-point (class 0) or floating point (class 1) of 4 or 8 bytes.
+int parse_type(const File& f, uint64_t o, uint64_t size, int& type, int& elem, bool& be) {
+  if (size < 8) return GS_H5_FORMAT;
+  const int cls = f.b[o] & 0x0f;
+  const uint8_t bits = f.b[o + 1];
+  elem = (int)f.u(o + 4, 4);
+  be = (bits & 1) != 0;
+  if (cls == 0) {
+    const bool sgn = (bits & 8) != 0;
+    if (!sgn) return GS_H5_UNSUPPORTED;
+    if (elem == 4) type = GS_H5_I32;
+    else if (elem == 8) type = GS_H5_I64;
+    else return GS_H5_UNSUPPORTED;
+  } else if (cls == 1) {
+    if (bits & 0x40) return GS_H5_UNSUPPORTED;  // VAX order
+    if (elem == 4) type = GS_H5_F32;
+    else if (elem == 8) type = GS_H5_F64;
+    else return GS_H5_UNSUPPORTED;
+  } else {
+    return GS_H5_UNSUPPORTED;
+  }
+  return 0;
+}
+
+// Dataspace message (versions 1 and 2): rank and current dimensions.
+int parse_space(const File& f, uint64_t o, uint64_t size, int& rank, int64_t* dims) {
+  if (size < 4) return GS_H5_FORMAT;
+  const int ver = f.b[o];
+  rank = f.b[o + 1];
+  if (rank > GS_H5_MAX_RANK) return GS_H5_UNSUPPORTED;
+  const uint64_t d0 = o + (ver == 1 ? 8 : 4);
+  if (ver != 1 && ver != 2) return GS_H5_UNSUPPORTED;
+  if (d0 + 8 * (uint64_t)rank > o + size) return GS_H5_FORMAT;
+  for (int i = 0; i < rank; i++) dims[i] = (int64_t)f.u(d0 + 8 * i, 8);
+  return 0;
+}
+
+int dataset(const File& f, const char* name, Dataset& d) {
+  uint64_t addr = 0;
+  int st = lookup(f, name, addr);
+  if (st) return st;
+  std::vector<Msg> ms;
+  if ((st = messages(f, addr, ms))) return st;
+  for (const Msg& m : ms) {
+    if (m.type == 0x0001) {
+      if ((st = parse_space(f, m.off, m.size, d.rank, d.dims))) return st;
+    } else if (m.type == 0x0003) {
+      if ((st = parse_type(f, m.off, m.size, d.type, d.elem, d.big_endian))) return st;
+    } else if (m.type == 0x000B) {
+      return GS_H5_UNSUPPORTED;  // filter pipeline
+    } else if (m.type == 0x0008) {
+      const int ver = f.b[m.off];
+      if (ver == 1 || ver == 2) {
+        if (m.size < 8) return GS_H5_FORMAT;
+        const int nd = f.b[m.off + 1], cls = f.b[m.off + 2];
+        if (cls == 1) {  // contiguous: address, then nd sizes of 4 bytes (the last the element size)
+          if (m.size < 16 + 4 * (uint64_t)nd) return GS_H5_FORMAT;
+          d.addr = f.u(m.off + 8, 8);
+          uint64_t n = 1;
+          for (int i = 0; i < nd; i++) n *= f.u(m.off + 16 + 4 * i, 4);
+          d.bytes = n;
+        } else if (cls == 0) {  // compact: sizes, then the data size and the data
+          const uint64_t s = m.off + 8 + 4 * (uint64_t)nd;
+          if (s + 4 > m.off + m.size) return GS_H5_FORMAT;
+          d.bytes = f.u(s, 4);
+          d.addr = s + 4;
+        } else {
+          return GS_H5_UNSUPPORTED;
+        }
+      } else if (ver == 3) {
+        const int cls = f.b[m.off + 1];
+        if (cls == 1) {
+          if (m.size < 18) return GS_H5_FORMAT;
+          d.addr = f.u(m.off + 2, 8);
+          d.bytes = f.u(m.off + 10, 8);
+        } else if (cls == 0) {
+          d.bytes = f.u(m.off + 2, 2);
+          d.addr = m.off + 4;
+        } else {
+          return GS_H5_UNSUPPORTED;
+        }
+      } else {
+        return GS_H5_UNSUPPORTED;
+      }
+      d.have_layout = true;
+    }
+  }
+  if (d.rank < 0 || !d.type || !d.have_layout) return GS_H5_FORMAT;
+  uint64_t n = 1;
+  for (int i = 0; i < d.rank; i++) {
+    if (d.dims[i] < 0 || (uint64_t)d.dims[i] > (1ull << 40)) return GS_H5_FORMAT;
+    n *= (uint64_t)d.dims[i];
+    if (n > f.b.size()) return GS_H5_FORMAT;  // more elements than bytes in the file
+  }
+  if (d.bytes < n * (uint64_t)d.elem || !f.in(d.addr, n * (uint64_t)d.elem)) return GS_H5_FORMAT;
+  d.bytes = n * (uint64_t)d.elem;
+  return 0;
+}
+
+int64_t count_of(const Dataset& d) {
+  int64_t n = 1;
+  for (int i = 0; i < d.rank; i++) n *= d.dims[i];
+  return n;
+}
+
+// Element k of a dataset converted to double / int64.
+double as_double(const File& f, const Dataset& d, int64_t k) {
+  uint8_t t[8];
+  const uint64_t o = d.addr + (uint64_t)k * d.elem;
+  for (int i = 0; i < d.elem; i++) t[i] = f.b[o + (d.big_endian ? d.elem - 1 - i : i)];
+  switch (d.type) {
+    case GS_H5_F64: {
+      double v;
+      std::memcpy(&v, t, 8);
+      return v;
+    }
+    case GS_H5_F32: {
+      float v;
+      std::memcpy(&v, t, 4);
+      return v;
+    }
+    case GS_H5_I32: {
+      int32_t v;
+      std::memcpy(&v, t, 4);
+      return v;
+    }
+    default: {
+      int64_t v;
+      std::memcpy(&v, t, 8);
+      return (double)v;
+    }
+  }
+}
+
+int64_t as_int(const File& f, const Dataset& d, int64_t k) {
+  uint8_t t[8];
+  const uint64_t o = d.addr + (uint64_t)k * d.elem;
+  for (int i = 0; i < d.elem; i++) t[i] = f.b[o + (d.big_endian ? d.elem - 1 - i : i)];
+  if (d.type == GS_H5_I32) {
+    int32_t v;
+    std::memcpy(&v, t, 4);
+    return v;
+  }
+  if (d.type == GS_H5_I64) {
+    int64_t v;
+    std::memcpy(&v, t, 8);
+    return v;
+  }
+  return (int64_t)as_double(f, d, k);
+}
+
+// Root-group attribute `name` (attribute message versions 1-3) as integers.
+int attribute(const File& f, const char* name, int64_t* out, int capacity) {
+  std::vector<Msg> ms;
+  int st = messages(f, f.root, ms);
+  if (st) return st;
+  for (const Msg& m : ms) {
+    if (m.type != 0x000C || m.size < 8) continue;
+    const int ver = f.b[m.off];
+    const uint64_t nlen = f.u(m.off + 2, 2), tlen = f.u(m.off + 4, 2), slen = f.u(m.off + 6, 2);
+    auto pad = [&](uint64_t x) { return ver == 1 ? (x + 7) / 8 * 8 : x; };
+    uint64_t o = m.off + 8 + (ver == 3 ? 1 : 0);
+    if (ver < 1 || ver > 3 || o + pad(nlen) + pad(tlen) + pad(slen) > m.off + m.size) return GS_H5_FORMAT;
+    const char* s = (const char*)&f.b[o];
+    if (nlen == 0 || strnlen(s, nlen) != nlen - 1 || std::strcmp(s, name) != 0) continue;
+    o += pad(nlen);
+    Dataset d;
+    if ((st = parse_type(f, o, tlen, d.type, d.elem, d.big_endian))) return st;
+    o += pad(tlen);
+    if ((st = parse_space(f, o, slen, d.rank, d.dims))) return st;
+    o += pad(slen);
+    d.addr = o;
+    for (int i = 0; i < d.rank; i++)
+      if (d.dims[i] < 0 || (uint64_t)d.dims[i] > m.size) return GS_H5_FORMAT;
+    const int64_t n = count_of(d);
+    if (n < 0 || (uint64_t)n > m.size || !f.in(o, (uint64_t)n * d.elem) || o + (uint64_t)n * d.elem > m.off + m.size) return GS_H5_FORMAT;
+    for (int64_t k = 0; k < n && k < capacity; k++) out[k] = as_int(f, d, k);
+    return (int)n;
+  }
+  return GS_H5_NOT_FOUND;
+}
+
+}  // namespace
+
+extern "C" int32_t gs_h5_dataset_info(const char* path, const char* name, int32_t* rank, int64_t* dims,
+                                      int32_t* type) {
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  Dataset d;
+  if ((st = dataset(f, name, d))) return st;
+  if (rank) *rank = d.rank;
+  if (dims)
+    for (int i = 0; i < GS_H5_MAX_RANK; i++) dims[i] = i < d.rank ? d.dims[i] : 0;
+  if (type) *type = d.type;
+  return 0;
+}
+
+extern "C" int64_t gs_h5_read_f64(const char* path, const char* name, double* out, int64_t capacity) {
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  Dataset d;
+  if ((st = dataset(f, name, d))) return st;
+  const int64_t n = count_of(d);
+  if (!out || capacity < n) return GS_H5_INVALID;
+  for (int64_t k = 0; k < n; k++) out[k] = as_double(f, d, k);
+  return n;
+}
+
+extern "C" int64_t gs_h5_read_i32(const char* path, const char* name, int32_t* out, int64_t capacity) {
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  Dataset d;
+  if ((st = dataset(f, name, d))) return st;
+  if (d.type != GS_H5_I32 && d.type != GS_H5_I64) return GS_H5_UNSUPPORTED;
+  const int64_t n = count_of(d);
+  if (!out || capacity < n) return GS_H5_INVALID;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t v = as_int(f, d, k);
+    if (v < INT32_MIN || v > INT32_MAX) return GS_H5_UNSUPPORTED;
+    out[k] = (int32_t)v;
+  }
+  return n;
+}
+
+extern "C" int32_t gs_h5_attribute_i64(const char* path, const char* name, int64_t* out, int32_t capacity) {
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  if (!name || (capacity > 0 && !out)) return GS_H5_INVALID;
+  return attribute(f, name, out, capacity);
+}
+
+extern "C" int64_t gs_h5_read_field(const char* path, int32_t dim, int32_t dofs, int64_t n_nodes, double* out) {
+  if ((dim != 2 && dim != 3) || dofs < 1 || n_nodes < 0 || !out) return GS_H5_INVALID;
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  // FeVariable.cxx:2233-2262, :2326-2338: no "checkpoint file version"
+  // attribute means the first layout, whose rows start with the node index.
+  int64_t ver = 1;
+  st = attribute(f, "checkpoint file version", &ver, 1);
+  if (st == GS_H5_NOT_FOUND) ver = 1;
+  else if (st < 0) return st;
+  if (ver != 1 && ver != 2) return GS_H5_UNSUPPORTED;
+  const int noffset = ver == 1 ? 1 : 0;
+  Dataset d;
+  if ((st = dataset(f, "data", d))) return st;
+  if (d.rank != 2 || d.dims[0] != n_nodes) return GS_H5_SHAPE;
+  // FeVariable.cxx:2348-2351: extra columns are the saved vertex coordinates
+  const int64_t cols = d.dims[1];
+  const bool coords = cols > dofs + noffset;
+  if (cols != dofs + noffset + (coords ? dim : 0)) return GS_H5_SHAPE;
+  const int64_t first = noffset + (coords ? dim : 0);
+  for (int64_t i = 0; i < n_nodes; i++)
+    for (int c = 0; c < dofs; c++) out[i * dofs + c] = as_double(f, d, i * cols + first + c);
+  return n_nodes;
+}
+
+extern "C" int64_t gs_h5_read_swarm(const char* path, const char* swarm, int32_t dim, int64_t capacity,
+                                    double* coord, int32_t* material) {
+  if ((dim != 2 && dim != 3) || !swarm) return GS_H5_INVALID;
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  // FileParticleLayout.cxx:395-409: "/<variable>" first, then the older
+  // "/<swarm>-<variable>" names
+  std::string pos = "Position", mat = "MaterialIndex";
+  Dataset dp;
+  st = dataset(f, pos.c_str(), dp);
+  if (st == GS_H5_NOT_FOUND) {
+    pos = std::string(swarm) + "-Position";
+    mat = std::string(swarm) + "-MaterialIndex";
+    dp = Dataset();
+    st = dataset(f, pos.c_str(), dp);
+  }
+  if (st) return st;
+  if (dp.rank != 2 || dp.dims[1] != dim) return GS_H5_SHAPE;
+  const int64_t P = dp.dims[0];
+  if (!coord && !material) return P;
+  if (capacity < P) return GS_H5_INVALID;
+  Dataset dm;
+  if (material) {
+    if ((st = dataset(f, mat.c_str(), dm))) return st;
+    if (count_of(dm) != P || (dm.type != GS_H5_I32 && dm.type != GS_H5_I64)) return GS_H5_SHAPE;
+  }
+  for (int64_t p = 0; p < P; p++) {
+    if (coord) {
+      for (int c = 0; c < dim; c++) coord[3 * p + c] = as_double(f, dp, p * dim + c);
+      if (dim == 2) coord[3 * p + 2] = 0.0;
+    }
+    if (material) material[p] = (int32_t)as_int(f, dm, p);
+  }
+  return P;
+}

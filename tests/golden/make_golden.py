@@ -17,6 +17,11 @@ available; the outputs are committed and the tests never read the reference).
                               StgDomain/Utils/tests/expected/testSobolGeneratorOutput.<degree>-<coeff>.expected
   spacefiller.json            StgDomain/Swarm/tests/expected/testSpaceFillerParticleLayoutOutput.expected
                               (cell, in-cell index, global index, coordinates as printed %7.5g)
+  rayleigh_taylor_step10/     the HDF5 checkpoint files (Mesh, VelocityField, PressureField, materialSwarm,
+                              timeInfo at step 10) of the reference's regression fixtures
+  lid_driven_pic_step10/      Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml/ and
+                              PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/, copied
+                              unchanged as data
 
 The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
 1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
@@ -95,6 +100,19 @@ def dvc_fixtures():
     json.dump(pc, open(os.path.join(OUT, "pcdvc_expected.json"), "w"), indent=1)
     print("pcdvc_expected.json", pc)
     sobol_fixtures()
+    regression_fixtures()
+
+
+def regression_fixtures():
+    import shutil
+    for src, name in (("Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml", "rayleigh_taylor_step10"),
+                      ("PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml", "lid_driven_pic_step10")):
+        dst = os.path.join(OUT, name)
+        os.makedirs(dst, exist_ok=True)
+        for fn in ("Mesh", "VelocityField", "PressureField", "materialSwarm", "timeInfo"):
+            shutil.copyfile(os.path.join(REF, src, fn + ".00010.h5"), os.path.join(dst, fn + ".00010.h5"))
+            os.chmod(os.path.join(dst, fn + ".00010.h5"), 0o644)
+        print(name, sorted(os.listdir(dst)))
 
 
 def c_array(src, name):

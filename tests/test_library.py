@@ -12,7 +12,8 @@ from conftest import ROOT
 
 def header_functions():
     names = set()
-    for h in ("gale_stokes.h", "gale_generate.h", "gale_learned.h", "gale_batch.h", "gale_weights.h"):
+    for h in ("gale_stokes.h", "gale_generate.h", "gale_learned.h", "gale_batch.h", "gale_weights.h",
+              "gale_checkpoint.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
         names |= set(re.findall(r"\b(gs_[A-Za-z0-9_]+)\s*\(", txt))
