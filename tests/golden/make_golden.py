@@ -19,9 +19,9 @@ available; the outputs are committed and the tests never read the reference).
                               (cell, in-cell index, global index, coordinates as printed %7.5g)
   rayleigh_taylor_step10/     the HDF5 checkpoint files (Mesh, VelocityField, PressureField, materialSwarm,
                               timeInfo at step 10) of the reference's regression fixtures
-  lid_driven_pic_step10/      Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml/ and
-                              PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/, copied
-                              unchanged as data
+  lid_driven_pic_step10/      Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml/,
+  stokes_multigrid_step10/    PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/ and
+                              StgFEM/SysTest/RegressionTests/expected/Multigrid.xml/, copied unchanged as data
 
 The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
 1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
@@ -105,11 +105,15 @@ def dvc_fixtures():
 
 def regression_fixtures():
     import shutil
-    for src, name in (("Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml", "rayleigh_taylor_step10"),
-                      ("PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml", "lid_driven_pic_step10")):
+    swarm = ("Mesh", "VelocityField", "PressureField", "materialSwarm", "timeInfo")
+    for src, name, files in (
+            ("Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml", "rayleigh_taylor_step10", swarm),
+            ("PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml", "lid_driven_pic_step10", swarm),
+            ("StgFEM/SysTest/RegressionTests/expected/Multigrid.xml", "stokes_multigrid_step10",
+             ("Mesh.linearMesh", "VelocityField", "PressureField", "timeInfo"))):
         dst = os.path.join(OUT, name)
         os.makedirs(dst, exist_ok=True)
-        for fn in ("Mesh", "VelocityField", "PressureField", "materialSwarm", "timeInfo"):
+        for fn in files:
             shutil.copyfile(os.path.join(REF, src, fn + ".00010.h5"), os.path.join(dst, fn + ".00010.h5"))
             os.chmod(os.path.join(dst, fn + ".00010.h5"), 0o644)
         print(name, sorted(os.listdir(dst)))

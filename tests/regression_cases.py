@@ -1,10 +1,12 @@
-"""The reference's PIC regression fixtures at step 10, as Stokes problems.
+"""The reference's regression fixtures at step 10, as Stokes problems.
 
 tests/golden/<case>_step10/ holds the HDF5 checkpoints (Mesh, VelocityField,
 PressureField, materialSwarm, timeInfo) that the reference's own regression
 suites compare against (Underworld/SysTest/RegressionTests/expected/
 RayleighTaylorBenchmark.xml, PICellerator/SysTest/RegressionTests/expected/
-LidDrivenPIC2D.xml).  The checkpoint is written after the step-10 solve and
+LidDrivenPIC2D.xml, StgFEM/SysTest/RegressionTests/expected/Multigrid.xml).
+The StgFEM case integrates at 2 x 2 Gauss points (no swarm, nothing
+advected).  For the PIC cases the checkpoint is written after the step-10 solve and
 the particle advection that follows it, so the swarm it holds is one RK2 step
 (TimeIntegrand.cxx:265-335) past the one that step's solve integrated over;
 load() undoes that step with the saved velocity field (fixed-point iteration
@@ -28,15 +30,25 @@ from conftest import GOLDEN
 # lidDrivenVelocityBCs.xml: vy = 0 bottom and top, vx = 0 sides, vx on the top
 # from StgFEM_StandardConditionFunctions_PartialLid_TopLayer
 # (StgFEM/plugins/StandardConditionFunctions/StandardConditionFunctions.cxx:547-567).
+# Multigrid.xml (StgFEM/SysTest/RegressionTests): 32 x 16 over 2 x 1, Gauss
+# swarm 2 x 2 (StgFEM/Apps/StgFEM_Components/GaussSwarm.xml), isoviscous, vx = 0
+# on the sides, vy = 0 top and bottom, vx = 1 on the whole top (the top wall is
+# the last entry of the CompositeVC, so it sets the corners).
 CASES = {
     "rayleigh_taylor": dict(folder="rayleigh_taylor_step10", res=(64, 64), lo=(0.0, 0.0), hi=(0.9142, 1.0),
                             gravity=1.0, materials=[(1.0, 1.0), (1.0, 0.0)],
                             walls=[("bottom", 0, 0.0), ("bottom", 1, 0.0), ("top", 0, 0.0), ("top", 1, 0.0),
-                                   ("left", 0, 0.0), ("right", 0, 0.0)], lid=False),
+                                   ("left", 0, 0.0), ("right", 0, 0.0)], lid=False, gauss=False, mesh="Mesh"),
     "lid_driven": dict(folder="lid_driven_pic_step10", res=(20, 20), lo=(0.0, 0.0), hi=(1.0, 1.0),
                        gravity=0.0, materials=[(1.0, 0.0)],
-                       walls=[("bottom", 1, 0.0), ("left", 0, 0.0), ("right", 0, 0.0), ("top", 1, 0.0)], lid=True),
+                       walls=[("bottom", 1, 0.0), ("left", 0, 0.0), ("right", 0, 0.0), ("top", 1, 0.0)], lid=True,
+                       gauss=False, mesh="Mesh"),
+    "stokes_multigrid": dict(folder="stokes_multigrid_step10", res=(32, 16), lo=(0.0, 0.0), hi=(2.0, 1.0),
+                             gravity=0.0, materials=[(1.0, 0.0)],
+                             walls=[("left", 0, 0.0), ("right", 0, 0.0), ("bottom", 1, 0.0), ("top", 1, 0.0),
+                                    ("top", 0, 1.0)], lid=False, gauss=True, mesh="Mesh.linearMesh"),
 }
+PIC = [k for k, c in CASES.items() if not c["gauss"]]
 
 
 def path(case, what):
@@ -61,16 +73,20 @@ def load(case, gale):
     res, lo, hi = c["res"], c["lo"], c["hi"]
     N = (res[0] + 1) * (res[1] + 1)
     E = res[0] * res[1]
-    x1, mat = gale.read_swarm(path(case, "materialSwarm"), "materialSwarm", 2)
     u = gale.read_field(path(case, "VelocityField"), 2, 2, N)
     p = gale.read_field(path(case, "PressureField"), 2, 1, E)[:, 0]
     dt = float(gale.h5_read(path(case, "timeInfo"), "Dt")[0])
-    x0 = x1[:, :2].copy()
-    for _ in range(40):
-        mid = x0 + 0.5 * dt * _interp(u, x0, res, lo, hi)
-        x0 = x1[:, :2] - dt * _interp(u, mid, res, lo, hi)
-    coord = np.zeros_like(x1)
-    coord[:, :2] = x0
+    if c["gauss"]:
+        coord = gale.layout_gauss(2, res, lo, hi)
+        mat = np.zeros(len(coord), np.int32)
+    else:
+        x1, mat = gale.read_swarm(path(case, "materialSwarm"), "materialSwarm", 2)
+        x0 = x1[:, :2].copy()
+        for _ in range(40):
+            mid = x0 + 0.5 * dt * _interp(u, x0, res, lo, hi)
+            x0 = x1[:, :2] - dt * _interp(u, mid, res, lo, hi)
+        coord = np.zeros_like(x1)
+        coord[:, :2] = x0
     mask, vals = gale.wall_bc(2, res, c["walls"])
     if c["lid"]:
         # PartialLid_TopLayer: vx = 1 where min + margin < x < max - margin,

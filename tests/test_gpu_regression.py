@@ -4,14 +4,14 @@ The HDF5 checkpoints of RayleighTaylorBenchmark and LidDrivenPIC2D are read
 with the library's checkpoint reader, the last advection is undone (see
 regression_cases.py), the problem is built with DVC weights (a 10 x 10
 sub-grid, as the reference's PCDVC) and solved with the reference's Uzawa
-settings (tolerance 1e-5).  The velocity and pressure must match the saved
+settings (tolerance 1e-5); StgFEM's Multigrid case integrates at Gauss points.  The velocity and pressure must match the saved
 fields inside the reference regression suites' tolerance (5e-3; measured
 levels are far below it), in FP64 and in mixed precision, and a tight solve
 must equal the oracle's to 1e-8."""
 import numpy as np
 import pytest
 
-from regression_cases import CASES, load, rel
+from regression_cases import CASES, PIC, load, rel
 
 pytestmark = pytest.mark.gpu
 
@@ -21,11 +21,13 @@ def _problem(gale, case, d):
     mats = [gale.material(eta0=e, density=r) for e, r in c["materials"]]
     pb = gale.Problem(dim=2, res=c["res"], lo=c["lo"], hi=c["hi"], coord=d["coord"], material_index=d["material"],
                       materials=mats, bc_mask=d["bc_mask"], bc_value=d["bc_value"], gravity=c["gravity"])
-    pb.set_weights(gale.WEIGHTS_DVC, 10)
+    if case in PIC:
+        pb.set_weights(gale.WEIGHTS_DVC, 10)
     return pb
 
 
-@pytest.mark.parametrize("case,tol_u,tol_p", [("rayleigh_taylor", 1e-3, 1e-3), ("lid_driven", 1e-4, 1e-3)])
+@pytest.mark.parametrize("case,tol_u,tol_p", [("rayleigh_taylor", 1e-3, 1e-3), ("lid_driven", 1e-4, 1e-3),
+                                              ("stokes_multigrid", 1e-4, 1e-3)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_reference_regression_fields(gale, case, tol_u, tol_p, precision):
     d = load(case, gale)
@@ -52,7 +54,8 @@ def test_reference_regression_problem_matches_oracle(gale, oracle, case):
     o = oracle.Problem(2, c["res"], c["lo"], c["hi"], d["coord"], d["material"],
                        [dict(eta0=e, density=r) for e, r in c["materials"]], bc_mask=d["bc_mask"],
                        bc_value=d["bc_value"], gravity=c["gravity"])
-    o.set_weights(1, (10, 10, 10))
+    if case in PIC:
+        o.set_weights(1, (10, 10, 10))
     _, gown, gxi, gw = pb.particles()
     oown, oxi, ow, _ = o.particles()
     assert np.array_equal(gown, oown) and np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
