@@ -46,7 +46,8 @@ EXPORTS = (
     "gs_solver_set_learned", "gs_lpc_selftest",
     "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
     "gs_batch_solution",
-    "gs_problem_set_weights", "gs_dvc_cells", "gs_dvc_max_particles", "gs_solver_particle_fields",
+    "gs_problem_set_weights", "gs_problem_set_strain_weakening", "gs_problem_set_plastic_strain",
+    "gs_problem_plastic_strain", "gs_dvc_cells", "gs_dvc_max_particles", "gs_solver_particle_fields",
     "gs_problem_population_control",
 )
 
@@ -148,6 +149,9 @@ def lib():
             "gs_relocate": (i32, [vp, vp]),
             "gs_problem_set_coords": (i32, [vp, vp]),
             "gs_problem_set_weights": (i32, [vp, i32, vp]),
+            "gs_problem_set_strain_weakening": (i32, [vp, i32, vp]),
+            "gs_problem_set_plastic_strain": (i32, [vp, vp]),
+            "gs_problem_plastic_strain": (i32, [vp, vp, vp]),
             "gs_dvc_cells": (i32, [i32, vp, i64, vp, vp, vp, vp]),
             "gs_dvc_max_particles": (i64, [i32, vp]),
             "gs_solver_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
@@ -563,6 +567,32 @@ class Problem:
         st = lib().gs_problem_set_weights(self._h, int(kind), _ptr(r))
         _check(st, "set_weights", allow=(E_ESCAPED,))
         return st
+
+    def set_strain_weakening(self, material, healingRate=0.0, initialSofteningStrain=0.0,
+                             finalSofteningStrain=float("inf"), cohesionAfterSoftening=0.0,
+                             frictionCoefficientAfterSoftening=0.0):
+        """StrainWeakening on `material`'s yield rheology, parameters under their
+        XML names (StrainWeakening.cxx:232-290; VonMises / DruckerPrager
+        after-softening values).  healingRate=None detaches it."""
+        if healingRate is None:
+            _check(lib().gs_problem_set_strain_weakening(self._h, int(material), None), "set_strain_weakening")
+            return
+        sw = np.array([healingRate, initialSofteningStrain, finalSofteningStrain, cohesionAfterSoftening,
+                       frictionCoefficientAfterSoftening], np.float64)
+        _check(lib().gs_problem_set_strain_weakening(self._h, int(material), _ptr(sw)), "set_strain_weakening")
+
+    def set_plastic_strain(self, w):
+        """Accumulated plastic strain of every particle (caller order)."""
+        w = np.ascontiguousarray(w, np.float64)
+        if w.shape != (self.P,):
+            raise GaleError(E_INVALID, "plastic strain size")
+        _check(lib().gs_problem_set_plastic_strain(self._h, _ptr(w)), "set_plastic_strain")
+
+    def plastic_strain(self):
+        """(accumulated plastic strain, its rate from the last constitutive pass), caller order."""
+        w, inc = np.zeros(self.P), np.zeros(self.P)
+        _check(lib().gs_problem_plastic_strain(self._h, _ptr(w), _ptr(inc)), "plastic_strain")
+        return w, inc
 
 
 class PcdvcParams(C.Structure):

@@ -58,6 +58,7 @@ struct gs_problem {
   hipStream_t st = nullptr;
   std::vector<gs_material> mats_h;
   MatDev* mats_d = nullptr;
+  std::vector<MatDev> mats_dh;  // host copy of mats_d
   double gravity = 0, ghat[3] = {0, 0, 0}, clamp_min = 0, clamp_max = 0;
   uint8_t* bcdof = nullptr;   // N*dim
   uint8_t* bcflag = nullptr;  // N (bit d = dof d is Dirichlet)
@@ -73,6 +74,8 @@ struct gs_problem {
   int* mat = nullptr;
   int* pid = nullptr;
   double* eta0p = nullptr;
+  double* pstr = nullptr;   // StrainWeakening: accumulated plastic strain (sorted), null when off
+  double* pinc = nullptr;   // its rate from the last constitutive pass (sorted)
   int* off = nullptr;
   unsigned* keys = nullptr;  // sorted owner of each sorted particle (E = outside)
   unsigned* keys2 = nullptr;
@@ -80,6 +83,8 @@ struct gs_problem {
   int* vals2 = nullptr;
   double* coord2 = nullptr;
   double* eta0p2 = nullptr;
+  double* pstr2 = nullptr;
+  double* pinc2 = nullptr;
   int* mat2 = nullptr;
   int* pid2 = nullptr;
   void* sort_tmp = nullptr;
@@ -203,8 +208,7 @@ static int32_t alloc_reducer(Reducer* rd) {
 // Carry the particle arrays along the sort permutation.
 __global__ void k_gather_particles(const int* __restrict__ perm, long long P, const double* __restrict__ c,
                                    double* __restrict__ c2, const int* __restrict__ m, int* __restrict__ m2,
-                                   const int* __restrict__ id, int* __restrict__ id2, const double* __restrict__ e0,
-                                   double* __restrict__ e02) {
+                                   const int* __restrict__ id, int* __restrict__ id2, PartDoubles x, PartDoubles x2) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= P) return;
   const long long j = perm[i];
@@ -213,7 +217,8 @@ __global__ void k_gather_particles(const int* __restrict__ perm, long long P, co
   c2[3 * i + 2] = c[3 * j + 2];
   m2[i] = m[j];
   id2[i] = id[j];
-  if (e0) e02[i] = e0[j];
+  for (int k = 0; k < kPartDoubles; k++)
+    if (x.a[k]) x2.a[k][i] = x.a[k][j];
 }
 
 // Locate every particle (bit-exact owner rule), sort by owner (stable radix
@@ -238,11 +243,14 @@ static int32_t relocate(gs_problem* pb) {
     CK(rocprim::radix_sort_pairs(pb->sort_tmp, bytes, pb->keys2, pb->keys, pb->vals2, pb->vals, (size_t)P, 0, end_bit,
                                  st));
     hipLaunchKernelGGL(k_gather_particles, dim3((unsigned)((P + kBlock - 1) / kBlock)), dim3(kBlock), 0, st, pb->vals, P,
-                       pb->coord, pb->coord2, pb->mat, pb->mat2, pb->pid, pb->pid2, pb->eta0p, pb->eta0p2);
+                       pb->coord, pb->coord2, pb->mat, pb->mat2, pb->pid, pb->pid2,
+                       PartDoubles{{pb->eta0p, pb->pstr, pb->pinc}}, PartDoubles{{pb->eta0p2, pb->pstr2, pb->pinc2}});
     std::swap(pb->coord, pb->coord2);
     std::swap(pb->mat, pb->mat2);
     std::swap(pb->pid, pb->pid2);
     std::swap(pb->eta0p, pb->eta0p2);
+    std::swap(pb->pstr, pb->pstr2);
+    std::swap(pb->pinc, pb->pinc2);
   }
   launch_offsets(reinterpret_cast<const int*>(pb->keys), P, pb->g.E, pb->off, st);
   launch_remap(pb->g, pb->coord, pb->off, pb->pk, pb->g.E, st);
@@ -322,8 +330,9 @@ extern "C" int32_t gs_problem_create(const gs_problem_desc* d, gs_problem** out)
   for (int i = 0; i < d->n_materials; i++) {
     const gs_material& m = d->materials[i];
     md[i] = MatDev{m.eta0, m.density, m.yield_type, 0, m.cohesion, m.friction, m.min_yield_stress, m.min_viscosity,
-                   m.max_strain_rate};
+                   m.max_strain_rate, 0.0, 0.0, 0.0, 0.0, 0.0};
   }
+  pb->mats_dh = md;
   std::vector<uint8_t> bcdof(ND, 0), bcflag(g.N, 0);
   std::vector<double> ubc(ND, 0.0);
   long long nfree = 0;
@@ -384,7 +393,7 @@ extern "C" void gs_problem_destroy(gs_problem* pb) {
   dfree(pb->mats_d); dfree(pb->bcdof); dfree(pb->bcflag); dfree(pb->ubc); dfree(pb->pk); dfree(pb->pk32);
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->off); dfree(pb->keys); dfree(pb->keys2); dfree(pb->vals);
-  dfree(pb->wp);
+  dfree(pb->wp); dfree(pb->pstr); dfree(pb->pstr2); dfree(pb->pinc); dfree(pb->pinc2);
   dfree(pb->vals2); dfree(pb->S); dfree(pb->rd.partials); dfree(pb->halo64);
   dfree(pb->halo32);
   if (pb->sort_tmp) (void)hipFree(pb->sort_tmp);
@@ -458,6 +467,71 @@ extern "C" int32_t gs_relocate(gs_problem* pb, int32_t* owner_out) {
   return st;
 }
 
+extern "C" int32_t gs_problem_set_strain_weakening(gs_problem* pb, int32_t material, const gs_strain_weakening* sw) {
+  if (!pb || material < 0 || material >= (int32_t)pb->mats_dh.size()) return GS_E_INVALID;
+  MatDev& m = pb->mats_dh[material];
+  if (sw) {
+    // StrainWeakening belongs to a yield rheology (YieldRheology.cxx:141-147)
+    if (m.yield_type == GS_YIELD_NONE) return GS_E_INVALID;
+    if (!(sw->final_softening_strain > sw->initial_softening_strain) || sw->healing_rate < 0.0) return GS_E_INVALID;
+    m.sw_on = 1;
+    m.sw_healing = sw->healing_rate;
+    m.sw_eps0 = sw->initial_softening_strain;
+    m.sw_eps1 = sw->final_softening_strain;
+    m.cohesion_after = sw->cohesion_after;
+    m.friction_after = sw->friction_after;
+  } else {
+    m.sw_on = 0;
+  }
+  CK(hipStreamSynchronize(pb->st));
+  if (sw && !pb->pstr) {
+    // _StrainWeakening_Initialise: no initial damage, zero increments
+    const long long PA = pb->P > 0 ? pb->P : 1;
+    CK(dalloc(&pb->pstr, PA));
+    CK(dalloc(&pb->pstr2, PA));
+    CK(dalloc(&pb->pinc, PA));
+    CK(dalloc(&pb->pinc2, PA));
+    CK(hipMemset(pb->pstr, 0, sizeof(double) * PA));
+    CK(hipMemset(pb->pinc, 0, sizeof(double) * PA));
+  }
+  CK(hipMemcpy(pb->mats_d, pb->mats_dh.data(), sizeof(MatDev) * pb->mats_dh.size(), hipMemcpyHostToDevice));
+  pb->gen++;  // solvers re-run their constitutive pass
+  return GS_OK;
+}
+
+extern "C" int32_t gs_problem_set_plastic_strain(gs_problem* pb, const double* w) {
+  if (!pb || !pb->pstr || (!w && pb->P > 0)) return GS_E_INVALID;
+  const long long P = pb->P;
+  if (P == 0) return GS_OK;
+  CK(hipStreamSynchronize(pb->st));
+  std::vector<int> pid(P);
+  std::vector<double> ws(P);
+  CK(hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < P; i++) ws[i] = w[pid[i]];
+  CK(hipMemcpy(pb->pstr, ws.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  pb->gen++;
+  return GS_OK;
+}
+
+extern "C" int32_t gs_problem_plastic_strain(gs_problem* pb, double* w, double* inc) {
+  if (!pb || !pb->pstr) return GS_E_INVALID;
+  const long long P = pb->P;
+  if (P == 0) return GS_OK;
+  CK(hipStreamSynchronize(pb->st));
+  std::vector<int> pid(P);
+  std::vector<double> v(P);
+  CK(hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost));
+  if (w) {
+    CK(hipMemcpy(v.data(), pb->pstr, sizeof(double) * P, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < P; i++) w[pid[i]] = v[i];
+  }
+  if (inc) {
+    CK(hipMemcpy(v.data(), pb->pinc, sizeof(double) * P, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < P; i++) inc[pid[i]] = v[i];
+  }
+  return GS_OK;
+}
+
 extern "C" int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const int32_t* res) {
   if (!pb || (kind != GS_WEIGHTS_CONSTANT && kind != GS_WEIGHTS_DVC)) return GS_E_INVALID;
   if (kind == GS_WEIGHTS_DVC) {
@@ -497,12 +571,12 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   unsigned char* del = nullptr;
   int *nsplit = nullptr, *sparent = nullptr, *dmap = nullptr, *dbase = nullptr;
   double* scoord = nullptr;
-  double *coordN = nullptr, *eta0N = nullptr;
+  double *coordN = nullptr, *eta0N = nullptr, *pstrN = nullptr, *pincN = nullptr;
   int* matN = nullptr;
   int32_t rc = GS_OK;
   auto cleanup = [&]() {
     dfree(del); dfree(nsplit); dfree(sparent); dfree(dmap); dfree(dbase); dfree(scoord);
-    dfree(coordN); dfree(eta0N); dfree(matN);
+    dfree(coordN); dfree(eta0N); dfree(matN); dfree(pstrN); dfree(pincN);
   };
   if (dalloc(&del, P > 0 ? P : 1) || dalloc(&nsplit, E) || dalloc(&sparent, E * slots) ||
       dalloc(&scoord, E * slots * 3)) {
@@ -540,13 +614,14 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   }
   if (dalloc(&dmap, P > 0 ? P : 1) || dalloc(&dbase, E) || dalloc(&coordN, 3 * (P2 > 0 ? P2 : 1)) ||
       dalloc(&matN, P2 > 0 ? P2 : 1) || (pb->eta0p && dalloc(&eta0N, P2 > 0 ? P2 : 1)) ||
+      (pb->pstr && (dalloc(&pstrN, P2 > 0 ? P2 : 1) || dalloc(&pincN, P2 > 0 ? P2 : 1))) ||
       hipMemcpy(dmap, map.data(), sizeof(int) * P, hipMemcpyHostToDevice) ||
       hipMemcpy(dbase, base.data(), sizeof(int) * E, hipMemcpyHostToDevice)) {
     cleanup();
     return GS_E_HIP;
   }
-  pcdvc_rebuild(P, E, dmap, pb->coord, pb->mat, pb->eta0p, nsplit, dbase, slots, scoord, sparent, nsurv, coordN, matN,
-                eta0N, st);
+  pcdvc_rebuild(P, E, dmap, pb->coord, pb->mat, PartDoubles{{pb->eta0p, pb->pstr, pb->pinc}}, nsplit, dbase, slots,
+                scoord, sparent, nsurv, coordN, matN, PartDoubles{{eta0N, pstrN, pincN}}, st);
   if (hipStreamSynchronize(st)) {
     cleanup();
     return GS_E_HIP;
@@ -555,13 +630,18 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->pk); dfree(pb->pk32); dfree(pb->keys); dfree(pb->keys2);
   dfree(pb->vals); dfree(pb->vals2); dfree(pb->wp);
+  dfree(pb->pstr); dfree(pb->pstr2); dfree(pb->pinc); dfree(pb->pinc2);
   pb->pk32_valid = false;
   pb->coord = coordN;
   pb->mat = matN;
   pb->eta0p = eta0N;
+  pb->pstr = pstrN;
+  pb->pinc = pincN;
   coordN = nullptr;
   matN = nullptr;
   eta0N = nullptr;
+  pstrN = nullptr;
+  pincN = nullptr;
   cleanup();
   pb->P = P2;
   const long long PA = P2 > 0 ? P2 : 1;
@@ -576,6 +656,10 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   CK(dalloc(&pb->vals2, PA));
   CK(dalloc(&pb->wp, PA));
   if (pb->eta0p) CK(dalloc(&pb->eta0p2, PA));
+  if (pb->pstr) {
+    CK(dalloc(&pb->pstr2, PA));
+    CK(dalloc(&pb->pinc2, PA));
+  }
   std::vector<int> ids(PA);
   for (long long i = 0; i < PA; i++) ids[i] = (int)i;
   CK(hipMemcpy(pb->pid, ids.data(), sizeof(int) * PA, hipMemcpyHostToDevice));
@@ -600,11 +684,12 @@ extern "C" int32_t gs_problem_set_coords(gs_problem* pb, const double* coord) {
   CK(hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost));
   CK(hipMemcpy(mat.data(), pb->mat, sizeof(int) * P, hipMemcpyDeviceToHost));
   for (long long i = 0; i < P; i++) mat_o[pid[i]] = mat[i];
-  if (pb->eta0p) {
+  for (double* arr : {pb->eta0p, pb->pstr, pb->pinc}) {
+    if (!arr) continue;
     std::vector<double> e0(P), e0o(P);
-    CK(hipMemcpy(e0.data(), pb->eta0p, sizeof(double) * P, hipMemcpyDeviceToHost));
+    CK(hipMemcpy(e0.data(), arr, sizeof(double) * P, hipMemcpyDeviceToHost));
     for (long long i = 0; i < P; i++) e0o[pid[i]] = e0[i];
-    CK(hipMemcpy(pb->eta0p, e0o.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+    CK(hipMemcpy(arr, e0o.data(), sizeof(double) * P, hipMemcpyHostToDevice));
   }
   CK(hipMemcpy(pb->mat, mat_o.data(), sizeof(int) * P, hipMemcpyHostToDevice));
   CK(hipMemcpy(pb->pid, ids.data(), sizeof(int) * P, hipMemcpyHostToDevice));
@@ -1092,6 +1177,8 @@ static int32_t setup_dev(gs_solver* s, const double* u_prev, const double* p_pre
   a.mats = pb->mats_d;
   a.mat = pb->mat;
   a.eta0p = pb->eta0p;
+  a.pstr = pb->pstr;
+  a.pinc = pb->pinc;
   a.wp = pb->wkind == GS_WEIGHTS_DVC ? pb->wp : nullptr;
   a.clamp_min = pb->clamp_min;
   a.clamp_max = pb->clamp_max;
@@ -1705,6 +1792,8 @@ extern "C" int32_t gs_advect_rk2(gs_solver* s, double dt) {
   unsigned esc;
   std::memcpy(&esc, pb->hS + S_ESC, sizeof(unsigned));
   s->setup_done = false;
+  // the other time integrand of the material swarm: StrainWeakening
+  if (pb->pstr) launch_weakening_step(pb->pstr, pb->pinc, pb->P, dt, pb->st);
   const int32_t rs = relocate(pb);
   if (esc) return GS_E_ESCAPED;
   return rs;

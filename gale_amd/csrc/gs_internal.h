@@ -20,8 +20,11 @@ struct Geo {
 // Material parameters on the device (one RheologyMaterial each).
 struct MatDev {
   double eta0, density;
-  int yield_type, pad_;
+  int yield_type, sw_on;  // sw_on: the yield rheology carries StrainWeakening
   double cohesion, friction, min_yield_stress, min_viscosity, max_strain_rate;
+  // StrainWeakening (healingRate, initial/finalSofteningStrain) and the
+  // after-softening cohesion / friction coefficient of the yield rheology
+  double sw_healing, sw_eps0, sw_eps1, cohesion_after, friction_after;
 };
 
 // Scalar slots kept on the device so that the CG / Uzawa loops never wait
@@ -67,6 +70,8 @@ struct SetupArgs {
   const int* mat;         // P (sorted)
   const double* eta0p;    // P (sorted) or null
   const double* wp;       // P (sorted) integration weights, or null: 2^dim / count
+  const double* pstr;     // P (sorted) accumulated plastic strain (StrainWeakening), or null
+  double* pinc;           // P (sorted) its rate from this pass, written when pstr is set
   double clamp_min, clamp_max;
   double g[3];            // gravity vector
 };
@@ -85,6 +90,8 @@ void launch_apply_Gt(const Geo& g, const double* x, double* q, hipStream_t st);
 void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f,
                   double* Q, double* eta_out, uint8_t* yielded, hipStream_t st);
 void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st);
+// StrainWeakening time step: w += dt * inc, then negative values set to 0.
+void launch_weakening_step(double* w, const double* inc, long long P, double dt, hipStream_t st);
 // Strain rate (P x nst), its second invariant (P) and stress (P x nst) of the
 // material points (sorted order) for velocity u and viscosity eta.
 void launch_particle_fields(const Geo& g, const double4* pk, const unsigned* keys, long long P, const double* u,

@@ -430,6 +430,20 @@ void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st)
   if (P > 0) hipLaunchKernelGGL(k_pack32, dim3(grid_for(P)), dim3(kBlock), 0, st, pk, pk32, P);
 }
 
+// StrainWeakening as a TimeIntegrand (StrainWeakening.cxx:432-441: the time
+// derivative is the increment of the last constitutive pass, so the RK2 step
+// is w + dt * inc) followed by _StrainWeakening_MakeValuesPositive (:443-468).
+__global__ void k_weakening_step(double* __restrict__ w, const double* __restrict__ inc, long long P, double dt) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const double v = w[p] + dt * inc[p];
+  w[p] = v < 0.0 ? 0.0 : v;
+}
+
+void launch_weakening_step(double* w, const double* inc, long long P, double dt, hipStream_t st) {
+  if (P > 0) hipLaunchKernelGGL(k_weakening_step, dim3(grid_for(P)), dim3(kBlock), 0, st, w, inc, P, dt);
+}
+
 // ---------------------------------------------------------------------------
 // G p (node-centric gather) and G^T u (element-centric).  G_e[a,d] is the
 // closed form of GradientStiffnessMatrixTerm.cxx:225-245 on an affine
@@ -527,8 +541,23 @@ __device__ __forceinline__ double sym_inv(int dim, const double* t) {
 
 // Yield-limited viscosity (YieldRheology.cxx:203-231, VonMises.cxx:207-291,
 // DruckerPrager.cxx:296-445).  L = velocity gradient at the point.
+// StrainWeakening_CalcRatio (StrainWeakening.cxx:497-519): softening ratio of
+// accumulated plastic strain w, clipped to [0, 1].
+__device__ inline double weakening_ratio(const MatDev& m, double w) {
+  if (w < 0.0) w = 0.0;
+  double r = (w - m.sw_eps0) / (m.sw_eps1 - m.sw_eps0);
+  if (r > 1.0) r = 1.0;
+  if (r < 0.0) r = 0.0;
+  return r;
+}
+
+// Viscosity after the yield rheology of material m.  With strain weakening
+// (ratio r) the cohesion and friction are interpolated towards their
+// after-softening values (VonMises.cxx:196-199, DruckerPrager.cxx:448-489),
+// and *crit / *ind return the yield criterion and indicator the
+// StrainWeakening increment uses.
 __device__ double yield_viscosity(int dim, const MatDev& m, double eta, const double (&L)[3][3], double pres,
-                                  int* yielded) {
+                                  double r, int* yielded, double* crit_out, double* ind_out) {
   double sr[6], stv[6];
   const int nst = dim == 2 ? 3 : 6;
   if (dim == 2) {
@@ -555,8 +584,10 @@ __device__ double yield_viscosity(int dim, const MatDev& m, double eta, const do
   }
   const double srII = sym_inv(dim, sr), tauII = sym_inv(dim, stv);
   *yielded = 0;
+  *ind_out = tauII;
+  const double C = m.sw_on ? m.cohesion * (1.0 - r) + m.cohesion_after * r : m.cohesion;
   if (m.yield_type == 1) {
-    const double mu = m.friction, C = m.cohesion;
+    const double mu = m.sw_on ? m.friction * (1.0 - r) + m.friction_after * r : m.friction;
     double factor, fs;
     if (dim == 2) {
       factor = 1 / sqrt(1 + mu * mu);
@@ -570,6 +601,7 @@ __device__ double yield_viscosity(int dim, const MatDev& m, double eta, const do
     double minys = m.min_yield_stress;
     if (minys == 0.0) minys = C;
     if (fs < minys * factor) fs = minys * factor;
+    *crit_out = fs;
     if (fs < tauII) {
       double v = (m.max_strain_rate > 0 && srII > m.max_strain_rate) ? fs / (2 * m.max_strain_rate)
                                                                       : fs / (2 * srII);
@@ -578,7 +610,8 @@ __device__ double yield_viscosity(int dim, const MatDev& m, double eta, const do
       return v;
     }
   } else if (m.yield_type == 2) {
-    const double crit = m.cohesion;
+    const double crit = C;
+    *crit_out = crit;
     if (crit < tauII) {
       const double beta = 1.0 - crit / tauII;
       double corr = -eta * beta;
@@ -648,13 +681,27 @@ __global__ __launch_bounds__(kBlock) void k_setup(Geo g, int c, long long mcount
       }
     }
     int yl = 0;
+    double inc = 0.0;
     if (need_u && m.yield_type != 0) {
       double L[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
       for (int q = 0; q < NEN; q++)
         for (int i = 0; i < DIM; i++)
           for (int j = 0; j < DIM; j++) L[i][j] += dN[j][q] * ue[q][i];
-      eta = yield_viscosity(DIM, m, eta, L, pres, &yl);
+      const bool sw = m.sw_on && a.pstr;
+      const double w = sw ? (a.pstr[p] < 0.0 ? 0.0 : a.pstr[p]) : 0.0;
+      double crit = 0.0, ind = 0.0;
+      const double eta_in = eta;
+      eta = yield_viscosity(DIM, m, eta, L, pres, sw ? weakening_ratio(m, w) : 0.0, &yl, &crit, &ind);
+      if (sw) {
+        // _StrainWeakening_CalcIncrementIsotropic (StrainWeakening.cxx:470-495),
+        // with the viscosity before yielding: growth from the plastic part of
+        // the strain rate, decay by healing
+        double beta = 1.0 - crit / ind;
+        if (beta < 0.0) beta = 0.0;
+        inc = (crit / eta_in) * (beta / (1.0 - beta) - m.sw_healing * w);
+      }
     }
+    if (a.pinc) a.pinc[p] = inc;
     if (a.clamp_min > 0 && eta < a.clamp_min) eta = a.clamp_min;
     if (a.clamp_max > 0 && eta > a.clamp_max) eta = a.clamp_max;
     const double cw = w * detJ;

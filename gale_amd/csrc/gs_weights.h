@@ -27,8 +27,14 @@ void pcdvc_decide(const Geo& g, const int* off, const int* pid, const int* mat, 
                   const PcdvcArgs& a, unsigned char* del, int* nsplit, double* scoord, int* sparent, hipStream_t st);
 // Rebuild the particle arrays in the new caller order: survivors to map[i]
 // (-1 = deleted), new particles after n_surv in cell order.
-void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, const double* eta0,
+// Optional per-particle double arrays that follow the particles (per-particle
+// eta0, StrainWeakening plastic strain and its rate); null entries are skipped.
+constexpr int kPartDoubles = 3;
+struct PartDoubles {
+  double* a[kPartDoubles];
+};
+void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, PartDoubles x,
                    const int* nsplit, const int* base, int slots, const double* scoord, const int* sparent,
-                   long long n_surv, double* coordN, int* matN, double* eta0N, hipStream_t st);
+                   long long n_surv, double* coordN, int* matN, PartDoubles xN, hipStream_t st);
 
 }  // namespace gs

@@ -371,21 +371,24 @@ __global__ void k_pcdvc_decide(Geo g, const int* __restrict__ off, const int* __
 }
 
 __global__ void k_pcdvc_compact(long long P, const int* __restrict__ map, const double* __restrict__ coord,
-                                const int* __restrict__ mat, const double* __restrict__ eta0,
-                                double* __restrict__ coordN, int* __restrict__ matN, double* __restrict__ eta0N) {
+                                const int* __restrict__ mat, PartDoubles x, double* __restrict__ coordN,
+                                int* __restrict__ matN, PartDoubles xN) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= P) return;
   const int m = map[i];
   if (m < 0) return;
   for (int d = 0; d < 3; d++) coordN[3LL * m + d] = coord[3 * i + d];
   matN[m] = mat[i];
-  if (eta0N) eta0N[m] = eta0[i];
+  for (int k = 0; k < kPartDoubles; k++)
+    if (xN.a[k]) xN.a[k][m] = x.a[k][i];
 }
 
+// A split copies its parent's particle state (Swarm_CopyParticleWithinSwarm
+// in PCDVC.cxx): material, per-particle eta0 and the StrainWeakening values.
 __global__ void k_pcdvc_append(long long E, const int* __restrict__ nsplit, const int* __restrict__ base, int slots,
                                const double* __restrict__ scoord, const int* __restrict__ sparent,
-                               const int* __restrict__ mat, const double* __restrict__ eta0, long long n_surv,
-                               double* __restrict__ coordN, int* __restrict__ matN, double* __restrict__ eta0N) {
+                               const int* __restrict__ mat, PartDoubles x, long long n_surv,
+                               double* __restrict__ coordN, int* __restrict__ matN, PartDoubles xN) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= E) return;
   for (int k = 0; k < nsplit[e]; k++) {
@@ -394,7 +397,8 @@ __global__ void k_pcdvc_append(long long E, const int* __restrict__ nsplit, cons
     for (int d = 0; d < 3; d++) coordN[3 * np + d] = scoord[3 * s + d];
     const int par = sparent[s];
     matN[np] = mat[par];
-    if (eta0N) eta0N[np] = eta0[par];
+    for (int q = 0; q < kPartDoubles; q++)
+      if (xN.a[q]) xN.a[q][np] = x.a[q][par];
   }
 }
 
@@ -407,15 +411,15 @@ void pcdvc_decide(const Geo& g, const int* off, const int* pid, const int* mat, 
                      a, del, nsplit, scoord, sparent);
 }
 
-void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, const double* eta0,
+void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, PartDoubles x,
                    const int* nsplit, const int* base, int slots, const double* scoord, const int* sparent,
-                   long long n_surv, double* coordN, int* matN, double* eta0N, hipStream_t st) {
+                   long long n_surv, double* coordN, int* matN, PartDoubles xN, hipStream_t st) {
   if (P > 0)
-    hipLaunchKernelGGL(k_pcdvc_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, map, coord, mat, eta0,
-                       coordN, matN, eta0N);
+    hipLaunchKernelGGL(k_pcdvc_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, map, coord, mat, x,
+                       coordN, matN, xN);
   if (E > 0)
     hipLaunchKernelGGL(k_pcdvc_append, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, E, nsplit, base, slots,
-                       scoord, sparent, mat, eta0, n_surv, coordN, matN, eta0N);
+                       scoord, sparent, mat, x, n_surv, coordN, matN, xN);
 }
 
 }  // namespace gs

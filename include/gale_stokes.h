@@ -57,6 +57,16 @@ typedef struct gs_material {
   double max_strain_rate;   /* DruckerPrager "maxStrainRate" (0 -> off) */
 } gs_material;
 
+/* StrainWeakening attached to a material's yield rheology
+ * (Underworld/Rheology/src/StrainWeakening.cxx; YieldRheology "StrainWeakening"). */
+typedef struct gs_strain_weakening {
+  double healing_rate;              /* StrainWeakening "healingRate" */
+  double initial_softening_strain;  /* "initialSofteningStrain" */
+  double final_softening_strain;    /* "finalSofteningStrain" (older name "softeningStrain") */
+  double cohesion_after;            /* VonMises / DruckerPrager "cohesionAfterSoftening" */
+  double friction_after;            /* DruckerPrager "frictionCoefficientAfterSoftening" */
+} gs_strain_weakening;
+
 /* Mesh (CartesianGenerator regular box), boundary conditions (WallVC as a
  * per-dof mask), gravity (BuoyancyForceTerm) and the material swarm. */
 typedef struct gs_problem_desc {
@@ -191,13 +201,35 @@ int32_t gs_compute_dt(gs_solver* s, double dt_factor, double* dt);
 /* TimeIntegrand_SecondOrder with SwarmAdvector (TimeIntegrand.cxx:265-335,
  * SwarmAdvector.cxx:261-280), then Swarm_UpdateAllParticleOwners
  * (SwarmClass.cxx:878-974) and the CoincidentMapper remap with
- * ConstantWeights, using the solver's velocity. */
+ * ConstantWeights, using the solver's velocity.  With strain weakening on,
+ * the plastic strain of every particle is integrated over the same dt first
+ * (StrainWeakening.cxx:432-468: w += dt * increment, negatives set to 0). */
 int32_t gs_advect_rk2(gs_solver* s, double dt);
 /* Owner update only (Mesh_RegularAlgorithms.cxx:191-224 rule, bit-exact);
  * owner_out (P, caller order) may be NULL. */
 int32_t gs_relocate(gs_problem* pb, int32_t* owner_out);
 /* Replace particle coordinates (caller order, P*3) and relocate. */
 int32_t gs_problem_set_coords(gs_problem* pb, const double* coord);
+
+/* Attach StrainWeakening to material `material`'s yield rheology (NULL
+ * detaches it).  The yield criterion then uses cohesion and friction
+ * interpolated towards their after-softening values by the ratio
+ * clip((w - initial) / (final - initial), 0, 1) of each particle's
+ * accumulated plastic strain w (VonMises.cxx:185-205,
+ * DruckerPrager.cxx:448-489, StrainWeakening_CalcRatio
+ * StrainWeakening.cxx:497-519), and every constitutive pass with a previous
+ * solution stores the rate (tau_y / eta) (beta / (1 - beta) - healing w),
+ * beta = max(0, 1 - tau_y / tau_II) (_StrainWeakening_CalcIncrementIsotropic,
+ * :470-495; YieldRheology.cxx:190-228).  The first call allocates w and its
+ * rate, both 0 (no initial damage).  GS_E_INVALID for a material without a
+ * yield rheology or final <= initial softening strain. */
+int32_t gs_problem_set_strain_weakening(gs_problem* pb, int32_t material, const gs_strain_weakening* sw);
+/* Set the accumulated plastic strain (P, caller order); the reference's
+ * initial damage (rand()-seeded, StrainWeakening.cxx:333-430) is set this way. */
+int32_t gs_problem_set_plastic_strain(gs_problem* pb, const double* w);
+/* Read the accumulated plastic strain and its rate from the last
+ * constitutive pass (P each, caller order; either may be NULL). */
+int32_t gs_problem_plastic_strain(gs_problem* pb, double* w, double* inc);
 
 /* Operator applies on full nodal vectors, unconstrained (no Dirichlet
  * masking), for operator-level parity and measurement.  DEVICE pointers on

@@ -117,6 +117,10 @@ def lib():
             "orc_advect_rk2": (i32, [vp, vp, d, vp]),
             "orc_dvc_cell": (i32, [i32, vp, i32, vp, vp, vp]),
             "orc_set_weights": (i32, [vp, i32, vp]),
+            "orc_set_strain_weakening": (i32, [vp, i32, vp]),
+            "orc_set_plastic_strain": (i32, [vp, vp]),
+            "orc_plastic_strain": (i32, [vp, vp, vp]),
+            "orc_weakening_step": (i32, [vp, d]),
             "orc_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
             "orc_population_control": (i32, [vp, vp, vp]),
         }
@@ -337,6 +341,37 @@ class Problem:
         st = lib().orc_set_weights(self._h, kind, r.ctypes.data)
         if st != 0:
             raise RuntimeError(f"orc_set_weights status {st}")
+
+    def set_strain_weakening(self, material, healing_rate=0.0, initial_softening_strain=0.0,
+                             final_softening_strain=float("inf"), cohesion_after=0.0, friction_after=0.0):
+        """StrainWeakening on `material`'s yield rheology (None-like: pass
+        healing_rate=None to detach)."""
+        if healing_rate is None:
+            st = lib().orc_set_strain_weakening(self._h, material, None)
+        else:
+            sw = np.array([healing_rate, initial_softening_strain, final_softening_strain, cohesion_after,
+                           friction_after], dtype=np.float64)
+            st = lib().orc_set_strain_weakening(self._h, material, sw.ctypes.data)
+        if st != 0:
+            raise RuntimeError(f"orc_set_strain_weakening status {st}")
+
+    def set_plastic_strain(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        assert w.shape == (self.P,)
+        if lib().orc_set_plastic_strain(self._h, w.ctypes.data) != 0:
+            raise RuntimeError("orc_set_plastic_strain")
+
+    def plastic_strain(self):
+        """(w, increment) in caller order."""
+        w = np.zeros(self.P)
+        inc = np.zeros(self.P)
+        if lib().orc_plastic_strain(self._h, w.ctypes.data, inc.ctypes.data) != 0:
+            raise RuntimeError("orc_plastic_strain")
+        return w, inc
+
+    def weakening_step(self, dt):
+        if lib().orc_weakening_step(self._h, float(dt)) != 0:
+            raise RuntimeError("orc_weakening_step")
 
     def particles(self):
         owner = np.zeros(self.P, dtype=np.int32)

@@ -245,6 +245,10 @@ struct orc_problem {
   double* coord;
   int32_t* material;
   double* eta0p;
+  double* pstr;                 /* StrainWeakening plastic strain (caller order), NULL when off */
+  double* pinc;                 /* its rate from the last constitutive evaluation */
+  orc_strain_weakening* sw;     /* per material */
+  int* sw_on;
   int32_t* owner;
   double* xi;
   double* weight;
@@ -702,23 +706,35 @@ int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int6
   double* coord2 = (double*)malloc(sizeof(double) * 3 * (P2 > 0 ? P2 : 1));
   int32_t* mat2 = (int32_t*)malloc(sizeof(int32_t) * (P2 > 0 ? P2 : 1));
   double* e02 = pb->eta0p ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
+  double* w2 = pb->pstr ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
+  double* i2 = pb->pstr ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
   int64_t k = 0;
   for (int64_t p = 0; p < pb->P; p++) {
     if (del[p]) continue;
     memcpy(coord2 + 3 * k, pb->coord + 3 * p, sizeof(double) * 3);
     mat2[k] = pb->material[p];
     if (e02) e02[k] = pb->eta0p[p];
+    if (w2) {
+      w2[k] = pb->pstr[p];
+      i2[k] = pb->pinc[p];
+    }
     k++;
   }
   for (int64_t j = 0; j < nnew; j++, k++) {
     memcpy(coord2 + 3 * k, ncoord + 3 * j, sizeof(double) * 3);
     mat2[k] = pb->material[nparent[j]];
     if (e02) e02[k] = pb->eta0p[nparent[j]];
+    if (w2) {   /* a split copies its parent, extensions included */
+      w2[k] = pb->pstr[nparent[j]];
+      i2[k] = pb->pinc[nparent[j]];
+    }
   }
-  free(pb->coord); free(pb->material); free(pb->eta0p);
+  free(pb->coord); free(pb->material); free(pb->eta0p); free(pb->pstr); free(pb->pinc);
   pb->coord = coord2;
   pb->material = mat2;
   pb->eta0p = e02;
+  pb->pstr = w2;
+  pb->pinc = i2;
   pb->P = P2;
   pb->owner = (int32_t*)realloc(pb->owner, sizeof(int32_t) * (P2 > 0 ? P2 : 1));
   pb->xi = (double*)realloc(pb->xi, sizeof(double) * dim * (P2 > 0 ? P2 : 1));
@@ -736,6 +752,53 @@ int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int6
   pb->wkind = 1;
   for (int d = 0; d < 3; d++) pb->wres[d] = res[d];
   return relocate_all(pb);
+}
+
+int32_t orc_set_strain_weakening(orc_problem* pb, int32_t material, const orc_strain_weakening* sw) {
+  if (material < 0 || material >= pb->nmat) return ORC_E_INVALID;
+  if (!pb->sw) {
+    pb->sw = (orc_strain_weakening*)calloc((size_t)pb->nmat, sizeof(orc_strain_weakening));
+    pb->sw_on = (int*)calloc((size_t)pb->nmat, sizeof(int));
+  }
+  if (!sw) {
+    pb->sw_on[material] = 0;
+    return ORC_OK;
+  }
+  if (pb->mats[material].yield_type == ORC_YIELD_NONE) return ORC_E_INVALID;
+  if (!(sw->final_softening_strain > sw->initial_softening_strain) || sw->healing_rate < 0.0) return ORC_E_INVALID;
+  pb->sw[material] = *sw;
+  pb->sw_on[material] = 1;
+  if (!pb->pstr) {
+    pb->pstr = (double*)calloc((size_t)(pb->P > 0 ? pb->P : 1), sizeof(double));
+    pb->pinc = (double*)calloc((size_t)(pb->P > 0 ? pb->P : 1), sizeof(double));
+  }
+  pb->has_visc = 0;
+  return ORC_OK;
+}
+
+int32_t orc_set_plastic_strain(orc_problem* pb, const double* w) {
+  if (!pb->pstr) return ORC_E_INVALID;
+  memcpy(pb->pstr, w, sizeof(double) * pb->P);
+  pb->has_visc = 0;
+  return ORC_OK;
+}
+
+int32_t orc_plastic_strain(const orc_problem* pb, double* w, double* inc) {
+  if (!pb->pstr) return ORC_E_INVALID;
+  if (w) memcpy(w, pb->pstr, sizeof(double) * pb->P);
+  if (inc) memcpy(inc, pb->pinc, sizeof(double) * pb->P);
+  return ORC_OK;
+}
+
+/* StrainWeakening.cxx:432-468: the TimeIntegrand's derivative is the stored
+ * increment, so each step adds dt * increment; then values below 0 become 0. */
+int32_t orc_weakening_step(orc_problem* pb, double dt) {
+  if (!pb->pstr) return ORC_E_INVALID;
+  for (int64_t p = 0; p < pb->P; p++) {
+    pb->pstr[p] += dt * pb->pinc[p];
+    if (pb->pstr[p] < 0.0) pb->pstr[p] = 0.0;
+  }
+  return ORC_OK;
 }
 
 int32_t orc_set_weights(orc_problem* pb, int32_t kind, const int32_t* res) {
@@ -815,7 +878,7 @@ void orc_problem_free(orc_problem* pb) {
   if (!pb) return;
   free_system(pb);
   free(pb->bc_mask); free(pb->bc_value); free(pb->mats); free(pb->coord); free(pb->material);
-  free(pb->eta0p); free(pb->owner); free(pb->xi); free(pb->weight); free(pb->cell_off);
+  free(pb->eta0p); free(pb->pstr); free(pb->pinc); free(pb->sw); free(pb->sw_on); free(pb->owner); free(pb->xi); free(pb->weight); free(pb->cell_off);
   free(pb->cell_list); free(pb->eta); free(pb->yielded);
   free(pb);
 }
@@ -868,12 +931,26 @@ static double sym_invariant(int dim, const double* t) {
   return sqrt(inv);
 }
 
-static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, const double* u_prev,
+static double particle_viscosity(orc_problem* pb, int64_t p, int64_t e, const double* u_prev,
                                  const double* p_prev, int yield_on, uint8_t* yielded) {
   const orc_material* m = &pb->mats[pb->material[p]];
   double eta = pb->eta0p ? pb->eta0p[p] : m->eta0;
   *yielded = 0;
+  const int swon = pb->pstr && pb->sw_on[pb->material[p]];
+  if (pb->pinc) pb->pinc[p] = 0.0;
   if (yield_on && m->yield_type != ORC_YIELD_NONE && u_prev) {
+    /* StrainWeakening_CalcRatio (StrainWeakening.cxx:497-519) clamps the
+     * stored value at 0 and clips the ratio to [0, 1] */
+    double ratio = 0.0, w = 0.0;
+    const orc_strain_weakening* sw = swon ? &pb->sw[pb->material[p]] : NULL;
+    if (sw) {
+      if (pb->pstr[p] < 0.0) pb->pstr[p] = 0.0;
+      w = pb->pstr[p];
+      ratio = (w - sw->initial_softening_strain) / (sw->final_softening_strain - sw->initial_softening_strain);
+      if (ratio > 1.0) ratio = 1.0;
+      if (ratio < 0.0) ratio = 0.0;
+    }
+    const double eta_before = eta;
     const int dim = pb->dim, nen = pb->nen;
     const double* xi = pb->xi + dim * p;
     double GNx[3][MAXNEN];
@@ -914,7 +991,9 @@ static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, co
     double crit;
     if (m->yield_type == ORC_YIELD_DRUCKER_PRAGER) {
       const double pres = p_prev ? p_prev[e] : 0.0;
-      const double mu = m->friction, C = m->cohesion;
+      /* DruckerPrager.cxx:448-489: effective cohesion and friction */
+      const double mu = sw ? m->friction * (1.0 - ratio) + sw->friction_after * ratio : m->friction;
+      const double C = sw ? m->cohesion * (1.0 - ratio) + sw->cohesion_after * ratio : m->cohesion;
       double factor, fs;
       if (dim == 2) {
         factor = 1 / sqrt(1 + mu * mu);
@@ -940,7 +1019,8 @@ static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, co
         *yielded = 1;
       }
     } else {
-      crit = m->cohesion;
+      /* VonMises.cxx:196-199 */
+      crit = sw ? m->cohesion * (1.0 - ratio) + sw->cohesion_after * ratio : m->cohesion;
       if (crit < tauII) {
         const double beta = 1.0 - crit / tauII;
         double corr = -eta * beta;
@@ -948,6 +1028,13 @@ static double particle_viscosity(const orc_problem* pb, int64_t p, int64_t e, co
         eta = eta + corr;
         *yielded = 1;
       }
+    }
+    if (sw) {
+      /* YieldRheology.cxx:190-228 assigns the increment before yielding
+       * changes the viscosity; StrainWeakening.cxx:470-495 */
+      double beta = 1.0 - crit / tauII;
+      if (beta < 0.0) beta = 0.0;
+      pb->pinc[p] = (crit / eta_before) * (beta / (1.0 - beta) - sw->healing_rate * w);
     }
   }
   if (pb->clamp_min > 0 && eta < pb->clamp_min) eta = pb->clamp_min;

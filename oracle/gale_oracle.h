@@ -121,6 +121,18 @@ int32_t orc_search_element(int32_t dim, const int32_t* res, const double* lo, co
 int64_t orc_layout_random(int32_t dim, const int32_t* res, const double* lo, const double* hi,
                           int32_t ppc, long seed, double* coord);
 
+/* --- StrainWeakening (Underworld/Rheology/src/StrainWeakening.cxx) --- */
+typedef struct orc_strain_weakening {
+  double healing_rate, initial_softening_strain, final_softening_strain, cohesion_after, friction_after;
+} orc_strain_weakening;
+/* Attach (sw != NULL) or detach strain weakening on a material with a yield
+ * rheology; the first attach allocates the plastic strain w and its rate (0). */
+int32_t orc_set_strain_weakening(orc_problem* pb, int32_t material, const orc_strain_weakening* sw);
+int32_t orc_set_plastic_strain(orc_problem* pb, const double* w);         /* P, caller order */
+int32_t orc_plastic_strain(const orc_problem* pb, double* w, double* inc); /* P each, caller order */
+/* The TimeIntegrand step of StrainWeakening and MakeValuesPositive. */
+int32_t orc_weakening_step(orc_problem* pb, double dt);
+
 /* --- Sobol sequence (StgDomain/Utils/src/SobolGenerator.cxx) --- */
 /* One generator: primitive polynomial of degree `degree` whose inner
  * coefficients are the bits of `coefficient`, initial direction numbers m[0..degree). */

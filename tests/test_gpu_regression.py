@@ -58,7 +58,11 @@ def test_reference_regression_problem_matches_oracle(gale, oracle, case):
         o.set_weights(1, (10, 10, 10))
     _, gown, gxi, gw = pb.particles()
     oown, oxi, ow, _ = o.particles()
-    assert np.array_equal(gown, oown) and np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
+    assert np.array_equal(gown, oown)
+    if case in PIC:   # DVC centroids and volumes: integer sub-cell work, bit-exact
+        assert np.array_equal(gxi, oxi) and np.array_equal(gw, ow)
+    else:             # affine inverse map against the oracle's Newton iteration
+        assert np.allclose(gxi, oxi, rtol=0, atol=1e-13) and np.array_equal(gw, ow)
     s = gale.Stokes_SLE_UzawaSolver(pb, tight(gale))
     u, p, rep = s.Solve()
     st, ou, op, _ = o.uzawa(otight(oracle))
