@@ -39,7 +39,7 @@ EXPORTS = (
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_wall_bc",
     "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
-    "gs_h5_read_swarm", "gs_shape_layout", "gs_gaussian_field",
+    "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time", "gs_shape_layout", "gs_gaussian_field",
     "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
     "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
     "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
@@ -175,6 +175,9 @@ def lib():
             "gs_h5_attribute_i64": (i32, [C.c_char_p, C.c_char_p, vp, i32]),
             "gs_h5_read_field": (i64, [C.c_char_p, i32, i32, i64, vp]),
             "gs_h5_read_swarm": (i64, [C.c_char_p, C.c_char_p, i32, i64, vp, vp]),
+            "gs_h5_write_field": (i32, [C.c_char_p, i32, i32, i64, vp, vp, vp]),
+            "gs_h5_write_swarm": (i32, [C.c_char_p, i32, i64, vp, vp]),
+            "gs_h5_write_time": (i32, [C.c_char_p, d, d, i32]),
             "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
             "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
             "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
@@ -355,6 +358,29 @@ def read_swarm(path, swarm, dim):
     mat = np.zeros(P, np.int32)
     _h5(lib().gs_h5_read_swarm(os.fsencode(path), swarm.encode(), dim, P, _ptr(coord), _ptr(mat)), "read_swarm")
     return coord, mat
+
+
+def write_field(path, dim, values, coords=None, mesh_res=None):
+    """FeVariable_SaveToFile, checkpoint version 2: values (n_nodes, dofs)."""
+    v = np.ascontiguousarray(values, np.float64)
+    v = v.reshape(v.shape[0], -1)
+    c = None if coords is None else np.ascontiguousarray(coords, np.float64).reshape(-1, 3)
+    r = None if mesh_res is None else np.ascontiguousarray(list(mesh_res)[:dim], np.int32)
+    _h5(lib().gs_h5_write_field(os.fsencode(path), dim, v.shape[1], v.shape[0], _ptr(v),
+                                None if c is None else _ptr(c), None if r is None else _ptr(r)), "write_field")
+
+
+def write_swarm(path, dim, coord, material):
+    """SwarmDump of a material swarm: coordinates (P, 3) and material indices (P,)."""
+    c = np.ascontiguousarray(coord, np.float64).reshape(-1, 3)
+    m = np.ascontiguousarray(material, np.int32).reshape(-1)
+    if len(m) != len(c):
+        raise GaleError(E_INVALID, "write_swarm sizes")
+    _h5(lib().gs_h5_write_swarm(os.fsencode(path), dim, len(c), _ptr(c), _ptr(m)), "write_swarm")
+
+
+def write_time(path, dt, time, nproc=1):
+    _h5(lib().gs_h5_write_time(os.fsencode(path), float(dt), float(time), int(nproc)), "write_time")
 
 
 def num_nodes(dim, res) -> int:

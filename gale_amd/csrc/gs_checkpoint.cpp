@@ -6,6 +6,8 @@
 // Anything outside that subset (chunked or filtered storage, version-2 object
 // headers, other datatypes) is refused with GS_H5_UNSUPPORTED, never guessed.
 // Every offset read from the file is bounds-checked against its size.
+#include <algorithm>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -447,4 +449,290 @@ extern "C" int64_t gs_h5_read_swarm(const char* path, const char* swarm, int32_t
     if (material) material[p] = (int32_t)as_int(f, dm, p);
   }
   return P;
+}
+
+// ---------------------------------------------------------------------------
+// Writing: files in the same subset (superblock 0, version-1 object headers,
+// one symbol-table root group holding contiguous datasets, root attributes),
+// laid out as FeVariable_SaveToFile (checkpoint version 2), SwarmDump and the
+// timeInfo writer name and shape their contents.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr uint64_t kUndef = ~0ull;
+constexpr int kLeafK = 4, kInternalK = 16;  // the library defaults the reference's files carry
+
+struct Buf {
+  std::vector<uint8_t> b;
+  uint64_t at() const { return b.size(); }
+  void u(uint64_t v, int n) {
+    for (int i = 0; i < n; i++) b.push_back((uint8_t)(v >> (8 * i)));
+  }
+  void bytes(const void* p, size_t n) {
+    const uint8_t* c = (const uint8_t*)p;
+    b.insert(b.end(), c, c + n);
+  }
+  void pad8() {
+    while (b.size() % 8) b.push_back(0);
+  }
+  void zeros(size_t n) { b.insert(b.end(), n, 0); }
+  void put(uint64_t off, uint64_t v, int n) {
+    for (int i = 0; i < n; i++) b[off + i] = (uint8_t)(v >> (8 * i));
+  }
+};
+
+// Datatype message bodies: little-endian IEEE double, native int32, and the
+// big-endian int32 the reference uses for its attributes.
+std::vector<uint8_t> type_f64() {
+  Buf t;
+  t.u(0x11, 1);                  // version 1, class 1 (floating point)
+  t.u(0x20, 1);                  // little endian, implied mantissa msb
+  t.u(63, 1);                    // sign bit position
+  t.u(0, 1);
+  t.u(8, 4);                     // size
+  t.u(0, 2);                     // bit offset
+  t.u(64, 2);                    // precision
+  t.u(52, 1);                    // exponent location
+  t.u(11, 1);                    // exponent size
+  t.u(0, 1);                     // mantissa location
+  t.u(52, 1);                    // mantissa size
+  t.u(1023, 4);                  // exponent bias
+  return t.b;
+}
+std::vector<uint8_t> type_i32(bool big_endian) {
+  Buf t;
+  t.u(0x10, 1);                  // version 1, class 0 (fixed point)
+  t.u(big_endian ? 0x09 : 0x08, 1);  // byte order, signed
+  t.u(0, 2);
+  t.u(4, 4);
+  t.u(0, 2);                     // bit offset
+  t.u(32, 2);                    // precision
+  return t.b;
+}
+std::vector<uint8_t> space_v1(const std::vector<uint64_t>& dims) {
+  Buf s;
+  s.u(1, 1);
+  s.u(dims.size(), 1);
+  s.u(0, 1);                     // no maximum dimensions
+  s.zeros(5);
+  for (uint64_t d : dims) s.u(d, 8);
+  return s.b;
+}
+
+struct Item {
+  std::string name;
+  std::vector<uint64_t> dims;
+  bool is_int = false, big_endian = false;
+  const void* data = nullptr;    // row-major, native layout
+  uint64_t nbytes = 0;
+};
+
+uint64_t count(const std::vector<uint64_t>& dims) {
+  uint64_t n = 1;
+  for (uint64_t d : dims) n *= d;
+  return n;
+}
+
+// One object-header message of type `type` with body `body`, padded to 8.
+void message(Buf& h, int type, const std::vector<uint8_t>& body) {
+  const uint64_t n = (body.size() + 7) / 8 * 8;
+  h.u(type, 2);
+  h.u(n, 2);
+  h.u(0, 1);
+  h.zeros(3);
+  h.bytes(body.data(), body.size());
+  h.zeros(n - body.size());
+}
+
+std::vector<uint8_t> attribute_body(const Item& a) {
+  Buf m;
+  const std::vector<uint8_t> t = a.is_int ? type_i32(a.big_endian) : type_f64();
+  const std::vector<uint8_t> s = space_v1(a.dims);
+  m.u(1, 1);
+  m.u(0, 1);
+  m.u(a.name.size() + 1, 2);
+  m.u(t.size(), 2);
+  m.u(s.size(), 2);
+  m.bytes(a.name.c_str(), a.name.size() + 1);
+  m.pad8();
+  m.bytes(t.data(), t.size());
+  m.pad8();
+  m.bytes(s.data(), s.size());
+  m.pad8();
+  const uint8_t* p = (const uint8_t*)a.data;
+  const int elem = a.is_int ? 4 : 8;
+  for (uint64_t k = 0; k < count(a.dims); k++)
+    for (int i = 0; i < elem; i++) m.u(p[k * elem + (a.big_endian ? elem - 1 - i : i)], 1);
+  return m.b;
+}
+
+void object_header(Buf& f, const std::vector<std::pair<int, std::vector<uint8_t>>>& msgs) {
+  Buf h;
+  for (auto& m : msgs) message(h, m.first, m.second);
+  f.u(1, 1);            // version
+  f.u(0, 1);
+  f.u(msgs.size(), 2);
+  f.u(1, 4);            // reference count
+  f.u(h.b.size(), 4);   // header data size
+  f.zeros(4);
+  f.bytes(h.b.data(), h.b.size());
+}
+
+int write_file(const char* path, std::vector<Item> sets, const std::vector<Item>& attrs) {
+  if (!path) return GS_H5_INVALID;
+  // symbol-table entries are kept sorted by name (the B-tree key order)
+  std::sort(sets.begin(), sets.end(), [](const Item& a, const Item& b) { return a.name < b.name; });
+  if ((int)sets.size() > 2 * kLeafK) return GS_H5_UNSUPPORTED;
+  Buf f;
+  f.zeros(96);                                  // superblock, filled in last
+  // local heap: the empty name at 0, then each dataset name padded to 8
+  std::vector<uint64_t> name_off;
+  Buf heap_data;
+  heap_data.zeros(8);
+  for (auto& s : sets) {
+    name_off.push_back(heap_data.at());
+    heap_data.bytes(s.name.c_str(), s.name.size() + 1);
+    heap_data.pad8();
+  }
+  // root object header: symbol table message + attributes
+  const uint64_t root = f.at();
+  std::vector<std::pair<int, std::vector<uint8_t>>> rmsgs;
+  rmsgs.push_back({0x11, std::vector<uint8_t>(16, 0)});
+  for (auto& a : attrs) rmsgs.push_back({0x0C, attribute_body(a)});
+  object_header(f, rmsgs);
+  const uint64_t stab_body = root + 16 + 8;     // first message's body
+  // B-tree (group node, level 0, one child), sized for 2K entries
+  const uint64_t tree = f.at();
+  f.bytes("TREE", 4);
+  f.u(0, 1);
+  f.u(0, 1);
+  f.u(sets.empty() ? 0 : 1, 2);
+  f.u(kUndef, 8);
+  f.u(kUndef, 8);
+  const uint64_t keys = f.at();
+  f.zeros((2 * kInternalK + 1) * 8 + 2 * kInternalK * 8);
+  // local heap header and data segment
+  const uint64_t heap = f.at();
+  f.bytes("HEAP", 4);
+  f.zeros(4);
+  f.u(heap_data.at(), 8);
+  f.u(1, 8);                                    // no free block
+  f.u(heap + 32, 8);
+  f.bytes(heap_data.b.data(), heap_data.at());
+  // dataset object headers
+  std::vector<uint64_t> ds_hdr, layout_at;
+  for (auto& s : sets) {
+    ds_hdr.push_back(f.at());
+    std::vector<uint8_t> lay(18, 0);
+    lay[0] = 3;
+    lay[1] = 1;                                 // contiguous; address and size patched below
+    object_header(f, {{0x0001, space_v1(s.dims)}, {0x0003, s.is_int ? type_i32(false) : type_f64()}, {0x0008, lay}});
+    layout_at.push_back(f.at() - 24 + 2);       // the layout message body is the last 24 bytes
+  }
+  // symbol-table node sized for 2 * leaf K entries
+  const uint64_t snod = f.at();
+  f.bytes("SNOD", 4);
+  f.u(1, 1);
+  f.u(0, 1);
+  f.u(sets.size(), 2);
+  for (size_t i = 0; i < sets.size(); i++) {
+    f.u(name_off[i], 8);
+    f.u(ds_hdr[i], 8);
+    f.u(0, 4);
+    f.zeros(4 + 16);
+  }
+  f.zeros((2 * kLeafK - sets.size()) * 40);
+  // raw data
+  for (size_t i = 0; i < sets.size(); i++) {
+    f.pad8();
+    f.put(layout_at[i], f.at(), 8);
+    f.put(layout_at[i] + 8, sets[i].nbytes, 8);
+    f.bytes(sets[i].data, sets[i].nbytes);
+  }
+  // patch B-tree keys/child, the symbol table message and the superblock
+  if (!sets.empty()) {
+    f.put(keys, 0, 8);
+    f.put(keys + 8, snod, 8);
+    f.put(keys + 16, name_off.back(), 8);
+  }
+  f.put(stab_body, tree, 8);
+  f.put(stab_body + 8, heap, 8);
+  Buf sb;
+  const uint8_t sig[8] = {0x89, 'H', 'D', 'F', '\r', '\n', 0x1a, '\n'};
+  sb.bytes(sig, 8);
+  sb.u(0, 1);   // superblock version
+  sb.u(0, 1);   // free-space version
+  sb.u(0, 1);   // root group symbol table entry version
+  sb.u(0, 1);
+  sb.u(0, 1);   // shared header message format version
+  sb.u(8, 1);   // size of offsets
+  sb.u(8, 1);   // size of lengths
+  sb.u(0, 1);
+  sb.u(kLeafK, 2);
+  sb.u(kInternalK, 2);
+  sb.u(0, 4);   // file consistency flags
+  sb.u(0, 8);   // base address
+  sb.u(kUndef, 8);
+  sb.u(f.at(), 8);  // end of file
+  sb.u(kUndef, 8);
+  sb.u(0, 8);   // root entry: link name offset
+  sb.u(root, 8);
+  sb.u(1, 4);   // cache type 1: scratch holds the B-tree and heap addresses
+  sb.u(0, 4);
+  sb.u(tree, 8);
+  sb.u(heap, 8);
+  std::memcpy(f.b.data(), sb.b.data(), 96);
+  FILE* fp = std::fopen(path, "wb");
+  if (!fp) return GS_H5_IO;
+  const size_t put = std::fwrite(f.b.data(), 1, f.b.size(), fp);
+  const int closed = std::fclose(fp);
+  return (put == f.b.size() && closed == 0) ? 0 : GS_H5_IO;
+}
+
+}  // namespace
+
+extern "C" int32_t gs_h5_write_field(const char* path, int32_t dim, int32_t dofs, int64_t n_nodes,
+                                     const double* values, const double* coords, const int32_t* mesh_res) {
+  if ((dim != 2 && dim != 3) || dofs < 1 || n_nodes < 0 || (n_nodes > 0 && !values)) return GS_H5_INVALID;
+  // FeVariable.cxx:1938-2130, checkpoint version 2: rows of [coordinates,] components
+  const int cols = dofs + (coords ? dim : 0);
+  std::vector<double> rows((size_t)n_nodes * cols);
+  for (int64_t i = 0; i < n_nodes; i++) {
+    int c = 0;
+    if (coords)
+      for (int d = 0; d < dim; d++) rows[i * cols + c++] = coords[3 * i + d];
+    for (int k = 0; k < dofs; k++) rows[i * cols + c++] = values[i * dofs + k];
+  }
+  const int32_t ver = 2, dims = dim;
+  std::vector<Item> attrs;
+  attrs.push_back({"checkpoint file version", {1}, true, true, &ver, 4});
+  attrs.push_back({"dimensions", {1}, true, true, &dims, 4});
+  if (mesh_res) attrs.push_back({"mesh resolution", {(uint64_t)dim}, true, true, mesh_res, 4ull * dim});
+  return write_file(path, {{"data", {(uint64_t)n_nodes, (uint64_t)cols}, false, false, rows.data(),
+                            rows.size() * sizeof(double)}},
+                    attrs);
+}
+
+extern "C" int32_t gs_h5_write_swarm(const char* path, int32_t dim, int64_t P, const double* coord,
+                                     const int32_t* material) {
+  if ((dim != 2 && dim != 3) || P < 0 || P > INT32_MAX || (P > 0 && (!coord || !material))) return GS_H5_INVALID;
+  // SwarmDump.cxx:213-330: "/Position" (P x dim), "/MaterialIndex" (P x 1),
+  // and the particle count as a root attribute
+  std::vector<double> pos((size_t)P * dim);
+  for (int64_t p = 0; p < P; p++)
+    for (int d = 0; d < dim; d++) pos[p * dim + d] = coord[3 * p + d];
+  const int32_t n = (int32_t)P;
+  return write_file(path,
+                    {{"Position", {(uint64_t)P, (uint64_t)dim}, false, false, pos.data(), pos.size() * sizeof(double)},
+                     {"MaterialIndex", {(uint64_t)P, 1}, true, false, material, (uint64_t)P * 4}},
+                    {{"Swarm Particle Count", {1}, true, true, &n, 4}});
+}
+
+extern "C" int32_t gs_h5_write_time(const char* path, double dt, double time, int32_t nproc) {
+  // the timeInfo file FileParticleLayout.cxx:565-597 reads back: Dt, currentTime, nproc
+  return write_file(path,
+                    {{"Dt", {1}, false, false, &dt, 8},
+                     {"currentTime", {1}, false, false, &time, 8},
+                     {"nproc", {1}, true, false, &nproc, 4}},
+                    {});
 }

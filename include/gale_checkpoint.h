@@ -8,8 +8,8 @@
  * version-1 object headers, a symbol-table root group, and contiguous or
  * compact datasets of signed 32/64-bit integers or 32/64-bit floats in either
  * byte order.  Other files (chunked or filtered datasets, version-2 object
- * headers, other types) return GS_H5_UNSUPPORTED.  Each call opens, reads and
- * closes the file; nothing is kept between calls.
+ * headers, other types) return GS_H5_UNSUPPORTED.  Each call opens, reads (or
+ * writes) and closes the file; nothing is kept between calls.
  */
 #ifndef GALE_CHECKPOINT_H
 #define GALE_CHECKPOINT_H
@@ -63,6 +63,27 @@ int64_t gs_h5_read_field(const char* path, int32_t dim, int32_t dofs, int64_t n_
  * both NULL returns P only.  Returns P or a GS_H5_* error. */
 int64_t gs_h5_read_swarm(const char* path, const char* swarm, int32_t dim, int64_t capacity, double* coord,
                          int32_t* material);
+
+/* Writers of the same subset: one symbol-table root group of contiguous
+ * datasets plus root attributes (at most 8 datasets per file).  Returns 0 or
+ * a GS_H5_* error.  The files are read back by the functions above. */
+
+/* FeVariable_SaveToFile (StgFEM/Discretisation/src/FeVariable.cxx:1938-2130),
+ * checkpoint version 2: "/data" of n_nodes rows, each the node's coordinates
+ * (coords P*3 as elsewhere in this library, or NULL for none) then its `dofs`
+ * components (values n_nodes*dofs); root attributes "checkpoint file
+ * version" = 2, "dimensions" and, if mesh_res is given, "mesh resolution". */
+int32_t gs_h5_write_field(const char* path, int32_t dim, int32_t dofs, int64_t n_nodes, const double* values,
+                          const double* coords, const int32_t* mesh_res);
+
+/* SwarmDump (StgDomain/Swarm/src/SwarmDump.cxx:213-330) of a material swarm:
+ * "/Position" (P x dim doubles from coord, P*3), "/MaterialIndex" (P x 1
+ * int32) and the root attribute "Swarm Particle Count". */
+int32_t gs_h5_write_swarm(const char* path, int32_t dim, int64_t P, const double* coord, const int32_t* material);
+
+/* The timeInfo file (read by FileParticleLayout.cxx:565-597 on restart):
+ * "/Dt", "/currentTime" (doubles) and "/nproc" (int32). */
+int32_t gs_h5_write_time(const char* path, double dt, double time, int32_t nproc);
 
 #ifdef __cplusplus
 }

@@ -136,3 +136,68 @@ def test_oracle_reproduces_reference_regression_fields(oracle, case, tol_u, tol_
         assert errs["constant"][0] > 5e-3
     else:   # Gauss points with their unit weights
         assert errs["constant"][0] < tol_u and errs["constant"][1] < tol_p
+
+
+def _same_content(a, b):
+    """Every root dataset and attribute of HDF5 files a and b agree in name,
+    shape, element type and value (read with the oracle's reader)."""
+    from oracle.h5 import H5File
+    ha, hb = H5File(a), H5File(b)
+    assert sorted(ha.names()) == sorted(hb.names())
+    for n in ha.names():
+        x, y = ha.dataset(n), hb.dataset(n)
+        assert x.shape == y.shape and x.dtype == y.dtype and np.array_equal(x, y), n
+    aa, ab = ha.attributes(), hb.attributes()
+    assert sorted(aa) == sorted(ab)
+    for n in aa:
+        assert aa[n].dtype == ab[n].dtype and np.array_equal(aa[n], ab[n]), n
+
+
+def test_writer_reproduces_reference_checkpoint_contents(tmp_path):
+    """Rewriting the reference's version-2 checkpoints from their own values
+    gives files with the same datasets, attributes, types and values."""
+    import gale_amd
+    src = os.path.join(GOLDEN, "lid_driven_pic_step10")
+    u = gale_amd.read_field(os.path.join(src, "VelocityField.00010.h5"), 2, 2, 441)
+    gale_amd.write_field(str(tmp_path / "v.h5"), 2, u, mesh_res=(20, 20))
+    _same_content(str(tmp_path / "v.h5"), os.path.join(src, "VelocityField.00010.h5"))
+    p = gale_amd.read_field(os.path.join(src, "PressureField.00010.h5"), 2, 1, 400)
+    gale_amd.write_field(str(tmp_path / "p.h5"), 2, p)
+    _same_content(str(tmp_path / "p.h5"), os.path.join(src, "PressureField.00010.h5"))
+    coord, mat = gale_amd.read_swarm(os.path.join(src, "materialSwarm.00010.h5"), "materialSwarm", 2)
+    gale_amd.write_swarm(str(tmp_path / "s.h5"), 2, coord, mat)
+    _same_content(str(tmp_path / "s.h5"), os.path.join(src, "materialSwarm.00010.h5"))
+    t = os.path.join(src, "timeInfo.00010.h5")
+    gale_amd.write_time(str(tmp_path / "t.h5"), gale_amd.h5_read(t, "Dt")[0], gale_amd.h5_read(t, "currentTime")[0],
+                        int(gale_amd.h5_read(t, "nproc")[0]))
+    _same_content(str(tmp_path / "t.h5"), t)
+
+
+@pytest.mark.parametrize("dim,dofs,coords", [(2, 2, False), (3, 3, True), (3, 1, False), (2, 4, True)])
+def test_writer_round_trip(tmp_path, dim, dofs, coords):
+    import gale_amd
+    from oracle.h5 import H5File
+    rng = np.random.default_rng(dim * 10 + dofs)
+    n = 97
+    v = rng.normal(size=(n, dofs))
+    c = rng.normal(size=(n, 3)) if coords else None
+    if c is not None and dim == 2:
+        c[:, 2] = 0.0
+    f = str(tmp_path / "f.h5")
+    gale_amd.write_field(f, dim, v, coords=c, mesh_res=(4, 5, 6)[:dim])
+    assert np.array_equal(gale_amd.read_field(f, dim, dofs, n), v)
+    raw = H5File(f).dataset("data")
+    assert raw.shape == (n, dofs + (dim if coords else 0))
+    if coords:
+        assert np.array_equal(raw[:, :dim], c[:, :dim])
+    assert list(gale_amd.h5_attribute(f, "mesh resolution")) == [4, 5, 6][:dim]
+    P = 1234
+    xc = np.zeros((P, 3))
+    xc[:, :dim] = rng.uniform(size=(P, dim))
+    m = rng.integers(0, 7, P).astype(np.int32)
+    s = str(tmp_path / "s.h5")
+    gale_amd.write_swarm(s, dim, xc, m)
+    x2, m2 = gale_amd.read_swarm(s, "materialSwarm", dim)
+    assert np.array_equal(x2, xc) and np.array_equal(m2, m)
+    gale_amd.write_swarm(s, dim, xc[:0], m[:0])          # an empty swarm is a valid file
+    assert gale_amd.read_swarm(s, "materialSwarm", dim)[0].shape == (0, 3)

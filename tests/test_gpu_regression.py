@@ -70,3 +70,36 @@ def test_reference_regression_problem_matches_oracle(gale, oracle, case):
     assert rel(u, ou) < 1e-8 and rel(p - p.mean(), op - op.mean()) < 1e-8
     s.close()
     pb.close()
+
+
+def test_checkpoint_restart_round_trip(gale, tmp_path):
+    """Solve, checkpoint (fields, swarm, time) with the library's writers,
+    read everything back into a new problem and solve again: the restarted
+    solve equals the original bit for bit."""
+    c = CASES["rayleigh_taylor"]
+    d = load("rayleigh_taylor", gale)
+    pb = _problem(gale, "rayleigh_taylor", d)
+    s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(tolerance=1e-5, maxIterations=5000))
+    u, p, rep = s.Solve()
+    assert rep.status == 0
+    coord = pb.particles()[0]
+    N, E = u.size // 2, p.size
+    gale.write_field(str(tmp_path / "VelocityField.h5"), 2, u.reshape(-1, 2), mesh_res=c["res"])
+    gale.write_field(str(tmp_path / "PressureField.h5"), 2, p.reshape(-1, 1))
+    gale.write_swarm(str(tmp_path / "materialSwarm.h5"), 2, coord, d["material"])
+    gale.write_time(str(tmp_path / "timeInfo.h5"), d["dt"], 93.8795, 1)
+    s.close()
+    pb.close()
+    coord2, mat2 = gale.read_swarm(str(tmp_path / "materialSwarm.h5"), "materialSwarm", 2)
+    assert np.array_equal(coord2, coord) and np.array_equal(mat2, d["material"])
+    u2 = gale.read_field(str(tmp_path / "VelocityField.h5"), 2, 2, N)
+    p2 = gale.read_field(str(tmp_path / "PressureField.h5"), 2, 1, E)[:, 0]
+    assert np.array_equal(u2.ravel(), u) and np.array_equal(p2, p)
+    assert gale.h5_read(str(tmp_path / "timeInfo.h5"), "Dt")[0] == d["dt"]
+    d2 = dict(d, coord=coord2, material=mat2)
+    pb2 = _problem(gale, "rayleigh_taylor", d2)
+    s2 = gale.Stokes_SLE_UzawaSolver(pb2, gale.uzawa_params(tolerance=1e-5, maxIterations=5000))
+    u3, p3, rep3 = s2.Solve()
+    assert rep3.status == 0 and np.array_equal(u3, u) and np.array_equal(p3, p)
+    s2.close()
+    pb2.close()
