@@ -230,6 +230,13 @@ def main():
     nbytes = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)
     ms32 = s.time_apply_K(G.APPLY_PARTICLES_FP32, args.apply_reps)
     nbytes32 = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P, b=4)
+    # the pressure operators of every Uzawa iteration: G p reads p (8 B per
+    # element) and the node flags (1 B per node) and writes dim values per node;
+    # G^T u reads dim values per node and writes one per element
+    ms_g = s.time_apply_K(G.APPLY_G, args.apply_reps)
+    bytes_g = 8 * cfg.E + 8 * cfg.dim * cfg.N + cfg.N
+    ms_gt = s.time_apply_K(G.APPLY_GT, args.apply_reps)
+    bytes_gt = 8 * cfg.dim * cfg.N + 8 * cfg.E
     achieved = bytes_m / (ms_m * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_apply_traffic.json")
@@ -367,6 +374,12 @@ def main():
                                     "kernel": "gs::k_tile_apply3<double,1> + gs::k_tile_halo3<double> (particle-form K apply)",
                                     "ms_per_apply": ms, "algorithmic_bytes": nbytes,
                                     "fp32_ms_per_apply": ms32, "fp32_algorithmic_bytes": nbytes32},
+        "roofline_G": {"bound": "hbm", "achieved": bytes_g / (ms_g * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": bytes_g / (ms_g * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "gs::k_apply_G (G p)",
+                       "ms_per_apply": ms_g, "algorithmic_bytes": bytes_g},
+        "roofline_Gt": {"bound": "hbm", "achieved": bytes_gt / (ms_gt * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": bytes_gt / (ms_gt * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "kernel": "gs::k_apply_Gt (G^T u)", "ms_per_apply": ms_gt, "algorithmic_bytes": bytes_gt},
         "cpu_baseline": cpu,
         "batch": batch,
         "dvc_weights": dvc,

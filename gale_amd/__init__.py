@@ -24,6 +24,7 @@ PC_NONE, PC_DIAGONAL, PC_LEARNED = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 INNER_JACOBI, INNER_MULTIGRID = 0, 1
 APPLY_PARTICLES_FP64, APPLY_PARTICLES_FP32, APPLY_MOMENTS_FP64, APPLY_MOMENTS_FP32 = 0, 1, 2, 3
+APPLY_G, APPLY_GT = 4, 5
 SHAPE_EVERYWHERE, SHAPE_SPHERE, SHAPE_BOX, SHAPE_BELOW_SINES = 0, 1, 2, 3
 WALLS = {"left": 0, "right": 1, "bottom": 2, "top": 3, "back": 4, "front": 5}
 

@@ -1856,12 +1856,16 @@ extern "C" int32_t gs_apply_K_moments(gs_solver* s, const double* x, double* y) 
 }
 
 extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, double* ms_per_apply) {
-  if (!s || n_rep <= 0 || !ms_per_apply || form < 0 || form > 3) return GS_E_INVALID;
+  if (!s || n_rep <= 0 || !ms_per_apply || form < 0 || form > GS_APPLY_GT) return GS_E_INVALID;
   gs_problem* pb = s->pb;
   hipStream_t st = pb->st;
   if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
-  if (form >= 2 && !s->mg.structured) CKS(mg_numeric(s));
-  if (form == 1 || form == 3) {
+  if ((form == 2 || form == 3) && !s->mg.structured) CKS(mg_numeric(s));
+  if (form == GS_APPLY_G) {
+    launch_fill(s->tmpE, pb->g.E, 1.0, st);
+  } else if (form == GS_APPLY_GT) {
+    launch_fill(s->cg_p, s->n, 1.0, st);
+  } else if (form == 1 || form == 3) {
     if (form == 1) CKS(ensure_pk32(s));
     if (!s->x32) CK(dalloc(&s->x32, s->n));
     if (!s->y32) CK(dalloc(&s->y32, s->n));
@@ -1874,7 +1878,9 @@ extern "C" int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, do
       case 0: launch_apply_K(pb->g, s->cg_p, s->cg_Ap, pb->pk, pb->off, nullptr, pb->halo64, st); break;
       case 1: launch_apply_K32(pb->g, s->x32, s->y32, pb->pk32, pb->off, nullptr, pb->halo32, st); break;
       case 2: launch_mom_apply<double>(s->mg.lv[0].g, s->cg_p, s->cg_Ap, s->mg.lv[0].M64, nullptr, s->mg.lv[0].h64, st); break;
-      default: launch_mom_apply<float>(s->mg.lv[0].g, s->x32, s->y32, s->mg.lv[0].M32, nullptr, s->mg.lv[0].h32, st); break;
+      case 3: launch_mom_apply<float>(s->mg.lv[0].g, s->x32, s->y32, s->mg.lv[0].M32, nullptr, s->mg.lv[0].h32, st); break;
+      case GS_APPLY_G: launch_apply_G(pb->g, s->tmpE, s->cg_Ap, pb->bcflag, st); break;
+      default: launch_apply_Gt(pb->g, s->cg_p, s->tmpE, st); break;
     }
   };
   one();

@@ -40,8 +40,10 @@ enum { GS_PREC_FP64 = 0, GS_PREC_MIXED = 1 };
 /* Inner velocity preconditioner: Jacobi, or a geometric multigrid V-cycle whose
  * levels are the Galerkin coarse operators of the particle-integrated K. */
 enum { GS_INNER_JACOBI = 0, GS_INNER_MULTIGRID = 1 };
-/* Operator forms timed by gs_time_apply_K. */
-enum { GS_APPLY_PARTICLES_FP64 = 0, GS_APPLY_PARTICLES_FP32 = 1, GS_APPLY_MOMENTS_FP64 = 2, GS_APPLY_MOMENTS_FP32 = 3 };
+/* Operator forms timed by gs_time_apply_K (G p and G^T u included, so every
+ * operator of the Uzawa loop can be measured the same way). */
+enum { GS_APPLY_PARTICLES_FP64 = 0, GS_APPLY_PARTICLES_FP32 = 1, GS_APPLY_MOMENTS_FP64 = 2, GS_APPLY_MOMENTS_FP32 = 3,
+       GS_APPLY_G = 4, GS_APPLY_GT = 5 };
 
 /* One RheologyMaterial: MaterialViscosity (+ optional yield rheology) and
  * the density used by BuoyancyForceTerm.  XML keys in comments. */
@@ -241,8 +243,8 @@ int32_t gs_apply_K_fp32(gs_solver* s, const float* x_dev, float* y_dev);
 int32_t gs_apply_G(gs_solver* s, const double* p_dev, double* y_dev);
 int32_t gs_apply_Gt(gs_solver* s, const double* x_dev, double* q_dev);
 int32_t gs_solver_sync(gs_solver* s);
-/* Average device time (ms) of n_rep K applies of the given form
- * (GS_APPLY_*), timed with HIP events on the solver's stream. */
+/* Average device time (ms) of n_rep applies of the given form (GS_APPLY_*:
+ * K in four forms, G p, G^T u), timed with HIP events on the solver's stream. */
 int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, double* ms_per_apply);
 /* K apply from the per-element particle moments (the form the solver uses;
  * equal to gs_apply_K up to rounding).  Device pointers, full nodal vectors. */
