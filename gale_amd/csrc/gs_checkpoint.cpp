@@ -204,6 +204,7 @@ int dataset(const File& f, const char* name, Dataset& d) {
     } else if (m.type == 0x000B) {
       return GS_H5_UNSUPPORTED;  // filter pipeline
     } else if (m.type == 0x0008) {
+      if (m.size < 1) return GS_H5_FORMAT;
       const int ver = f.b[m.off];
       if (ver == 1 || ver == 2) {
         if (m.size < 8) return GS_H5_FORMAT;
@@ -223,14 +224,17 @@ int dataset(const File& f, const char* name, Dataset& d) {
           return GS_H5_UNSUPPORTED;
         }
       } else if (ver == 3) {
+        if (m.size < 2) return GS_H5_FORMAT;
         const int cls = f.b[m.off + 1];
         if (cls == 1) {
           if (m.size < 18) return GS_H5_FORMAT;
           d.addr = f.u(m.off + 2, 8);
           d.bytes = f.u(m.off + 10, 8);
         } else if (cls == 0) {
+          if (m.size < 4) return GS_H5_FORMAT;
           d.bytes = f.u(m.off + 2, 2);
           d.addr = m.off + 4;
+          if (d.bytes > m.size - 4) return GS_H5_FORMAT;
         } else {
           return GS_H5_UNSUPPORTED;
         }

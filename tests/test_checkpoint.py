@@ -201,3 +201,26 @@ def test_writer_round_trip(tmp_path, dim, dofs, coords):
     assert np.array_equal(x2, xc) and np.array_equal(m2, m)
     gale_amd.write_swarm(s, dim, xc[:0], m[:0])          # an empty swarm is a valid file
     assert gale_amd.read_swarm(s, "materialSwarm", dim)[0].shape == (0, 3)
+
+
+def test_reader_survives_corrupted_metadata(tmp_path):
+    """Random byte changes in the metadata of a checkpoint end in a clean
+    error or a successful read, never a crash or a read outside the file."""
+    import gale_amd
+    src = open(os.path.join(GOLDEN, "lid_driven_pic_step10", "VelocityField.00010.h5"), "rb").read()
+    meta = 2208                                  # bytes before the raw data
+    rng = np.random.default_rng(0)
+    t = tmp_path / "c.h5"
+    outcomes = {"ok": 0, "error": 0}
+    for trial in range(400):
+        b = bytearray(src)
+        for _ in range(rng.integers(1, 6)):
+            b[rng.integers(0, meta)] = rng.integers(0, 256)
+        t.write_bytes(bytes(b))
+        try:
+            gale_amd.read_field(str(t), 2, 2, 441)
+            gale_amd.h5_attribute(str(t), "mesh resolution")
+            outcomes["ok"] += 1
+        except gale_amd.GaleError:
+            outcomes["error"] += 1
+    assert outcomes["error"] > 0 and sum(outcomes.values()) == 400
