@@ -31,6 +31,11 @@
  *     GetNumberByIndex consistency check, and the (0,1)-sequence property
  *     (every dyadic interval of length 2^-k holds one point of each run of
  *     2^k); SpaceFiller layout: testSpaceFillerParticleLayoutOutput.expected.
+ *   - StrainWeakening: with no healing the increment of a yielding point is
+ *     its plastic strain rate 2 e_II - tau_y / eta (linear velocity fields,
+ *     exact strain rates); healing alone gives -(tau_y / eta) h w; a problem
+ *     at or past the final softening strain solves bit-identically to the
+ *     after-softening material (tests/test_weakening_oracle.py).
  * Functions without a check of their own are listed as "parity unpinned" in
  * DESIGN.md.
  */
