@@ -38,7 +38,7 @@ EXPORTS = (
     "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
-    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_wall_bc",
+    "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_crand_stream", "gs_initial_damage", "gs_wall_bc",
     "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
     "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time", "gs_shape_layout", "gs_gaussian_field",
     "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
@@ -169,6 +169,8 @@ def lib():
             "gs_layout_random": (i64, [i32, vp, vp, vp, i32, i64, vp]),
             "gs_layout_gauss": (i64, [i32, vp, vp, vp, vp]),
             "gs_sobol_sequence": (i32, [i32, i64, i64, vp]),
+            "gs_crand_stream": (i32, [i64, i64, vp]),
+            "gs_initial_damage": (i32, [i32, i64, vp, vp, vp, vp]),
             "gs_layout_spacefiller": (i64, [i32, vp, vp, vp, i64, d, i32, vp]),
             "gs_h5_dataset_info": (i32, [C.c_char_p, C.c_char_p, vp, vp, vp]),
             "gs_h5_read_f64": (i64, [C.c_char_p, C.c_char_p, vp, i64]),
@@ -382,6 +384,36 @@ def write_swarm(path, dim, coord, material):
 
 def write_time(path, dt, time, nproc=1):
     _h5(lib().gs_h5_write_time(os.fsencode(path), float(dt), float(time), int(nproc)), "write_time")
+
+
+def crand_stream(seed: int, count: int) -> np.ndarray:
+    """The C library's rand() after srand(seed) (the generator StrainWeakening's initial damage uses)."""
+    out = np.zeros(count, np.int32)
+    if lib().gs_crand_stream(int(seed), int(count), _ptr(out)) != 0:
+        raise GaleError(E_INVALID, "crand_stream")
+    return out
+
+
+class InitialDamageParams(C.Structure):
+    _fields_ = [("fraction", C.c_double), ("factor", C.c_double), ("final_softening_strain", C.c_double),
+                ("wavenumber", C.c_double), ("wavenumber_sin_i", C.c_double), ("wavenumber_cos_i", C.c_double),
+                ("wavenumber_sin_k", C.c_double), ("wavenumber_cos_k", C.c_double), ("seed", C.c_int64)]
+
+
+def initial_damage(dim, coord, initialDamageFraction, finalSofteningStrain, randomSeed=0, initialDamageFactor=1.0,
+                   initialDamageWavenumber=-1.0, initialDamageWavenumberSinI=-1.0, initialDamageWavenumberCosI=-1.0,
+                   initialDamageWavenumberSinK=-1.0, initialDamageWavenumberCosK=-1.0, initialStrainShape=None):
+    """StrainWeakening's initial plastic strain of the particles (in layout
+    order), XML names; feed it to Problem.set_plastic_strain."""
+    c = np.ascontiguousarray(coord, np.float64).reshape(-1, 3)
+    d = InitialDamageParams(initialDamageFraction, initialDamageFactor, finalSofteningStrain, initialDamageWavenumber,
+                            initialDamageWavenumberSinI, initialDamageWavenumberCosI, initialDamageWavenumberSinK,
+                            initialDamageWavenumberCosK, int(randomSeed))
+    w = np.zeros(len(c))
+    shp = None if initialStrainShape is None else C.byref(initialStrainShape)
+    if lib().gs_initial_damage(dim, len(c), _ptr(c), C.byref(d), shp, _ptr(w)) != 0:
+        raise GaleError(E_INVALID, "initial_damage")
+    return w
 
 
 def num_nodes(dim, res) -> int:

@@ -331,6 +331,83 @@ extern "C" int32_t gs_wall_bc(int32_t dim, const int32_t* res, int32_t wall, int
   return 0;
 }
 
+namespace {
+
+// Stg_Shape_IsCoordInside for the shapes of gs_shape; -1 for an unknown type.
+int shape_contains(int dim, const gs_shape& sh, const double* x) {
+  switch (sh.type) {
+    case GS_SHAPE_EVERYWHERE:
+      return 1;
+    case GS_SHAPE_SPHERE: {  // Sphere.cxx:189-208
+      double v = 0.0;
+      for (int d = 0; d < dim; d++) {
+        const double t = x[d] - sh.p[d];
+        v += t * t;
+      }
+      return v <= sh.p[3] * sh.p[3];
+    }
+    case GS_SHAPE_BOX: {  // Box.cxx:209-221: |x - centre| <= width/2 on every axis
+      for (int d = 0; d < dim; d++) {
+        const double c = 0.5 * (sh.p[d] + sh.p[3 + d]);
+        const double w = sh.p[3 + d] - sh.p[d];
+        if (std::fabs(x[d] - c) > 0.5 * w) return 0;
+      }
+      return 1;
+    }
+    case GS_SHAPE_BELOW_SINES: {
+      // below the surface x_v = p[0] + sum_k amp_k sin(2 pi (kx x + kz z) + phase_k)
+      const int v = sh.axis;
+      double level = sh.p[0];
+      for (int k = 0; k < sh.n_modes && k < 8; k++) {
+        const double* m = sh.modes + 4 * k;
+        const int a0 = v == 0 ? 1 : 0;
+        const int a1 = dim == 3 ? (v == 2 ? 1 : 2) : -1;
+        double arg = m[1] * x[a0];
+        if (a1 >= 0) arg += m[2] * x[a1];
+        level += m[0] * std::sin(2.0 * M_PI * arg + m[3]);
+      }
+      return x[v] < level;
+    }
+    default:
+      return -1;
+  }
+}
+
+// The C library's rand() as the reference's StrainWeakening draws it
+// (glibc's additive feedback generator): r_0 = seed (0 taken as 1), r_i =
+// 16807 r_{i-1} mod (2^31 - 1) for i < 31, r_i = r_{i-31} for 31 <= i < 34,
+// then r_i = r_{i-3} + r_{i-31} mod 2^32; the k-th output is r_{k+344} >> 1.
+class CRand {
+ public:
+  explicit CRand(unsigned seed) {
+    int32_t w = (int32_t)(seed == 0 ? 1u : seed);
+    r_[0] = (uint32_t)w;
+    for (int i = 1; i < 31; i++) {
+      const int64_t hi = w / 127773, lo = w % 127773;
+      int64_t v = 16807 * lo - 2836 * hi;  // Schrage's method for 16807 w mod (2^31 - 1)
+      if (v < 0) v += 2147483647;
+      w = (int32_t)v;
+      r_[i] = (uint32_t)w;
+    }
+    for (int i = 31; i < 34; i++) r_[i] = r_[i - 31];
+    n_ = 34;
+    for (int k = 0; k < 310; k++) step();
+  }
+  int32_t next() { return (int32_t)(step() >> 1); }
+
+ private:
+  uint32_t step() {  // r_n = r_{n-3} + r_{n-31}, over a ring of 34
+    const uint32_t v = r_[(n_ - 3) % 34] + r_[(n_ - 31) % 34];
+    r_[n_ % 34] = v;
+    n_++;
+    return v;
+  }
+  uint32_t r_[34];
+  uint64_t n_;
+};
+
+}  // namespace
+
 // Material layout: for each shape in order, particles inside take its
 // material (Material.cxx:210-228, the last registered material wins).
 extern "C" int32_t gs_shape_layout(int32_t dim, int64_t P, const double* coord, int32_t n_shapes,
@@ -339,50 +416,53 @@ extern "C" int32_t gs_shape_layout(int32_t dim, int64_t P, const double* coord, 
   for (int s = 0; s < n_shapes; s++) {
     const gs_shape& sh = shapes[s];
     for (int64_t p = 0; p < P; p++) {
-      const double* x = coord + 3 * p;
-      bool in = false;
-      switch (sh.type) {
-        case GS_SHAPE_EVERYWHERE:
-          in = true;
-          break;
-        case GS_SHAPE_SPHERE: {  // Sphere.cxx:189-208
-          double v = 0.0;
-          for (int d = 0; d < dim; d++) {
-            const double t = x[d] - sh.p[d];
-            v += t * t;
-          }
-          in = v <= sh.p[3] * sh.p[3];
-          break;
-        }
-        case GS_SHAPE_BOX: {  // Box.cxx:209-221: |x - centre| <= width/2 on every axis
-          in = true;
-          for (int d = 0; d < dim; d++) {
-            const double c = 0.5 * (sh.p[d] + sh.p[3 + d]);
-            const double w = sh.p[3 + d] - sh.p[d];
-            if (std::fabs(x[d] - c) > 0.5 * w) in = false;
-          }
-          break;
-        }
-        case GS_SHAPE_BELOW_SINES: {
-          // below the surface x_v = p[0] + sum_k amp_k sin(2 pi (kx x + kz z) + phase_k)
-          const int v = sh.axis;
-          double level = sh.p[0];
-          for (int k = 0; k < sh.n_modes && k < 8; k++) {
-            const double* m = sh.modes + 4 * k;
-            const int a0 = v == 0 ? 1 : 0;
-            const int a1 = dim == 3 ? (v == 2 ? 1 : 2) : -1;
-            double arg = m[1] * x[a0];
-            if (a1 >= 0) arg += m[2] * x[a1];
-            level += m[0] * std::sin(2.0 * M_PI * arg + m[3]);
-          }
-          in = x[v] < level;
-          break;
-        }
-        default:
-          return 6;
-      }
+      const int in = shape_contains(dim, sh, coord + 3 * p);
+      if (in < 0) return 6;
       if (in) material[p] = sh.material;
     }
+  }
+  return 0;
+}
+
+extern "C" int32_t gs_crand_stream(int64_t seed, int64_t count, int32_t* out) {
+  if (seed < 0 || seed > 0xffffffffLL || count < 0 || (count > 0 && !out)) return 6;
+  CRand g((unsigned)seed);
+  for (int64_t i = 0; i < count; i++) out[i] = g.next();
+  return 0;
+}
+
+// _StrainWeakening_Initialise (StrainWeakening.cxx:333-430), the initial
+// damage loop: particles in order; with probability "initialDamageFraction"
+// (one draw) a particle inside the optional shape gets
+// initialDamageFactor * draw * finalSofteningStrain / RAND_MAX, modulated by
+// the squared half-cosine / half-sine profiles of the wavenumbers that are set.
+extern "C" int32_t gs_initial_damage(int32_t dim, int64_t P, const double* coord, const gs_initial_damage_params* d,
+                                     const gs_shape* shape, double* w) {
+#pragma clang fp contract(off)
+  if ((dim != 2 && dim != 3) || P < 0 || !d || (P > 0 && (!coord || !w))) return 6;
+  if (d->seed < 0 || d->seed > 0xffffffffLL) return 6;
+  if (shape && shape_contains(dim, *shape, coord) < 0 && P > 0) return 6;
+  const double kRandMax = 2147483647.0;
+  CRand g((unsigned)d->seed);
+  for (int64_t p = 0; p < P; p++) {
+    const double* x = coord + 3 * p;
+    double v = 0.0;
+    if ((double)g.next() < kRandMax * d->fraction) {
+      if (shape && !shape_contains(dim, *shape, x)) {
+        w[p] = 0.0;
+        continue;
+      }
+      const double a = d->factor * (double)g.next();
+      const double b = a * d->final_softening_strain;
+      v = b / kRandMax;
+      if (d->wavenumber > 0.0 && d->wavenumber_cos_i == -1.0)
+        v *= std::pow(0.5 + 0.5 * std::cos(M_PI * x[0] * d->wavenumber), 2.0);
+      if (d->wavenumber_cos_i > 0.0) v *= std::pow(0.5 + 0.5 * std::cos(M_PI * x[0] * d->wavenumber_cos_i), 2.0);
+      if (d->wavenumber_sin_i > 0.0) v *= std::pow(0.5 + 0.5 * std::sin(M_PI * x[0] * d->wavenumber_sin_i), 2.0);
+      if (d->wavenumber_cos_k > 0.0) v *= std::pow(0.5 + 0.5 * std::cos(M_PI * x[2] * d->wavenumber_cos_k), 2.0);
+      if (d->wavenumber_sin_k > 0.0) v *= std::pow(0.5 + 0.5 * std::sin(M_PI * x[2] * d->wavenumber_sin_k), 2.0);
+    }
+    w[p] = v;
   }
   return 0;
 }

@@ -75,6 +75,33 @@ typedef struct gs_shape {
 int32_t gs_shape_layout(int32_t dim, int64_t P, const double* coord, int32_t n_shapes, const gs_shape* shapes,
                         int32_t* material);
 
+/* The C library's rand() sequence after srand(seed) (glibc's generator, the
+ * one the reference's StrainWeakening initial damage draws from): count
+ * values into out.  Returns 0 or 6 (bad arguments). */
+int32_t gs_crand_stream(int64_t seed, int64_t count, int32_t* out);
+
+/* StrainWeakening initial damage (StrainWeakening.cxx:232-290 keys,
+ * _StrainWeakening_Initialise :333-430).  A wavenumber <= 0 is off (-1 is
+ * the reference's default); `wavenumber` applies only when wavenumber_cos_i
+ * is -1, as in the reference. */
+typedef struct gs_initial_damage_params {
+  double fraction;           /* "initialDamageFraction" */
+  double factor;             /* "initialDamageFactor" (1) */
+  double final_softening_strain; /* "finalSofteningStrain" */
+  double wavenumber;         /* "initialDamageWavenumber" */
+  double wavenumber_sin_i;   /* "initialDamageWavenumberSinI" */
+  double wavenumber_cos_i;   /* "initialDamageWavenumberCosI" */
+  double wavenumber_sin_k;   /* "initialDamageWavenumberSinK" */
+  double wavenumber_cos_k;   /* "initialDamageWavenumberCosK" */
+  int64_t seed;              /* "randomSeed" (srand argument) */
+} gs_initial_damage_params;
+
+/* Initial plastic strain of P particles (coordinates P*3, in the order the
+ * layout created them) into w; shape (may be NULL) is "initialStrainShape".
+ * Feed w to gs_problem_set_plastic_strain.  Returns 0 or 6. */
+int32_t gs_initial_damage(int32_t dim, int64_t P, const double* coord, const gs_initial_damage_params* d,
+                          const gs_shape* shape, double* w);
+
 /* Seeded Gaussian random field on a node grid (white noise filtered by a
  * separable Gaussian with covariance length corr), min-max scaled to
  * [vmin, vmax].  No reference counterpart: used to build viscosity fields. */

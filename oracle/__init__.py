@@ -90,6 +90,7 @@ def lib():
             "orc_search_element": (i32, [i32, vp, vp, vp, vp, vp]),
             "orc_layout_random": (i64, [i32, vp, vp, vp, i32, C.c_long, vp]),
             "orc_sobol_sequence": (i32, [C.c_uint, C.c_uint, vp, i64, vp, vp]),
+            "orc_initial_damage": (i32, [i64, vp, vp, C.c_long, vp, vp]),
             "orc_layout_spacefiller": (i64, [i32, vp, vp, vp, i64, d, vp, vp, vp, vp]),
             "orc_problem_new": (vp, [vp]),
             "orc_problem_free": (None, [vp]),
@@ -223,6 +224,17 @@ def layout_spacefiller(dim, res, lo, hi, generators, total=0, avg_per_cell=0.0):
                                      _p(m), _p(coord))
     assert n == P
     return coord
+
+
+def initial_damage(coord, fraction, final_softening_strain, seed, factor=1.0, wavenumber=-1.0, sin_i=-1.0,
+                   cos_i=-1.0, sin_k=-1.0, cos_k=-1.0, inside=None):
+    """StrainWeakening.cxx:333-430 initial damage, drawing from the C library's rand()."""
+    c = np.ascontiguousarray(coord, dtype=np.float64).reshape(-1, 3)
+    prm = np.array([fraction, factor, final_softening_strain, wavenumber, sin_i, cos_i, sin_k, cos_k], np.float64)
+    ins = None if inside is None else np.ascontiguousarray(inside, dtype=np.uint8)
+    w = np.zeros(len(c))
+    lib().orc_initial_damage(len(c), _p(c), _p(prm), int(seed), None if ins is None else _p(ins), _p(w))
+    return w
 
 
 def dvc_cell(dim, res, xi):

@@ -35,7 +35,9 @@
  *     its plastic strain rate 2 e_II - tau_y / eta (linear velocity fields,
  *     exact strain rates); healing alone gives -(tau_y / eta) h w; a problem
  *     at or past the final softening strain solves bit-identically to the
- *     after-softening material (tests/test_weakening_oracle.py).
+ *     after-softening material (tests/test_weakening_oracle.py).  The
+ *     initial damage calls the C library's srand/rand itself (the generator
+ *     the reference uses), closed form at fraction 1.
  * Functions without a check of their own are listed as "parity unpinned" in
  * DESIGN.md.
  */
@@ -802,6 +804,34 @@ int32_t orc_weakening_step(orc_problem* pb, double dt) {
   for (int64_t p = 0; p < pb->P; p++) {
     pb->pstr[p] += dt * pb->pinc[p];
     if (pb->pstr[p] < 0.0) pb->pstr[p] = 0.0;
+  }
+  return ORC_OK;
+}
+
+/* StrainWeakening.cxx:333-430 (loadFromCheckPoint false): srand(randomSeed),
+ * then per particle one draw against initialDamageFraction and, when it
+ * passes and the particle is inside the shape, a second draw scaled by the
+ * factor and the final softening strain, times the wavenumber profiles. */
+int32_t orc_initial_damage(int64_t P, const double* coord, const double* prm, long seed, const uint8_t* inside,
+                           double* w) {
+  const double kPi = 3.14159265358979323846; /* M_PI of <math.h> */
+  srand((unsigned)seed);
+  for (int64_t p = 0; p < P; p++) {
+    const double* x = coord + 3 * p;
+    double v = 0.0;
+    if (rand() < RAND_MAX * prm[0]) {
+      if (inside && !inside[p]) {
+        w[p] = v;
+        continue;
+      }
+      v = prm[1] * rand() * prm[2] / RAND_MAX;
+      if (prm[3] > 0.0 && prm[5] == -1.0) v *= pow(0.5 + 0.5 * cos(kPi * x[0] * prm[3]), 2.0);
+      if (prm[5] > 0.0) v *= pow(0.5 + 0.5 * cos(kPi * x[0] * prm[5]), 2.0);
+      if (prm[4] > 0.0) v *= pow(0.5 + 0.5 * sin(kPi * x[0] * prm[4]), 2.0);
+      if (prm[7] > 0.0) v *= pow(0.5 + 0.5 * cos(kPi * x[2] * prm[7]), 2.0);
+      if (prm[6] > 0.0) v *= pow(0.5 + 0.5 * sin(kPi * x[2] * prm[6]), 2.0);
+    }
+    w[p] = v;
   }
   return ORC_OK;
 }

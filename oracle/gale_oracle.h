@@ -132,6 +132,12 @@ int32_t orc_set_plastic_strain(orc_problem* pb, const double* w);         /* P, 
 int32_t orc_plastic_strain(const orc_problem* pb, double* w, double* inc); /* P each, caller order */
 /* The TimeIntegrand step of StrainWeakening and MakeValuesPositive. */
 int32_t orc_weakening_step(orc_problem* pb, double dt);
+/* _StrainWeakening_Initialise's initial damage with the C library's
+ * srand/rand: prm = {fraction, factor, finalSofteningStrain, wavenumber,
+ * sinI, cosI, sinK, cosK}; inside (may be NULL) flags the particles inside
+ * initialStrainShape. */
+int32_t orc_initial_damage(int64_t P, const double* coord, const double* prm, long seed, const uint8_t* inside,
+                           double* w);
 
 /* --- Sobol sequence (StgDomain/Utils/src/SobolGenerator.cxx) --- */
 /* One generator: primitive polynomial of degree `degree` whose inner
