@@ -283,7 +283,9 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
   __shared__ T sy[3][NB];
   __shared__ T sih[TX + TY + TZ];  // 1 / (4 h) of the tile's columns, rows, layers
   const long long tile = xcd_remap(blockIdx.x, gridDim.x);
-  const int tx = (int)(tile % tg.n[0]), ty = (int)((tile / tg.n[0]) % tg.n[1]), tz = (int)(tile / ((long long)tg.n[0] * tg.n[1]));
+  const unsigned tq = (unsigned)tile / (unsigned)tg.n[0];  // the grid has < 2^32 tiles
+  const int tx = (int)((unsigned)tile - tq * (unsigned)tg.n[0]), ty = (int)(tq % (unsigned)tg.n[1]),
+            tz = (int)(tq / (unsigned)tg.n[1]);
   const int x0 = tx * TX, y0 = ty * TY, z0 = tz * TZ;
   const int nbx = min(TX, g.res[0] - x0) + 1, nby = min(TY, g.res[1] - y0) + 1, nbz = min(TZ, g.res[2] - z0) + 1;
   const long long sy_ = g.nn[0], sz_ = g.nn[0] * g.nn[1];
@@ -366,7 +368,18 @@ __global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __r
   constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
-  const int i = (int)(n % g.nn[0]), j = (int)((n / g.nn[0]) % g.nn[1]), k = (int)(n / (g.nn[0] * g.nn[1]));
+  int i, j, k;
+  if (g.N < (1LL << 31)) {  // 32-bit lattice indices (64-bit division is a long software sequence)
+    const unsigned un = (unsigned)n, nx = (unsigned)g.nn[0], ny = (unsigned)g.nn[1];
+    const unsigned q = un / nx;
+    i = (int)(un - q * nx);
+    j = (int)(q % ny);
+    k = (int)(q / ny);
+  } else {
+    i = (int)(n % g.nn[0]);
+    j = (int)((n / g.nn[0]) % g.nn[1]);
+    k = (int)(n / (g.nn[0] * g.nn[1]));
+  }
   const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1), tz = min(k / TZ, tg.n[2] - 1);
   const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0), onz = (k == tz * TZ && tz > 0);
   if (EPI == 0 && !(onx | ony | onz)) return;
