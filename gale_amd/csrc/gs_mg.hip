@@ -241,12 +241,28 @@ void launch_mom_diag(const Geo& g, const double* Mom, double* diag, hipStream_t 
 // ---------------------------------------------------------------------------
 // transfers: P = Q1 interpolation coarse -> fine, R = P^T (full weighting)
 // ---------------------------------------------------------------------------
+// Lattice indices of node n (x fastest); 32-bit division when the node count
+// allows it (64-bit division is a long software sequence on the GPU).
+__device__ __forceinline__ void node_ijk(const Geo& g, long long n, long long& i, long long& j, long long& k) {
+  if (g.N < (1LL << 31)) {
+    const unsigned un = (unsigned)n, nx = (unsigned)g.nn[0], ny = (unsigned)g.nn[1];
+    const unsigned q = un / nx;
+    i = un - q * nx;
+    j = q % ny;
+    k = q / ny;
+  } else {
+    i = n % g.nn[0];
+    j = (n / g.nn[0]) % g.nn[1];
+    k = n / (g.nn[0] * g.nn[1]);
+  }
+}
 template <typename T>
 __global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ bcflag_c) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= gc.N) return;
   const int dim = gc.dim;
-  const long long I = n % gc.nn[0], J = (n / gc.nn[0]) % gc.nn[1], K = n / (gc.nn[0] * gc.nn[1]);
+  long long I, J, K;
+  node_ijk(gc, n, I, J, K);
   T acc[3] = {0, 0, 0};
   const int kz0 = dim == 3 ? -1 : 0, kz1 = dim == 3 ? 1 : 0;
   for (int dz = kz0; dz <= kz1; dz++) {
@@ -275,7 +291,8 @@ __global__ void k_prolong_add(Geo gc, Geo gf, const T* __restrict__ xc, T* __res
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= gf.N) return;
   const int dim = gf.dim;
-  const long long i = n % gf.nn[0], j = (n / gf.nn[0]) % gf.nn[1], k = n / (gf.nn[0] * gf.nn[1]);
+  long long i, j, k;
+  node_ijk(gf, n, i, j, k);
   T acc[3] = {0, 0, 0};
   const int nz = (dim == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
   const T wz = nz == 2 ? (T)0.5 : (T)1, wy = ny == 2 ? (T)0.5 : (T)1, wx = nx == 2 ? (T)0.5 : (T)1;

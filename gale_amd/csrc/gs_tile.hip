@@ -31,6 +31,24 @@ __device__ __forceinline__ double vcoord_t(const Geo& g, int d, long long idx) {
   return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
 }
 
+// Lattice indices of entry n of an nx x ny x nz lattice (x fastest); 32-bit
+// division when the lattice is small enough (64-bit division is a long
+// software sequence on the GPU).
+__device__ __forceinline__ void lattice3(long long n, long long nx, long long ny, long long total, long long& i,
+                                         long long& j, long long& k) {
+  if (total < (1LL << 31)) {
+    const unsigned un = (unsigned)n, ux = (unsigned)nx, uy = (unsigned)ny;
+    const unsigned q = un / ux;
+    i = un - q * ux;
+    j = q % uy;
+    k = q / uy;
+  } else {
+    i = n % nx;
+    j = (n / nx) % ny;
+    k = n / (nx * ny);
+  }
+}
+
 template <typename T>
 struct V4;
 template <>
@@ -609,7 +627,8 @@ __device__ void bot_kx(const BotLevel<T>& L, const T* __restrict__ x, T* sy) {
     T ye[8][3];
     long long n0 = 0;
     if (act) {
-      const long long gi = e % g.res[0], gj = (e / g.res[0]) % g.res[1], gk = e / ((long long)g.res[0] * g.res[1]);
+      long long gi, gj, gk;
+      lattice3(e, g.res[0], g.res[1], E, gi, gj, gk);
       n0 = gi + sy_ * gj + sz_ * gk;
       const T ix = (T)(0.25 / (vcoord_t(g, 0, gi + 1) - vcoord_t(g, 0, gi)));
       const T iy = (T)(0.25 / (vcoord_t(g, 1, gj + 1) - vcoord_t(g, 1, gj)));
@@ -696,7 +715,8 @@ template <typename T, int BS>
 __device__ void bot_restrict(const BotLevel<T>& C, const BotLevel<T>& F) {
   const Geo &gc = C.g, &gf = F.g;
   for (long long n = threadIdx.x; n < gc.N; n += BS) {
-    const long long I = n % gc.nn[0], J = (n / gc.nn[0]) % gc.nn[1], K = n / (gc.nn[0] * gc.nn[1]);
+    long long I, J, K;
+    lattice3(n, gc.nn[0], gc.nn[1], gc.N, I, J, K);
     T acc[3] = {0, 0, 0};
     for (int dz = -1; dz <= 1; dz++) {
       const long long k = 2 * K + dz;
@@ -726,7 +746,8 @@ template <typename T, int BS>
 __device__ void bot_prolong(const BotLevel<T>& C, const T* __restrict__ xc, const BotLevel<T>& F, T* xf) {
   const Geo &gc = C.g, &gf = F.g;
   for (long long n = threadIdx.x; n < gf.N; n += BS) {
-    const long long i = n % gf.nn[0], j = (n / gf.nn[0]) % gf.nn[1], k = n / (gf.nn[0] * gf.nn[1]);
+    long long i, j, k;
+    lattice3(n, gf.nn[0], gf.nn[1], gf.N, i, j, k);
     T acc[3] = {0, 0, 0};
     const int nz = (k & 1) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
     const T wz = nz == 2 ? (T)0.5 : (T)1, wy = ny == 2 ? (T)0.5 : (T)1, wx = nx == 2 ? (T)0.5 : (T)1;
