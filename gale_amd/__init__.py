@@ -40,7 +40,8 @@ EXPORTS = (
     "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_crand_stream", "gs_initial_damage", "gs_wall_bc",
     "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
-    "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time", "gs_shape_layout", "gs_gaussian_field",
+    "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time",
+    "gs_h5_write_swarm_vars", "gs_h5_read_swarm_var", "gs_shape_layout", "gs_gaussian_field",
     "gs_lpc_create", "gs_lpc_destroy", "gs_lpc_num_params", "gs_lpc_get_params", "gs_lpc_set_params",
     "gs_lpc_forward", "gs_lpc_backward", "gs_lpc_loss", "gs_lpc_dataset_create", "gs_lpc_dataset_destroy",
     "gs_lpc_dataset_size", "gs_lpc_dataset_get", "gs_solver_record_pairs", "gs_lpc_train", "gs_lpc_eval",
@@ -181,6 +182,8 @@ def lib():
             "gs_h5_write_field": (i32, [C.c_char_p, i32, i32, i64, vp, vp, vp]),
             "gs_h5_write_swarm": (i32, [C.c_char_p, i32, i64, vp, vp]),
             "gs_h5_write_time": (i32, [C.c_char_p, d, d, i32]),
+            "gs_h5_write_swarm_vars": (i32, [C.c_char_p, i32, i64, vp, vp, i32, vp, vp]),
+            "gs_h5_read_swarm_var": (i64, [C.c_char_p, C.c_char_p, C.c_char_p, i64, vp]),
             "gs_wall_bc": (i32, [i32, vp, i32, i32, d, vp, vp]),
             "gs_shape_layout": (i32, [i32, i64, vp, i32, vp, vp]),
             "gs_gaussian_field": (i32, [i32, vp, vp, d, i64, d, d, vp]),
@@ -373,13 +376,31 @@ def write_field(path, dim, values, coords=None, mesh_res=None):
                                 None if c is None else _ptr(c), None if r is None else _ptr(r)), "write_field")
 
 
-def write_swarm(path, dim, coord, material):
-    """SwarmDump of a material swarm: coordinates (P, 3) and material indices (P,)."""
+def write_swarm(path, dim, coord, material, variables=None):
+    """SwarmDump of a material swarm: coordinates (P, 3), material indices (P,)
+    and optional further scalar variables {name: (P,) values}."""
     c = np.ascontiguousarray(coord, np.float64).reshape(-1, 3)
     m = np.ascontiguousarray(material, np.int32).reshape(-1)
     if len(m) != len(c):
         raise GaleError(E_INVALID, "write_swarm sizes")
-    _h5(lib().gs_h5_write_swarm(os.fsencode(path), dim, len(c), _ptr(c), _ptr(m)), "write_swarm")
+    if not variables:
+        _h5(lib().gs_h5_write_swarm(os.fsencode(path), dim, len(c), _ptr(c), _ptr(m)), "write_swarm")
+        return
+    arrs = [np.ascontiguousarray(v, np.float64).reshape(-1) for v in variables.values()]
+    if any(len(a) != len(c) for a in arrs):
+        raise GaleError(E_INVALID, "write_swarm variable sizes")
+    names = (C.c_char_p * len(arrs))(*[k.encode() for k in variables])
+    ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+    _h5(lib().gs_h5_write_swarm_vars(os.fsencode(path), dim, len(c), _ptr(c), _ptr(m), len(arrs), names, ptrs),
+        "write_swarm")
+
+
+def read_swarm_variable(path, swarm, name):
+    """One scalar swarm variable (e.g. "PostFailureWeakening") of a SwarmDump file."""
+    n = _h5(lib().gs_h5_read_swarm_var(os.fsencode(path), swarm.encode(), name.encode(), 0, None), name)
+    out = np.zeros(n)
+    _h5(lib().gs_h5_read_swarm_var(os.fsencode(path), swarm.encode(), name.encode(), n, _ptr(out)), name)
+    return out
 
 
 def write_time(path, dt, time, nproc=1):

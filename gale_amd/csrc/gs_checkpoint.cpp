@@ -717,19 +717,55 @@ extern "C" int32_t gs_h5_write_field(const char* path, int32_t dim, int32_t dofs
                     attrs);
 }
 
-extern "C" int32_t gs_h5_write_swarm(const char* path, int32_t dim, int64_t P, const double* coord,
-                                     const int32_t* material) {
+extern "C" int32_t gs_h5_write_swarm_vars(const char* path, int32_t dim, int64_t P, const double* coord,
+                                          const int32_t* material, int32_t n_vars, const char* const* names,
+                                          const double* const* values) {
   if ((dim != 2 && dim != 3) || P < 0 || P > INT32_MAX || (P > 0 && (!coord || !material))) return GS_H5_INVALID;
+  if (n_vars < 0 || n_vars > 2 * kLeafK - 2 || (n_vars > 0 && (!names || !values))) return GS_H5_INVALID;
   // SwarmDump.cxx:213-330: "/Position" (P x dim), "/MaterialIndex" (P x 1),
-  // and the particle count as a root attribute
+  // one "/<variable>" per further swarm variable, and the particle count as
+  // a root attribute
   std::vector<double> pos((size_t)P * dim);
   for (int64_t p = 0; p < P; p++)
     for (int d = 0; d < dim; d++) pos[p * dim + d] = coord[3 * p + d];
   const int32_t n = (int32_t)P;
-  return write_file(path,
-                    {{"Position", {(uint64_t)P, (uint64_t)dim}, false, false, pos.data(), pos.size() * sizeof(double)},
-                     {"MaterialIndex", {(uint64_t)P, 1}, true, false, material, (uint64_t)P * 4}},
-                    {{"Swarm Particle Count", {1}, true, true, &n, 4}});
+  std::vector<Item> sets = {
+      {"Position", {(uint64_t)P, (uint64_t)dim}, false, false, pos.data(), pos.size() * sizeof(double)},
+      {"MaterialIndex", {(uint64_t)P, 1}, true, false, material, (uint64_t)P * 4}};
+  for (int v = 0; v < n_vars; v++) {
+    if (!names[v] || !names[v][0] || (P > 0 && !values[v])) return GS_H5_INVALID;
+    for (const Item& it : sets)
+      if (it.name == names[v]) return GS_H5_INVALID;
+    sets.push_back({names[v], {(uint64_t)P, 1}, false, false, values[v], (uint64_t)P * 8});
+  }
+  return write_file(path, sets, {{"Swarm Particle Count", {1}, true, true, &n, 4}});
+}
+
+extern "C" int32_t gs_h5_write_swarm(const char* path, int32_t dim, int64_t P, const double* coord,
+                                     const int32_t* material) {
+  return gs_h5_write_swarm_vars(path, dim, P, coord, material, 0, nullptr, nullptr);
+}
+
+extern "C" int64_t gs_h5_read_swarm_var(const char* path, const char* swarm, const char* name, int64_t capacity,
+                                        double* out) {
+  if (!swarm || !name) return GS_H5_INVALID;
+  File f;
+  int st = load(path, f);
+  if (st) return st;
+  // FileParticleLayout.cxx:395-409 naming: "/<variable>", then "/<swarm>-<variable>"
+  Dataset d;
+  st = dataset(f, name, d);
+  if (st == GS_H5_NOT_FOUND) {
+    d = Dataset();
+    st = dataset(f, (std::string(swarm) + "-" + name).c_str(), d);
+  }
+  if (st) return st;
+  const int64_t n = count_of(d);
+  if (d.rank < 1 || (d.rank == 2 && d.dims[1] != 1) || d.rank > 2) return GS_H5_SHAPE;
+  if (!out) return n;
+  if (capacity < n) return GS_H5_INVALID;
+  for (int64_t k = 0; k < n; k++) out[k] = as_double(f, d, k);
+  return n;
 }
 
 extern "C" int32_t gs_h5_write_time(const char* path, double dt, double time, int32_t nproc) {

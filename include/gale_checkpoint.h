@@ -80,6 +80,15 @@ int32_t gs_h5_write_field(const char* path, int32_t dim, int32_t dofs, int64_t n
  * "/Position" (P x dim doubles from coord, P*3), "/MaterialIndex" (P x 1
  * int32) and the root attribute "Swarm Particle Count". */
 int32_t gs_h5_write_swarm(const char* path, int32_t dim, int64_t P, const double* coord, const int32_t* material);
+/* The same with n_vars (at most 6) further scalar swarm variables, each a
+ * "/<name>" dataset of P doubles (e.g. "PostFailureWeakening", the plastic
+ * strain of StrainWeakening). */
+int32_t gs_h5_write_swarm_vars(const char* path, int32_t dim, int64_t P, const double* coord, const int32_t* material,
+                               int32_t n_vars, const char* const* names, const double* const* values);
+/* One scalar swarm variable (P values) by name, as FileParticleLayout looks it
+ * up: "/<name>", then "/<swarm>-<name>".  out NULL returns P only.  Returns P
+ * or a GS_H5_* error. */
+int64_t gs_h5_read_swarm_var(const char* path, const char* swarm, const char* name, int64_t capacity, double* out);
 
 /* The timeInfo file (read by FileParticleLayout.cxx:565-597 on restart):
  * "/Dt", "/currentTime" (doubles) and "/nproc" (int32). */

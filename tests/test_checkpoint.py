@@ -224,3 +224,27 @@ def test_reader_survives_corrupted_metadata(tmp_path):
         except gale_amd.GaleError:
             outcomes["error"] += 1
     assert outcomes["error"] > 0 and sum(outcomes.values()) == 400
+
+
+def test_swarm_variables_round_trip(tmp_path):
+    import gale_amd
+    rng = np.random.default_rng(4)
+    P = 777
+    c = np.zeros((P, 3))
+    c[:, :3] = rng.uniform(size=(P, 3))
+    m = rng.integers(0, 4, P).astype(np.int32)
+    w, eta = rng.uniform(size=P), rng.uniform(size=P)
+    f = str(tmp_path / "s.h5")
+    gale_amd.write_swarm(f, 3, c, m, {"PostFailureWeakening": w, "Viscosity": eta})
+    assert np.array_equal(gale_amd.read_swarm_variable(f, "materialSwarm", "PostFailureWeakening"), w)
+    assert np.array_equal(gale_amd.read_swarm_variable(f, "materialSwarm", "Viscosity"), eta)
+    c2, m2 = gale_amd.read_swarm(f, "materialSwarm", 3)
+    assert np.array_equal(c2, c) and np.array_equal(m2, m)
+    with pytest.raises(gale_amd.GaleError, match="not found"):
+        gale_amd.read_swarm_variable(f, "materialSwarm", "Missing")
+    with pytest.raises(gale_amd.GaleError):
+        gale_amd.write_swarm(f, 3, c, m, {"Position": w})          # clashes with the coordinates
+    # the older "<swarm>-<variable>" names of the reference's fixtures resolve too
+    src = os.path.join(GOLDEN, "rayleigh_taylor_step10", "materialSwarm.00010.h5")
+    idx = gale_amd.read_swarm_variable(src, "materialSwarm", "MaterialIndex")
+    assert np.array_equal(idx.astype(np.int32), gale_amd.read_swarm(src, "materialSwarm", 2)[1])

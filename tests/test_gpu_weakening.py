@@ -111,3 +111,49 @@ def test_fully_softened_equals_softened_material(gale):
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     g.close()
     g2.close()
+
+
+def test_weakening_restart_from_checkpoint(gale, tmp_path):
+    """The plastic strain goes into the swarm checkpoint beside the positions
+    ("PostFailureWeakening"); a problem rebuilt from the file solves exactly as
+    the original and stores the same rates."""
+    from gale_amd import configs
+    from test_gpu_parity import tight
+    cfg = configs.lithosphere_2d(nx=32, ny=8, ppc=10)
+    lid = cfg.materials[1]
+    sw = dict(healingRate=0.05, finalSofteningStrain=0.02, cohesionAfterSoftening=0.3 * lid["cohesion"],
+              frictionCoefficientAfterSoftening=0.5 * lid["friction"])
+    g = gale.Problem(**cfg.problem_kwargs())
+    g.set_strain_weakening(1, **sw)
+    g.set_plastic_strain(np.random.default_rng(2).uniform(0.0, 0.03, g.P))
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale, tolerance=1e-11))
+    u, p, rep = s.NonLinearExecute(nonLinearTolerance=1e-4, nonLinearMaxIterations=20)
+    w, inc = g.plastic_strain()
+    coord = g.particles()[0]
+    f = str(tmp_path / "materialSwarm.h5")
+    gale.write_swarm(f, 2, coord, cfg.material, {"PostFailureWeakening": w})
+    s.close()
+    g.close()
+    coord2, mat2 = gale.read_swarm(f, "materialSwarm", 2)
+    kw = cfg.problem_kwargs()
+    kw.update(coord=coord2, material_index=mat2)
+    g2 = gale.Problem(**kw)
+    g2.set_strain_weakening(1, **sw)
+    g2.set_plastic_strain(gale.read_swarm_variable(f, "materialSwarm", "PostFailureWeakening"))
+    s2 = gale.Stokes_SLE_UzawaSolver(g2, tight(gale, tolerance=1e-11))
+    w0, _ = g2.plastic_strain()
+    assert np.array_equal(w0, w)
+    u2, p2, rep2 = s2.NonLinearExecute(nonLinearTolerance=1e-4, nonLinearMaxIterations=20)
+    w2, inc2 = g2.plastic_strain()
+    # the first problem had solved once before its checkpoint: compare the rates of a solve from the same state
+    assert rep2.status == 0 and np.array_equal(w2, w)
+    g3 = gale.Problem(**kw)
+    g3.set_strain_weakening(1, **sw)
+    g3.set_plastic_strain(w)
+    s3 = gale.Stokes_SLE_UzawaSolver(g3, tight(gale, tolerance=1e-11))
+    u3, p3, _ = s3.NonLinearExecute(nonLinearTolerance=1e-4, nonLinearMaxIterations=20)
+    assert np.array_equal(u3, u2) and np.array_equal(g3.plastic_strain()[1], inc2)
+    s2.close()
+    s3.close()
+    g2.close()
+    g3.close()
