@@ -49,6 +49,7 @@ EXPORTS = (
     "gs_batch_create", "gs_batch_destroy", "gs_batch_set_tile", "gs_batch_autotune", "gs_batch_solve",
     "gs_batch_solution",
     "gs_problem_set_weights", "gs_problem_set_strain_weakening", "gs_problem_set_plastic_strain",
+    "gs_problem_set_strain_limit",
     "gs_problem_plastic_strain", "gs_dvc_cells", "gs_dvc_max_particles", "gs_solver_particle_fields",
     "gs_problem_population_control",
 )
@@ -153,6 +154,7 @@ def lib():
             "gs_problem_set_weights": (i32, [vp, i32, vp]),
             "gs_problem_set_strain_weakening": (i32, [vp, i32, vp]),
             "gs_problem_set_plastic_strain": (i32, [vp, vp]),
+            "gs_problem_set_strain_limit": (i32, [vp, i32, vp, d]),
             "gs_problem_plastic_strain": (i32, [vp, vp, vp]),
             "gs_dvc_cells": (i32, [i32, vp, i64, vp, vp, vp, vp]),
             "gs_dvc_max_particles": (i64, [i32, vp]),
@@ -660,6 +662,16 @@ class Problem:
         sw = np.array([healingRate, initialSofteningStrain, finalSofteningStrain, cohesionAfterSoftening,
                        frictionCoefficientAfterSoftening], np.float64)
         _check(lib().gs_problem_set_strain_weakening(self._h, int(material), _ptr(sw)), "set_strain_weakening")
+
+    def set_strain_limit(self, shape=None, strainLimit=0.0):
+        """StrainWeakening "strainLimitedShape" (a sphere(), box() or
+        everywhere() shape) and "strainLimit"; shape=None removes it."""
+        if shape is None:
+            _check(lib().gs_problem_set_strain_limit(self._h, -1, None, 0.0), "set_strain_limit")
+            return
+        prm = np.array([shape.p[i] for i in range(6)], np.float64)
+        _check(lib().gs_problem_set_strain_limit(self._h, int(shape.type), _ptr(prm), float(strainLimit)),
+               "set_strain_limit")
 
     def set_plastic_strain(self, w):
         """Accumulated plastic strain of every particle (caller order)."""

@@ -76,6 +76,7 @@ struct gs_problem {
   double* eta0p = nullptr;
   double* pstr = nullptr;   // StrainWeakening: accumulated plastic strain (sorted), null when off
   double* pinc = nullptr;   // its rate from the last constitutive pass (sorted)
+  StrainLimit slim{-1, 0, {0, 0, 0, 0, 0, 0}, 0.0};  // strain-limited region (type -1: none)
   int* off = nullptr;
   unsigned* keys = nullptr;  // sorted owner of each sorted particle (E = outside)
   unsigned* keys2 = nullptr;
@@ -496,6 +497,16 @@ extern "C" int32_t gs_problem_set_strain_weakening(gs_problem* pb, int32_t mater
   }
   CK(hipMemcpy(pb->mats_d, pb->mats_dh.data(), sizeof(MatDev) * pb->mats_dh.size(), hipMemcpyHostToDevice));
   pb->gen++;  // solvers re-run their constitutive pass
+  return GS_OK;
+}
+
+extern "C" int32_t gs_problem_set_strain_limit(gs_problem* pb, int32_t shape_type, const double* shape_params,
+                                               double limit) {
+  if (!pb || shape_type < -1 || shape_type > 2 || (shape_type > 0 && !shape_params)) return GS_E_INVALID;
+  pb->slim.type = shape_type;
+  pb->slim.dim = pb->dim;
+  for (int i = 0; i < 6; i++) pb->slim.p[i] = shape_type > 0 ? shape_params[i] : 0.0;
+  pb->slim.limit = limit;
   return GS_OK;
 }
 
@@ -1793,7 +1804,7 @@ extern "C" int32_t gs_advect_rk2(gs_solver* s, double dt) {
   std::memcpy(&esc, pb->hS + S_ESC, sizeof(unsigned));
   s->setup_done = false;
   // the other time integrand of the material swarm: StrainWeakening
-  if (pb->pstr) launch_weakening_step(pb->pstr, pb->pinc, pb->P, dt, pb->st);
+  if (pb->pstr) launch_weakening_step(pb->pstr, pb->pinc, pb->coord, pb->P, dt, pb->slim, pb->st);
   const int32_t rs = relocate(pb);
   if (esc) return GS_E_ESCAPED;
   return rs;

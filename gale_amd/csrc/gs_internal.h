@@ -90,8 +90,18 @@ void launch_apply_Gt(const Geo& g, const double* x, double* q, hipStream_t st);
 void launch_setup(const Geo& g, const SetupArgs& a, double4* pk, const int* off, double* diag, double* f,
                   double* Q, double* eta_out, uint8_t* yielded, hipStream_t st);
 void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st);
-// StrainWeakening time step: w += dt * inc, then negative values set to 0.
-void launch_weakening_step(double* w, const double* inc, long long P, double dt, hipStream_t st);
+// StrainWeakening's strain-limited region: particles inside get `limit`
+// after each step (type: -1 none, 0 everywhere, 1 sphere p[0..2] centre p[3]
+// radius, 2 box p[0..2] start p[3..5] end).
+struct StrainLimit {
+  int type, dim;
+  double p[6];
+  double limit;
+};
+// StrainWeakening time step: w += dt * inc, the strain-limited region set,
+// then negative values set to 0.  coord: P*3 (same order as w).
+void launch_weakening_step(double* w, const double* inc, const double* coord, long long P, double dt,
+                           const StrainLimit& lim, hipStream_t st);
 // Strain rate (P x nst), its second invariant (P) and stress (P x nst) of the
 // material points (sorted order) for velocity u and viscosity eta.
 void launch_particle_fields(const Geo& g, const double4* pk, const unsigned* keys, long long P, const double* u,

@@ -433,15 +433,40 @@ void launch_pack32(const double4* pk, float4* pk32, long long P, hipStream_t st)
 // StrainWeakening as a TimeIntegrand (StrainWeakening.cxx:432-441: the time
 // derivative is the increment of the last constitutive pass, so the RK2 step
 // is w + dt * inc) followed by _StrainWeakening_MakeValuesPositive (:443-468).
-__global__ void k_weakening_step(double* __restrict__ w, const double* __restrict__ inc, long long P, double dt) {
+// The strain-limited shape test uses the same arithmetic as the host's
+// Sphere / Box containment (Sphere.cxx:189-208, Box.cxx:209-221).
+__device__ bool strain_limit_inside(const StrainLimit& L, const double* x) {
+  if (L.type == 0) return true;
+  if (L.type == 1) {
+    double v = 0.0;
+    for (int d = 0; d < L.dim; d++) {
+      const double t = x[d] - L.p[d];
+      v += t * t;
+    }
+    return v <= L.p[3] * L.p[3];
+  }
+  for (int d = 0; d < L.dim; d++) {
+    const double c = 0.5 * (L.p[d] + L.p[3 + d]);
+    const double wd = L.p[3 + d] - L.p[d];
+    if (fabs(x[d] - c) > 0.5 * wd) return false;
+  }
+  return true;
+}
+
+__global__ void k_weakening_step(double* __restrict__ w, const double* __restrict__ inc,
+                                 const double* __restrict__ coord, long long P, double dt, StrainLimit lim) {
+#pragma clang fp contract(off)
   const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (p >= P) return;
-  const double v = w[p] + dt * inc[p];
+  double v = w[p] + dt * inc[p];
+  if (lim.type >= 0 && strain_limit_inside(lim, coord + 3 * p)) v = lim.limit;
   w[p] = v < 0.0 ? 0.0 : v;
 }
 
-void launch_weakening_step(double* w, const double* inc, long long P, double dt, hipStream_t st) {
-  if (P > 0) hipLaunchKernelGGL(k_weakening_step, dim3(grid_for(P)), dim3(kBlock), 0, st, w, inc, P, dt);
+void launch_weakening_step(double* w, const double* inc, const double* coord, long long P, double dt,
+                           const StrainLimit& lim, hipStream_t st) {
+  if (P > 0)
+    hipLaunchKernelGGL(k_weakening_step, dim3(grid_for(P)), dim3(kBlock), 0, st, w, inc, coord, P, dt, lim);
 }
 
 // ---------------------------------------------------------------------------

@@ -226,6 +226,14 @@ int32_t gs_problem_set_coords(gs_problem* pb, const double* coord);
  * rate, both 0 (no initial damage).  GS_E_INVALID for a material without a
  * yield rheology or final <= initial softening strain. */
 int32_t gs_problem_set_strain_weakening(gs_problem* pb, int32_t material, const gs_strain_weakening* sw);
+/* StrainWeakening "strainLimitedShape" / "strainLimit"
+ * (_StrainWeakening_MakeValuesPositive, StrainWeakening.cxx:443-468): after
+ * each time step the particles inside the shape get the plastic strain
+ * `limit` (then negatives become 0).  shape_type: -1 none, 0 everywhere,
+ * 1 sphere (params: centre x, y, z, radius), 2 box (start x, y, z, end x, y,
+ * z) — GS_SHAPE_* of gale_generate.h, containment as in Sphere.cxx:189-208
+ * and Box.cxx:209-221. */
+int32_t gs_problem_set_strain_limit(gs_problem* pb, int32_t shape_type, const double* shape_params, double limit);
 /* Set the accumulated plastic strain (P, caller order); the reference's
  * initial damage (rand()-seeded, StrainWeakening.cxx:333-430) is set this way. */
 int32_t gs_problem_set_plastic_strain(gs_problem* pb, const double* w);

@@ -122,6 +122,7 @@ def lib():
             "orc_set_plastic_strain": (i32, [vp, vp]),
             "orc_plastic_strain": (i32, [vp, vp, vp]),
             "orc_weakening_step": (i32, [vp, d]),
+            "orc_weakening_step_limited": (i32, [vp, d, vp, d]),
             "orc_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
             "orc_population_control": (i32, [vp, vp, vp]),
         }
@@ -381,8 +382,10 @@ class Problem:
             raise RuntimeError("orc_plastic_strain")
         return w, inc
 
-    def weakening_step(self, dt):
-        if lib().orc_weakening_step(self._h, float(dt)) != 0:
+    def weakening_step(self, dt, inside=None, limit=0.0):
+        """inside: optional (P,) flags of the strain-limited region (positions after the step)."""
+        ins = None if inside is None else np.ascontiguousarray(inside, dtype=np.uint8)
+        if lib().orc_weakening_step_limited(self._h, float(dt), _p(ins), float(limit)) != 0:
             raise RuntimeError("orc_weakening_step")
 
     def particles(self):

@@ -799,14 +799,17 @@ int32_t orc_plastic_strain(const orc_problem* pb, double* w, double* inc) {
 
 /* StrainWeakening.cxx:432-468: the TimeIntegrand's derivative is the stored
  * increment, so each step adds dt * increment; then values below 0 become 0. */
-int32_t orc_weakening_step(orc_problem* pb, double dt) {
+int32_t orc_weakening_step_limited(orc_problem* pb, double dt, const uint8_t* inside, double limit) {
   if (!pb->pstr) return ORC_E_INVALID;
   for (int64_t p = 0; p < pb->P; p++) {
     pb->pstr[p] += dt * pb->pinc[p];
+    if (inside && inside[p]) pb->pstr[p] = limit; /* StrainWeakening.cxx:455-461 */
     if (pb->pstr[p] < 0.0) pb->pstr[p] = 0.0;
   }
   return ORC_OK;
 }
+
+int32_t orc_weakening_step(orc_problem* pb, double dt) { return orc_weakening_step_limited(pb, dt, NULL, 0.0); }
 
 /* StrainWeakening.cxx:333-430 (loadFromCheckPoint false): srand(randomSeed),
  * then per particle one draw against initialDamageFraction and, when it

@@ -132,6 +132,9 @@ int32_t orc_set_plastic_strain(orc_problem* pb, const double* w);         /* P, 
 int32_t orc_plastic_strain(const orc_problem* pb, double* w, double* inc); /* P each, caller order */
 /* The TimeIntegrand step of StrainWeakening and MakeValuesPositive. */
 int32_t orc_weakening_step(orc_problem* pb, double dt);
+/* The same with the strain-limited region: particles flagged in inside get
+ * `limit` before the clamp at 0. */
+int32_t orc_weakening_step_limited(orc_problem* pb, double dt, const uint8_t* inside, double limit);
 /* _StrainWeakening_Initialise's initial damage with the C library's
  * srand/rand: prm = {fraction, factor, finalSofteningStrain, wavenumber,
  * sinI, cosI, sinK, cosK}; inside (may be NULL) flags the particles inside

@@ -157,3 +157,31 @@ def test_weakening_restart_from_checkpoint(gale, tmp_path):
     s3.close()
     g2.close()
     g3.close()
+
+
+def test_strain_limited_region(gale, oracle):
+    """StrainWeakening's strainLimitedShape: after the step every particle in
+    the box holds strainLimit exactly, the others the integrated value."""
+    from gale_amd import configs
+    from test_gpu_parity import otight, tight
+    cfg = configs.lithosphere_2d(nx=32, ny=8, ppc=10)
+    lid = cfg.materials[1]
+    g, o = _pair(gale, oracle, cfg, lid["cohesion"], lid["friction"])
+    box = gale.box(0, (1.5, 0.7, 0.0), (2.5, 1.0, 0.0))
+    g.set_strain_limit(box, strainLimit=0.5)
+    s = gale.Stokes_SLE_UzawaSolver(g, tight(gale, tolerance=1e-11))
+    s.NonLinearExecute(nonLinearTolerance=1e-4, nonLinearMaxIterations=20)
+    o.picard(otight(oracle, tolerance=1e-11), tolerance=1e-4, max_iterations=20)
+    x = g.particles()[0][:, 0]
+    dt = min(s.compute_dt(0.5), 0.25 * min(x.min() - cfg.lo[0], cfg.hi[0] - x.max()))
+    assert s.advect_rk2(dt) == 0
+    coord = g.particles()[0]
+    c, wd = np.array([2.0, 0.85]), np.array([1.0, 0.3])
+    inside = np.all(np.abs(coord[:, :2] - c) <= 0.5 * wd, axis=1)
+    o.weakening_step(dt, inside=inside, limit=0.5)
+    gw, _ = g.plastic_strain()
+    ow, _ = o.plastic_strain()
+    assert inside.any() and np.all(gw[inside] == 0.5) and np.array_equal(gw[inside], ow[inside])
+    assert np.allclose(gw[~inside], ow[~inside], rtol=1e-6, atol=1e-12)
+    s.close()
+    g.close()

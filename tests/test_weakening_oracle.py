@@ -126,3 +126,15 @@ def test_weakening_step_and_population_control(oracle):
     assert o.P == P0 + added - deleted
     for p in range(o.P):
         assert w2[p] in before[owner[p]] or p < P0 - deleted
+
+
+def test_strain_limited_step(oracle):
+    o = _vm_problem(oracle)
+    o.set_strain_weakening(0, healing_rate=0.5, final_softening_strain=1.0, cohesion_after=0.1)
+    w0 = np.linspace(-0.2, 1.0, o.P)
+    o.set_plastic_strain(w0)
+    inside = np.arange(o.P) % 3 == 0
+    o.weakening_step(0.0, inside=inside, limit=0.25)
+    w1, _ = o.plastic_strain()
+    assert np.all(w1[inside] == 0.25)
+    assert np.array_equal(w1[~inside], np.maximum(w0[~inside], 0.0))
