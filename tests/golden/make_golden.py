@@ -20,8 +20,10 @@ available; the outputs are committed and the tests never read the reference).
   rayleigh_taylor_step10/     the HDF5 checkpoint files (Mesh, VelocityField, PressureField, materialSwarm,
                               timeInfo at step 10) of the reference's regression fixtures
   lid_driven_pic_step10/      Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml/,
-  stokes_multigrid_step10/    PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/ and
-                              StgFEM/SysTest/RegressionTests/expected/Multigrid.xml/, copied unchanged as data
+  stokes_multigrid_step10/    PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml/,
+  rayleigh_taylor_lowres_step5/  StgFEM/SysTest/RegressionTests/expected/Multigrid.xml/ and
+                              Underworld/SysTest/RegressionTests/lowres_expected/RayleighTaylorBenchmark.xml/
+                              (the suite's own low-resolution run, step 5), copied unchanged as data
 
 The .matrix files are PETSc binary AIJ matrices (big-endian int32 header
 1211216, M, N, nnz; row lengths; column indices; big-endian float64 values);
@@ -110,12 +112,16 @@ def regression_fixtures():
             ("Underworld/SysTest/RegressionTests/expected/RayleighTaylorBenchmark.xml", "rayleigh_taylor_step10", swarm),
             ("PICellerator/SysTest/RegressionTests/expected/LidDrivenPIC2D.xml", "lid_driven_pic_step10", swarm),
             ("StgFEM/SysTest/RegressionTests/expected/Multigrid.xml", "stokes_multigrid_step10",
-             ("Mesh.linearMesh", "VelocityField", "PressureField", "timeInfo"))):
+             ("Mesh.linearMesh", "VelocityField", "PressureField", "timeInfo")),
+            ("Underworld/SysTest/RegressionTests/lowres_expected/RayleighTaylorBenchmark.xml",
+             "rayleigh_taylor_lowres_step5",
+             ("Mesh.linearMesh", "VelocityField", "PressureField", "materialSwarm", "timeInfo"))):
         dst = os.path.join(OUT, name)
         os.makedirs(dst, exist_ok=True)
+        step = "00005" if name.endswith("step5") else "00010"
         for fn in files:
-            shutil.copyfile(os.path.join(REF, src, fn + ".00010.h5"), os.path.join(dst, fn + ".00010.h5"))
-            os.chmod(os.path.join(dst, fn + ".00010.h5"), 0o644)
+            shutil.copyfile(os.path.join(REF, src, fn + "." + step + ".h5"), os.path.join(dst, fn + "." + step + ".h5"))
+            os.chmod(os.path.join(dst, fn + "." + step + ".h5"), 0o644)
         print(name, sorted(os.listdir(dst)))
 
 

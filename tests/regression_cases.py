@@ -43,6 +43,12 @@ CASES = {
                        gravity=0.0, materials=[(1.0, 0.0)],
                        walls=[("bottom", 1, 0.0), ("left", 0, 0.0), ("right", 0, 0.0), ("top", 1, 0.0)], lid=True,
                        gauss=False, mesh="Mesh"),
+    # the regression suite's own low-resolution run (lowres_np1.dat: 10 x 10), checkpoint at step 5
+    "rayleigh_taylor_lowres": dict(folder="rayleigh_taylor_lowres_step5", res=(10, 10), lo=(0.0, 0.0),
+                                   hi=(0.9142, 1.0), gravity=1.0, materials=[(1.0, 1.0), (1.0, 0.0)],
+                                   walls=[("bottom", 0, 0.0), ("bottom", 1, 0.0), ("top", 0, 0.0), ("top", 1, 0.0),
+                                          ("left", 0, 0.0), ("right", 0, 0.0)], lid=False, gauss=False,
+                                   mesh="Mesh.linearMesh", step="00005"),
     "stokes_multigrid": dict(folder="stokes_multigrid_step10", res=(32, 16), lo=(0.0, 0.0), hi=(2.0, 1.0),
                              gravity=0.0, materials=[(1.0, 0.0)],
                              walls=[("left", 0, 0.0), ("right", 0, 0.0), ("bottom", 1, 0.0), ("top", 1, 0.0),
@@ -52,7 +58,7 @@ PIC = [k for k, c in CASES.items() if not c["gauss"]]
 
 
 def path(case, what):
-    return os.path.join(GOLDEN, CASES[case]["folder"], what + ".00010.h5")
+    return os.path.join(GOLDEN, CASES[case]["folder"], what + "." + CASES[case].get("step", "00010") + ".h5")
 
 
 def _interp(U, pts, res, lo, hi):

@@ -15,11 +15,11 @@ import pytest
 from conftest import GOLDEN
 from regression_cases import CASES, PIC, load, path, rel
 
-FILES = sorted(glob.glob(os.path.join(GOLDEN, "*_step10", "*.h5")))
+FILES = sorted(glob.glob(os.path.join(GOLDEN, "*_step10", "*.h5")) + glob.glob(os.path.join(GOLDEN, "*_step5", "*.h5")))
 
 
 def test_fixture_files_present():
-    assert len(FILES) == 14
+    assert len(FILES) == 19
 
 
 @pytest.mark.parametrize("fn", FILES, ids=[os.path.relpath(f, GOLDEN) for f in FILES])
@@ -109,7 +109,7 @@ def test_reader_errors(tmp_path):
 
 
 @pytest.mark.parametrize("case,tol_u,tol_p", [("rayleigh_taylor", 1e-3, 1e-3), ("lid_driven", 1e-4, 1e-3),
-                                              ("stokes_multigrid", 1e-6, 1e-5)])
+                                              ("rayleigh_taylor_lowres", 1e-3, 1e-3), ("stokes_multigrid", 1e-6, 1e-5)])
 def test_oracle_reproduces_reference_regression_fields(oracle, case, tol_u, tol_p):
     """The oracle's Stokes solve on the checkpointed swarm (last advection
     undone) with DVC weights reproduces the reference's saved step-10 fields

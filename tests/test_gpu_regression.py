@@ -27,7 +27,7 @@ def _problem(gale, case, d):
 
 
 @pytest.mark.parametrize("case,tol_u,tol_p", [("rayleigh_taylor", 1e-3, 1e-3), ("lid_driven", 1e-4, 1e-3),
-                                              ("stokes_multigrid", 1e-4, 1e-3)])
+                                              ("rayleigh_taylor_lowres", 1e-3, 1e-3), ("stokes_multigrid", 1e-4, 1e-3)])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_reference_regression_fields(gale, case, tol_u, tol_p, precision):
     d = load(case, gale)
