@@ -195,7 +195,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is a design in DESIGN.md, not built).", file=sys.stderr)
+              "(the multi-GPU sharding is a design only, not built).", file=sys.stderr)
         sys.exit(2)
 
     if args.profile_apply:
@@ -274,7 +274,8 @@ def main():
 
     # learned Schur preconditioner + mixed precision (FP32 V-cycle inside the
     # FP64 flexible inner CG), trained offline on 32^3 fields of the same
-    # generator (gale_amd/lpc_train.py); the plain solve above is its judge.
+    # generator (gale_amd/lpc_train.py); the plain solve above is the
+    # check every learned solve must meet.
     learned = None
     wpath = args.weights
     if not args.no_learned and os.path.exists(wpath):

@@ -38,8 +38,9 @@
  *     after-softening material (tests/test_weakening_oracle.py).  The
  *     initial damage calls the C library's srand/rand itself (the generator
  *     the reference uses), closed form at fraction 1.
- * Functions without a check of their own are listed as "parity unpinned" in
- * DESIGN.md.
+ * Parity unpinned: the iterate counts of the GMRES(30)/ILU(0) inner solve
+ * (PETSc's internals are not in the reference tree); the solution it returns
+ * is pinned by the direct solve (tests/test_oracle_pins.py).
  */
 #include "gale_oracle.h"
 
