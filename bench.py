@@ -3,11 +3,18 @@
 One step = one Stokes solve (Uzawa to 1e-6 relative residual, reference
 settings) of the 3D random-viscosity workload (96^3 Q1-P0 elements, 16
 particles per cell, log10 eta in [-3, 3]) with all inputs resident in HBM.
-Prints one JSON line (see the "metric" field).  Extra objects:
+--steps / --warmup apply to the headline solve (learned Schur preconditioner
++ mixed precision); the plain FP64 Uzawa it is compared with runs once.
+Prints one JSON line (see the "metric" field) as soon as the headline and the
+roofline exist, and prints it again, extended, after each later leg (plain
+solve, ablation, yielded solve, CPU baseline, batched solves, DVC weights);
+the last line printed is the complete one.  A wall-clock budget (--budget)
+skips the extra legs that would not fit and records why.  Extra objects:
 
   roofline      the dominant kernel (the FP64 matrix-free K apply) timed with
-                HIP events on the solver's stream: algorithmic bytes per apply
-                / measured apply time, against the 8 TB/s HBM peak.
+                HIP events on the solver's stream, cold (operand copies cycled
+                past the last-level cache): algorithmic bytes per apply /
+                measured apply time, against the 8 TB/s HBM peak.
   cpu_baseline  the CPU oracle (assembled CSR + reference Uzawa with the
                 PETSc-default GMRES(30)+ILU(0) inner solve) on a bounded
                 sample of the same generator, timed on the host.
@@ -144,9 +151,12 @@ def bench_batch(G, configs, count):
 
 def profile_apply(args):
     """Calibration copy of known size (torch elementwise copy, 256 MiB read +
-    256 MiB written per call) followed by the four K apply forms, so that
-    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per dispatch can be scaled to
-    bytes.  torch is imported before the library so both share one HIP runtime."""
+    256 MiB written per call) followed by the four K apply forms and the two
+    pressure operators, each timed cold (operand copies cycled so every apply
+    streams from HBM), so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per
+    dispatch can be scaled to bytes and the kernel-trace averages compare with
+    the bench's roofline.  torch is imported before the library so both share
+    one HIP runtime."""
     import torch
     n = 32 * 1024 * 1024
     a = torch.ones(n, dtype=torch.float64, device="cuda")
@@ -154,40 +164,130 @@ def profile_apply(args):
     for _ in range(3):
         b.copy_(a)
     torch.cuda.synchronize()
+    del a, b
     import gale_amd as G
     from gale_amd import configs
     cfg = configs.random_viscosity_3d(res=args.res, ppc=args.ppc)
     pb = G.Problem(**cfg.problem_kwargs())
     s = G.Stokes_SLE_UzawaSolver(pb, G.uzawa_params(tolerance=1e-6))
     s.setup()
-    for form in (G.APPLY_MOMENTS_FP64, G.APPLY_MOMENTS_FP32, G.APPLY_PARTICLES_FP64, G.APPLY_PARTICLES_FP32):
-        ms = s.time_apply_K(form, 3)
-        print(f"# form {form}: {ms:.4f} ms", file=sys.stderr, flush=True)
+    for form in (G.APPLY_MOMENTS_FP64, G.APPLY_MOMENTS_FP32, G.APPLY_PARTICLES_FP64, G.APPLY_PARTICLES_FP32,
+                 G.APPLY_G, G.APPLY_GT):
+        ms = s.time_apply(form, args.apply_reps, cold=True)
+        print(f"# form {form}: {ms:.4f} ms (cold)", file=sys.stderr, flush=True)
     print(json.dumps({"copy_bytes_each_way": n * 8, "res": args.res, "ppc": args.ppc,
                       "moments_bytes": k_moments_bytes(cfg.dim, cfg.N, cfg.E),
                       "particles_bytes": k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)}), flush=True)
 
 
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def roofline_obj(nbytes, ms, kernel, **extra):
+    gbs = nbytes / (ms * 1e-3) / 1e9
+    d = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+         "kernel": kernel, "ms_per_apply": ms, "algorithmic_bytes": nbytes}
+    d.update(extra)
+    return d
+
+
+class Emitter:
+    """Holds the one JSON line.  It is printed as soon as the headline and the
+    roofline exist and printed again, complete so far, after each later leg,
+    so a run stopped by a time limit still leaves a valid line."""
+
+    def __init__(self, budget_s):
+        self.t0 = time.perf_counter()
+        self.budget = budget_s
+        self.line = None
+        self.skipped = {}
+
+    def left(self):
+        return self.budget - (time.perf_counter() - self.t0)
+
+    def fits(self, name, est_s):
+        if self.left() >= est_s:
+            return True
+        self.skipped[name] = f"skipped: {self.left():.0f} s of the {self.budget:.0f} s bench budget left, " \
+                             f"leg needs ~{est_s:.0f} s"
+        print(f"# {name}: {self.skipped[name]}", file=sys.stderr, flush=True)
+        return False
+
+    def emit(self):
+        if self.line is None:
+            return
+        self.line["config"]["skipped_legs"] = dict(self.skipped)
+        self.line["config"]["bench_wall_s"] = time.perf_counter() - self.t0
+        print(json.dumps(self.line), flush=True)
+
+
+def solve_once(sv, p0=None):
+    sv.set_pressure(p0)
+    u, p, rep = sv.Solve(raise_on_error=False)
+    return u, p, rep
+
+
+def leg_dict(rep, precision, **extra):
+    d = {"solve_s": float(rep.solve_seconds), "status": int(rep.status), "uzawa_iterations": int(rep.uzawa_iterations),
+         "inner_iterations": int(rep.inner_iterations), "rel_residual": float(rep.rel_residual),
+         "div_u": float(rep.div_u), "momentum_residual": float(rep.momentum_residual),
+         "learned_fallback": int(rep.learned_fallback), "verify_rounds": int(rep.verify_rounds),
+         "precision": precision}
+    d.update(extra)
+    return d
+
+
+def rel_diff(a, b, mean_free=False):
+    if mean_free:
+        a, b = a - a.mean(), b - b.mean()
+    den = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (den if den > 0 else 1.0))
+
+
+PREC_PLAIN = "FP64 throughout (Uzawa, inner flexible CG, multigrid V-cycle, K apply)"
+PREC_MIXED = "FP64 Uzawa, inner CG vectors, dots and residuals; FP32 multigrid V-cycle (K applies) as preconditioner"
+PREC_NET = "network BF16 (MFMA, FP32 accumulation)"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=3, help="timed steps of the headline (learned) solve")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed warmup solves of the headline (learned) solve")
     ap.add_argument("--res", type=int, default=96)
     ap.add_argument("--ppc", type=int, default=16)
-    ap.add_argument("--apply-reps", type=int, default=20)
-    ap.add_argument("--cpu-res", type=int, default=20)
+    ap.add_argument("--apply-reps", type=int, default=30)
+    ap.add_argument("--cpu-res", type=int, default=16)
+    ap.add_argument("--budget", type=float, default=360.0,
+                    help="wall-clock budget (s) of the whole run; extra legs that would not fit are skipped")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--max-outer", type=int, default=20000)
     ap.add_argument("--plain-max-outer", type=int, default=0,
                     help="cap of the plain solve only (profiling the learned solve); 0 = --max-outer")
     ap.add_argument("--mg-smooth", type=int, default=2)
     ap.add_argument("--weights", default=os.path.join(ROOT, "gale_amd", "lpc_weights.npy"))
-    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true")
+    ap.add_argument("--no-yield", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--no-dvc", action="store_true", help="skip the DVC weights measurement")
     ap.add_argument("--only-dvc", action="store_true", help="measure only the DVC weights of the workload")
+    ap.add_argument("--only-learned", action="store_true",
+                    help="only the headline learned solve and the applies (profiling)")
     ap.add_argument("--inner-relax", type=float, default=0.1,
                     help="learned solve: cap of the relaxed inner tolerance (0 = reference 0.1 x tol throughout)")
     ap.add_argument("--profile-apply", action="store_true",
@@ -202,6 +302,7 @@ def main():
         profile_apply(args)
         return
 
+    em = Emitter(args.budget)
     import gale_amd as G
     from gale_amd import configs
 
@@ -213,31 +314,32 @@ def main():
     if args.only_dvc:
         print(json.dumps({"dvc_weights": bench_dvc(G, pb, cfg, population_control=True)}), flush=True)
         return
-    params = G.uzawa_params(tolerance=1e-6, maxIterations=args.plain_max_outer or args.max_outer, inner_max_iterations=20000,
-                            mg_smooth=args.mg_smooth, monitor=100)
+    params = G.uzawa_params(tolerance=1e-6, maxIterations=args.plain_max_outer or args.max_outer,
+                            inner_max_iterations=20000, mg_smooth=args.mg_smooth, monitor=500)
     s = G.Stokes_SLE_UzawaSolver(pb, params)
     s.setup()  # first linear solve of the Picard sequence: no yielding yet
 
     # roofline: the solve's dominant kernel is the FP64 moment-form K apply
-    # (every inner FCG iteration and every fine-level smoothing step); the
-    # particle-form apply (the per-material-point integration) is reported
-    # beside it.  Both timed with HIP events on the solver's stream.
-    ms_m = s.time_apply_K(G.APPLY_MOMENTS_FP64, args.apply_reps)
+    # (every inner FCG iteration and every fine-level smoothing step), timed
+    # cold with HIP events on the solver's stream (operand copies cycled so
+    # each apply streams from HBM, as inside a solve); the other forms and
+    # the pressure operators are reported beside it the same way.
+    ms_m = s.time_apply(G.APPLY_MOMENTS_FP64, args.apply_reps, cold=True)
+    ms_m_warm = s.time_apply(G.APPLY_MOMENTS_FP64, args.apply_reps, cold=False)
     bytes_m = k_moments_bytes(cfg.dim, cfg.N, cfg.E)
-    ms_m32 = s.time_apply_K(G.APPLY_MOMENTS_FP32, args.apply_reps)
+    ms_m32 = s.time_apply(G.APPLY_MOMENTS_FP32, args.apply_reps, cold=True)
     bytes_m32 = k_moments_bytes(cfg.dim, cfg.N, cfg.E, b=4)
-    ms = s.time_apply_K(G.APPLY_PARTICLES_FP64, args.apply_reps)
-    nbytes = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)
-    ms32 = s.time_apply_K(G.APPLY_PARTICLES_FP32, args.apply_reps)
-    nbytes32 = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P, b=4)
-    # the pressure operators of every Uzawa iteration: G p reads p (8 B per
-    # element) and the node flags (1 B per node) and writes dim values per node;
-    # G^T u reads dim values per node and writes one per element
-    ms_g = s.time_apply_K(G.APPLY_G, args.apply_reps)
+    ms_p = s.time_apply(G.APPLY_PARTICLES_FP64, args.apply_reps, cold=True)
+    bytes_p = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)
+    ms_p32 = s.time_apply(G.APPLY_PARTICLES_FP32, args.apply_reps, cold=True)
+    bytes_p32 = k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P, b=4)
+    # G p reads p (8 B per element) and the node flags (1 B per node) and
+    # writes dim values per node; G^T u reads dim values per node and writes
+    # one per element
+    ms_g = s.time_apply(G.APPLY_G, args.apply_reps, cold=True)
     bytes_g = 8 * cfg.E + 8 * cfg.dim * cfg.N + cfg.N
-    ms_gt = s.time_apply_K(G.APPLY_GT, args.apply_reps)
+    ms_gt = s.time_apply(G.APPLY_GT, args.apply_reps, cold=True)
     bytes_gt = 8 * cfg.dim * cfg.N + 8 * cfg.E
-    achieved = bytes_m / (ms_m * 1e-3) / 1e9
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "k_apply_traffic.json")
     if os.path.exists(tpath):
@@ -247,84 +349,171 @@ def main():
                 traffic = tr.get("bytes_per_apply")
         except Exception:
             traffic = None
-    print(f"# K apply moments fp64 {ms_m:.4f} ms ({achieved:.0f} GB/s), fp32 {ms_m32:.4f} ms; particles fp64 "
-          f"{ms:.4f} ms ({nbytes / ms / 1e6:.0f} GB/s), fp32 {ms32:.4f} ms", file=sys.stderr, flush=True)
-
-    for w in range(args.warmup):
-        s.set_pressure(None)
-        u, p, rep = s.Solve()
-        print(f"# warmup {w}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations}",
-              file=sys.stderr, flush=True)
-    times, reps = [], []
-    s.sync()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        s.set_pressure(None)
-        u, p, rep = s.Solve(raise_on_error=False)
-        times.append(rep.solve_seconds)
-        reps.append(rep.as_dict())
-        print(f"# step {k}: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations} "
-              f"rel={rep.rel_residual:.3e}", file=sys.stderr, flush=True)
-    s.sync()
-    wall = time.perf_counter() - t0
-    solve_s = float(np.mean(times))
-    last = reps[-1]
+    print(f"# K apply moments fp64 {ms_m:.4f} ms cold ({bytes_m / ms_m / 1e6:.0f} GB/s), {ms_m_warm:.4f} ms warm; "
+          f"fp32 {ms_m32:.4f} ms; particles fp64 {ms_p:.4f} ms, fp32 {ms_p32:.4f} ms; G {ms_g:.4f} ms, "
+          f"G^T {ms_gt:.4f} ms", file=sys.stderr, flush=True)
     n_u = pb.N * cfg.dim
     dofs = n_u + cfg.E
+    model, nproc, usable = cpu_info()
 
-    # learned Schur preconditioner + mixed precision (FP32 V-cycle inside the
-    # FP64 flexible inner CG), trained offline on 32^3 fields of the same
-    # generator (gale_amd/lpc_train.py); the plain solve above is the
+    line = {
+        "metric": METRIC, "value": None, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64+f32+bf16", "data": "synthetic",
+        "config": {"workload": f"random_viscosity_3d {args.res}^3 Q1-P0, {args.ppc} ppc, log10 eta GRF [-3,3] "
+                               f"+ plastic lid (first, linear solve of the Picard sequence), free slip, "
+                               f"Uzawa to 1e-6",
+                   "elements": cfg.E, "particles": cfg.P, "dofs": dofs, "generate_s": t_gen},
+        "roofline": roofline_obj(bytes_m, ms_m, "gs::k_tile_apply3<double,0> + gs::k_tile_halo3<double> "
+                                 "(moment-form FP64 K apply, fine level; cold: operand copies cycled past the "
+                                 "256 MB last-level cache)", traffic=traffic, ms_per_apply_warm=ms_m_warm),
+        "roofline_moments_fp32": roofline_obj(bytes_m32, ms_m32, "moment-form FP32 K apply (V-cycle fine level)"),
+        "roofline_particle_apply": roofline_obj(bytes_p, ms_p, "gs::k_tile_apply3<double,1> + halo (particle-form "
+                                                "FP64 K apply)", fp32_ms_per_apply=ms_p32,
+                                                fp32_algorithmic_bytes=bytes_p32),
+        "roofline_G": roofline_obj(bytes_g, ms_g, "gs::k_apply_G (G p)"),
+        "roofline_Gt": roofline_obj(bytes_gt, ms_gt, "gs::k_apply_Gt (G^T u)"),
+        "cpu_baseline": None,
+    }
+    line["roofline"]["frac"] = line["roofline"]["achieved"] / HBM_PEAK_GBS
+    em.line = line
+    C = line["config"]
+
+    # headline: learned Schur preconditioner + mixed precision (FP32 V-cycle
+    # inside the FP64 flexible inner CG) + relaxed inner tolerance with a
+    # true-residual check, network trained on 32^3 fields of the same
+    # generator (gale_amd/lpc_train.py).  The plain FP64 solve below is the
     # check every learned solve must meet.
-    learned = None
     wpath = args.weights
-    if not args.no_learned and os.path.exists(wpath):
+    net = None
+    if os.path.exists(wpath):
         net = G.LearnedPreconditioner(seed=1)
         net.set_params(np.load(wpath))
-        meta = {}
         mpath = os.path.splitext(wpath)[0] + ".json"
         if os.path.exists(mpath):
-            meta = {k: v for k, v in json.load(open(mpath)).items() if k != "loss_curve"}
-        sl = G.Stokes_SLE_UzawaSolver(pb, G.uzawa_params(tolerance=1e-6, maxIterations=args.max_outer,
+            C["training"] = {k: v for k, v in json.load(open(mpath)).items() if k != "loss_curve"}
+
+    def learned_solver(q_every=0, precision=G.PREC_MIXED, relax=args.inner_relax):
+        sv = G.Stokes_SLE_UzawaSolver(pb, G.uzawa_params(tolerance=1e-6, maxIterations=args.max_outer,
                                                          inner_max_iterations=20000, mg_smooth=args.mg_smooth,
-                                                         monitor=100, learned=True, precision=G.PREC_MIXED,
-                                                         inner_relax_max=args.inner_relax))
-        sl.set_learned(net)
+                                                         monitor=500, learned=True, precision=precision,
+                                                         inner_relax_max=relax, learned_q_every=q_every))
+        sv.set_learned(net)
+        return sv
+
+    ul = pl = None
+    learned = None
+    if net is not None:
+        sl = learned_solver()
         for w in range(args.warmup):
-            sl.set_pressure(None)
-            _, _, rl = sl.Solve(raise_on_error=False)
-            print(f"# learned warmup {w}: {rl.solve_seconds:.3f}s uzawa={rl.uzawa_iterations}",
+            _, _, rl = solve_once(sl)
+            print(f"# learned warmup {w}: {rl.solve_seconds:.3f}s uzawa={rl.uzawa_iterations} status={rl.status}",
                   file=sys.stderr, flush=True)
         ltimes = []
         sl.sync()
         t0 = time.perf_counter()
         for k in range(args.steps):
-            sl.set_pressure(None)
-            ul, pl, rl = sl.Solve(raise_on_error=False)
+            ul, pl, rl = solve_once(sl)
             ltimes.append(rl.solve_seconds)
             print(f"# learned step {k}: {rl.solve_seconds:.3f}s uzawa={rl.uzawa_iterations} "
                   f"inner={rl.inner_iterations} fallback={rl.learned_fallback} rel={rl.rel_residual:.3e}",
                   file=sys.stderr, flush=True)
         sl.sync()
         lwall = time.perf_counter() - t0
-        pm, plm = p - p.mean(), pl - pl.mean()
-        learned = {"solve_s": float(np.mean(ltimes)), "status": int(rl.status),
-                   "uzawa_iterations": int(rl.uzawa_iterations), "inner_iterations": int(rl.inner_iterations),
-                   "fallback_at": int(rl.learned_fallback), "rel_residual_true": float(rl.rel_residual),
-                   "verify_rounds": int(rl.verify_rounds), "inner_relax_max": args.inner_relax,
-                   "u_rel_diff_vs_plain": float(np.linalg.norm(ul - u) / np.linalg.norm(u)),
-                   "p_rel_diff_vs_plain": float(np.linalg.norm(plm - pm) / np.linalg.norm(pm)),
-                   "speedup_vs_plain": solve_s / float(np.mean(ltimes)), "wall_s": lwall, "training": meta}
+        learned = leg_dict(rl, PREC_MIXED + "; " + PREC_NET + "; inner rtol relaxed up to "
+                           f"{args.inner_relax} with a true-residual check", wall_s=lwall,
+                           solve_s_each=[float(v) for v in ltimes])
+        learned["solve_s"] = float(np.mean(ltimes))
+        C["learned"] = learned
+        if learned["status"] == 0:
+            line["value"] = learned["solve_s"]
+            line["ms_per_step"] = lwall / max(args.steps, 1) * 1e3
+            C["value_is"] = ("learned Schur preconditioner + mixed precision (FP32 V-cycle in FP64 flexible inner "
+                             "CG) + relaxed inner tolerance with a true-residual check; mean device time per solve")
+            C["dof_per_s"] = dofs / learned["solve_s"]
+            em.emit()
         sl.close()
+    if args.only_learned:
+        em.emit()
+        return
 
-    batch = None
-    if not args.no_batch:
-        batch = bench_batch(G, configs, args.batch_count)
-    dvc = None if args.no_dvc else bench_dvc(G, pb, cfg)
+    # plain FP64 GPU Uzawa (the reference algorithm), timed once: its ~40 s
+    # would otherwise dominate the run
+    u, p, rep = solve_once(s)
+    C["plain_runs"] = 1
+    C["plain"] = leg_dict(rep, PREC_PLAIN, method="reference Uzawa, diagonal Q, inner FP64 flexible CG + "
+                          "multigrid V-cycle at 0.1 x tol", dof_per_s=dofs / rep.solve_seconds)
+    C["plain_fp64_solve_s"] = float(rep.solve_seconds)
+    print(f"# plain: {rep.solve_seconds:.3f}s uzawa={rep.uzawa_iterations} inner={rep.inner_iterations} "
+          f"rel={rep.rel_residual:.3e} status={rep.status}", file=sys.stderr, flush=True)
+    if line["value"] is None:
+        line["value"] = float(rep.solve_seconds)
+        line["ms_per_step"] = rep.solve_seconds * 1e3
+        line["steps"], line["warmup"] = 1, 0
+        C["value_is"] = "plain FP64 Uzawa (the learned solve did not converge or no network was found)"
+        C["dof_per_s"] = dofs / rep.solve_seconds
+    if learned is not None and learned["status"] == 0:
+        learned["speedup_vs_plain"] = rep.solve_seconds / learned["solve_s"]
+        learned["u_rel_diff_vs_plain"] = rel_diff(ul, u)
+        learned["p_rel_diff_vs_plain"] = rel_diff(pl, p, mean_free=True)
+    em.emit()
 
-    cpu = None
-    if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+    # ablation: what each change contributes, each solve once
+    if net is not None and not args.no_ablation:
+        abl = {}
+        legs = [
+            ("no_network_mixed_relaxed", dict(q_every=1), PREC_MIXED + "; no network (every iteration Q^-1 r); "
+             "relaxed inner rtol; same truncated flexible outer loop as the headline"),
+            ("network_mixed_no_relax", dict(relax=0.0), PREC_MIXED + "; " + PREC_NET + "; inner rtol 0.1 x tol"),
+            ("network_fp64_no_relax", dict(relax=0.0, precision=G.PREC_FP64), PREC_PLAIN + "; " + PREC_NET +
+             "; inner rtol 0.1 x tol"),
+        ]
+        est = {"no_network_mixed_relaxed": 30.0, "network_mixed_no_relax": 15.0, "network_fp64_no_relax": 20.0}
+        for name, kw, prec in legs:
+            if not em.fits(f"ablation.{name}", est[name]):
+                continue
+            sv = learned_solver(**kw)
+            ua, pa, ra = solve_once(sv)
+            sv.close()
+            abl[name] = leg_dict(ra, prec, speedup_vs_plain=rep.solve_seconds / ra.solve_seconds,
+                                 u_rel_diff_vs_plain=rel_diff(ua, u), p_rel_diff_vs_plain=rel_diff(pa, p, True))
+            print(f"# ablation {name}: {ra.solve_seconds:.3f}s uzawa={ra.uzawa_iterations} "
+                  f"inner={ra.inner_iterations} status={ra.status}", file=sys.stderr, flush=True)
+            C["ablation"] = abl
+            if learned is not None and name == "no_network_mixed_relaxed" and ra.status == 0:
+                learned["network_speedup"] = ra.solve_seconds / learned["solve_s"]
+            em.emit()
+
+    # a yielded solve: the second (Picard) solve of the sequence, with the
+    # Drucker-Prager lid yielding from the first solution, warm-started from
+    # its pressure as the reference keeps pSolnVec
+    if not args.no_yield and em.fits("yielded", 10.0):
+        yl = {}
+        if net is not None and ul is not None:
+            sv = learned_solver()
+            sv.setup(ul, pl, yield_on=True)
+            uy, py, ry = solve_once(sv, pl)
+            sv.close()
+            yl["learned"] = leg_dict(ry, PREC_MIXED + "; " + PREC_NET + "; relaxed inner rtol")
+            print(f"# yielded learned: {ry.solve_seconds:.3f}s uzawa={ry.uzawa_iterations} status={ry.status}",
+                  file=sys.stderr, flush=True)
+        C["yielded"] = yl
+        em.emit()
+        if em.fits("yielded.plain", 1.3 * rep.solve_seconds + 10):
+            s.setup(u, p, yield_on=True)
+            uyp, pyp, ryp = solve_once(s, p)
+            yl["plain"] = leg_dict(ryp, PREC_PLAIN)
+            if "learned" in yl and yl["learned"]["status"] == 0 and ryp.status == 0:
+                yl["learned"]["speedup_vs_plain"] = ryp.solve_seconds / yl["learned"]["solve_s"]
+                yl["learned"]["u_rel_diff_vs_plain"] = rel_diff(uy, uyp)
+                yl["learned"]["p_rel_diff_vs_plain"] = rel_diff(py, pyp, True)
+            print(f"# yielded plain: {ryp.solve_seconds:.3f}s uzawa={ryp.uzawa_iterations} status={ryp.status}",
+                  file=sys.stderr, flush=True)
+            em.emit()
+
+    if not args.no_cpu and em.fits("cpu_baseline", 60.0):
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or usable
+        threads = max(1, min(threads, usable))
         t = time.perf_counter()
         cb = cpu_baseline(args.cpu_res, threads)
         print(f"# cpu baseline {cb} in {time.perf_counter() - t:.1f}s", file=sys.stderr, flush=True)
@@ -332,60 +521,25 @@ def main():
         cfg_s = configs.random_viscosity_3d(res=args.cpu_res, ppc=16)
         ps = G.Problem(**cfg_s.problem_kwargs())
         ss = G.Stokes_SLE_UzawaSolver(ps, G.uzawa_params(tolerance=1e-6, inner_max_iterations=20000))
-        _, _, rs = ss.Solve()
-        cpu = {"value": cb["solve_s"], "unit": "s", "cores": threads, "kind": "port",
-               "sample": f"random_viscosity_3d at {args.cpu_res}^3 ({cb['E']} elements, {cb['P']} particles): "
-                         f"oracle assembled CSR + reference Uzawa, GMRES(30)+ILU(0) inner (PETSc defaults), "
-                         f"{cb['uzawa_iterations']} Uzawa its; assembly {cb['assemble_s']:.2f}s not included",
-               "gpu_same_sample_s": rs.solve_seconds,
-               "status": cb["status"]}
-    ok_learned = learned is not None and learned["status"] == 0
-    line = {
-        "metric": METRIC,
-        "value": learned["solve_s"] if ok_learned else solve_s,
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": (learned["wall_s"] if ok_learned else wall) / max(args.steps, 1) * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": f"random_viscosity_3d {args.res}^3 Q1-P0, {args.ppc} ppc, log10 eta GRF [-3,3] "
-                               f"+ plastic lid (first, linear solve), free slip, Uzawa to 1e-6",
-                   "value_is": ("learned Schur preconditioner + mixed precision (FP32 V-cycle in FP64 flexible "
-                                "inner CG) + relaxed inner tolerance with a true-residual check"
-                                if ok_learned else "plain FP64 Uzawa (diagonal Q)"),
-                   "plain_fp64_solve_s": solve_s,
-                   "plain": "diagonal Q, inner FP64 flexible CG + multigrid V-cycle at 0.1 x tol",
-                   "learned": learned,
-                   "elements": cfg.E, "particles": cfg.P, "dofs": dofs,
-                   "uzawa_iterations": last["uzawa_iterations"], "inner_iterations": last["inner_iterations"],
-                   "rel_residual": last["rel_residual"], "dof_per_s": dofs / solve_s,
-                   "setup_s": last["setup_seconds"], "k_applies": last["k_applies"],
-                   "k_moments_fp32_ms": ms_m32, "k_moments_fp32_gbs": bytes_m32 / (ms_m32 * 1e-3) / 1e9},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "gs::k_tile_apply3<double,0> + gs::k_tile_halo3<double> (moment-form K apply, fine level)",
-                     "ms_per_apply": ms_m, "algorithmic_bytes": bytes_m},
-        "roofline_particle_apply": {"bound": "hbm", "achieved": nbytes / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                                    "unit": "GB/s", "frac": nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                                    "kernel": "gs::k_tile_apply3<double,1> + gs::k_tile_halo3<double> (particle-form K apply)",
-                                    "ms_per_apply": ms, "algorithmic_bytes": nbytes,
-                                    "fp32_ms_per_apply": ms32, "fp32_algorithmic_bytes": nbytes32},
-        "roofline_G": {"bound": "hbm", "achieved": bytes_g / (ms_g * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": bytes_g / (ms_g * 1e-3) / 1e9 / HBM_PEAK_GBS, "kernel": "gs::k_apply_G (G p)",
-                       "ms_per_apply": ms_g, "algorithmic_bytes": bytes_g},
-        "roofline_Gt": {"bound": "hbm", "achieved": bytes_gt / (ms_gt * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": bytes_gt / (ms_gt * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                        "kernel": "gs::k_apply_Gt (G^T u)", "ms_per_apply": ms_gt, "algorithmic_bytes": bytes_gt},
-        "cpu_baseline": cpu,
-        "batch": batch,
-        "dvc_weights": dvc,
-    }
-    print(json.dumps(line), flush=True)
+        _, _, rs = ss.Solve(raise_on_error=False)
+        ss.close()
+        ps.close()
+        line["cpu_baseline"] = {
+            "value": cb["solve_s"], "unit": "s", "cores": threads, "kind": "port",
+            "sample": f"random_viscosity_3d at {args.cpu_res}^3 ({cb['E']} elements, {cb['P']} particles), same "
+                      f"generator as the workload: oracle assembled CSR + reference Uzawa, GMRES(30)+ILU(0) inner "
+                      f"(PETSc defaults), {cb['uzawa_iterations']} Uzawa its, OpenMP; assembly "
+                      f"{cb['assemble_s']:.2f}s not included",
+            "gpu_same_sample_plain_fp64_s": rs.solve_seconds, "status": cb["status"],
+            "host_cpu_model": model, "host_nproc": nproc, "host_usable_cpus": usable}
+        em.emit()
+
+    if not args.no_batch and em.fits("batch", 60.0):
+        C["batch"] = bench_batch(G, configs, args.batch_count)
+        em.emit()
+    if not args.no_dvc and em.fits("dvc_weights", 20.0):
+        C["dvc_weights"] = bench_dvc(G, pb, cfg)
+    em.emit()
 
 
 if __name__ == "__main__":

@@ -35,9 +35,9 @@ WEIGHTS_CONSTANT, WEIGHTS_DVC = 0, 1
 EXPORTS = (
     "gs_strerror", "gs_version", "gs_device_count", "gs_problem_create", "gs_problem_destroy", "gs_problem_info",
     "gs_problem_particles", "gs_solver_create", "gs_solver_set_params", "gs_solver_destroy", "gs_solver_setup",
-    "gs_stokes_solve", "gs_solver_set_pressure", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
+    "gs_stokes_solve", "gs_solver_set_pressure", "gs_solver_set_solution", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
-    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_solver_setup_arrays", "gs_apply_K_moments",
+    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_time_apply", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_crand_stream", "gs_initial_damage", "gs_wall_bc",
     "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
     "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time",
@@ -144,6 +144,7 @@ def lib():
             "gs_solver_setup": (i32, [vp, vp, vp, i32]),
             "gs_stokes_solve": (i32, [vp, vp, vp, vp]),
             "gs_solver_set_pressure": (i32, [vp, vp]),
+            "gs_solver_set_solution": (i32, [vp, vp, vp]),
             "gs_picard_solve": (i32, [vp, vp, vp, vp, vp]),
             "gs_get_rhs": (i32, [vp, vp]),
             "gs_form_residual": (i32, [vp, vp, vp]),
@@ -167,6 +168,7 @@ def lib():
             "gs_apply_Gt": (i32, [vp, vp, vp]),
             "gs_solver_sync": (i32, [vp]),
             "gs_time_apply_K": (i32, [vp, i32, i32, vp]),
+            "gs_time_apply": (i32, [vp, i32, i32, i32, vp]),
             "gs_solver_setup_arrays": (i32, [vp, vp, vp, vp, vp, vp]),
             "gs_rng_stream": (d, [i64, i64, vp]),
             "gs_layout_random": (i64, [i32, vp, vp, vp, i32, i64, vp]),
@@ -786,6 +788,15 @@ class Stokes_SLE_UzawaSolver:
         p = _f64(p)
         _check(lib().gs_solver_set_pressure(self._h, _ptr(p)))
 
+    def set_solution(self, u, p):
+        """Restart from a previous solution (full nodal velocity, pressure):
+        the next setup yields from it, as the reference does when it loads
+        from a checkpoint."""
+        u, p = _f64(u), _f64(p)
+        if u is None or p is None or u.size != self.n or p.size != self.np:
+            raise GaleError(E_INVALID, "set_solution: u (N*dim) and p (E) are required")
+        _check(lib().gs_solver_set_solution(self._h, _ptr(u), _ptr(p)), "set_solution")
+
     def NonLinearExecute(self, nonLinearTolerance=0.01, nonLinearMaxIterations=500, nonLinearMinIterations=1,
                          killNonConvergent=False, raise_on_error=True):
         pp = PicardParams(nonLinearTolerance, nonLinearMaxIterations, nonLinearMinIterations,
@@ -871,6 +882,13 @@ class Stokes_SLE_UzawaSolver:
     def time_apply_K(self, form=APPLY_PARTICLES_FP64, n_rep=20):
         ms = C.c_double()
         _check(lib().gs_time_apply_K(self._h, form, n_rep, C.byref(ms)), "time_apply_K")
+        return ms.value
+
+    def time_apply(self, form=APPLY_MOMENTS_FP64, n_rep=20, cold=True):
+        """Average device ms per apply; cold=True cycles through operand copies
+        larger than the last-level cache so each apply streams from HBM."""
+        ms = C.c_double()
+        _check(lib().gs_time_apply(self._h, form, n_rep, 1 if cold else 0, C.byref(ms)), "time_apply")
         return ms.value
 
 

@@ -127,6 +127,27 @@ def lithosphere_2d(nx=256, ny=64, ppc=40, seed=13) -> Config:
                   eta_clamp=(1e-3, 1e3), picard=dict(nonLinearTolerance=1e-4, nonLinearMaxIterations=20))
 
 
+def lithosphere_3d(nx=16, ny=4, nz=4, ppc=10, seed=13) -> Config:
+    """3D counterpart of lithosphere_2d: a 4 x 1 x 1 box, lid y >= 0.75
+    Drucker-Prager (cohesion 1e-2, friction angle 30 deg, eta 1e2, minimum
+    viscosity 1e-3), mantle eta 1, a weak 0.05 x 0.05 x 1 seed of eta 1e-2 at
+    x = 2.0, y = 0.875 through the depth; vx -1 / +1 on the left / right
+    walls, vy 0 on the base, vz 0 on the back and front walls, free top;
+    rho 0; viscosity clamp [1e-3, 1e3]; Picard <= 20 to 1e-4."""
+    dim, r = 3, (nx, ny, nz)
+    lo, hi = (0.0, 0.0, 0.0), (4.0, 1.0, 1.0)
+    coord = layout_random(dim, r, lo, hi, ppc, seed)
+    lid = material(eta0=1e2, density=0.0, yield_type=YIELD_DRUCKER_PRAGER, cohesion=1e-2,
+                   friction=math.tan(math.radians(30.0)), min_viscosity=1e-3)
+    mats = [material(eta0=1.0), lid, material(eta0=1e-2)]
+    mat = shape_layout(dim, coord, [everywhere(0), box(1, (0.0, 0.75, 0.0), (4.0, 1.0, 1.0)),
+                                    box(2, (1.975, 0.85, 0.0), (2.025, 0.9, 1.0))])
+    mask, vals = wall_bc(dim, r, [("left", 0, -1.0), ("right", 0, 1.0), ("bottom", 1, 0.0), ("back", 2, 0.0),
+                                  ("front", 2, 0.0)])
+    return Config("lithosphere_3d", dim, r, lo, hi, coord, mat, mats, mask, vals, gravity=0.0,
+                  eta_clamp=(1e-3, 1e3), picard=dict(nonLinearTolerance=1e-4, nonLinearMaxIterations=20))
+
+
 def rayleigh_taylor_3d(res=64, ppc=20, seed=13, mode_seed=1003, steps=10) -> Config:
     """3D Rayleigh-Taylor: unit cube, dense layer (rho 1, eta 1e2) over light
     (rho 0, eta 1); interface at y = 0.5 + four seeded sine modes whose
@@ -181,6 +202,7 @@ def random_viscosity_3d(res=96, ppc=16, seed=13, field_seed=1005, corr=0.1) -> C
 BUILDERS = {
     "sinker_2d": sinker_2d,
     "lithosphere_2d": lithosphere_2d,
+    "lithosphere_3d": lithosphere_3d,
     "rayleigh_taylor_3d": rayleigh_taylor_3d,
     "random_viscosity_3d": random_viscosity_3d,
 }

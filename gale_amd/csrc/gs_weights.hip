@@ -337,7 +337,7 @@ __global__ void k_pcdvc_decide(Geo g, const int* __restrict__ off, const int* __
   if (mD > n - 4) mD = n / 2;
   const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
                             g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
-  int ns = 0, nd = 0, last = -1;
+  int ns = 0, last = -1;
   for (int t = 0; t < n; t++) {
     // next particle in pid order
     int j = -1;
@@ -362,11 +362,21 @@ __global__ void k_pcdvc_decide(Geo g, const int* __restrict__ off, const int* __
       sparent[(long long)e * a.slots + ns] = b + j;
       ns++;
     }
-    if (wj < a.min_w && nd < mD) {
-      del[b + j] = 1;
-      nd++;
-    }
   }
+  // deletions: of the particles below minW, the reference sorts the list by
+  // swarm index in descending order and deletes its first maxDeletions
+  // (compare_indexOnCPU, PCDVC.cxx:611-620, :974, :1015-1021), i.e. the
+  // candidates with the highest indices
+  int ncand = 0;
+  for (int j = 0; j < n; j++) ncand += w[b + j] < a.min_w ? 1 : 0;
+  const int nD = mD < ncand ? mD : ncand;
+  if (nD > 0)
+    for (int j = 0; j < n; j++) {
+      if (!(w[b + j] < a.min_w)) continue;
+      int higher = 0;
+      for (int k = 0; k < n; k++) higher += (w[b + k] < a.min_w && pid[b + k] > pid[b + j]) ? 1 : 0;
+      if (higher < nD) del[b + j] = 1;
+    }
   nsplit[e] = ns;
 }
 

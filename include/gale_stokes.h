@@ -190,8 +190,20 @@ int32_t gs_stokes_solve(gs_solver* s, double* u, double* p, gs_report* rep);
 int32_t gs_solver_set_pressure(gs_solver* s, const double* p);
 
 /* SystemLinearEquations_NonLinearExecute (SystemLinearEquations.cxx:735-920):
- * setup + solve, first iteration without yielding, then Picard. */
+ * setup + solve, then Picard.  The first solve yields only when a previous
+ * solution exists -- after this solver's first nonlinear execute (the
+ * reference's sle->hasExecuted, :794, read as previousSolutionExists by
+ * ConstitutiveMatrixCartesian.cxx:270-273 and YieldRheology.cxx:203-204) or
+ * after gs_solver_set_solution -- and then yields from the stored solution.
+ * gs_stokes_solve follows the same rule when it runs the setup itself; it
+ * does not mark a previous solution (a linear execute does not set
+ * hasExecuted). */
 int32_t gs_picard_solve(gs_solver* s, const gs_picard_params* pp, double* u, double* p, gs_report* rep);
+/* Restart: load a previous solution (u: full nodal velocity N*dim with the
+ * Dirichlet values; p: E) and mark it as existing, as a checkpoint restart
+ * does (ConstitutiveMatrix.cxx:127-130, context->loadFromCheckPoint): the
+ * next setup yields from it, and p is the next Uzawa initial guess. */
+int32_t gs_solver_set_solution(gs_solver* s, const double* u, const double* p);
 
 /* _Stokes_SLE_UzawaSolver_GetRhs: fhat = G^T K^-1 f - h  (E). */
 int32_t gs_get_rhs(gs_solver* s, double* fhat);
@@ -254,6 +266,12 @@ int32_t gs_solver_sync(gs_solver* s);
 /* Average device time (ms) of n_rep applies of the given form (GS_APPLY_*:
  * K in four forms, G p, G^T u), timed with HIP events on the solver's stream. */
 int32_t gs_time_apply_K(gs_solver* s, int32_t form, int32_t n_rep, double* ms_per_apply);
+/* The same with cold != 0: the applies cycle through copies of their operands
+ * (input and output vectors, moments or particle stream) totalling more than
+ * three times the 256 MB last-level cache, so each apply streams from HBM as
+ * it does inside a solve.  cold = 0 is gs_time_apply_K.  No reference
+ * counterpart (measurement only). */
+int32_t gs_time_apply(gs_solver* s, int32_t form, int32_t n_rep, int32_t cold, double* ms_per_apply);
 /* K apply from the per-element particle moments (the form the solver uses;
  * equal to gs_apply_K up to rounding).  Device pointers, full nodal vectors. */
 int32_t gs_apply_K_moments(gs_solver* s, const double* x_dev, double* y_dev);

@@ -113,6 +113,7 @@ def lib():
             "orc_uzawa_Q": (None, [vp, vp]),
             "orc_uzawa": (i32, [vp, vp, vp, vp, vp, vp]),
             "orc_picard": (i32, [vp, vp, vp, vp, vp, vp]),
+            "orc_picard_from": (i32, [vp, vp, vp, vp, vp, vp, vp, vp]),
             "orc_compute_dt": (d, [vp, vp, d]),
             "orc_interpolate_velocity": (i32, [vp, vp, vp, vp]),
             "orc_advect_rk2": (i32, [vp, vp, d, vp]),
@@ -482,13 +483,19 @@ class Problem:
         st = lib().orc_uzawa(self._h, C.byref(params), _p(p0), _p(u), _p(p), C.byref(rep))
         return st, u, p, rep
 
-    def picard(self, params=None, tolerance=0.01, max_iterations=500, min_iterations=1, kill=0, **kw):
+    def picard(self, params=None, tolerance=0.01, max_iterations=500, min_iterations=1, kill=0, u_prev=None,
+               p_prev=None, **kw):
+        """Picard loop; u_prev (full nodal) / p_prev: the previous solution
+        (a previous time step's, or a restart's): the first solve then yields
+        from it and starts from p_prev."""
         params = params or uzawa_params(**kw)
         pp = PicardParams(tolerance, max_iterations, min_iterations, kill, 0)
         u = np.zeros(self.N * self.dim)
         p = np.zeros(self.E)
         rep = Report()
-        st = lib().orc_picard(self._h, C.byref(params), C.byref(pp), _p(u), _p(p), C.byref(rep))
+        u_prev, p_prev = _f64(u_prev), _f64(p_prev)
+        st = lib().orc_picard_from(self._h, C.byref(params), C.byref(pp), _p(u_prev), _p(p_prev), _p(u), _p(p),
+                                   C.byref(rep))
         return st, u, p, rep
 
     def compute_dt(self, u, dt_factor=1.0):

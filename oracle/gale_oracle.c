@@ -645,8 +645,8 @@ static int relocate_all(orc_problem* pb) {
  * nump/2 when it exceeds nump - 4; the first maxSplits particles (in cell
  * order) with weight above maxW are split -- a copy of the material point is
  * put at the centroid of the parent's region, at the global position
- * 0.5 ((1 - xi) a + (1 + xi) b) per axis -- and the first maxDeletions with
- * weight below minW are deleted.  The particle order afterwards: the
+ * 0.5 ((1 - xi) a + (1 + xi) b) per axis -- and of the particles with weight
+ * below minW the maxDeletions with the highest swarm indices are deleted.  The particle order afterwards: the
  * surviving particles in their previous order, then the new ones in cell
  * order and split order.  counts = {added, deleted}. */
 int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int64_t* counts) {
@@ -703,7 +703,19 @@ int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int6
         nparent[nnew++] = p;
         ns++;
       }
-      if (ws[q] < minW && nd < mD) {
+    }
+    /* deletions: the candidates below minW sorted by swarm index, highest
+     * first (compare_indexOnCPU, PCDVC.cxx:611-620, qsort :974), and the first
+     * maxDeletions of that list deleted (:1015-1021) */
+    int64_t ncand = 0;
+    for (int64_t q = 0; q < n; q++) ncand += ws[q] < minW;
+    const int64_t nD = mD < ncand ? mD : ncand;
+    for (int64_t q = 0; q < n && nD > 0; q++) {
+      if (!(ws[q] < minW)) continue;
+      const int64_t p = pb->cell_list[b + q];
+      int64_t higher = 0;
+      for (int64_t t = 0; t < n; t++) higher += ws[t] < minW && pb->cell_list[b + t] > p;
+      if (higher < nD) {
         del[p] = 1;
         nd++;
       }
@@ -1806,12 +1818,19 @@ int32_t orc_uzawa(orc_problem* pb, const orc_uzawa_params* up, const double* p0,
   return st;
 }
 
-/* Picard loop: SystemLinearEquations.cxx:735-920.  Iteration 0 is linear
- * (YieldRheology.cxx:203-204); the pressure vector carries over between
- * linear solves as the Uzawa initial guess; the residual is
+/* Picard loop: SystemLinearEquations.cxx:735-920.  The first solve yields
+ * only when a previous solution exists (YieldRheology.cxx:203-204 with
+ * previousSolutionExists = sle->hasExecuted, ConstitutiveMatrixCartesian.cxx
+ * :270-273; hasExecuted is set after the first solve of the first nonlinear
+ * execute, SystemLinearEquations.cxx:794, and stays set for later time steps,
+ * or is set up front on a restart, ConstitutiveMatrix.cxx:127-130).
+ * u_prev_full / p_prev: that previous solution (NULL = none: the first solve
+ * is linear and starts from p = 0); p_prev is also the first Uzawa initial
+ * guess, as the reference keeps pSolnVec.  The pressure vector carries over
+ * between linear solves as the Uzawa initial guess; the residual is
  * |u_prev - u| / |u| over the free velocity dofs. */
-int32_t orc_picard(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp, double* u_full,
-                   double* p, orc_report* rep) {
+int32_t orc_picard_from(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp,
+                        const double* u_prev_full, const double* p_prev, double* u_full, double* p, orc_report* rep) {
   memset(rep, 0, sizeof(*rep));
   const int64_t ND = pb->N * pb->dim;
   double* uf_full = (double*)calloc(ND, sizeof(double));
@@ -1819,8 +1838,12 @@ int32_t orc_picard(orc_problem* pb, const orc_uzawa_params* up, const orc_picard
   double *uprev = NULL, *ucur = NULL;
   orc_report r1;
   int32_t st;
+  if (u_prev_full && p_prev) memcpy(q, p_prev, sizeof(double) * pb->E);
   double ta = now_s();
-  st = orc_assemble(pb, NULL, NULL, 0, up->nthreads);
+  if (u_prev_full && p_prev)
+    st = orc_assemble(pb, u_prev_full, p_prev, 1, up->nthreads);
+  else
+    st = orc_assemble(pb, NULL, NULL, 0, up->nthreads);
   rep->assemble_seconds += now_s() - ta;
   if (st) goto out;
   memset(&r1, 0, sizeof(r1));
@@ -1874,6 +1897,11 @@ out:
   free(uprev);
   free(ucur);
   return st;
+}
+
+int32_t orc_picard(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp, double* u_full,
+                   double* p, orc_report* rep) {
+  return orc_picard_from(pb, up, pp, NULL, NULL, u_full, p, rep);
 }
 
 /* ------------------------------------------------------------------------ */

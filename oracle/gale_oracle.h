@@ -226,6 +226,10 @@ int32_t orc_uzawa(orc_problem* pb, const orc_uzawa_params* up, const double* p0,
                   double* u_full, double* p, orc_report* rep);
 int32_t orc_picard(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp,
                    double* u_full, double* p, orc_report* rep);
+/* The same with a previous solution (u_prev_full N*dim, p_prev E; NULL =
+ * none): the first solve then yields from it and starts from p_prev. */
+int32_t orc_picard_from(orc_problem* pb, const orc_uzawa_params* up, const orc_picard_params* pp,
+                        const double* u_prev_full, const double* p_prev, double* u_full, double* p, orc_report* rep);
 
 /* time stepping */
 double orc_compute_dt(orc_problem* pb, const double* u_full, double dt_factor);

@@ -181,7 +181,10 @@ def expected_population_control(coord, mat, maxS=3, maxD=2, upT=25.0, lowT=0.6):
         if mD > n - 4:
             mD = n // 2
         s_ = [q for q in range(n) if w[q] > upT * 4 / 100][:mS]
-        d_ = [q for q in range(n) if w[q] < lowT * 4 / 100][:mD]
+        # deletion candidates sorted by swarm index, highest first; the first
+        # maxDeletions go (compare_indexOnCPU + qsort, PCDVC.cxx:611-620, :974)
+        cand = [q for q in range(n) if w[q] < lowT * 4 / 100]
+        d_ = sorted(cand, key=lambda q: idx[q], reverse=True)[:mD]
         for q in d_:
             keep[idx[q]] = False
         for q in s_:
@@ -199,8 +202,11 @@ def test_oracle_population_control_designed_cell(oracle):
     # cell 0 by construction: its far particle splits, two clustered ones go
     cent, w = oracle.dvc_cell(2, (10, 10), 2.0 * coord[:16, :2] - 1.0)
     assert [i for i in range(16) if w[i] > 1.0][:3] == [15]
-    assert len([i for i in range(16) if w[i] < 0.024]) >= 2
+    cand = [i for i in range(16) if w[i] < 0.024]
+    assert len(cand) >= 3  # more candidates than maxDeletions = 2: the choice matters
     keep, new_xy, new_mat = expected_population_control(coord, mat)
+    # the two highest-indexed candidates go, the lower ones stay
+    assert not keep[cand[-1]] and not keep[cand[-2]] and all(keep[i] for i in cand[:-2])
     added, deleted = o.population_control()
     owner, xi, wt, coord2 = o.particles()
     assert (added, deleted) == (len(new_xy), int((~keep).sum()))

@@ -247,3 +247,33 @@ def test_shipped_network_relaxed_solve_accuracy(gale):
     assert r1.status == 0 and r1.rel_residual <= 1e-6 and r1.verify_rounds >= 1
     assert rel(u1, ur) <= 2 * rel(u0, ur) + 1e-6
     assert rel(p1, pr) <= 2 * rel(p0, pr) + 1e-3
+
+
+def test_shipped_network_gain_over_same_loop_without_it(gale):
+    """What the trained network gains: the learned solve against the same
+    truncated flexible outer loop (mixed precision, relaxed inner solves,
+    true-residual check) with every iteration preconditioned by Q alone, on a
+    32^3 field of the generator that training did not see.  Both reach the
+    1e-6 true residual; the network needs at most half the Uzawa iterations
+    and fewer inner iterations."""
+    import os
+    from conftest import ROOT
+    net = gale.LearnedPreconditioner(seed=1)
+    net.set_params(np.load(os.path.join(ROOT, "gale_amd", "lpc_weights.npy")))
+    cfg = _rv(32, 4243, ppc=8)
+    pb = gale.Problem(**cfg.problem_kwargs())
+    out = {}
+    for q_every in (0, 1):
+        s = gale.Stokes_SLE_UzawaSolver(pb, gale.uzawa_params(tolerance=1e-6, maxIterations=40000, learned=True,
+                                                              precision=gale.PREC_MIXED, inner_relax_max=0.1,
+                                                              learned_q_every=q_every))
+        s.set_learned(net)
+        u, p, rep = s.Solve()
+        s.close()
+        out[q_every] = rep
+        print(f"{'network' if q_every == 0 else 'Q only'}: uzawa {rep.uzawa_iterations} inner "
+              f"{rep.inner_iterations} {rep.solve_seconds:.3f}s rel {rep.rel_residual:.2e}", flush=True)
+        assert rep.status == 0 and rep.rel_residual <= 1e-6
+    assert out[0].uzawa_iterations <= 0.5 * out[1].uzawa_iterations
+    assert out[0].inner_iterations < out[1].inner_iterations
+    pb.close()

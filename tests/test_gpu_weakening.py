@@ -34,9 +34,13 @@ def test_weakening_time_steps_match_oracle(gale, oracle):
     g.set_plastic_strain(w0)
     o.set_plastic_strain(w0)
     s = gale.Stokes_SLE_UzawaSolver(g, tight(gale, tolerance=1e-11))
+    ou = op = None
     for step in range(3):
+        # from the second step on the first solve yields from the previous
+        # step's solution (previousSolutionExists = sle->hasExecuted)
         u, p, rep = s.NonLinearExecute(nonLinearTolerance=1e-4, nonLinearMaxIterations=20)
-        st, ou, op, orep = o.picard(otight(oracle, tolerance=1e-11), tolerance=1e-4, max_iterations=20)
+        st, ou, op, orep = o.picard(otight(oracle, tolerance=1e-11), tolerance=1e-4, max_iterations=20,
+                                    u_prev=ou, p_prev=op)
         print(f"step {step}: picard {rep.picard_iterations} / {orep.picard_iterations}", flush=True)
         assert st == 0 and rep.status == 0 and rep.picard_iterations == orep.picard_iterations
         assert rel(u, ou) < 1e-8

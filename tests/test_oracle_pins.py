@@ -217,6 +217,93 @@ def test_drucker_prager_2d_closed_form(oracle):
     np.testing.assert_allclose(pb.viscosity(u, p, 1)[1], C / np.sqrt(1 + mu * mu) / gamma, rtol=1e-13)
 
 
+def test_drucker_prager_3d_closed_form(oracle):
+    # 3D simple shear u = (gamma y, 0, 0): strain-rate invariant gamma/2,
+    # stress invariant eta gamma.  DruckerPrager.cxx:376-405 in 3D:
+    # f = 2 cos(phi) / (sqrt(3) (3 - sin(phi))), tau_y = mu f p + 3 C f,
+    # clamped below at minimumYieldStress f (minimumYieldStress = C when 0);
+    # the yielded viscosity is tau_y / (2 eps_II) = tau_y / gamma.
+    gamma, eta0, C, mu, pres = 4.0, 10.0, 0.5, np.tan(np.radians(30)), 2.0
+    cphi = 1.0 / np.sqrt(1.0 + mu * mu)
+    sphi = mu * cphi
+    f = 2.0 * cphi / (np.sqrt(3.0) * (3.0 - sphi))
+    res = (3, 3, 2)
+    coord = oracle.layout_random(3, res, (0, 0, 0), (1, 1, 1), 5, 7)
+
+    def visc(mats, p_elem):
+        pb = oracle.Problem(3, res, (0, 0, 0), (1, 1, 1), coord, np.zeros(len(coord), np.int32), mats)
+        X = node_coords(3, res)
+        u = np.stack([gamma * X[:, 1], 0 * X[:, 0], 0 * X[:, 0]], 1).reshape(-1)
+        return pb.viscosity(u, np.full(pb.E, p_elem), 1)
+
+    mats = [dict(eta0=eta0, yield_type=1, cohesion=C, friction=mu, min_viscosity=1e-3)]
+    st, eta, yl = visc(mats, pres)
+    tau_y = mu * f * pres + 3.0 * C * f
+    assert tau_y < eta0 * gamma and yl.all()
+    np.testing.assert_allclose(eta, tau_y / gamma, rtol=1e-13)
+    # the 2D criterion would give a different value: the 3D branch is the one taken
+    assert abs(tau_y - (mu * pres + C) * cphi) > 1e-3
+    # tension below the minimum yield stress: clamp at C f (no factor 3)
+    st, eta, yl = visc(mats, -100.0)
+    np.testing.assert_allclose(eta, C * f / gamma, rtol=1e-13)
+    # an explicit minimumYieldStress replaces C in the clamp
+    mats2 = [dict(mats[0], min_yield_stress=0.8)]
+    st, eta, yl = visc(mats2, -100.0)
+    np.testing.assert_allclose(eta, 0.8 * f / gamma, rtol=1e-13)
+    # minimum viscosity floor
+    mats3 = [dict(mats[0], min_viscosity=1.0)]
+    np.testing.assert_allclose(visc(mats3, pres)[1], 1.0)
+    # below yield: unchanged
+    mats4 = [dict(mats[0], cohesion=50.0)]
+    st, eta, yl = visc(mats4, pres)
+    np.testing.assert_array_equal(eta, eta0)
+    assert not yl.any()
+
+
+def _q_closed_form(dim, h, eta):
+    """UzawaPreconditionerTerm.cxx:195-215 on one Q1-P0 element with exact
+    integration: Q_e = sum over the element's velocity dofs v of
+    G_e[v]^2 / K_e[v][v] (the element's own K diagonal).  On a box of sides
+    h, trilinear N_a: |int dN_a/dx_d| = prod_{c != d} h_c / 2 and
+    int (dN_a/dx_c)^2 = prod_{c' != c} (h_c' / 3) / h_c, with the isotropic
+    D = eta diag(2, .., 2, 1, .., 1): K_dd = eta sum_c (1 + [c == d]) int (dN_a/dx_c)^2."""
+    h = np.asarray(h, float)
+    q = 0.0
+    for d in range(dim):
+        area = np.prod([h[c] for c in range(dim) if c != d])
+        g2 = (area / 2 ** (dim - 1)) ** 2
+        kdd = 0.0
+        for c in range(dim):
+            ic = np.prod([h[k] / 3.0 for k in range(dim) if k != c]) / h[c]
+            kdd += (2.0 if c == d else 1.0) * ic
+        q += 2 ** dim * g2 / (eta * kdd)
+    return q
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_uzawa_Q_closed_form(oracle, dim):
+    """The oracle's Uzawa preconditioner equals the element-level closed form
+    on every element -- also on boundary elements with Dirichlet dofs, where
+    an assembled (global) diagonal of K or dropping the Dirichlet dofs would
+    give other values -- with an element-wise viscosity and unequal sides."""
+    res = (4, 3, 2)[:dim]
+    hi = (1.0, 0.6, 0.35)[:dim]
+    c = gauss_coords(dim, res, (0, 0, 0), hi + (1.0,) * (3 - dim))
+    E = int(np.prod(res))
+    eta_e = np.array([1.0 + 0.5 * e for e in range(E)])
+    mat = np.repeat(np.arange(E, dtype=np.int32), 2 ** dim)
+    mats = [dict(eta0=float(v), density=1.0) for v in eta_e]
+    pb = oracle.Problem(dim, res, (0,) * dim, hi, c, mat, mats, bc_mask=free_slip_mask(dim, res))
+    pb.assemble()
+    q = pb.uzawa_Q()
+    h = [hi[d] / res[d] for d in range(dim)]
+    exact = np.array([_q_closed_form(dim, h, v) for v in eta_e])
+    np.testing.assert_allclose(q, exact, rtol=1e-12)
+    # the square / cube special cases: 2 h^2 / eta (2D), 27 h^3 / (8 eta) (3D)
+    assert abs(_q_closed_form(2, (0.5, 0.5), 2.0) - 2 * 0.25 / 2.0) < 1e-15
+    assert abs(_q_closed_form(3, (0.5,) * 3, 2.0) - 27 * 0.125 / 16.0) < 1e-15
+
+
 def test_ownership_rule_on_face_lattice(oracle):
     """Mesh_RegularAlgorithms rule: floor((x-min)/sep); a point whose
     fractional part (in element units) is within 1e-12 of a face goes to the
