@@ -175,6 +175,12 @@ def profile_apply(args):
                  G.APPLY_G, G.APPLY_GT):
         ms = s.time_apply(form, args.apply_reps, cold=True)
         print(f"# form {form}: {ms:.4f} ms (cold)", file=sys.stderr, flush=True)
+    # a few Uzawa iterations of the plain solve, so the solve's vector,
+    # reduction and multigrid kernels are profiled as they run inside it
+    s.set_params(G.uzawa_params(tolerance=1e-6, maxIterations=args.profile_outer, inner_max_iterations=20000))
+    _, _, rep = s.Solve(raise_on_error=False)
+    print(f"# capped solve: {rep.uzawa_iterations} Uzawa its, {rep.inner_iterations} inner, "
+          f"{rep.solve_seconds:.3f} s", file=sys.stderr, flush=True)
     print(json.dumps({"copy_bytes_each_way": n * 8, "res": args.res, "ppc": args.ppc,
                       "moments_bytes": k_moments_bytes(cfg.dim, cfg.N, cfg.E),
                       "particles_bytes": k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)}), flush=True)
@@ -291,7 +297,10 @@ def main():
     ap.add_argument("--inner-relax", type=float, default=0.1,
                     help="learned solve: cap of the relaxed inner tolerance (0 = reference 0.1 x tol throughout)")
     ap.add_argument("--profile-apply", action="store_true",
-                    help="only run a calibration copy and the K applies (for rocprofv3 --pmc traffic counts)")
+                    help="only run a calibration copy, the operator applies (cold) and a capped plain solve "
+                         "(for rocprofv3 kernel traces and --pmc traffic counts)")
+    ap.add_argument("--profile-outer", type=int, default=3,
+                    help="Uzawa iterations of the capped solve of --profile-apply")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not supported "
@@ -485,14 +494,19 @@ def main():
             em.emit()
 
     # a yielded solve: the second (Picard) solve of the sequence, with the
-    # Drucker-Prager lid yielding from the first solution, warm-started from
-    # its pressure as the reference keeps pSolnVec
+    # Drucker-Prager lid yielding from the first solution (the plain FP64
+    # one, for both solvers, so both solve the same problem), warm-started
+    # from its pressure as the reference keeps pSolnVec
     if not args.no_yield and em.fits("yielded", 10.0):
         yl = {}
-        if net is not None and ul is not None:
+        if net is not None:
             sv = learned_solver()
-            sv.setup(ul, pl, yield_on=True)
-            uy, py, ry = solve_once(sv, pl)
+            sv.setup(u, p, yield_on=True)
+            eta_y = sv.setup_arrays()["eta"]
+            lid = cfg.material == 1
+            yl["lid_points_yielded"] = int(np.sum(eta_y[lid] < 1e2))
+            yl["lid_points"] = int(np.sum(lid))
+            uy, py, ry = solve_once(sv, p)
             sv.close()
             yl["learned"] = leg_dict(ry, PREC_MIXED + "; " + PREC_NET + "; relaxed inner rtol")
             print(f"# yielded learned: {ry.solve_seconds:.3f}s uzawa={ry.uzawa_iterations} status={ry.status}",
