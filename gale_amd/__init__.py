@@ -37,7 +37,7 @@ EXPORTS = (
     "gs_problem_particles", "gs_solver_create", "gs_solver_set_params", "gs_solver_destroy", "gs_solver_setup",
     "gs_stokes_solve", "gs_solver_set_pressure", "gs_solver_set_solution", "gs_picard_solve", "gs_get_rhs", "gs_form_residual",
     "gs_compute_dt", "gs_advect_rk2", "gs_relocate", "gs_problem_set_coords", "gs_apply_K", "gs_apply_K_fp32",
-    "gs_apply_G", "gs_apply_Gt", "gs_solver_sync", "gs_time_apply_K", "gs_time_apply", "gs_solver_setup_arrays", "gs_apply_K_moments",
+    "gs_apply_G", "gs_apply_Gt", "gs_apply_K_noalloc", "gs_solver_sync", "gs_time_apply_K", "gs_time_apply", "gs_solver_setup_arrays", "gs_apply_K_moments",
     "gs_rng_stream", "gs_layout_random", "gs_layout_gauss", "gs_sobol_sequence", "gs_layout_spacefiller", "gs_crand_stream", "gs_initial_damage", "gs_wall_bc",
     "gs_h5_dataset_info", "gs_h5_read_f64", "gs_h5_read_i32", "gs_h5_attribute_i64", "gs_h5_read_field",
     "gs_h5_read_swarm", "gs_h5_write_field", "gs_h5_write_swarm", "gs_h5_write_time",
@@ -162,6 +162,7 @@ def lib():
             "gs_solver_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
             "gs_problem_population_control": (i32, [vp, vp, vp]),
             "gs_apply_K": (i32, [vp, vp, vp]),
+            "gs_apply_K_noalloc": (i32, [vp, vp, vp]),
             "gs_apply_K_fp32": (i32, [vp, vp, vp]),
             "gs_apply_K_moments": (i32, [vp, vp, vp]),
             "gs_apply_G": (i32, [vp, vp, vp]),
@@ -862,6 +863,11 @@ class Stokes_SLE_UzawaSolver:
     # device-pointer operator applies (torch tensors on cuda)
     def apply_K(self, x, y):
         _check(lib().gs_apply_K(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "apply_K")
+
+    def apply_K_noalloc(self, x, y):
+        """gs_apply_K without any allocation (static per-device scratch)."""
+        _check(lib().gs_apply_K_noalloc(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())),
+               "apply_K_noalloc")
 
     def apply_K_fp32(self, x, y):
         _check(lib().gs_apply_K_fp32(self._h, C.c_void_p(x.data_ptr()), C.c_void_p(y.data_ptr())), "apply_K_fp32")

@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
@@ -100,6 +101,8 @@ struct gs_problem {
   int wres[3] = {10, 10, 10};       // DVC sub-grid
   double* wp = nullptr;             // DVC weight of each (sorted) particle
   unsigned long long gen = 0;       // bumped by every relocation (solvers re-run their setup)
+  int device = 0;                   // HIP device current at creation (the problem's buffers live there)
+  void* scratch = nullptr;          // that device's address of the no-allocation scratch array
 };
 
 
@@ -288,6 +291,38 @@ static int32_t relocate(gs_problem* pb) {
   return esc ? GS_E_ESCAPED : GS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation K apply (gs_apply_K_noalloc): the tile-face
+// partial sums of one apply live in this file-scope device array instead of
+// a buffer the library allocates.  Each device holds its own copy until the
+// process ends.  In host code `scratch` only stands for the symbol: its
+// device address is resolved once per device with hipGetSymbolAddress (with
+// that device current, at problem creation) and only that address is handed
+// to kernels.  Every call on a device shares that device's array, so calls
+// that overlap on one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_scratch_mu;
+static std::vector<void*> g_scratch_ptr;  // per device id
+
+static int32_t scratch_address(int dev, void** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) return GS_E_INVALID;
+  if ((size_t)dev >= g_scratch_ptr.size()) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      std::fprintf(stderr, "gale_stokes: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+      return GS_E_HIP;
+    }
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return GS_OK;
+}
+
 extern "C" int32_t gs_problem_create(const gs_problem_desc* d, gs_problem** out) {
   if (!d || !out) return GS_E_INVALID;
   *out = nullptr;
@@ -330,6 +365,8 @@ extern "C" int32_t gs_problem_create(const gs_problem_desc* d, gs_problem** out)
     gs_problem_destroy(pb);
     return s;
   };
+  if (hipGetDevice(&pb->device) != hipSuccess) return fail(GS_E_HIP);
+  if (scratch_address(pb->device, &pb->scratch) != GS_OK) return fail(GS_E_HIP);
   if (hipStreamCreate(&pb->st) != hipSuccess) return fail(GS_E_HIP);
   const long long ND = g.N * g.dim;
   std::vector<MatDev> md(d->n_materials);
@@ -1833,6 +1870,27 @@ extern "C" int32_t gs_apply_K(gs_solver* s, const double* x, double* y) {
   if (!s || !x || !y) return GS_E_INVALID;
   if (needs_setup(s)) CKS(setup_dev(s, nullptr, nullptr, 0));
   launch_apply_K(s->pb->g, x, y, s->pb->pk, s->pb->off, nullptr, s->pb->halo64, s->pb->st);
+  s->k_applies++;
+  return GS_OK;
+}
+
+extern "C" int32_t gs_apply_K_noalloc(gs_solver* s, const double* x, double* y) {
+  if (!s || !x || !y) return GS_E_INVALID;
+  gs_problem* pb = s->pb;
+  const size_t need = sizeof(double) * (size_t)tile_halo_size(pb->g);
+  if (need > kNoAllocScratchBytes) {
+    std::fprintf(stderr,
+                 "gale_stokes: gs_apply_K_noalloc: this mesh needs %zu bytes of tile-face scratch, more than the "
+                 "%zu-byte static scratch; use gs_apply_K\n", need, kNoAllocScratchBytes);
+    return GS_E_INVALID;
+  }
+  if (needs_setup(s)) {
+    std::fprintf(stderr, "gale_stokes: gs_apply_K_noalloc: the solver is not set up for the problem's current "
+                         "particles (gs_solver_setup allocates; call it first)\n");
+    return GS_E_INVALID;
+  }
+  if (!pb->scratch) return GS_E_HIP;
+  launch_apply_K(pb->g, x, y, pb->pk, pb->off, nullptr, static_cast<double*>(pb->scratch), pb->st);
   s->k_applies++;
   return GS_OK;
 }

@@ -259,6 +259,19 @@ int32_t gs_problem_plastic_strain(gs_problem* pb, double* w, double* inc);
  * gs_solver_sync.  K uses the viscosities of the last gs_solver_setup.
  *   K: x, y (N*dim)   G: p (E) -> y (N*dim)   G^T: x (N*dim) -> q (E) */
 int32_t gs_apply_K(gs_solver* s, const double* x_dev, double* y_dev);
+/* gs_apply_K that allocates nothing: no hipMalloc, hipFree or allocating
+ * library call.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  The
+ * apply's scratch (the tile-face partial sums, 8 * 405 * 3 bytes per 8x8x4
+ * element tile in 3D) lives in a static 64 MiB device array, one per device,
+ * resolved when the problem is created; larger meshes (more than 6904 such
+ * tiles, about 121^3 elements) are refused with GS_E_INVALID and a message.  The
+ * solver must already be set up for the problem's current particles
+ * (gs_solver_setup, which does allocate), otherwise GS_E_INVALID.  Every call
+ * on one device shares that device's scratch: calls that overlap on one
+ * device (different streams or threads) race.  Same result as gs_apply_K,
+ * bit for bit. */
+int32_t gs_apply_K_noalloc(gs_solver* s, const double* x_dev, double* y_dev);
 int32_t gs_apply_K_fp32(gs_solver* s, const float* x_dev, float* y_dev);
 int32_t gs_apply_G(gs_solver* s, const double* p_dev, double* y_dev);
 int32_t gs_apply_Gt(gs_solver* s, const double* x_dev, double* q_dev);
