@@ -128,7 +128,7 @@ struct MGLevel {
 
 struct MGData {
   bool structured = false;
-  bool bottom = true;  // single-workgroup bottom of the cycle (3D) where levels fit
+  bool bottom = false;  // single-workgroup bottom of the cycle (3D) where levels fit
   int L = 0;  // index of the coarsest level
   std::vector<MGLevel> lv;
   int nfc = 0;
@@ -875,9 +875,8 @@ static void mg_free(MGData& mg) {
 }
 
 static int32_t mg_alloc_level(MGLevel& L) {
-  const int nm = n_moments(L.g.dim);
   L.n = L.g.N * L.g.dim;
-  CK(dalloc(&L.M64, L.g.E * nm)); CK(dalloc(&L.M32, L.g.E * nm));
+  CK(dalloc(&L.M64, mom_count<double>(L.g))); CK(dalloc(&L.M32, mom_count<float>(L.g)));
   CK(dalloc(&L.diag, L.n)); CK(dalloc(&L.dinv64, L.n)); CK(dalloc(&L.dinv32, L.n));
   CK(dalloc(&L.b, L.n)); CK(dalloc(&L.x, L.n)); CK(dalloc(&L.r, L.n)); CK(dalloc(&L.d, L.n)); CK(dalloc(&L.Ax, L.n));
   CK(dalloc(&L.bf, L.n)); CK(dalloc(&L.xf, L.n)); CK(dalloc(&L.rf, L.n)); CK(dalloc(&L.df, L.n));
@@ -903,8 +902,24 @@ static int32_t mg_structure(gs_solver* s) {
   CKS(mg_alloc_level(mg.lv.back()));
   std::vector<uint8_t> flag(pb->g.N);
   CK(hipMemcpy(flag.data(), pb->bcflag, pb->g.N, hipMemcpyDeviceToHost));
+  // 3D: coarsening stops at the first level small enough for the dense
+  // coarsest solve (at most 2000 free dofs): an exact solve there needs fewer
+  // inner iterations than smoothing further down, and its matrix-vector
+  // product is one wide launch (96^3: 96-48-24-12-6, 6^3 dense; 9 % less time
+  // per inner iteration and 8 % fewer inner iterations than going on to 3^3).
+  // 2D levels coarsen as far as the mesh allows (the batched solver runs each
+  // instance's dense coarsest solve inside one workgroup, where a small one
+  // is cheaper).  GALE_MG_DENSE_STOP=0 / 1 overrides (A/B timing).
+  const char* dstop = std::getenv("GALE_MG_DENSE_STOP");
+  const bool dense_stop = dstop ? std::atoi(dstop) != 0 : dim == 3;
   for (int lev = 0; lev < 16; lev++) {
     const Geo& gf = mg.lv.back().g;
+    if (dense_stop && lev > 0) {
+      long long nfree = 0;
+      for (long long n = 0; n < gf.N; n++)
+        for (int d = 0; d < dim; d++) nfree += ((flag[n] >> d) & 1u) ? 0 : 1;
+      if (nfree <= 2000) break;
+    }
     bool ok = true;
     for (int d = 0; d < dim; d++) ok = ok && gf.res[d] % 2 == 0 && gf.res[d] / 2 >= 2;
     if (!ok) break;
@@ -968,8 +983,12 @@ static int32_t mg_structure(gs_solver* s) {
     mg.nfc = -1;  // no dense coarse solve: Chebyshev smoothing on the coarsest level
   }
   {
-    const char* ev = std::getenv("GALE_MG_BOTTOM");  // 0: multi-launch bottom (A/B timing)
-    mg.bottom = !(ev && std::atoi(ev) == 0);
+    // The single-workgroup bottom kernel is off by default: with the dense
+    // solve at the first level that allows it, the levels above run faster as
+    // wide launches than in one workgroup (96^3: 1.03 vs 1.78 ms per inner
+    // iteration).  GALE_MG_BOTTOM=1 turns it on (A/B timing).
+    const char* ev = std::getenv("GALE_MG_BOTTOM");
+    mg.bottom = ev && std::atoi(ev) != 0;
   }
   mg.structured = true;
   return GS_OK;
@@ -1016,12 +1035,12 @@ static int32_t mg_numeric(gs_solver* s) {
   hipStream_t st = pb->st;
   MGData& mg = s->mg;
   if (!mg.structured) CKS(mg_structure(s));
-  const int dim = pb->dim, nm = n_moments(dim);
+  const int dim = pb->dim;
   launch_moments_fine(mg.lv[0].g, pb->pk, pb->off, mg.lv[0].M64, st);
   for (int l = 1; l <= mg.L; l++) launch_moments_coarsen(mg.lv[l].g, mg.lv[l - 1].g, mg.lv[l - 1].M64, mg.lv[l].M64, st);
   for (int l = 0; l <= mg.L; l++) {
     MGLevel& L = mg.lv[l];
-    launch_d2f(L.M64, L.M32, L.g.E * nm, st);
+    launch_mom_d2f(L.g, L.M64, L.M32, st);
     launch_mom_diag(L.g, L.M64, L.diag, st);
     launch_make_dinv(L.diag, L.bcdof, L.dinv64, L.n, st);
     launch_d2f(L.dinv64, L.dinv32, L.n, st);
@@ -1960,7 +1979,6 @@ extern "C" int32_t gs_time_apply(gs_solver* s, int32_t form, int32_t n_rep, int3
     if (!s->y32) CK(dalloc(&s->y32, s->n));
   }
   const long long n = s->n, E = pb->g.E, P = pb->P;
-  const int nmom = n_moments(pb->dim);
   // operand set 0 is the solver's own buffers; x_bytes / y_bytes / in_bytes
   // are the sizes of the input vector, output vector and streamed operand
   void* x0 = nullptr;
@@ -1970,8 +1988,8 @@ extern "C" int32_t gs_time_apply(gs_solver* s, int32_t form, int32_t n_rep, int3
   switch (form) {
     case 0: x0 = s->cg_p; y0 = s->cg_Ap; in0 = pb->pk; xb = yb = 8 * n; ib = sizeof(double4) * P; break;
     case 1: x0 = s->x32; y0 = s->y32; in0 = pb->pk32; xb = yb = 4 * n; ib = sizeof(float4) * P; break;
-    case 2: x0 = s->cg_p; y0 = s->cg_Ap; in0 = s->mg.lv[0].M64; xb = yb = 8 * n; ib = 8ull * nmom * E; break;
-    case 3: x0 = s->x32; y0 = s->y32; in0 = s->mg.lv[0].M32; xb = yb = 4 * n; ib = 4ull * nmom * E; break;
+    case 2: x0 = s->cg_p; y0 = s->cg_Ap; in0 = s->mg.lv[0].M64; xb = yb = 8 * n; ib = 8ull * mom_count<double>(pb->g); break;
+    case 3: x0 = s->x32; y0 = s->y32; in0 = s->mg.lv[0].M32; xb = yb = 4 * n; ib = 4ull * mom_count<float>(pb->g); break;
     case GS_APPLY_G: x0 = s->tmpE; y0 = s->cg_Ap; xb = 8 * E; yb = 8 * n; break;
     default: x0 = s->cg_p; y0 = s->tmpE; xb = 8 * n; yb = 8 * E; break;
   }

@@ -8,8 +8,11 @@ namespace gs {
 void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double* M, hipStream_t st);
 void launch_moments_coarsen(const Geo& gc, const Geo& gf, const double* Mf, double* Mc, hipStream_t st);
 void launch_d2f(const double* a, float* b, long long n, hipStream_t st);
+// FP32 copy of a level's moments, layout-aware (gs_tile.h mom_index).
+void launch_mom_d2f(const Geo& g, const double* a, float* b, hipStream_t st);
 void launch_f2d(const float* a, double* b, long long n, hipStream_t st);
-// Moment-form K apply (tiled, gs_tile.hip); Mom is SoA, M[k * E + e]; halo
+// Moment-form K apply (tiled, gs_tile.hip); Mom in the layout of gs_tile.h
+// (mom_index: tile-major in 3D, SoA in 2D); halo
 // needs tile_halo_size(g) values.
 template <typename T>
 void launch_mom_apply(const Geo& g, const T* x, T* y, const T* Mom, const uint8_t* bcflag, T* halo, hipStream_t st);

@@ -79,7 +79,7 @@ __global__ void k_moments_fine(Geo g, const double4* __restrict__ pk, const int*
     }
   }
 #pragma unroll
-  for (int k = 0; k < NM; k++) M[k * g.E + e] = m[k];
+  for (int k = 0; k < NM; k++) M[mom_index<double>(g, k, e)] = m[k];
 }
 
 // Parent moments from children: child local r_f maps to parent r_c = (r_f + sg)/2,
@@ -105,7 +105,7 @@ __global__ void k_moments_coarsen(Geo gc, Geo gf, const double* __restrict__ Mf,
       T[d][2][0] = 0.25; T[d][2][1] = 0.5 * s; T[d][2][2] = 0.25;  // (r+s)^2/4 = (r^2 + 2 s r + 1)/4
     }
     double m[NM];
-    for (int k = 0; k < NM; k++) m[k] = Mf[k * gf.E + e];
+    for (int k = 0; k < NM; k++) m[k] = Mf[mom_index<double>(gf, k, e)];
     if (DIM == 2) {
       for (int b = 0; b < 3; b++)
         for (int a = 0; a < 3; a++) {
@@ -128,7 +128,7 @@ __global__ void k_moments_coarsen(Geo gc, Geo gf, const double* __restrict__ Mf,
     }
   }
 #pragma unroll
-  for (int k = 0; k < NM; k++) Mc[k * gc.E + E] = acc[k];
+  for (int k = 0; k < NM; k++) Mc[mom_index<double>(gc, k, E)] = acc[k];
 }
 
 void launch_moments_fine(const Geo& g, const double4* pk, const int* off, double* M, hipStream_t st) {
@@ -149,6 +149,23 @@ __global__ void k_convert(const S* __restrict__ a, T* __restrict__ b, long long 
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     b[i] = (T)a[i];
 }
+// FP32 copy of a level's moments (the two precisions group the moments
+// differently in 3D, so the copy goes through the element and moment index).
+template <int DIM>
+__global__ void k_mom_d2f(Geo g, const double* __restrict__ a, float* __restrict__ b) {
+  constexpr int NM = DIM == 2 ? 9 : 27;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= g.E) return;
+#pragma unroll
+  for (int k = 0; k < NM; k++) b[mom_index<float>(g, k, e)] = (float)a[mom_index<double>(g, k, e)];
+}
+void launch_mom_d2f(const Geo& g, const double* a, float* b, hipStream_t st) {
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_mom_d2f<3>, dim3(gridn(g.E)), dim3(kBlock), 0, st, g, a, b);
+  else
+    hipLaunchKernelGGL(k_mom_d2f<2>, dim3(gridn(g.E)), dim3(kBlock), 0, st, g, a, b);
+}
+
 void launch_d2f(const double* a, float* b, long long n, hipStream_t st) {
   if (n > 0) hipLaunchKernelGGL((k_convert<double, float>), dim3(gridn(n) < 8192 ? gridn(n) : 8192), dim3(kBlock), 0, st, a, b, n);
 }
@@ -187,7 +204,7 @@ __global__ void k_mom_diag(Geo g, int c, long long mcount, const double* __restr
   for (int d = 0; d < DIM; d++) inv[d] = 2.0 / (vcoord_mg(g, d, ijk[d] + 1) - vcoord_mg(g, d, ijk[d]));
   const double cN = DIM == 2 ? 0.25 : 0.125;
   double M[NM];
-  for (int k = 0; k < NM; k++) M[k] = Mom[k * g.E + e];
+  for (int k = 0; k < NM; k++) M[k] = Mom[mom_index<double>(g, k, e)];
   const long long sy = g.nn[0], sz = g.nn[0] * g.nn[1];
   const long long n0 = ijk[0] + sy * ijk[1] + sz * ijk[2];
   for (int a = 0; a < NEN; a++) {

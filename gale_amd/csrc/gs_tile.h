@@ -69,6 +69,38 @@ void mg_bottom3(const BotArgs<float>& a, hipStream_t st);
 void mg_bottom3(const BotArgs<double>& a, hipStream_t st);
 
 TileGrid tile_grid(const Geo& g);
+
+// Element moments (27 per 3D element, 9 per 2D element).
+// 3D: tile-major.  The 256 elements of one 8x8x4 apply tile are contiguous
+// for every moment, and consecutive moments are grouped by G = 16 / sizeof(T)
+// (pairs in FP64, quads in FP32) so one 16-byte load brings G moments of one
+// element:
+//   base(e) = tile(e) * 27 * 256
+//   k < G * (27 / G):  base(e) + G * (k / G) * 256 + G * local(e) + k % G
+//   otherwise:         base(e) + k * 256 + local(e)
+// with tile(e) the apply tile of element e and local(e) = lx + 8 ly + 64 lz
+// its place in it; tiles cut by the mesh edge keep all 256 slots.  A
+// workgroup applying one tile streams 27 * 256 contiguous values.
+// 2D: SoA, M[k * E + e].
+template <typename T>
+__host__ __device__ inline long long mom_count(const Geo& g) {
+  if (g.dim != 3) return 9 * g.E;
+  const long long tx = (g.res[0] + kTile3X - 1) / kTile3X, ty = (g.res[1] + kTile3Y - 1) / kTile3Y,
+                  tz = (g.res[2] + kTile3Z - 1) / kTile3Z;
+  return tx * ty * tz * 27 * kTileThreads;
+}
+template <typename T>
+__host__ __device__ inline long long mom_index(const Geo& g, int k, long long e) {
+  if (g.dim != 3) return (long long)k * g.E + e;
+  constexpr int G = 16 / (int)sizeof(T);
+  const long long ex = e % g.res[0], ey = (e / g.res[0]) % g.res[1], ez = e / ((long long)g.res[0] * g.res[1]);
+  const long long tnx = (g.res[0] + kTile3X - 1) / kTile3X, tny = (g.res[1] + kTile3Y - 1) / kTile3Y;
+  const long long tile = ex / kTile3X + tnx * (ey / kTile3Y + tny * (ez / kTile3Z));
+  const int local = (int)(ex % kTile3X + kTile3X * (ey % kTile3Y + kTile3Y * (ez % kTile3Z)));
+  const long long base = tile * 27 * kTileThreads;
+  if (k < G * (27 / G)) return base + (long long)G * (k / G) * kTileThreads + G * local + k % G;
+  return base + (long long)k * kTileThreads + local;
+}
 // Halo buffer length (values) for one apply on this mesh.
 long long tile_halo_size(const Geo& g);
 

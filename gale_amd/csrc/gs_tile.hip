@@ -187,12 +187,58 @@ __device__ __forceinline__ void element_particles3(const T (&A)[3][4], const T (
   }
 }
 
-// Stress moments as contractions of the gradient coefficients with the 27
-// particle moments (moments stored SoA: M[k * E + e]).
-template <typename T>
-__device__ __forceinline__ void load_moments3(const T* __restrict__ Mom, long long E, long long e, T (&M)[27]) {
+// The 27 moments of the element at place `local` of apply tile `tile`
+// (tile-major layout, gs_tile.h): one 16-byte load per group of
+// G = 16 / sizeof(T) moments, consecutive threads reading consecutive
+// 16-byte slots.  NT marks the loads non-temporal: a stream larger than the
+// last-level cache is read once per apply and should not evict the node
+// vectors; a small level's moments are reused by the next applies of the
+// cycle and are loaded normally.
+typedef double dvec2_t __attribute__((ext_vector_type(2)));
+typedef float fvec4_t __attribute__((ext_vector_type(4)));
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_mom(const V* p) {
+  if (NT) return __builtin_nontemporal_load(p);
+  return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void load_tile_moments3(const double* __restrict__ Mom, long long tile, int local,
+                                                   double (&M)[27]) {
+  const double* base = Mom + tile * 27 * kTileThreads;
+  const dvec2_t* b2 = reinterpret_cast<const dvec2_t*>(base) + local;
 #pragma unroll
-  for (int k = 0; k < 27; k++) M[k] = Mom[k * E + e];
+  for (int j = 0; j < 13; j++) {
+    const dvec2_t v = ld_mom<NT>(b2 + j * kTileThreads);
+    M[2 * j] = v.x;
+    M[2 * j + 1] = v.y;
+  }
+  M[26] = ld_mom<NT>(base + 26 * kTileThreads + local);
+}
+template <bool NT>
+__device__ __forceinline__ void load_tile_moments3(const float* __restrict__ Mom, long long tile, int local,
+                                                   float (&M)[27]) {
+  const float* base = Mom + tile * 27 * kTileThreads;
+  const fvec4_t* b4 = reinterpret_cast<const fvec4_t*>(base) + local;
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    const fvec4_t v = ld_mom<NT>(b4 + j * kTileThreads);
+    M[4 * j] = v.x;
+    M[4 * j + 1] = v.y;
+    M[4 * j + 2] = v.z;
+    M[4 * j + 3] = v.w;
+  }
+#pragma unroll
+  for (int k = 24; k < 27; k++) M[k] = ld_mom<NT>(base + k * kTileThreads + local);
+}
+
+// The 27 moments of the element at lattice place (gi, gj, gk).
+template <typename T>
+__device__ __forceinline__ void load_moments3(const Geo& g, const T* __restrict__ Mom, int gi, int gj, int gk,
+                                              T (&M)[27]) {
+  const int tnx = (g.res[0] + kTile3X - 1) / kTile3X, tny = (g.res[1] + kTile3Y - 1) / kTile3Y;
+  const long long tile = gi / kTile3X + (long long)tnx * (gj / kTile3Y + (long long)tny * (gk / kTile3Z));
+  const int local = gi % kTile3X + kTile3X * (gj % kTile3Y + kTile3Y * (gk % kTile3Z));
+  load_tile_moments3<false>(Mom, tile, local, M);
 }
 
 template <typename T>
@@ -292,7 +338,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
                                                               const T* __restrict__ Mom,
                                                               const typename V4<T>::type* __restrict__ pk,
                                                               const int* __restrict__ off,
-                                                              const uint8_t* __restrict__ bcflag) {
+                                                              const uint8_t* __restrict__ bcflag, int mom_nt) {
   constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
   constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
   // node box, one array per component (consecutive elements hit consecutive
@@ -313,7 +359,12 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
   // the element's moments are requested before the node box, so both
   // memory round trips overlap
   T M[27];
-  if (FORM == 0 && active) load_moments3(Mom, g.E, e, M);
+  if (FORM == 0 && active) {
+    if (mom_nt)
+      load_tile_moments3<true>(Mom, tile, (int)threadIdx.x, M);
+    else
+      load_tile_moments3<false>(Mom, tile, (int)threadIdx.x, M);
+  }
   for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
     const int c = q % 3, nd = q / 3;
     const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
@@ -558,8 +609,11 @@ static void tile_launch(const Geo& g, const T* x, T* y, T* halo, const T* Mom, c
   const TileGrid tg = tile_grid(g);
   const unsigned nbn = (unsigned)((g.N + kBlock - 1) / kBlock);
   if (g.dim == 3) {
+    // non-temporal moment loads when the moments exceed a quarter of the
+    // 256 MB last-level cache (they are then streamed, not reused)
+    const int nt = FORM == 0 && (double)mom_count<T>(g) * sizeof(T) > 64.0e6 ? 1 : 0;
     hipLaunchKernelGGL((k_tile_apply3<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
-                       y, halo, Mom, pk, off, bcflag);
+                       y, halo, Mom, pk, off, bcflag, nt);
     hipLaunchKernelGGL((k_tile_halo3<T, EPI>), dim3(nbn), dim3(kBlock), 0, st, g, tg, x, y, halo, bcflag, ep);
   } else {
     hipLaunchKernelGGL((k_tile_apply2<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
@@ -643,7 +697,7 @@ __device__ void bot_kx(const BotLevel<T>& L, const T* __restrict__ x, T* sy) {
       T A[3][4], B[3][4], C[3][4];
       grad_coeffs3(u, ix, iy, iz, A, B, C);
       T M[27], m[30];
-      load_moments3(L.M, E, e, M);
+      load_moments3(g, L.M, (int)gi, (int)gj, (int)gk, M);
       element_moments3(A, B, C, M, m);
       epilogue3(m, ix, iy, iz, ye);
     }
