@@ -150,19 +150,22 @@ def bench_batch(G, configs, count):
 
 
 def profile_apply(args):
-    """Calibration copy of known size (torch elementwise copy, 256 MiB read +
-    256 MiB written per call) followed by the four K apply forms and the two
+    """Calibration copy of known size (torch elementwise copy, 1 GiB read +
+    1 GiB written) followed by the four K apply forms and the two
     pressure operators, each timed cold (operand copies cycled so every apply
     streams from HBM), so that rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE per
     dispatch can be scaled to bytes and the kernel-trace averages compare with
     the bench's roofline.  torch is imported before the library so both share
     one HIP runtime."""
     import torch
-    n = 32 * 1024 * 1024
+    # calibration: one elementwise copy of 1 GiB (4x the last-level cache, so
+    # every byte moves to and from HBM): FETCH_SIZE and WRITE_SIZE of that one
+    # dispatch against 1 GiB read and 1 GiB written
+    n = 128 * 1024 * 1024
     a = torch.ones(n, dtype=torch.float64, device="cuda")
     b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
     torch.cuda.synchronize()
     del a, b
     import gale_amd as G
@@ -181,7 +184,7 @@ def profile_apply(args):
     _, _, rep = s.Solve(raise_on_error=False)
     print(f"# capped solve: {rep.uzawa_iterations} Uzawa its, {rep.inner_iterations} inner, "
           f"{rep.solve_seconds:.3f} s", file=sys.stderr, flush=True)
-    print(json.dumps({"copy_bytes_each_way": n * 8, "res": args.res, "ppc": args.ppc,
+    print(json.dumps({"calibration_copy_bytes_each_way": n * 8, "res": args.res, "ppc": args.ppc,
                       "moments_bytes": k_moments_bytes(cfg.dim, cfg.N, cfg.E),
                       "particles_bytes": k_apply_bytes(cfg.dim, cfg.N, cfg.E, cfg.P)}), flush=True)
 

@@ -474,48 +474,102 @@ void launch_weakening_step(double* w, const double* inc, const double* coord, lo
 // closed form of GradientStiffnessMatrixTerm.cxx:225-245 on an affine
 // element: -int_e dN_a/dx_d = -s_ad * prod_{d'!=d} h_d' / 2^(dim-1).
 // ---------------------------------------------------------------------------
+// y = G p on the nodes: each node gathers the pressures of its (up to 2^dim)
+// elements, all loaded at once (elements outside the mesh read as 0 and add
+// exact zeros); element spacings are taken once per axis; 32-bit lattice
+// indices when the mesh allows.  Sum order: element layers, rows, columns.
+template <int DIM>
 __global__ void k_apply_G(Geo g, const double* __restrict__ p, double* __restrict__ y,
                           const uint8_t* __restrict__ bcflag) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
-  const long long i = n % g.nn[0], j = (n / g.nn[0]) % g.nn[1], k = n / (g.nn[0] * g.nn[1]);
+  long long i, j, k;
+  if (g.N < (1LL << 31)) {
+    const unsigned un = (unsigned)n, nx = (unsigned)g.nn[0], ny = (unsigned)g.nn[1];
+    const unsigned q = un / nx;
+    i = un - q * nx;
+    j = q % ny;
+    k = q / ny;
+  } else {
+    i = n % g.nn[0];
+    j = (n / g.nn[0]) % g.nn[1];
+    k = n / (g.nn[0] * g.nn[1]);
+  }
+  const unsigned bf = bcflag ? bcflag[n] : 0u;
+  constexpr int NZ = DIM == 3 ? 2 : 1;
+  double pe[NZ][2][2];
+#pragma unroll
+  for (int az = 0; az < NZ; az++) {
+    const long long kk = DIM == 3 ? k - 1 + az : 0;
+#pragma unroll
+    for (int ay = 0; ay < 2; ay++) {
+      const long long jj = j - 1 + ay;
+#pragma unroll
+      for (int ax = 0; ax < 2; ax++) {
+        const long long ii = i - 1 + ax;
+        const bool in = kk >= 0 && kk < g.res[2] && jj >= 0 && jj < g.res[1] && ii >= 0 && ii < g.res[0];
+        pe[az][ay][ax] = in ? p[ii + (long long)g.res[0] * (jj + (long long)g.res[1] * kk)] : 0.0;
+      }
+    }
+  }
+  // element spacings of the columns / rows / layers i-1 and i
+  double h[3][2];
+  for (int d = 0; d < DIM; d++) {
+    const long long c = d == 0 ? i : (d == 1 ? j : k);
+    const double v0 = vcoord(g, d, c - 1), v1 = vcoord(g, d, c), v2 = vcoord(g, d, c + 1);
+    h[d][0] = v1 - v0;
+    h[d][1] = v2 - v1;
+  }
   double acc[3] = {0.0, 0.0, 0.0};
-  const long long k0 = g.dim == 3 ? k - 1 : 0, k1 = g.dim == 3 ? k : 0;
-  for (long long kk = k0; kk <= k1; kk++) {
-    if (kk < 0 || kk >= g.res[2]) continue;
-    const double hz = g.dim == 3 ? vcoord(g, 2, kk + 1) - vcoord(g, 2, kk) : 1.0;
-    const double sz = (kk == k - 1) ? 1.0 : -1.0;
-    for (long long jj = j - 1; jj <= j; jj++) {
-      if (jj < 0 || jj >= g.res[1]) continue;
-      const double hy = vcoord(g, 1, jj + 1) - vcoord(g, 1, jj);
-      const double sy = (jj == j - 1) ? 1.0 : -1.0;
-      for (long long ii = i - 1; ii <= i; ii++) {
-        if (ii < 0 || ii >= g.res[0]) continue;
-        const double hx = vcoord(g, 0, ii + 1) - vcoord(g, 0, ii);
-        const double sx = (ii == i - 1) ? 1.0 : -1.0;
-        const double pe = p[ii + (long long)g.res[0] * (jj + (long long)g.res[1] * kk)];
-        if (g.dim == 2) {
-          acc[0] -= sx * (hy * 0.5) * pe;
-          acc[1] -= sy * (hx * 0.5) * pe;
+#pragma unroll
+  for (int az = 0; az < NZ; az++) {
+    const double hz = DIM == 3 ? h[2][az] : 1.0;
+    const double sz = az == 0 ? 1.0 : -1.0;
+#pragma unroll
+    for (int ay = 0; ay < 2; ay++) {
+      const double hy = h[1][ay];
+      const double sy = ay == 0 ? 1.0 : -1.0;
+#pragma unroll
+      for (int ax = 0; ax < 2; ax++) {
+        const double hx = h[0][ax];
+        const double sx = ax == 0 ? 1.0 : -1.0;
+        const double v = pe[az][ay][ax];
+        if (DIM == 2) {
+          acc[0] -= sx * (hy * 0.5) * v;
+          acc[1] -= sy * (hx * 0.5) * v;
         } else {
-          acc[0] -= sx * (hy * hz * 0.25) * pe;
-          acc[1] -= sy * (hx * hz * 0.25) * pe;
-          acc[2] -= sz * (hx * hy * 0.25) * pe;
+          acc[0] -= sx * (hy * hz * 0.25) * v;
+          acc[1] -= sy * (hx * hz * 0.25) * v;
+          acc[2] -= sz * (hx * hy * 0.25) * v;
         }
       }
     }
   }
-  const unsigned bf = bcflag ? bcflag[n] : 0u;
-  for (int d = 0; d < g.dim; d++) y[n * g.dim + d] = (bf >> d) & 1u ? 0.0 : acc[d];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) y[n * DIM + d] = (bf >> d) & 1u ? 0.0 : acc[d];
 }
 void launch_apply_G(const Geo& g, const double* p, double* y, const uint8_t* bcflag, hipStream_t st) {
-  hipLaunchKernelGGL(k_apply_G, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g, p, y, bcflag);
+  if (g.dim == 3)
+    hipLaunchKernelGGL(k_apply_G<3>, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g, p, y, bcflag);
+  else
+    hipLaunchKernelGGL(k_apply_G<2>, dim3(grid_for(g.N)), dim3(kBlock), 0, st, g, p, y, bcflag);
 }
 
 __global__ void k_apply_Gt(Geo g, const double* __restrict__ x, double* __restrict__ q) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= g.E) return;
-  const long long i = e % g.res[0], j = (e / g.res[0]) % g.res[1], k = g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0;
+  long long i, j, k;
+  if (g.E < (1LL << 31)) {  // 32-bit lattice indices (64-bit division is a long software sequence)
+    const unsigned ue = (unsigned)e, rx = (unsigned)g.res[0], ry = (unsigned)g.res[1];
+    const unsigned qq = ue / rx;
+    i = ue - qq * rx;
+    j = qq % ry;
+    k = g.dim == 3 ? qq / ry : 0;
+  } else {
+    i = e % g.res[0];
+    j = (e / g.res[0]) % g.res[1];
+    k = g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0;
+  }
   const double hx = vcoord(g, 0, i + 1) - vcoord(g, 0, i);
   const double hy = vcoord(g, 1, j + 1) - vcoord(g, 1, j);
   const long long n0 = i + g.nn[0] * (j + g.nn[1] * k);

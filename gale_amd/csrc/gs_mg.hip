@@ -273,15 +273,16 @@ __device__ __forceinline__ void node_ijk(const Geo& g, long long n, long long& i
     k = n / (g.nn[0] * g.nn[1]);
   }
 }
-template <typename T>
+// Full-weighting restriction b_C = R r_F on the nodes of the coarse level
+// (3 x 3 (x 3) stencil, weights 1 and 1/2 per axis, in stencil order).
+template <typename T, int DIM>
 __global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ bcflag_c) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= gc.N) return;
-  const int dim = gc.dim;
   long long I, J, K;
   node_ijk(gc, n, I, J, K);
   T acc[3] = {0, 0, 0};
-  const int kz0 = dim == 3 ? -1 : 0, kz1 = dim == 3 ? 1 : 0;
+  const int kz0 = DIM == 3 ? -1 : 0, kz1 = DIM == 3 ? 1 : 0;
   for (int dz = kz0; dz <= kz1; dz++) {
     const long long k = 2 * K + dz;
     if (k < 0 || k >= gf.nn[2]) continue;
@@ -295,47 +296,71 @@ __global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restri
         if (i < 0 || i >= gf.nn[0]) continue;
         const T w = wz * wy * (dx == 0 ? (T)1 : (T)0.5);
         const long long f = i + gf.nn[0] * (j + gf.nn[1] * k);
-        for (int d = 0; d < dim; d++) acc[d] += w * rf[f * dim + d];
+        for (int d = 0; d < DIM; d++) acc[d] += w * rf[f * DIM + d];
       }
     }
   }
   const unsigned bf = bcflag_c ? bcflag_c[n] : 0u;
-  for (int d = 0; d < dim; d++) bc[n * dim + d] = ((bf >> d) & 1u) ? (T)0 : acc[d];
+  for (int d = 0; d < DIM; d++) bc[n * DIM + d] = ((bf >> d) & 1u) ? (T)0 : acc[d];
 }
 
-template <typename T>
+// x_F += P x_C (Q1 interpolation) on the nodes of the fine level: the up to
+// 2^dim coarse parents and the fine value are loaded at once.
+template <typename T, int DIM>
 __global__ void k_prolong_add(Geo gc, Geo gf, const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ bcflag_f) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= gf.N) return;
-  const int dim = gf.dim;
   long long i, j, k;
   node_ijk(gf, n, i, j, k);
-  T acc[3] = {0, 0, 0};
-  const int nz = (dim == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
+  const int nz = (DIM == 3 && (k & 1)) ? 2 : 1, ny = (j & 1) ? 2 : 1, nx = (i & 1) ? 2 : 1;
   const T wz = nz == 2 ? (T)0.5 : (T)1, wy = ny == 2 ? (T)0.5 : (T)1, wx = nx == 2 ? (T)0.5 : (T)1;
-  for (int a = 0; a < nz; a++) {
-    const long long K = dim == 3 ? (k >> 1) + a : 0;
-    for (int b = 0; b < ny; b++) {
-      const long long J = (j >> 1) + b;
-      for (int cc = 0; cc < nx; cc++) {
-        const long long I = (i >> 1) + cc;
-        const long long cn = I + gc.nn[0] * (J + gc.nn[1] * K);
-        for (int d = 0; d < dim; d++) acc[d] += wx * wy * wz * xc[cn * dim + d];
-      }
-    }
-  }
+  constexpr int NA = DIM == 3 ? 2 : 1;
+  T v[NA][2][2][DIM];
+  T xv[DIM];
   const unsigned bf = bcflag_f ? bcflag_f[n] : 0u;
-  for (int d = 0; d < dim; d++)
-    if (!((bf >> d) & 1u)) xf[n * dim + d] += acc[d];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) xv[d] = xf[n * DIM + d];
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++) {
+        const bool in = a < nz && b < ny && cc < nx;
+        const long long K = DIM == 3 ? (k >> 1) + a : 0, J = (j >> 1) + b, I = (i >> 1) + cc;
+        const long long cn = in ? I + gc.nn[0] * (J + gc.nn[1] * K) : 0;
+#pragma unroll
+        for (int d = 0; d < DIM; d++) v[a][b][cc][d] = in ? xc[cn * DIM + d] : (T)0;
+      }
+  T acc[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; d++) acc[d] = 0;
+#pragma unroll
+  for (int a = 0; a < NA; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++) acc[d] += wx * wy * wz * v[a][b][cc][d];
+#pragma unroll
+  for (int d = 0; d < DIM; d++)
+    if (!((bf >> d) & 1u)) xf[n * DIM + d] = xv[d] + acc[d];
 }
 
 template <typename T>
 void launch_restrict(const Geo& gc, const Geo& gf, const T* rf, T* bc, const uint8_t* bcflag_c, hipStream_t st) {
-  hipLaunchKernelGGL(k_restrict<T>, dim3(gridn(gc.N)), dim3(kBlock), 0, st, gc, gf, rf, bc, bcflag_c);
+  if (gc.dim == 3)
+    hipLaunchKernelGGL((k_restrict<T, 3>), dim3(gridn(gc.N)), dim3(kBlock), 0, st, gc, gf, rf, bc, bcflag_c);
+  else
+    hipLaunchKernelGGL((k_restrict<T, 2>), dim3(gridn(gc.N)), dim3(kBlock), 0, st, gc, gf, rf, bc, bcflag_c);
 }
 template <typename T>
 void launch_prolong_add(const Geo& gc, const Geo& gf, const T* xc, T* xf, const uint8_t* bcflag_f, hipStream_t st) {
-  hipLaunchKernelGGL(k_prolong_add<T>, dim3(gridn(gf.N)), dim3(kBlock), 0, st, gc, gf, xc, xf, bcflag_f);
+  if (gf.dim == 3)
+    hipLaunchKernelGGL((k_prolong_add<T, 3>), dim3(gridn(gf.N)), dim3(kBlock), 0, st, gc, gf, xc, xf, bcflag_f);
+  else
+    hipLaunchKernelGGL((k_prolong_add<T, 2>), dim3(gridn(gf.N)), dim3(kBlock), 0, st, gc, gf, xc, xf, bcflag_f);
 }
 template void launch_restrict<double>(const Geo&, const Geo&, const double*, double*, const uint8_t*, hipStream_t);
 template void launch_restrict<float>(const Geo&, const Geo&, const float*, float*, const uint8_t*, hipStream_t);

@@ -162,8 +162,23 @@ __device__ __forceinline__ void element_particles3(const T (&A)[3][4], const T (
                                                    T (&m)[30]) {
 #pragma unroll
   for (int k = 0; k < 30; k++) m[k] = 0;
-  for (int p = pb; p < pe; p++) {
-    const typename V4<T>::type q = pk[p];
+  // particles in batches of four whose loads are issued together (one
+  // memory round trip per batch); a slot past the element's last particle
+  // carries weight 0 and adds exact zeros
+  constexpr int PB = 4;
+  for (int p0 = pb; p0 < pe; p0 += PB) {
+    typename V4<T>::type qb[PB];
+#pragma unroll
+    for (int k = 0; k < PB; k++) {
+      if (p0 + k < pe) {
+        qb[k] = pk[p0 + k];
+      } else {
+        qb[k].x = 0; qb[k].y = 0; qb[k].z = 0; qb[k].w = 0;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PB; k++) {
+    const typename V4<T>::type q = qb[k];
     const T r = q.x, s = q.y, tt = q.z, w = q.w;
     const T st = s * tt, rt = r * tt, rs = r * s;
     const T uxx = A[0][0] + A[0][1] * s + A[0][2] * tt + A[0][3] * st;
@@ -184,6 +199,7 @@ __device__ __forceinline__ void element_particles3(const T (&A)[3][4], const T (
     m[12] += sxy; m[13] += sxy * r; m[14] += sxy * s; m[15] += sxy * tt; m[16] += sxy * rt; m[17] += sxy * st;
     m[18] += sxz; m[19] += sxz * r; m[20] += sxz * s; m[21] += sxz * tt; m[22] += sxz * rs; m[23] += sxz * st;
     m[24] += syz; m[25] += syz * r; m[26] += syz * s; m[27] += syz * tt; m[28] += syz * rs; m[29] += syz * rt;
+    }
   }
 }
 
@@ -365,13 +381,34 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
     else
       load_tile_moments3<false>(Mom, tile, (int)threadIdx.x, M);
   }
-  for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
-    const int c = q % 3, nd = q / 3;
-    const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
-    T v = 0;
-    if (lx < nbx && ly < nby && lz < nbz) v = x[((x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz)) * 3 + c];
-    sx[c][nd] = v;
-    sy[c][nd] = 0;
+  // Node box: every thread's (at most NQ) values and their Dirichlet flags
+  // are requested at once, so the box costs one memory round trip; the
+  // flags are kept for the stores at the end (same value-to-thread map).
+  constexpr int NQ = (NB * 3 + kTileThreads - 1) / kTileThreads;
+  T vq[NQ];
+  unsigned mq = 0;  // bit it: value it is a Dirichlet dof
+#pragma unroll
+  for (int it = 0; it < NQ; it++) {
+    const int q = threadIdx.x + it * kTileThreads;
+    vq[it] = 0;
+    if (q < NB * 3) {
+      const int c = q % 3, nd = q / 3;
+      const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
+      if (lx < nbx && ly < nby && lz < nbz) {
+        const long long gn = (x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz);
+        vq[it] = x[gn * 3 + c];
+        if (bcflag) mq |= ((bcflag[gn] >> c) & 1u) << it;
+      }
+    }
+  }
+#pragma unroll
+  for (int it = 0; it < NQ; it++) {
+    const int q = threadIdx.x + it * kTileThreads;
+    if (q < NB * 3) {
+      const int c = q % 3, nd = q / 3;
+      sx[c][nd] = vq[it];
+      sy[c][nd] = 0;
+    }
   }
   if (threadIdx.x < TX + TY + TZ) {
     const int t = threadIdx.x;
@@ -412,7 +449,10 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
     __syncthreads();
   }
   const bool lastx = tx == tg.n[0] - 1, lasty = ty == tg.n[1] - 1, lastz = tz == tg.n[2] - 1;
-  for (int q = threadIdx.x; q < NB * 3; q += kTileThreads) {
+#pragma unroll
+  for (int it = 0; it < NQ; it++) {
+    const int q = threadIdx.x + it * kTileThreads;
+    if (q >= NB * 3) continue;
     const int c = q % 3, nd = q / 3;
     const int lx = nd % NBX, ly = (nd / NBX) % NBY, lz = nd / (NBX * NBY);
     if (lx >= nbx || ly >= nby || lz >= nbz) continue;
@@ -421,8 +461,7 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
       halo[(tile * NB + nd) * 3 + c] = sy[c][nd];
     } else {
       const long long gn = (x0 + lx) + sy_ * (y0 + ly) + sz_ * (z0 + lz);
-      const unsigned bf = bcflag ? bcflag[gn] : 0u;
-      y[gn * 3 + c] = ((bf >> c) & 1u) ? (T)0 : sy[c][nd];
+      y[gn * 3 + c] = ((mq >> it) & 1u) ? (T)0 : sy[c][nd];
     }
   }
 }
