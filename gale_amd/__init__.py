@@ -1031,13 +1031,14 @@ class BatchParams(C.Structure):
     _fields_ = [
         ("tolerance", C.c_double), ("max_iterations", C.c_int32), ("max_inner", C.c_int32),
         ("inner_rtol_factor", C.c_double), ("mg_smooth", C.c_int32), ("use_graph", C.c_int32),
+        ("inner_relax_max", C.c_double), ("flexible_window", C.c_int32), ("pad_", C.c_int32),
     ]
 
 
 class BatchResult(C.Structure):
     _fields_ = [
         ("status", C.c_int32), ("uzawa_iterations", C.c_int32), ("inner_iterations", C.c_int64),
-        ("rel_residual", C.c_double),
+        ("rel_residual", C.c_double), ("seconds", C.c_double),
     ]
 
 
@@ -1045,8 +1046,11 @@ TILE_CANDIDATES = [(256, 16), (256, 32), (512, 32), (512, 64), (1024, 32), (1024
 
 
 def batch_params(tolerance=1e-6, maxIterations=5000, max_inner=20000, inner_rtol_factor=0.1, mg_smooth=2,
-                 use_graph=False) -> BatchParams:
-    return BatchParams(tolerance, maxIterations, max_inner, inner_rtol_factor, mg_smooth, int(bool(use_graph)))
+                 use_graph=False, inner_relax_max=0.0, flexible_window=8) -> BatchParams:
+    """inner_relax_max > 0 selects the inexact (relaxed) Uzawa loop with a
+    flexible outer iteration and a true-residual check; 0 the reference loop."""
+    return BatchParams(tolerance, maxIterations, max_inner, inner_rtol_factor, mg_smooth, int(bool(use_graph)),
+                       inner_relax_max, flexible_window, 0)
 
 
 class BatchSolver:
@@ -1087,7 +1091,8 @@ class BatchSolver:
         if raise_on_error:
             _check(st, "gs_batch_solve")
         return st, [dict(status=r.status, uzawa_iterations=r.uzawa_iterations,
-                         inner_iterations=r.inner_iterations, rel_residual=r.rel_residual) for r in res], sec.value
+                         inner_iterations=r.inner_iterations, rel_residual=r.rel_residual, seconds=r.seconds)
+                    for r in res], sec.value
 
     def solution(self, i):
         nx, ny = self.shapes[i]

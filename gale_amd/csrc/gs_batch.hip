@@ -85,14 +85,19 @@ struct BArgs {
   double *u, *vstar, *ft, *cr, *cz, *cp, *cAp;
   double *p, *r, *s, *q, *z;
   int* status;
+  unsigned long long* itime;  // per instance: device wall-clock ticks of its solve
   int* outer;
   long long* inner;
   double* relres;
   unsigned* queue;
+  double *fs, *fq;   // relaxed loop: per-instance rings of window directions and their Schur images
   // parameters
   double tol, irt;
   int max_outer, max_inner, nu;
+  double relax;      // > 0: relaxed inner tolerance with a truncated flexible outer loop (0: reference loop)
+  int window;        // directions of the flexible outer loop
 };
+constexpr int kBatchMaxWin = 16;
 
 template <int BS>
 __device__ double block_sum(double v, double* sh) {
@@ -250,6 +255,7 @@ struct Inst {
   const double *M, *f, *dinv, *Qinv;
   const uint8_t* bc;
   double *u, *vstar, *ft, *cr, *cz, *cp, *cAp, *p, *r, *s, *q, *z;
+  double *fs, *fq;
 };
 
 // Chebyshev-Jacobi smoothing on one level (FP32), window [0.1, 1.1] lmax,
@@ -514,6 +520,135 @@ __device__ int uzawa_one(const Inst& W, const BArgs& A, int* outer_out, long lon
   return err;
 }
 
+// The same solve with inexact inner solves: the inner tolerance is relaxed as
+// the outer residual falls, rtol_k = min(relax, max(irt, irt |r_0| / |r_k|))
+// (the inexact-Krylov relaxation rule), and the outer iteration is flexible CG
+// on S = G^T K^-1 G with each new direction S-orthogonalised against the last
+// `window` ones, which keeps it converging with inexact applications of S.
+// When the recursive residual reaches the tolerance, the true residual is
+// recomputed from an accurate inner solve u = K^-1 (f - G p) (the reference's
+// STEP 3) and the loop resumes from it if needed; a round that stagnates
+// (less than 5 % residual drop over 50 iterations) also ends in that check.
+// Each new round relaxes ten times less and the fifth not at all (then the
+// loop is flexible CG with the reference inner tolerance).  The reported
+// residual is the true one.  Same firewalls as uzawa_one.
+template <int BS, int TX>
+__device__ int uzawa_relaxed(const Inst& W, const BArgs& A, int* outer_out, long long* inner_out, double* relres_out,
+                             float* lds32, double* lds64, double* sh, double* sqv) {
+  const long long N2 = W.N2, E = W.E;
+  long long inner = 0;
+  const double qRecip = 1.0 / sqrt((double)E);
+  double nfree = 0.0;
+  for (long long i = threadIdx.x; i < N2; i += BS) nfree += W.dinv[i] != 0.0 ? 1.0 : 0.0;
+  nfree = block_sum<BS>(nfree, sh);
+  const double ff = bdot<BS>(W.f, W.f, N2, sh);
+  *outer_out = 0;
+  *inner_out = 0;
+  *relres_out = 0.0;
+  if (sqrt(ff) / sqrt(nfree > 0 ? nfree : 1.0) <= 1e-99) return GS_E_ZERO_FORCE;
+  int st;
+  if ((st = inner_solve<BS, TX>(A, W, W.f, W.vstar, A.tol, &inner, lds32, lds64, sh))) return st;
+  bapply_Gt<BS>(W.I, W.vstar, W.z);
+  const double scale = sqrt(bdot<BS>(W.z, W.z, E, sh)) * qRecip;
+  if (!isfinite(scale)) return GS_E_NAN;
+  for (long long e = threadIdx.x; e < E; e += BS) W.p[e] = 0.0;
+  __syncthreads();
+  if ((st = inner_solve<BS, TX>(A, W, W.f, W.u, A.irt, &inner, lds32, lds64, sh))) return st;
+  bapply_Gt<BS>(W.I, W.u, W.r);
+  double relRes = sqrt(bdot<BS>(W.r, W.r, E, sh)) * qRecip / scale;
+  const int m = A.window;
+  int it = 0, err = 0;
+  constexpr int kRounds = 5, kStagWin = 50;
+  for (int round = 0; round < kRounds; round++) {
+    // each round relaxes ten times less; the last one not at all
+    const double relax = round == kRounds - 1 ? 0.0 : A.relax * pow(0.1, (double)round);
+    const double relax_ref = relRes;
+    const int base = it;
+    double stag_ref = relRes;  // residual at the start of the current stagnation window
+    int stag_it = it;
+    bool conv = relRes <= A.tol && it >= 1;
+    while (!conv) {
+      for (long long e = threadIdx.x; e < E; e += BS) W.z[e] = W.Qinv[e] * W.r[e];
+      __syncthreads();
+      const int k = it - base, c = k < m ? k : m, slot = k % m;
+      double* sk = W.fs + (long long)slot * E;
+      double* qk = W.fq + (long long)slot * E;
+      double beta[kBatchMaxWin];
+#pragma unroll
+      for (int j = 0; j < kBatchMaxWin; j++) {
+        beta[j] = 0.0;
+        if (j < c) beta[j] = bdot<BS>(W.z, W.fq + (long long)j * E, E, sh) / sqv[j];
+      }
+      for (long long e = threadIdx.x; e < E; e += BS) {
+        double v = W.z[e];
+#pragma unroll
+        for (int j = 0; j < kBatchMaxWin; j++)
+          if (j < c) v -= beta[j] * W.fs[(long long)j * E + e];
+        sk[e] = v;
+      }
+      __syncthreads();
+      bapply_G<BS>(W.I, sk, W.ft, W.bc);
+      double irt_k = A.irt;
+      if (relax > 0 && relRes > 0) irt_k = fmin(relax, fmax(A.irt, A.irt * relax_ref / relRes));
+      if ((err = inner_solve<BS, TX>(A, W, W.ft, W.vstar, irt_k, &inner, lds32, lds64, sh))) break;
+      bapply_Gt<BS>(W.I, W.vstar, qk);
+      const double sq = bdot<BS>(sk, qk, E, sh);
+      const double sr = bdot<BS>(sk, W.r, E, sh);
+      if (threadIdx.x == 0) sqv[slot] = sq;
+      const double alpha = sr / sq;
+      if (!isfinite(sq) || !isfinite(alpha)) {
+        err = GS_E_NAN;
+        break;
+      }
+      double rr = 0.0;
+      for (long long e = threadIdx.x; e < E; e += BS) {
+        W.p[e] += alpha * sk[e];
+        const double ri = W.r[e] - alpha * qk[e];
+        W.r[e] = ri;
+        rr += ri * ri;
+      }
+      for (long long i = threadIdx.x; i < N2; i += BS) W.u[i] -= alpha * W.vstar[i];
+      rr = block_sum<BS>(rr, sh);
+      relRes = sqrt(rr) * qRecip / scale;
+      if (!isfinite(relRes)) {
+        err = GS_E_NAN;
+        break;
+      }
+      if (!(it < A.max_outer)) {
+        err = GS_E_MAXIT;
+        break;
+      }
+      it++;
+      conv = relRes <= A.tol;
+      // stagnation (less than 5 % drop over kStagWin iterations) while still
+      // relaxing: end the round, re-check the true residual, relax less
+      if (!conv && relax > 0 && it - stag_it >= kStagWin) {
+        if (relRes > 0.95 * stag_ref) break;
+        stag_ref = relRes;
+        stag_it = it;
+      }
+    }
+    if (err) break;
+    // true residual: u = K^-1 (f - G p) with the reference inner tolerance
+    bapply_G<BS>(W.I, W.p, W.ft, W.bc);
+    for (long long i = threadIdx.x; i < N2; i += BS) W.ft[i] = W.f[i] - W.ft[i];
+    __syncthreads();
+    if ((err = inner_solve<BS, TX>(A, W, W.ft, W.u, A.irt, &inner, lds32, lds64, sh))) break;
+    bapply_Gt<BS>(W.I, W.u, W.r);
+    relRes = sqrt(bdot<BS>(W.r, W.r, E, sh)) * qRecip / scale;
+    if (!isfinite(relRes)) {
+      err = GS_E_NAN;
+      break;
+    }
+    if (relRes <= A.tol) break;
+    if (round == kRounds - 1) err = GS_E_MAXIT;
+  }
+  *outer_out = it;
+  *inner_out = inner;
+  *relres_out = relRes;
+  return err;
+}
+
 template <int BS, int TX>
 __global__ __launch_bounds__(BS) void k_batch_uzawa(BArgs A) {
   constexpr int TY = BS / TX;
@@ -521,6 +656,7 @@ __global__ __launch_bounds__(BS) void k_batch_uzawa(BArgs A) {
   __shared__ double lds64[2 * NB2];
   __shared__ float lds32[2 * NB2];
   __shared__ double sh[BS / 64];
+  __shared__ double sqv[kBatchMaxWin];
   __shared__ int cur;
   for (;;) {
     if (threadIdx.x == 0) {
@@ -553,11 +689,16 @@ __global__ __launch_bounds__(BS) void k_batch_uzawa(BArgs A) {
     W.s = A.s + eo;
     W.q = A.q + eo;
     W.z = A.z + eo;
+    W.fs = A.fs ? A.fs + (long long)A.window * eo : nullptr;
+    W.fq = A.fq ? A.fq + (long long)A.window * eo : nullptr;
     int outer = 0;
     long long inner = 0;
     double rel = 0.0;
-    const int st = uzawa_one<BS, TX>(W, A, &outer, &inner, &rel, lds32, lds64, sh);
+    const unsigned long long t0 = wall_clock64();
+    const int st = A.relax > 0 ? uzawa_relaxed<BS, TX>(W, A, &outer, &inner, &rel, lds32, lds64, sh, sqv)
+                               : uzawa_one<BS, TX>(W, A, &outer, &inner, &rel, lds32, lds64, sh);
     if (threadIdx.x == 0) {
+      A.itime[i] = wall_clock64() - t0;
       A.status[i] = st;
       A.outer[i] = outer;
       A.inner[i] = inner;
@@ -596,8 +737,12 @@ struct gs_batch {
   double *u = nullptr, *vstar = nullptr, *ft = nullptr, *cr = nullptr, *cz = nullptr, *cp = nullptr, *cAp = nullptr;
   double *p = nullptr, *r = nullptr, *s = nullptr, *q = nullptr, *z = nullptr;
   int *status = nullptr, *outer = nullptr;
+  unsigned long long* itime = nullptr;
+  double clock_khz = 100000.0;  // wall_clock64 rate
   long long* inner = nullptr;
   double* relres = nullptr;
+  double *fs = nullptr, *fq = nullptr;  // relaxed loop direction rings (window x elements)
+  int fwin = 0;
   unsigned* queue = nullptr;
   int tile = 4;  // index into kTiles
   int ncu = 256;
@@ -627,7 +772,8 @@ extern "C" void gs_batch_destroy(gs_batch* b) {
   free_graph(b);
   void* ptrs[] = {b->inst, b->lev, b->order, b->M, b->f, b->dinv, b->Qinv, b->bc, b->LM, b->Ldinv, b->Lb, b->Lx,
                   b->Lr, b->Ld, b->LAx, b->Lbc, b->Kinv, b->freeidx, b->u, b->vstar, b->ft, b->cr, b->cz, b->cp,
-                  b->cAp, b->p, b->r, b->s, b->q, b->z, b->status, b->outer, b->inner, b->relres, b->queue};
+                  b->cAp, b->p, b->r, b->s, b->q, b->z, b->status, b->outer, b->inner, b->relres, b->queue,
+                  b->fs, b->fq, b->itime};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (b->e0) (void)hipEventDestroy(b->e0);
@@ -664,6 +810,9 @@ extern "C" int32_t gs_batch_create(int32_t count, const gs_problem_desc* descs, 
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
         prop.multiProcessorCount > 0)
       b->ncu = prop.multiProcessorCount;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0)
+      b->clock_khz = khz;
   }
   // per-instance setup through the single-problem path: particle location,
   // constitutive pass, right-hand side, multigrid levels (Galerkin moments,
@@ -733,7 +882,7 @@ extern "C" int32_t gs_batch_create(int32_t count, const gs_problem_desc* descs, 
       balloc(&b->vstar, N2) || balloc(&b->ft, N2) || balloc(&b->cr, N2) || balloc(&b->cz, N2) ||
       balloc(&b->cp, N2) || balloc(&b->cAp, N2) || balloc(&b->p, eo) || balloc(&b->r, eo) || balloc(&b->s, eo) ||
       balloc(&b->q, eo) || balloc(&b->z, eo) || balloc(&b->status, count) || balloc(&b->outer, count) ||
-      balloc(&b->inner, count) || balloc(&b->relres, count) || balloc(&b->queue, 1))
+      balloc(&b->inner, count) || balloc(&b->relres, count) || balloc(&b->queue, 1) || balloc(&b->itime, count))
     return fail(GS_E_HIP);
   for (int i = 0; i < count; i++) {
     const BInst& I = b->hinst[i];
@@ -816,6 +965,7 @@ static BArgs make_args(gs_batch* b, const gs_batch_params* p) {
   A.q = b->q;
   A.z = b->z;
   A.status = b->status;
+  A.itime = b->itime;
   A.outer = b->outer;
   A.inner = b->inner;
   A.relres = b->relres;
@@ -825,7 +975,27 @@ static BArgs make_args(gs_batch* b, const gs_batch_params* p) {
   A.max_outer = p->max_iterations > 0 ? p->max_iterations : 1000;
   A.max_inner = p->max_inner > 0 ? p->max_inner : 20000;
   A.nu = p->mg_smooth > 0 ? p->mg_smooth : 2;
+  A.relax = p->inner_relax_max > 0 ? p->inner_relax_max : 0.0;
+  A.window = p->flexible_window > 0 ? std::min(p->flexible_window, kBatchMaxWin) : 8;
+  A.fs = b->fs;
+  A.fq = b->fq;
   return A;
+}
+
+// Direction rings of the relaxed loop (allocated on first use, before any
+// graph capture).
+static int32_t ensure_rings(gs_batch* b, const gs_batch_params* p) {
+  if (!(p->inner_relax_max > 0)) return GS_OK;
+  const int m = p->flexible_window > 0 ? std::min(p->flexible_window, kBatchMaxWin) : 8;
+  if (b->fs && b->fwin == m) return GS_OK;
+  if (b->fs) (void)hipFree(b->fs);
+  if (b->fq) (void)hipFree(b->fq);
+  b->fs = b->fq = nullptr;
+  b->fwin = 0;
+  BCK(balloc(&b->fs, (long long)m * b->elems));
+  BCK(balloc(&b->fq, (long long)m * b->elems));
+  b->fwin = m;
+  return GS_OK;
 }
 
 static void launch_batch(gs_batch* b, const BArgs& A, int tile, hipStream_t st) {
@@ -851,11 +1021,17 @@ static void launch_batch(gs_batch* b, const BArgs& A, int tile, hipStream_t st) 
 
 static bool same_params(const gs_batch_params& a, const gs_batch_params& c) {
   return a.tolerance == c.tolerance && a.max_iterations == c.max_iterations && a.max_inner == c.max_inner &&
-         a.inner_rtol_factor == c.inner_rtol_factor && a.mg_smooth == c.mg_smooth;
+         a.inner_rtol_factor == c.inner_rtol_factor && a.mg_smooth == c.mg_smooth &&
+         a.inner_relax_max == c.inner_relax_max && a.flexible_window == c.flexible_window;
 }
 
 extern "C" int32_t gs_batch_solve(gs_batch* b, const gs_batch_params* p, gs_batch_result* results, double* seconds) {
   if (!b || !p || !(p->tolerance > 0)) return GS_E_INVALID;
+  if (p->flexible_window < 0 || p->flexible_window > kBatchMaxWin) return GS_E_INVALID;
+  {
+    const int32_t s = ensure_rings(b, p);
+    if (s != GS_OK) return s;
+  }
   const BArgs A = make_args(b, p);
   if (p->use_graph) {
     if (!b->exec || b->gtile != b->tile || !same_params(b->gparams, *p)) {
@@ -883,13 +1059,15 @@ extern "C" int32_t gs_batch_solve(gs_batch* b, const gs_batch_params* p, gs_batc
   std::vector<int> st(b->B), ou(b->B);
   std::vector<long long> in(b->B);
   std::vector<double> rr(b->B);
+  std::vector<unsigned long long> tk(b->B);
   BCK(hipMemcpy(st.data(), b->status, sizeof(int) * b->B, hipMemcpyDeviceToHost));
+  BCK(hipMemcpy(tk.data(), b->itime, sizeof(unsigned long long) * b->B, hipMemcpyDeviceToHost));
   BCK(hipMemcpy(ou.data(), b->outer, sizeof(int) * b->B, hipMemcpyDeviceToHost));
   BCK(hipMemcpy(in.data(), b->inner, sizeof(long long) * b->B, hipMemcpyDeviceToHost));
   BCK(hipMemcpy(rr.data(), b->relres, sizeof(double) * b->B, hipMemcpyDeviceToHost));
   int32_t worst = GS_OK;
   for (int i = 0; i < b->B; i++) {
-    if (results) results[i] = gs_batch_result{st[i], ou[i], (int64_t)in[i], rr[i]};
+    if (results) results[i] = gs_batch_result{st[i], ou[i], (int64_t)in[i], rr[i], (double)tk[i] / (b->clock_khz * 1e3)};
     if (st[i] != GS_OK && worst == GS_OK) worst = st[i];
   }
   return worst;
@@ -897,6 +1075,11 @@ extern "C" int32_t gs_batch_solve(gs_batch* b, const gs_batch_params* p, gs_batc
 
 extern "C" int32_t gs_batch_autotune(gs_batch* b, const gs_batch_params* p, int32_t* block, int32_t* tx, float* ms) {
   if (!b || !p) return GS_E_INVALID;
+  if (p->flexible_window < 0 || p->flexible_window > kBatchMaxWin) return GS_E_INVALID;
+  {
+    const int32_t s = ensure_rings(b, p);
+    if (s != GS_OK) return s;
+  }
   gs_batch_params q = *p;
   q.use_graph = 0;
   if (q.max_iterations <= 0 || q.max_iterations > 3) q.max_iterations = 3;  // a few Uzawa iterations per instance

@@ -39,6 +39,13 @@ typedef struct gs_batch_params {
   double inner_rtol_factor;  /* inner rtol = factor * tolerance (reference: 0.1) */
   int32_t mg_smooth;         /* Chebyshev-Jacobi steps before and after each coarse correction (0 = 2) */
   int32_t use_graph;         /* 1: capture the launch once as a HIP graph and replay it */
+  double inner_relax_max;    /* > 0: inexact Uzawa -- the inner rtol grows as the outer residual falls,
+                                min(inner_relax_max, max(irt, irt |r_0| / |r_k|)), the outer loop is flexible
+                                CG on the Schur complement S-orthogonalised against the last flexible_window
+                                directions, and convergence is re-checked on the true residual (accurate
+                                inner solve); 0 = the reference loop */
+  int32_t flexible_window;   /* directions of the flexible outer loop (0 = 8, at most 16) */
+  int32_t pad_;
 } gs_batch_params;
 
 typedef struct gs_batch_result {
@@ -46,6 +53,7 @@ typedef struct gs_batch_result {
   int32_t uzawa_iterations;
   int64_t inner_iterations;
   double rel_residual;
+  double seconds;            /* device time the instance's solve took inside the launch */
 } gs_batch_result;
 
 /* count problem descriptions (2D); each instance is set up on the device. */

@@ -73,3 +73,37 @@ def test_batch_autotune_and_tiles(gale):
         else:
             for (u, p), (ur, pr) in zip(sol, ref):
                 assert rel(u, ur) < 1e-5 and rel(p - p.mean(), pr - pr.mean()) < 1e-5
+
+
+def test_batch_relaxed_loop_matches_reference(gale):
+    """The inexact loop (relaxed inner tolerance, flexible outer CG, true
+    residual re-checked): every instance reaches the 1e-6 true residual and
+    agrees with the reference loop's solution within the stated tolerance
+    (both at a 1e-6 Schur residual: u 1e-3, p 1e-3); it needs fewer inner
+    iterations."""
+    cfgs, bs = _batch(gale, 24, (16, 96), seed=80)
+    st, r0, t0 = bs.solve(gale.batch_params(tolerance=1e-6))
+    assert st == 0
+    ref = [bs.solution(i) for i in range(len(cfgs))]
+    st, r1, t1 = bs.solve(gale.batch_params(tolerance=1e-6, inner_relax_max=0.03, flexible_window=8))
+    print(f"reference loop {t0:.3f}s inner {sum(r['inner_iterations'] for r in r0)}; relaxed {t1:.3f}s inner "
+          f"{sum(r['inner_iterations'] for r in r1)}", flush=True)
+    assert st == 0 and all(r["status"] == 0 and r["rel_residual"] <= 1e-6 for r in r1)
+    assert sum(r["inner_iterations"] for r in r1) < sum(r["inner_iterations"] for r in r0)
+    for i in range(len(cfgs)):
+        u, p = bs.solution(i)
+        ur, pr = ref[i]
+        assert rel(u, ur) < 1e-3 and rel(p - p.mean(), pr - pr.mean()) < 1e-3, (i, rel(u, ur))
+    assert all(r["seconds"] > 0 for r in r1)
+    # the relaxed loop is also exact at a tight tolerance against the single-problem solver
+    st, r2, _ = bs.solve(gale.batch_params(tolerance=1e-10, inner_rtol_factor=1e-3, inner_relax_max=0.03))
+    assert st == 0
+    for i in (0, 5, 11):
+        u, p = bs.solution(i)
+        pr_ = gale.Problem(**cfgs[i].problem_kwargs())
+        s = gale.Stokes_SLE_UzawaSolver(pr_, gale.uzawa_params(tolerance=1e-10, inner_rtol_override=1e-13,
+                                                               maxIterations=20000, inner_max_iterations=200000))
+        us, ps, _ = s.Solve()
+        assert rel(u, us) < 1e-7 and rel(p - p.mean(), ps - ps.mean()) < 1e-7
+        s.close()
+        pr_.close()
