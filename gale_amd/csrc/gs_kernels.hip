@@ -481,7 +481,9 @@ void launch_weakening_step(double* w, const double* inc, const double* coord, lo
 template <int DIM>
 __global__ void k_apply_G(Geo g, const double* __restrict__ p, double* __restrict__ y,
                           const uint8_t* __restrict__ bcflag) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // blocks remapped so each XCD gathers from a contiguous slab (its L2 serves
+  // the pressures shared by neighbouring nodes)
+  const long long n = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= g.N) return;
   long long i, j, k;
   if (g.N < (1LL << 31)) {
@@ -556,7 +558,8 @@ void launch_apply_G(const Geo& g, const double* p, double* y, const uint8_t* bcf
 }
 
 __global__ void k_apply_Gt(Geo g, const double* __restrict__ x, double* __restrict__ q) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  // blocks remapped so each XCD gathers from a contiguous slab of nodes
+  const long long e = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (e >= g.E) return;
   long long i, j, k;
   if (g.E < (1LL << 31)) {  // 32-bit lattice indices (64-bit division is a long software sequence)

@@ -277,7 +277,7 @@ __device__ __forceinline__ void node_ijk(const Geo& g, long long n, long long& i
 // (3 x 3 (x 3) stencil, weights 1 and 1/2 per axis, in stencil order).
 template <typename T, int DIM>
 __global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restrict__ bc, const uint8_t* __restrict__ bcflag_c) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long n = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= gc.N) return;
   long long I, J, K;
   node_ijk(gc, n, I, J, K);
@@ -308,7 +308,7 @@ __global__ void k_restrict(Geo gc, Geo gf, const T* __restrict__ rf, T* __restri
 // 2^dim coarse parents and the fine value are loaded at once.
 template <typename T, int DIM>
 __global__ void k_prolong_add(Geo gc, Geo gf, const T* __restrict__ xc, T* __restrict__ xf, const uint8_t* __restrict__ bcflag_f) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long n = xcd_block_mg(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (n >= gf.N) return;
   long long i, j, k;
   node_ijk(gf, n, i, j, k);

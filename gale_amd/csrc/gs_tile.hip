@@ -466,18 +466,18 @@ __global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg
   }
 }
 
-// Adds the halo partial sums of up to seven lower neighbour tiles.  With a
-// fused node update (EPI != 0) every node passes through here, so the update
-// streams all vectors once after the apply.
-template <typename T, int EPI>
-__global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __restrict__ y,
-                             const T* __restrict__ halo, const uint8_t* __restrict__ bcflag, TileEpi<T> ep) {
+// Completes the tile-face nodes of a plain apply (no fused update): one
+// thread per node, only nodes on a lower tile face (about 40 %) do any work,
+// adding the halo partial sums of up to seven lower neighbour tiles.
+template <typename T>
+__global__ void k_tile_face3(Geo g, TileGrid tg, T* __restrict__ y, const T* __restrict__ halo,
+                             const uint8_t* __restrict__ bcflag) {
   constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
   constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= g.N) return;
   int i, j, k;
-  if (g.N < (1LL << 31)) {  // 32-bit lattice indices (64-bit division is a long software sequence)
+  if (g.N < (1LL << 31)) {
     const unsigned un = (unsigned)n, nx = (unsigned)g.nn[0], ny = (unsigned)g.nn[1];
     const unsigned q = un / nx;
     i = (int)(un - q * nx);
@@ -490,7 +490,7 @@ __global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __r
   }
   const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1), tz = min(k / TZ, tg.n[2] - 1);
   const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0), onz = (k == tz * TZ && tz > 0);
-  if (EPI == 0 && !(onx | ony | onz)) return;
+  if (!(onx | ony | onz)) return;
   T acc[3] = {0, 0, 0};
   for (int az = 0; az <= onz; az++)
     for (int ay = 0; ay <= ony; ay++)
@@ -505,14 +505,62 @@ __global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __r
         acc[2] += h[2];
       }
   const unsigned bf = bcflag ? bcflag[n] : 0u;
-  if (EPI == 0) {
-    for (int c = 0; c < 3; c++)
-      if (!((bf >> c) & 1u)) y[n * 3 + c] += acc[c];
+  for (int c = 0; c < 3; c++)
+    if (!((bf >> c) & 1u)) y[n * 3 + c] += acc[c];
+}
+
+// Completes an apply with a fused node update (EPI != 0; EPI == 0 is
+// k_tile_face3): every dof passes through here, so the update streams all
+// vectors once after the apply, adding on tile-face nodes the halo partial
+// sums of up to seven lower neighbour tiles.  One thread per dof (node-major:
+// dof i = 3 n + c), so every vector access of a wavefront is one contiguous
+// run (a thread per node with three strided accesses moved about twice the
+// bytes); blocks are remapped so each XCD works on a contiguous range.
+template <typename T, int EPI>
+__global__ void k_tile_halo3(Geo g, TileGrid tg, const T* __restrict__ x, T* __restrict__ y,
+                             const T* __restrict__ halo, const uint8_t* __restrict__ bcflag, TileEpi<T> ep) {
+  constexpr int TX = kTile3X, TY = kTile3Y, TZ = kTile3Z;
+  constexpr int NBX = TX + 1, NBY = TY + 1, NBZ = TZ + 1, NB = NBX * NBY * NBZ;
+  const long long id = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (id >= g.N * 3) return;
+  long long n;
+  int c, i, j, k;
+  if (g.N * 3 < (1LL << 31)) {  // 32-bit lattice indices (64-bit division is a long software sequence)
+    const unsigned uid = (unsigned)id;
+    const unsigned un = uid / 3u;
+    c = (int)(uid - 3u * un);
+    n = un;
+    const unsigned nx = (unsigned)g.nn[0], ny = (unsigned)g.nn[1];
+    const unsigned q = un / nx;
+    i = (int)(un - q * nx);
+    j = (int)(q % ny);
+    k = (int)(q / ny);
   } else {
-    for (int c = 0; c < 3; c++) {
-      const T ax = ((bf >> c) & 1u) ? (T)0 : y[n * 3 + c] + acc[c];
-      epi_store<T, EPI>(ep, y, n * 3 + c, ax, x[n * 3 + c]);
-    }
+    n = id / 3;
+    c = (int)(id - 3 * n);
+    i = (int)(n % g.nn[0]);
+    j = (int)((n / g.nn[0]) % g.nn[1]);
+    k = (int)(n / (g.nn[0] * g.nn[1]));
+  }
+  const int tx = min(i / TX, tg.n[0] - 1), ty = min(j / TY, tg.n[1] - 1), tz = min(k / TZ, tg.n[2] - 1);
+  const int onx = (i == tx * TX && tx > 0), ony = (j == ty * TY && ty > 0), onz = (k == tz * TZ && tz > 0);
+  if (EPI == 0 && !(onx | ony | onz)) return;
+  T acc = 0;
+  for (int az = 0; az <= onz; az++)
+    for (int ay = 0; ay <= ony; ay++)
+      for (int ax = 0; ax <= onx; ax++) {
+        if (!(ax | ay | az)) continue;
+        const int ux = tx - ax, uy = ty - ay, uz = tz - az;
+        const long long t2 = ux + (long long)tg.n[0] * (uy + (long long)tg.n[1] * uz);
+        const int ln = (i - ux * TX) + NBX * ((j - uy * TY) + NBY * (k - uz * TZ));
+        acc += halo[(t2 * NB + ln) * 3 + c];
+      }
+  const bool fixed = bcflag ? ((bcflag[n] >> c) & 1u) : false;
+  if (EPI == 0) {
+    if (!fixed) y[id] += acc;
+  } else {
+    const T ax = fixed ? (T)0 : y[id] + acc;
+    epi_store<T, EPI>(ep, y, id, ax, x[id]);
   }
 }
 
@@ -653,7 +701,12 @@ static void tile_launch(const Geo& g, const T* x, T* y, T* halo, const T* Mom, c
     const int nt = FORM == 0 && (double)mom_count<T>(g) * sizeof(T) > 64.0e6 ? 1 : 0;
     hipLaunchKernelGGL((k_tile_apply3<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
                        y, halo, Mom, pk, off, bcflag, nt);
-    hipLaunchKernelGGL((k_tile_halo3<T, EPI>), dim3(nbn), dim3(kBlock), 0, st, g, tg, x, y, halo, bcflag, ep);
+    if (EPI == 0) {
+      hipLaunchKernelGGL((k_tile_face3<T>), dim3(nbn), dim3(kBlock), 0, st, g, tg, y, halo, bcflag);
+    } else {
+      const unsigned nbd = (unsigned)((g.N * 3 + kBlock - 1) / kBlock);
+      hipLaunchKernelGGL((k_tile_halo3<T, EPI>), dim3(nbd), dim3(kBlock), 0, st, g, tg, x, y, halo, bcflag, ep);
+    }
   } else {
     hipLaunchKernelGGL((k_tile_apply2<T, FORM>), dim3((unsigned)tg.ntiles), dim3(kTileThreads), 0, st, g, tg, x,
                        y, halo, Mom, pk, off, bcflag);
