@@ -366,15 +366,17 @@ def main():
     bytes_g = 8 * cfg.E + 8 * cfg.dim * cfg.N + cfg.N
     ms_gt = s.time_apply(G.APPLY_GT, args.apply_reps, cold=True)
     bytes_gt = 8 * cfg.dim * cfg.N + 8 * cfg.E
-    traffic = None
+    # measured HBM bytes per launch (FETCH_SIZE + WRITE_SIZE passes of
+    # rocprofv3, scaled by a 1 GiB copy) of this workload, per form
+    traffic = {}
     tpath = os.path.join(ROOT, "profiles", "k_apply_traffic.json")
     if os.path.exists(tpath):
         try:
             tr = json.load(open(tpath))
             if tr.get("res") == args.res and tr.get("ppc") == args.ppc:
-                traffic = tr.get("bytes_per_apply")
+                traffic = {k: v.get("bytes_per_apply") for k, v in tr.get("forms", {}).items()}
         except Exception:
-            traffic = None
+            traffic = {}
     print(f"# K apply moments fp64 {ms_m:.4f} ms cold ({bytes_m / ms_m / 1e6:.0f} GB/s), {ms_m_warm:.4f} ms warm; "
           f"fp32 {ms_m32:.4f} ms; particles fp64 {ms_p:.4f} ms, fp32 {ms_p32:.4f} ms; G {ms_g:.4f} ms, "
           f"G^T {ms_gt:.4f} ms", file=sys.stderr, flush=True)
@@ -390,15 +392,20 @@ def main():
                                f"+ plastic lid (first, linear solve of the Picard sequence), free slip, "
                                f"Uzawa to 1e-6",
                    "elements": cfg.E, "particles": cfg.P, "dofs": dofs, "generate_s": t_gen},
-        "roofline": roofline_obj(bytes_m, ms_m, "gs::k_tile_apply3<double,0> + gs::k_tile_halo3<double> "
+        "roofline": roofline_obj(bytes_m, ms_m, "gs::k_tile_apply3<double,0> + gs::k_tile_face3<double> "
                                  "(moment-form FP64 K apply, fine level; cold: operand copies cycled past the "
-                                 "256 MB last-level cache)", traffic=traffic, ms_per_apply_warm=ms_m_warm),
-        "roofline_moments_fp32": roofline_obj(bytes_m32, ms_m32, "moment-form FP32 K apply (V-cycle fine level)"),
-        "roofline_particle_apply": roofline_obj(bytes_p, ms_p, "gs::k_tile_apply3<double,1> + halo (particle-form "
-                                                "FP64 K apply)", fp32_ms_per_apply=ms_p32,
-                                                fp32_algorithmic_bytes=bytes_p32),
-        "roofline_G": roofline_obj(bytes_g, ms_g, "gs::k_apply_G (G p)"),
-        "roofline_Gt": roofline_obj(bytes_gt, ms_gt, "gs::k_apply_Gt (G^T u)"),
+                                 "256 MB last-level cache)", traffic=traffic.get("moments_fp64"),
+                                 ms_per_apply_warm=ms_m_warm),
+        "roofline_moments_fp32": roofline_obj(bytes_m32, ms_m32, "gs::k_tile_apply3<float,0> + "
+                                              "gs::k_tile_face3<float> (moment-form FP32 K apply, V-cycle fine "
+                                              "level)", traffic=traffic.get("moments_fp32")),
+        "roofline_particle_apply": roofline_obj(bytes_p, ms_p, "gs::k_tile_apply3<double,1> + "
+                                                "gs::k_tile_face3<double> (particle-form FP64 K apply)",
+                                                traffic=traffic.get("particles_fp64"), fp32_ms_per_apply=ms_p32,
+                                                fp32_algorithmic_bytes=bytes_p32,
+                                                fp32_traffic=traffic.get("particles_fp32")),
+        "roofline_G": roofline_obj(bytes_g, ms_g, "gs::k_apply_G<3> (G p)", traffic=traffic.get("G")),
+        "roofline_Gt": roofline_obj(bytes_gt, ms_gt, "gs::k_apply_Gt (G^T u)", traffic=traffic.get("Gt")),
         "cpu_baseline": None,
     }
     line["roofline"]["frac"] = line["roofline"]["achieved"] / HBM_PEAK_GBS
