@@ -118,6 +118,7 @@ def lib():
             "orc_interpolate_velocity": (i32, [vp, vp, vp, vp]),
             "orc_advect_rk2": (i32, [vp, vp, d, vp]),
             "orc_dvc_cell": (i32, [i32, vp, i32, vp, vp, vp]),
+            "orc_dvc_cell_owners": (i32, [i32, vp, i32, vp, vp, vp, vp]),
             "orc_set_weights": (i32, [vp, i32, vp]),
             "orc_set_strain_weakening": (i32, [vp, i32, vp]),
             "orc_set_plastic_strain": (i32, [vp, vp]),
@@ -126,6 +127,7 @@ def lib():
             "orc_weakening_step_limited": (i32, [vp, d, vp, d]),
             "orc_particle_fields": (i32, [vp, vp, vp, vp, vp, vp]),
             "orc_population_control": (i32, [vp, vp, vp]),
+            "orc_get_cell_tables": (None, [vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -254,6 +256,21 @@ def dvc_cell(dim, res, xi):
     return cent, w
 
 
+def dvc_cell_owners(dim, res, xi):
+    """dvc_cell plus the owning particle of every sub-cell (k = i + r0 (j + r1 l))."""
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, dim)
+    n = xi.shape[0]
+    r = np.array(list(res) + [1] * (3 - len(res)), dtype=np.int32)
+    cent = np.zeros((n, dim))
+    w = np.zeros(n)
+    own = np.zeros(int(np.prod(r)), np.int32)
+    st = lib().orc_dvc_cell_owners(dim, r.ctypes.data, n, xi.ctypes.data, cent.ctypes.data, w.ctypes.data,
+                                   own.ctypes.data)
+    if st != 0:
+        raise RuntimeError(f"orc_dvc_cell status {st}")
+    return cent, w, own
+
+
 def uzawa_params(tolerance=1e-5, max_iterations=1000, min_iterations=1, use_absolute_tolerance=0,
                  use_preconditioner=1, inner=0, inner_max_iterations=0, inner_rtol_factor=0.1,
                  inner_rtol_override=0.0, nthreads=1):
@@ -319,19 +336,26 @@ class Problem:
             self._h = None
 
     def population_control(self, upperT=25.0, lowerT=0.6, maxSplits=3, maxDeletions=2,
-                           splitInInterfaceCells=False, deleteInInterfaceCells=False, resolution=(10, 10, 10)):
+                           splitInInterfaceCells=False, deleteInInterfaceCells=False, resolution=(10, 10, 10),
+                           Inflow=False, particlesPerCell=25, Threshold=0.8, CentPosRatio=0.01, rand_seed=1,
+                           rand_skip=0):
+        """One PCDVC pass (the reference's XML names); returns (added,
+        deleted) and sets self.rand_draws to the rand() calls it made."""
         class _P(C.Structure):
             _fields_ = [("upper_t", C.c_double), ("lower_t", C.c_double), ("max_splits", C.c_int32),
                         ("max_deletions", C.c_int32), ("split_in_interface_cells", C.c_int32),
-                        ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("pad_", C.c_int32)]
+                        ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("inflow", C.c_int32),
+                        ("particles_per_cell", C.c_int32), ("pad_", C.c_int32), ("threshold", C.c_double),
+                        ("cent_pos_ratio", C.c_double), ("rand_seed", C.c_int64), ("rand_skip", C.c_int64)]
         r = list(resolution) + [10] * (3 - len(resolution))
         p = _P(upperT, lowerT, maxSplits, maxDeletions, int(splitInInterfaceCells), int(deleteInInterfaceCells),
-               (C.c_int32 * 3)(*r), 0)
-        counts = np.zeros(2, np.int64)
+               (C.c_int32 * 3)(*r), int(Inflow), particlesPerCell, 0, Threshold, CentPosRatio, rand_seed, rand_skip)
+        counts = np.zeros(3, np.int64)
         st = lib().orc_population_control(self._h, C.byref(p), _p(counts))
         if st != 0:
             raise RuntimeError(f"orc_population_control status {st}")
         self.P = int(lib().orc_num_particles(self._h))
+        self.rand_draws = int(counts[2])
         return int(counts[0]), int(counts[1])
 
     def particle_fields(self, u_full):
@@ -396,6 +420,13 @@ class Problem:
         coord = np.zeros((self.P, 3))
         lib().orc_get_particles(self._h, _p(owner), _p(xi), _p(w), _p(coord))
         return owner, xi, w, coord
+
+    def cell_tables(self):
+        """Each cell's particles (caller indices) in the cell table's order."""
+        off = np.zeros(int(lib().orc_num_elements(self._h)) + 1, np.int64)
+        lst = np.zeros(max(self.P, 1), np.int64)
+        lib().orc_get_cell_tables(self._h, _p(off), _p(lst))
+        return [lst[off[e]:off[e + 1]].copy() for e in range(len(off) - 1)]
 
     def set_coords(self, coord):
         c = _f64(coord).reshape(-1, 3)

@@ -466,7 +466,9 @@ static double dvc_contest(int dim, const double* c, const double* o, const doubl
   return s;
 }
 
-int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w) {
+/* owner_out (may be NULL) receives the owning particle of every sub-cell. */
+static int32_t dvc_cell_impl(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w,
+                             int32_t* owner_out) {
   if (dim != 2 && dim != 3) return ORC_E_INVALID;
   int r[3] = {res[0], res[1], dim == 3 ? res[2] : 1};
   for (int d = 0; d < dim; d++)
@@ -579,10 +581,21 @@ int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double
   }
 #undef DVC_CENTRE
 #undef DVC_REBOUND
+  if (owner_out)
+    for (int c = 0; c < nc; c++) owner_out[c] = owner[c];
   for (int i = 0; i < nump; i++) free(bound[i]);
   free(bound); free(nbound); free(took); free(owner); free(mark);
   for (int d = 0; d < 3; d++) free(ctr[d]);
   return ORC_OK;
+}
+
+int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w) {
+  return dvc_cell_impl(dim, res, nump, xi, cent, w, NULL);
+}
+
+int32_t orc_dvc_cell_owners(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w,
+                            int32_t* owner) {
+  return dvc_cell_impl(dim, res, nump, xi, cent, w, owner);
 }
 
 /* Stable counting sort of particles by owning element: the cell tables that
@@ -637,141 +650,446 @@ static int relocate_all(orc_problem* pb) {
   return ORC_OK;
 }
 
-/* PCDVC population control (PCDVC.cxx:656-1090 / 1092-1512, Inflow off),
- * then DVC weights.  Per cell, on the DVC of its particles' local
- * coordinates: maxW = upperT% and minW = lowerT% of the cell volume; in a
- * cell holding more than one material no particle is deleted (split) unless
- * deleteInInterfaceCells (splitInInterfaceCells); maxDeletions is cut to
- * nump/2 when it exceeds nump - 4; the first maxSplits particles (in cell
- * order) with weight above maxW are split -- a copy of the material point is
- * put at the centroid of the parent's region, at the global position
- * 0.5 ((1 - xi) a + (1 + xi) b) per axis -- and of the particles with weight
- * below minW the maxDeletions with the highest swarm indices are deleted.  The particle order afterwards: the
- * surviving particles in their previous order, then the new ones in cell
- * order and split order.  counts = {added, deleted}. */
+/* PCDVC population control followed by the DVC weights it leaves
+ * (PCDVC.cxx:656-1090 in 3D, 1092-1512 in 2D), restated as the reference
+ * runs it: one pass over the cells in order (WeightsCalculator.cxx:195-197)
+ * on a swarm held as a global particle array (indexOnCPU) plus one table per
+ * cell, both built as CoincidentMapper builds them (particles added to their
+ * cell in index order, CoincidentMapper.cxx:207-240).
+ *
+ * Swarm operations (PCDVC.cxx:339-610, SwarmClass.cxx:976-990):
+ *  - split: a copy of the parent appended to the global array and to the end
+ *    of the cell's table, at local coordinate xi (global position
+ *    0.5 ((1 - xi) a + (1 + xi) b) per axis);
+ *  - delete: the cell's last table entry takes the freed table slot, and the
+ *    global array's last particle takes the freed index (its table entry is
+ *    rewritten to that index).
+ *
+ * Per cell, on the DVC of the particles' local coordinates in table order:
+ *  Inflow (when on): the last particle's squared distance to its centroid is
+ *  compared with CentPosRatio x 2 (the loop keeps only the last one, :783-787);
+ *  when it is larger, or nump / ParticlesPerCell < Threshold, ParticlesPerCell
+ *  particles are added, each a split of the owner of a random sub-cell j =
+ *  (int)(nc rand() / (RAND_MAX + 1.0)) placed at that sub-cell's centre, then
+ *  when the original count is above 5 that many particles are deleted at
+ *  random table slots (int)(nump rand() / (RAND_MAX + 1.0)), and the DVC is
+ *  redone.  Each particle's list of owned sub-cells is then built as the
+ *  reference builds it: (int)(w / da' + 0.5) entries (da' = round(1 / da)),
+ *  filled from the end in sub-cell order (:898-918).
+ *  Then: maxW = upperT %, minW = lowerT % of the cell volume; a cell with more
+ *  than one material splits (deletes) nothing unless splitInInterfaceCells
+ *  (deleteInInterfaceCells); maxDeletions above nump - 4 becomes nump / 2
+ *  (nump - 4 in 2D with Inflow, :1387); the first maxSplits particles above
+ *  maxW split, at their region's centroid, or with Inflow at the centre of a
+ *  random entry of their sub-cell list; the candidates below minW, sorted by
+ *  index, highest first (compare_indexOnCPU, :611-620), are deleted up to
+ *  maxDeletions; the DVC is redone when anything changed.  Every particle of
+ *  the cell then moves to its region's centroid with its region's volume as
+ *  weight.
+ *
+ * Afterwards the caller order is the global array's order and the cell
+ * tables keep the order the pass left (the weights belong to that order until
+ * the next relocation rebuilds the tables).  rand() is the C library's,
+ * seeded with srand(rand_seed) and advanced rand_skip times first.
+ * counts = {added, deleted, rand() calls}. */
+typedef struct {
+  int64_t n, cap;
+  int64_t* v;
+} pc_list;
+
+static void pc_push(pc_list* l, int64_t x) {
+  if (l->n == l->cap) {
+    l->cap = l->cap ? 2 * l->cap : 8;
+    l->v = (int64_t*)realloc(l->v, sizeof(int64_t) * l->cap);
+  }
+  l->v[l->n++] = x;
+}
+
+typedef struct {
+  int dim;
+  int64_t n, cap;
+  double* coord;  /* 3 per particle */
+  int32_t* mat;
+  double* eta0;   /* optional per-particle values, NULL when absent */
+  double* pstr;
+  double* pinc;
+  double* xi;     /* 3 per particle: the integration point's local coordinate */
+  double* w;
+  int64_t* cell;
+} pc_swarm;
+
+static void pc_reserve(pc_swarm* s, int64_t need) {
+  if (need <= s->cap) return;
+  int64_t c = s->cap ? s->cap : 16;
+  while (c < need) c *= 2;
+  s->coord = (double*)realloc(s->coord, sizeof(double) * 3 * c);
+  s->mat = (int32_t*)realloc(s->mat, sizeof(int32_t) * c);
+  if (s->eta0) s->eta0 = (double*)realloc(s->eta0, sizeof(double) * c);
+  if (s->pstr) s->pstr = (double*)realloc(s->pstr, sizeof(double) * c);
+  if (s->pinc) s->pinc = (double*)realloc(s->pinc, sizeof(double) * c);
+  s->xi = (double*)realloc(s->xi, sizeof(double) * 3 * c);
+  s->w = (double*)realloc(s->w, sizeof(double) * c);
+  s->cell = (int64_t*)realloc(s->cell, sizeof(int64_t) * c);
+  s->cap = c;
+}
+
+static void pc_copy(pc_swarm* s, int64_t to, int64_t from) {
+  memcpy(s->coord + 3 * to, s->coord + 3 * from, sizeof(double) * 3);
+  s->mat[to] = s->mat[from];
+  if (s->eta0) s->eta0[to] = s->eta0[from];
+  if (s->pstr) {
+    s->pstr[to] = s->pstr[from];
+    s->pinc[to] = s->pinc[from];
+  }
+  memcpy(s->xi + 3 * to, s->xi + 3 * from, sizeof(double) * 3);
+  s->w[to] = s->w[from];
+  s->cell[to] = s->cell[from];
+}
+
+/* splitIntParticleByIndexWithinCell (PCDVC.cxx:547-610) */
+static void pc_split(const orc_problem* pb, pc_swarm* s, pc_list* T, int64_t e, int64_t slot, const double* xi) {
+  pc_reserve(s, s->n + 1);
+  const int64_t k = s->n++;
+  pc_copy(s, k, T[e].v[slot]);
+  int32_t ijk[3];
+  elem_ijk(pb, e, ijk);
+  for (int d = 0; d < 3; d++) {
+    double x = 0.0;
+    if (d < s->dim) {
+      const double a = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d]);
+      const double b = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d] + 1);
+      x = 0.5 * ((1.0 - xi[d]) * a + (1.0 + xi[d]) * b);
+    }
+    s->coord[3 * k + d] = x;
+    s->xi[3 * k + d] = d < s->dim ? xi[d] : 0.0;
+  }
+  pc_push(&T[e], k);
+}
+
+/* Removal of table slot `slot` of cell e and of its particle from the global
+ * array (Swarm_RemoveParticleFromCell, then the last particle takes the freed
+ * index: PCDVC.cxx:363-437). */
+static void pc_delete_slot(pc_swarm* s, pc_list* T, int64_t e, int64_t slot) {
+  const int64_t gi = T[e].v[slot];
+  T[e].v[slot] = T[e].v[T[e].n - 1];
+  T[e].n--;
+  const int64_t last = s->n - 1;
+  if (gi != last) {
+    pc_list* tl = &T[s->cell[last]];
+    for (int64_t q = 0; q < tl->n; q++)
+      if (tl->v[q] == last) {
+        tl->v[q] = gi;
+        break;
+      }
+    pc_copy(s, gi, last);
+  }
+  s->n--;
+}
+
+static double pc_draw(int64_t* ndraw) {
+  (*ndraw)++;
+  return rand() / (RAND_MAX + 1.0);
+}
+
+/* the cell's DVC on its table order; owner of every sub-cell into own */
+static void pc_dvc(const pc_swarm* s, const pc_list* t, const int32_t* res, double* xs, double* cs, double* ws,
+                   int32_t* own) {
+  const int dim = s->dim;
+  for (int64_t q = 0; q < t->n; q++)
+    for (int d = 0; d < dim; d++) xs[q * dim + d] = s->xi[3 * t->v[q] + d];
+  dvc_cell_impl(dim, res, (int32_t)t->n, xs, cs, ws, own);
+}
+
 int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* pp, int64_t* counts) {
   const int dim = pb->dim;
+  const int32_t res[3] = {pp->res[0], pp->res[1], dim == 3 ? pp->res[2] : 1};
+  for (int d = 0; d < 3; d++)
+    if (res[d] < 1) return ORC_E_INVALID;
+  if (pp->max_splits < 0 || pp->max_deletions < 0) return ORC_E_INVALID;
+  const int inflow = pp->inflow != 0;
+  const int64_t ppc = pp->particles_per_cell;
+  if (inflow && ppc < 0) return ORC_E_INVALID;
+  double upT = pp->upper_t, lowT = pp->lower_t;
+  if (upT < lowT) { /* PCDVC.cxx:265-270 */
+    lowT = 0.6;
+    upT = 25.0;
+  }
   const double V = dim == 2 ? 4.0 : 8.0;
-  const double maxW = pp->upper_t * V / 100.0, minW = pp->lower_t * V / 100.0;
-  int32_t res[3] = {pp->res[0], pp->res[1], dim == 3 ? pp->res[2] : 1};
-  int64_t maxc = 0;
+  const double maxW = upT * V / 100.0, minW = lowT * V / 100.0;
+  const int nc = res[0] * res[1] * res[2];
+  /* sub-cell centres and volume, as _DVCWeights_ConstructGrid(2D) builds them */
+  double h[3], *ctr[3];
+  for (int d = 0; d < 3; d++) {
+    h[d] = d < dim ? 2.0 / res[d] : 0.0;
+    ctr[d] = (double*)malloc(sizeof(double) * res[d]);
+    double x = -1.0 - h[d] / 2.0;
+    for (int i = 0; i < res[d]; i++) {
+      x = x + h[d];
+      ctr[d][i] = x;
+    }
+  }
+  const double da = dim == 2 ? h[0] * h[1] : h[0] * h[1] * h[2];
+  const int one_oda = (int)(1.0 / da + 0.5);
+  if (inflow) {
+    /* a list length above the owned count would read entries the reference
+     * never wrote: such sub-grids are refused */
+    for (int k = 1; k <= nc; k++)
+      if ((int)(k * da * one_oda + 0.5) > k) {
+        for (int d = 0; d < 3; d++) free(ctr[d]);
+        return ORC_E_INVALID;
+      }
+  }
+  /* the swarm and the cell tables as the mapper leaves them */
+  pc_swarm s;
+  memset(&s, 0, sizeof s);
+  s.dim = dim;
+  if (pb->eta0p) s.eta0 = (double*)malloc(sizeof(double));
+  if (pb->pstr) {
+    s.pstr = (double*)malloc(sizeof(double));
+    s.pinc = (double*)malloc(sizeof(double));
+  }
+  pc_reserve(&s, pb->P > 0 ? pb->P : 1);
+  s.n = pb->P;
+  pc_list* T = (pc_list*)calloc((size_t)(pb->E > 0 ? pb->E : 1), sizeof(pc_list));
   for (int64_t e = 0; e < pb->E; e++)
-    if (pb->cell_off[e + 1] - pb->cell_off[e] > maxc) maxc = pb->cell_off[e + 1] - pb->cell_off[e];
-  uint8_t* del = (uint8_t*)calloc(pb->P > 0 ? pb->P : 1, 1);
-  int64_t cap = 64, nnew = 0;
-  double* ncoord = (double*)malloc(sizeof(double) * 3 * cap);
-  int64_t* nparent = (int64_t*)malloc(sizeof(int64_t) * cap);
-  double* xs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
-  double* cs = (double*)malloc(sizeof(double) * dim * (maxc > 0 ? maxc : 1));
-  double* ws = (double*)malloc(sizeof(double) * (maxc > 0 ? maxc : 1));
-  int64_t ndel = 0;
-  for (int64_t e = 0; e < pb->E; e++) {
-    const int64_t b = pb->cell_off[e], n = pb->cell_off[e + 1] - b;
-    if (n == 0) continue;
-    for (int64_t q = 0; q < n; q++) global_to_local(pb, e, pb->coord + 3 * pb->cell_list[b + q], xs + dim * q);
-    orc_dvc_cell(dim, res, (int32_t)n, xs, cs, ws);
-    int interface = 0;
-    for (int64_t q = 1; q < n; q++)
-      if (pb->material[pb->cell_list[b + q]] != pb->material[pb->cell_list[b]]) interface = 1;
-    int64_t mS = pp->max_splits, mD = pp->max_deletions;
-    if (interface) {
-      if (!pp->delete_in_interface_cells) mD = 0;
-      if (!pp->split_in_interface_cells) mS = 0;
+    for (int64_t q = pb->cell_off[e]; q < pb->cell_off[e + 1]; q++) {
+      const int64_t p = pb->cell_list[q];
+      memcpy(s.coord + 3 * p, pb->coord + 3 * p, sizeof(double) * 3);
+      s.mat[p] = pb->material[p];
+      if (s.eta0) s.eta0[p] = pb->eta0p[p];
+      if (s.pstr) {
+        s.pstr[p] = pb->pstr[p];
+        s.pinc[p] = pb->pinc[p];
+      }
+      double x[3] = {0.0, 0.0, 0.0};
+      global_to_local(pb, e, pb->coord + 3 * p, x);
+      memcpy(s.xi + 3 * p, x, sizeof x);
+      s.w[p] = 0.0;
+      s.cell[p] = e;
+      pc_push(&T[e], p);
     }
-    if (mD > n - 4) mD = n / 2;
-    int64_t ns = 0, nd = 0;
-    int32_t ijk[3] = {(int32_t)(e % pb->res[0]), (int32_t)((e / pb->res[0]) % pb->res[1]),
-                      dim == 3 ? (int32_t)(e / ((int64_t)pb->res[0] * pb->res[1])) : 0};
-    for (int64_t q = 0; q < n; q++) {
-      const int64_t p = pb->cell_list[b + q];
-      if (ws[q] > maxW && ns < mS) {
-        if (nnew == cap) {
-          cap *= 2;
-          ncoord = (double*)realloc(ncoord, sizeof(double) * 3 * cap);
-          nparent = (int64_t*)realloc(nparent, sizeof(int64_t) * cap);
+  for (int64_t e = 0; e < pb->E; e++) /* index order, as the mapper adds them */
+    for (int64_t a = 1; a < T[e].n; a++)
+      for (int64_t b = a; b > 0 && T[e].v[b - 1] > T[e].v[b]; b--) {
+        const int64_t t = T[e].v[b];
+        T[e].v[b] = T[e].v[b - 1];
+        T[e].v[b - 1] = t;
+      }
+  srand((unsigned)pp->rand_seed);
+  for (int64_t i = 0; i < pp->rand_skip; i++) (void)rand();
+  int64_t ndraw = 0, added = 0, deleted = 0;
+  int32_t st = ORC_OK;
+  int64_t cap = 16;
+  double* xs = (double*)malloc(sizeof(double) * dim * cap);
+  double* cs = (double*)malloc(sizeof(double) * dim * cap);
+  double* ws = (double*)malloc(sizeof(double) * cap);
+  int32_t* own = (int32_t*)malloc(sizeof(int32_t) * nc);
+  int32_t* lcount = (int32_t*)malloc(sizeof(int32_t) * cap);
+  int32_t* lsize = (int32_t*)malloc(sizeof(int32_t) * cap);
+  int32_t* lists = (int32_t*)malloc(sizeof(int32_t) * (size_t)cap * nc);
+  int64_t* dcand = (int64_t*)malloc(sizeof(int64_t) * cap);
+#define PC_FIT(nn)                                                              \
+  do {                                                                          \
+    if ((nn) > cap) {                                                           \
+      while (cap < (nn)) cap *= 2;                                              \
+      xs = (double*)realloc(xs, sizeof(double) * dim * cap);                    \
+      cs = (double*)realloc(cs, sizeof(double) * dim * cap);                    \
+      ws = (double*)realloc(ws, sizeof(double) * cap);                          \
+      lcount = (int32_t*)realloc(lcount, sizeof(int32_t) * cap);                \
+      lsize = (int32_t*)realloc(lsize, sizeof(int32_t) * cap);                  \
+      lists = (int32_t*)realloc(lists, sizeof(int32_t) * (size_t)cap * nc);     \
+      dcand = (int64_t*)realloc(dcand, sizeof(int64_t) * cap);                  \
+    }                                                                           \
+  } while (0)
+#define PC_CENTRE(j, out)                                     \
+  do {                                                        \
+    const int ci_ = (j) % res[0], cj_ = ((j) / res[0]) % res[1]; \
+    const int ck_ = (j) / (res[0] * res[1]);                  \
+    (out)[0] = ctr[0][ci_];                                   \
+    (out)[1] = ctr[1][cj_];                                   \
+    (out)[2] = dim == 3 ? ctr[2][ck_] : 0.0;                  \
+  } while (0)
+  for (int64_t e = 0; e < pb->E && st == ORC_OK; e++) {
+    pc_list* t = &T[e];
+    int64_t nump = t->n;
+    if (nump == 0) continue;
+    PC_FIT(nump + ppc + pp->max_splits + 1);
+    pc_dvc(&s, t, res, xs, cs, ws, own);
+    if (inflow) {
+      const int64_t nump_orig = nump;
+      double dist = 0.0;
+      {
+        const int64_t i = nump_orig - 1;
+        for (int d = 0; d < dim; d++) dist += (cs[i * dim + d] - xs[i * dim + d]) * (cs[i * dim + d] - xs[i * dim + d]);
+      }
+      const int flag = dist > pp->cent_pos_ratio * 2.0;
+      if ((1.0 * nump_orig) / ppc < pp->threshold || flag) {
+        int64_t changes = 0;
+        for (int64_t i = 0; i < ppc; i++) {
+          const int j = (int)(nc * pc_draw(&ndraw));
+          double x[3];
+          PC_CENTRE(j, x);
+          pc_split(pb, &s, T, e, own[j], x);
+          nump++;
+          changes++;
+          added++;
         }
-        for (int d = 0; d < 3; d++) {
-          double x = 0.0;
-          if (d < dim) {
-            const double a = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d]);
-            const double bb = orc_vertex_coord(pb->res[d], pb->lo[d], pb->hi[d], ijk[d] + 1);
-            const double xi = cs[q * dim + d];
-            x = 0.5 * ((1.0 - xi) * a + (1.0 + xi) * bb);
+        const int64_t del_num = nump - ppc;
+        if (del_num > 5)
+          for (int64_t i = 0; i < del_num; i++) {
+            const int64_t j = (int64_t)(nump * pc_draw(&ndraw));
+            pc_delete_slot(&s, T, e, j);
+            nump--;
+            changes++;
+            deleted++;
           }
-          ncoord[nnew * 3 + d] = x;
+        if (changes) {
+          if (nump < 3) {
+            st = ORC_E_INVALID; /* the reference's under-resolved-cell firewall */
+            break;
+          }
+          pc_dvc(&s, t, res, xs, cs, ws, own);
         }
-        nparent[nnew++] = p;
-        ns++;
+      }
+      /* owned sub-cell lists, filled from the end in sub-cell order */
+      for (int64_t i = 0; i < nump; i++) {
+        lcount[i] = (int32_t)(ws[i] * one_oda + 0.5);
+        lsize[i] = lcount[i];
+      }
+      for (int c = 0; c < nc; c++) {
+        const int o = own[c];
+        if (lcount[o] > 0) {
+          lists[(size_t)o * nc + lcount[o] - 1] = c;
+          lcount[o]--;
+        }
       }
     }
-    /* deletions: the candidates below minW sorted by swarm index, highest
-     * first (compare_indexOnCPU, PCDVC.cxx:611-620, qsort :974), and the first
-     * maxDeletions of that list deleted (:1015-1021) */
-    int64_t ncand = 0;
-    for (int64_t q = 0; q < n; q++) ncand += ws[q] < minW;
-    const int64_t nD = mD < ncand ? mD : ncand;
-    for (int64_t q = 0; q < n && nD > 0; q++) {
-      if (!(ws[q] < minW)) continue;
-      const int64_t p = pb->cell_list[b + q];
-      int64_t higher = 0;
-      for (int64_t t = 0; t < n; t++) higher += ws[t] < minW && pb->cell_list[b + t] > p;
-      if (higher < nD) {
-        del[p] = 1;
-        nd++;
+    /* the normal pass */
+    int64_t mS = pp->max_splits, mD = pp->max_deletions;
+    const int32_t m0 = s.mat[t->v[0]];
+    for (int64_t i = 0; i < nump; i++)
+      if (s.mat[t->v[i]] != m0) {
+        if (!pp->delete_in_interface_cells) mD = 0;
+        if (!pp->split_in_interface_cells) mS = 0;
+        break;
       }
+    int64_t nsplit = 0, ndc = 0;
+    int64_t* splits = (int64_t*)malloc(sizeof(int64_t) * (nump > 0 ? nump : 1));
+    for (int64_t i = 0; i < nump; i++) {
+      if (ws[i] > maxW) splits[nsplit++] = i;
+      if (ws[i] < minW) dcand[ndc++] = t->v[i];
     }
-    ndel += nd;
-  }
-  const int64_t P2 = pb->P - ndel + nnew;
-  double* coord2 = (double*)malloc(sizeof(double) * 3 * (P2 > 0 ? P2 : 1));
-  int32_t* mat2 = (int32_t*)malloc(sizeof(int32_t) * (P2 > 0 ? P2 : 1));
-  double* e02 = pb->eta0p ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
-  double* w2 = pb->pstr ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
-  double* i2 = pb->pstr ? (double*)malloc(sizeof(double) * (P2 > 0 ? P2 : 1)) : NULL;
-  int64_t k = 0;
-  for (int64_t p = 0; p < pb->P; p++) {
-    if (del[p]) continue;
-    memcpy(coord2 + 3 * k, pb->coord + 3 * p, sizeof(double) * 3);
-    mat2[k] = pb->material[p];
-    if (e02) e02[k] = pb->eta0p[p];
-    if (w2) {
-      w2[k] = pb->pstr[p];
-      i2[k] = pb->pinc[p];
+    for (int64_t a = 1; a < ndc; a++) /* indexOnCPU, highest first */
+      for (int64_t b = a; b > 0 && dcand[b - 1] < dcand[b]; b--) {
+        const int64_t x = dcand[b];
+        dcand[b] = dcand[b - 1];
+        dcand[b - 1] = x;
+      }
+    if (mD > nump - 4) mD = (dim == 2 && inflow) ? nump - 4 : nump / 2;
+    const int64_t cs_n = mS < nsplit ? mS : nsplit;
+    for (int64_t i = 0; i < cs_n; i++) {
+      const int64_t q = splits[i];
+      double x[3] = {0.0, 0.0, 0.0};
+      if (!inflow) {
+        for (int d = 0; d < dim; d++) x[d] = cs[q * dim + d];
+      } else {
+        const int j = (int)(lsize[q] * pc_draw(&ndraw));
+        PC_CENTRE(lists[(size_t)q * nc + j], x);
+      }
+      pc_split(pb, &s, T, e, q, x);
+      nump++;
+      added++;
     }
-    k++;
-  }
-  for (int64_t j = 0; j < nnew; j++, k++) {
-    memcpy(coord2 + 3 * k, ncoord + 3 * j, sizeof(double) * 3);
-    mat2[k] = pb->material[nparent[j]];
-    if (e02) e02[k] = pb->eta0p[nparent[j]];
-    if (w2) {   /* a split copies its parent, extensions included */
-      w2[k] = pb->pstr[nparent[j]];
-      i2[k] = pb->pinc[nparent[j]];
+    free(splits);
+    const int64_t cd_n = mD < ndc ? mD : ndc;
+    for (int64_t i = 0; i < cd_n; i++) {
+      int64_t slot = 0;
+      while (slot < t->n && t->v[slot] != dcand[i]) slot++;
+      if (slot == t->n) { /* cannot happen: candidates are deleted highest index first */
+        st = ORC_E_INVALID;
+        break;
+      }
+      pc_delete_slot(&s, T, e, slot);
+      nump--;
+      deleted++;
+    }
+    if (st != ORC_OK) break;
+    if (cs_n > 0 || cd_n > 0) {
+      if (nump < 3) {
+        st = ORC_E_INVALID;
+        break;
+      }
+      pc_dvc(&s, t, res, xs, cs, ws, own);
+    }
+    for (int64_t i = 0; i < nump; i++) {
+      const int64_t p = t->v[i];
+      for (int d = 0; d < dim; d++) s.xi[3 * p + d] = cs[i * dim + d];
+      s.w[p] = ws[i];
     }
   }
+#undef PC_FIT
+#undef PC_CENTRE
+  free(xs); free(cs); free(ws); free(own); free(lcount); free(lsize); free(lists); free(dcand);
+  for (int d = 0; d < 3; d++) free(ctr[d]);
+  if (st != ORC_OK) {
+    for (int64_t e = 0; e < pb->E; e++) free(T[e].v);
+    free(T);
+    free(s.coord); free(s.mat); free(s.eta0); free(s.pstr); free(s.pinc); free(s.xi); free(s.w); free(s.cell);
+    return st;
+  }
+  /* the problem takes the swarm: caller order = global order, cell lists in
+   * table order, integration points and weights from the pass */
+  const int64_t P2 = s.n;
+  const int64_t PA = P2 > 0 ? P2 : 1;
   free(pb->coord); free(pb->material); free(pb->eta0p); free(pb->pstr); free(pb->pinc);
-  pb->coord = coord2;
-  pb->material = mat2;
-  pb->eta0p = e02;
-  pb->pstr = w2;
-  pb->pinc = i2;
-  pb->P = P2;
-  pb->owner = (int32_t*)realloc(pb->owner, sizeof(int32_t) * (P2 > 0 ? P2 : 1));
-  pb->xi = (double*)realloc(pb->xi, sizeof(double) * dim * (P2 > 0 ? P2 : 1));
-  pb->weight = (double*)realloc(pb->weight, sizeof(double) * (P2 > 0 ? P2 : 1));
-  pb->cell_list = (int64_t*)realloc(pb->cell_list, sizeof(int64_t) * (P2 > 0 ? P2 : 1));
-  free(pb->eta); free(pb->yielded);
-  pb->eta = (double*)calloc(P2 > 0 ? P2 : 1, sizeof(double));
-  pb->yielded = (uint8_t*)calloc(P2 > 0 ? P2 : 1, 1);
-  pb->has_visc = 0;
-  free(del); free(ncoord); free(nparent); free(xs); free(cs); free(ws);
-  if (counts) {
-    counts[0] = nnew;
-    counts[1] = ndel;
+  pb->coord = (double*)malloc(sizeof(double) * 3 * PA);
+  memcpy(pb->coord, s.coord, sizeof(double) * 3 * P2);
+  pb->material = (int32_t*)malloc(sizeof(int32_t) * PA);
+  memcpy(pb->material, s.mat, sizeof(int32_t) * P2);
+  pb->eta0p = NULL;
+  pb->pstr = NULL;
+  pb->pinc = NULL;
+  if (s.eta0) {
+    pb->eta0p = (double*)malloc(sizeof(double) * PA);
+    memcpy(pb->eta0p, s.eta0, sizeof(double) * P2);
   }
+  if (s.pstr) {
+    pb->pstr = (double*)malloc(sizeof(double) * PA);
+    pb->pinc = (double*)malloc(sizeof(double) * PA);
+    memcpy(pb->pstr, s.pstr, sizeof(double) * P2);
+    memcpy(pb->pinc, s.pinc, sizeof(double) * P2);
+  }
+  pb->P = P2;
+  pb->owner = (int32_t*)realloc(pb->owner, sizeof(int32_t) * PA);
+  pb->xi = (double*)realloc(pb->xi, sizeof(double) * dim * PA);
+  pb->weight = (double*)realloc(pb->weight, sizeof(double) * PA);
+  pb->cell_list = (int64_t*)realloc(pb->cell_list, sizeof(int64_t) * PA);
+  pb->cell_off[0] = 0;
+  for (int64_t e = 0; e < pb->E; e++) {
+    pb->cell_off[e + 1] = pb->cell_off[e] + T[e].n;
+    for (int64_t q = 0; q < T[e].n; q++) pb->cell_list[pb->cell_off[e] + q] = T[e].v[q];
+  }
+  for (int64_t p = 0; p < P2; p++) {
+    pb->owner[p] = (int32_t)s.cell[p];
+    for (int d = 0; d < dim; d++) pb->xi[dim * p + d] = s.xi[3 * p + d];
+    pb->weight[p] = s.w[p];
+  }
+  for (int64_t e = 0; e < pb->E; e++) free(T[e].v);
+  free(T);
+  free(s.coord); free(s.mat); free(s.eta0); free(s.pstr); free(s.pinc); free(s.xi); free(s.w); free(s.cell);
+  free(pb->eta); free(pb->yielded);
+  pb->eta = (double*)calloc(PA, sizeof(double));
+  pb->yielded = (uint8_t*)calloc(PA, 1);
+  pb->has_visc = 0;
+  pb->assembled = 0;
   pb->wkind = 1;
   for (int d = 0; d < 3; d++) pb->wres[d] = res[d];
-  return relocate_all(pb);
+  if (counts) {
+    counts[0] = added;
+    counts[1] = deleted;
+    counts[2] = ndraw;
+  }
+  return ORC_OK;
 }
 
 int32_t orc_set_strain_weakening(orc_problem* pb, int32_t material, const orc_strain_weakening* sw) {
@@ -932,6 +1250,11 @@ void orc_problem_free(orc_problem* pb) {
   free(pb->eta0p); free(pb->pstr); free(pb->pinc); free(pb->sw); free(pb->sw_on); free(pb->owner); free(pb->xi); free(pb->weight); free(pb->cell_off);
   free(pb->cell_list); free(pb->eta); free(pb->yielded);
   free(pb);
+}
+
+void orc_get_cell_tables(const orc_problem* pb, int64_t* cell_off, int64_t* cell_list) {
+  memcpy(cell_off, pb->cell_off, sizeof(int64_t) * (pb->E + 1));
+  memcpy(cell_list, pb->cell_list, sizeof(int64_t) * pb->P);
 }
 
 void orc_get_particles(const orc_problem* pb, int32_t* owner, double* xi, double* weight, double* coord) {

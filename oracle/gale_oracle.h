@@ -176,6 +176,9 @@ int32_t orc_particle_fields(orc_problem* pb, const double* u_full, double* sr, d
 /* One FE cell: sub-grid res (x, y[, z]) over [-1,1]^dim; xi: nump x dim local
  * coordinates in cell-table order; cent (nump x dim) and w (nump) out. */
 int32_t orc_dvc_cell(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w);
+/* the same, with the owning particle of every sub-cell (sub-cell k = i + r0 (j + r1 l)) */
+int32_t orc_dvc_cell_owners(int32_t dim, const int32_t* res, int32_t nump, const double* xi, double* cent, double* w,
+                            int32_t* owner);
 
 /* --- problem --- */
 orc_problem* orc_problem_new(const orc_problem_desc* d);
@@ -188,14 +191,22 @@ typedef struct orc_pcdvc_params {
   double upper_t, lower_t;            /* "upperT", "lowerT" (percent of the cell volume) */
   int32_t max_splits, max_deletions;  /* "maxSplits", "maxDeletions" */
   int32_t split_in_interface_cells, delete_in_interface_cells;
-  int32_t res[3], pad_;
+  int32_t res[3];
+  int32_t inflow;                    /* "Inflow" */
+  int32_t particles_per_cell;        /* the root "particlesPerCell" */
+  int32_t pad_;
+  double threshold, cent_pos_ratio;  /* "Threshold", "CentPosRatio" */
+  int64_t rand_seed, rand_skip;      /* srand(rand_seed), then rand_skip draws discarded */
 } orc_pcdvc_params;
+/* counts (may be NULL) = {added, deleted, rand() calls} */
 int32_t orc_population_control(orc_problem* pb, const orc_pcdvc_params* p, int64_t* counts);
 void orc_problem_free(orc_problem* pb);
 int64_t orc_num_nodes(const orc_problem* pb);
 int64_t orc_num_elements(const orc_problem* pb);
 int64_t orc_num_particles(const orc_problem* pb);
 void orc_get_particles(const orc_problem* pb, int32_t* owner, double* xi, double* weight, double* coord);
+/* cell tables: cell_off[E+1], cell_list[P] (particle indices in each cell's table order) */
+void orc_get_cell_tables(const orc_problem* pb, int64_t* cell_off, int64_t* cell_list);
 int32_t orc_set_coords(orc_problem* pb, const double* coord);  /* relocate + remap after moving particles */
 
 /* per-particle effective viscosity (ConstitutiveMatrix_Assemble chain) */

@@ -194,6 +194,128 @@ def expected_population_control(coord, mat, maxS=3, maxD=2, upT=25.0, lowT=0.6):
     return keep, np.array(new_xy).reshape(-1, 2), np.array(new_mat, np.int32)
 
 
+class RefSwarm:
+    """The reference's swarm bookkeeping on the 2 x 2 unit-cell mesh, for
+    tiny cases: a global particle list (indexOnCPU) and one table per cell
+    built in index order (CoincidentMapper.cxx:207-240); a split appends to
+    both (PCDVC.cxx:547-610); a deletion moves the cell's last table entry
+    into the freed slot (SwarmClass.cxx:976-990) and the global last particle
+    into the freed index, rewriting its table entry (PCDVC.cxx:363-437).
+    Cells are processed in order (WeightsCalculator.cxx:195-197) as
+    PCDVC.cxx:1092-1512 does in 2D, Inflow included; rand() is the C
+    library's own; each cell's DVC comes from the oracle's growth (pinned by
+    the fixtures above)."""
+
+    def __init__(self, coord, mat, res=(10, 10)):
+        self.res = res
+        self.G = []
+        for p in range(len(coord)):
+            e = int(coord[p, 0] >= 1.0) + 2 * int(coord[p, 1] >= 1.0)
+            o = np.array([e % 2, e // 2], float)
+            self.G.append(dict(x=coord[p, :2].copy(), m=int(mat[p]), e=e, xi=2.0 * (coord[p, :2] - o) - 1.0))
+        self.T = [[p for p in range(len(coord)) if self.G[p]["e"] == e] for e in range(4)]
+
+    def split(self, e, slot, xi):
+        par = self.G[self.T[e][slot]]
+        o = np.array([e % 2, e // 2], float)
+        xi = np.array(xi, float)
+        self.G.append(dict(x=0.5 * ((1.0 - xi) * o + (1.0 + xi) * (o + 1.0)), m=par["m"], e=e, xi=xi))
+        self.T[e].append(len(self.G) - 1)
+
+    def delete_slot(self, e, slot):
+        gi = self.T[e][slot]
+        self.T[e][slot] = self.T[e][-1]
+        self.T[e].pop()
+        last = len(self.G) - 1
+        if gi != last:
+            tl = self.T[self.G[last]["e"]]
+            tl[tl.index(last)] = gi
+            self.G[gi] = self.G[last]
+        self.G.pop()
+
+    def centre(self, j):
+        out = []
+        for d, k in enumerate((j % self.res[0], j // self.res[0])):
+            h = 2.0 / self.res[d]
+            x = -1.0 - h / 2.0
+            for _ in range(k + 1):
+                x = x + h
+            out.append(x)
+        return np.array(out)
+
+    def dvc(self, e):
+        import oracle as O
+        xi = np.array([self.G[p]["xi"] for p in self.T[e]]).reshape(-1, 2)
+        return O.dvc_cell_owners(2, self.res, xi)
+
+    def run(self, maxS=3, maxD=2, upT=25.0, lowT=0.6, inflow=False, ppc=25, thresh=0.8, cpr=0.01, seed=1, skip=0):
+        import ctypes
+        libc = ctypes.CDLL(None)
+        libc.srand(ctypes.c_uint(seed))
+        for _ in range(skip):
+            libc.rand()
+        draws = [0]
+
+        def r():
+            draws[0] += 1
+            return libc.rand() / 2147483648.0
+        nc = self.res[0] * self.res[1]
+        da = (2.0 / self.res[0]) * (2.0 / self.res[1])
+        one_oda = int(1.0 / da + 0.5)
+        self.triggered = []
+        for e in range(4):
+            nump = len(self.T[e])
+            cent, w, own = self.dvc(e)
+            if inflow:
+                x_last = self.G[self.T[e][-1]]["xi"]
+                dist = float(((x_last - cent[-1]) ** 2).sum())
+                if nump / ppc < thresh or dist > cpr * 2.0:
+                    self.triggered.append(e)
+                    n0 = nump
+                    for _ in range(ppc):
+                        j = int(nc * r())
+                        self.split(e, int(own[j]), self.centre(j))
+                    nump += ppc
+                    if n0 > 5:
+                        for _ in range(n0):
+                            self.delete_slot(e, int(nump * r()))
+                            nump -= 1
+                    cent, w, own = self.dvc(e)
+                lists = {}
+                for i in range(nump):
+                    k = int(w[i] * one_oda + 0.5)
+                    lists[i] = [c for c in range(nc) if own[c] == i][:k][::-1]
+            mS, mD = maxS, maxD
+            if len({self.G[p]["m"] for p in self.T[e]}) > 1:
+                mS, mD = 0, 0
+            splits = [q for q in range(nump) if w[q] > upT * 4 / 100]
+            cand = sorted([self.T[e][q] for q in range(nump) if w[q] < lowT * 4 / 100], reverse=True)
+            if mD > nump - 4:
+                mD = nump - 4 if inflow else nump // 2
+            for q in splits[:mS]:
+                if inflow:
+                    self.split(e, q, self.centre(lists[q][int(len(lists[q]) * r())]))
+                else:
+                    self.split(e, q, cent[q])
+            for gi in cand[:mD]:
+                self.delete_slot(e, self.T[e].index(gi))
+            cent, w, own = self.dvc(e)
+            for i, p in enumerate(self.T[e]):
+                self.G[p]["cent"], self.G[p]["w"] = cent[i], w[i]
+        return draws[0]
+
+
+def _check_against_model(o, ref):
+    owner, xi, wt, coord2 = o.particles()
+    assert o.P == len(ref.G)
+    assert [list(t) for t in o.cell_tables()] == ref.T
+    assert np.array_equal(coord2[:, :2], np.array([g["x"] for g in ref.G]))
+    assert np.array_equal(wt, np.array([g["w"] for g in ref.G]))
+    assert np.array_equal(xi, np.array([g["cent"] for g in ref.G]))
+    for e in range(4):  # every cell's weights sum to its volume
+        assert abs(wt[owner == e].sum() - 4.0) < 1e-12
+
+
 def test_oracle_population_control_designed_cell(oracle):
     rng = np.random.default_rng(2)
     coord = cells_layout(rng, True)
@@ -211,12 +333,98 @@ def test_oracle_population_control_designed_cell(oracle):
     owner, xi, wt, coord2 = o.particles()
     assert (added, deleted) == (len(new_xy), int((~keep).sum()))
     assert o.P == len(coord) + added - deleted
-    ns = int(keep.sum())
-    assert np.array_equal(coord2[:ns], coord[keep])
-    assert np.array_equal(coord2[ns:, :2], new_xy)
-    # weights are the DVC of the new particle sets: every cell sums to 4
-    for e in range(4):
-        assert abs(wt[owner == e].sum() - 4.0) < 1e-12
+    got = {tuple(c[:2]) for c in coord2}
+    assert got == {tuple(c[:2]) for c in coord[keep]} | {tuple(c) for c in new_xy}
+    # By hand.  Cells hold particles 16e..16e+15; cell 0 deletes a = 11 and
+    # b = 12 and splits 15, cell 2 deletes its slot 11 (particle 43), cell 3
+    # its slot 6 (particle 54), cell 1 nothing.
+    a, b = cand[-2], cand[-1]
+    assert (a, b) == (11, 12)
+    for e, want in ((1, []), (2, [11]), (3, [6])):
+        _, we = oracle.dvc_cell(2, (10, 10), 2.0 * (coord[16 * e:16 * e + 16, :2] - [e % 2, e // 2]) - 1.0)
+        assert [i for i in range(16) if we[i] < 0.024] == want
+    # Cell 0: the copy of 15 becomes index 64 in table slot 16.  Deleting b:
+    # slot 12 takes the table's last entry (64) and index 12 the global last
+    # (64 itself), so slot 12 names 12, now the copy.  Deleting a: slot 11
+    # takes the table's last entry (15), index 11 the global last (63, slot 15
+    # of cell 3).  Cell 2 deleting 43: slot 11 takes 47, index 43 takes the
+    # global last 62 (slot 14 of cell 3).  Cell 3, table [48..61, 43, 11],
+    # deleting slot 6 (54): slot 6 takes 11, index 54 takes the global last
+    # 61 (slot 13).
+    T = o.cell_tables()
+    assert list(T[0]) == list(range(11)) + [15, 12, 13, 14]
+    assert list(T[1]) == list(range(16, 32))
+    assert list(T[2]) == list(range(32, 43)) + [47, 44, 45, 46]
+    assert list(T[3]) == [48, 49, 50, 51, 52, 53, 11, 55, 56, 57, 58, 59, 60, 54, 43]
+    np.testing.assert_array_equal(coord2[12, :2], new_xy[0])
+    np.testing.assert_array_equal(coord2[11], coord[63])
+    np.testing.assert_array_equal(coord2[43], coord[62])
+    np.testing.assert_array_equal(coord2[54], coord[61])
+    assert o.P == 61
+    ref = RefSwarm(coord, mat)
+    assert ref.run() == 0
+    _check_against_model(o, ref)
+
+
+def test_oracle_population_control_matches_swarm_model(oracle):
+    """Uneven random layouts (some sparse cells split, some crowded cells
+    lose particles), two passes with the moved particles relocated between
+    them: the oracle equals the swarm model."""
+    for seed in (3, 4, 5):
+        rng = np.random.default_rng(seed)
+        pts, mats = [], []
+        for e, n in enumerate((6, 30, 12, 45)):
+            o_ = np.array([e % 2, e // 2], float)
+            c = rng.uniform(0.0, 1.0, (n, 2))
+            if n > 20:
+                c[: n // 2] = 0.5 + 0.08 * rng.uniform(-1.0, 1.0, (n // 2, 2))
+            pts.append(c + o_)
+        coord = np.c_[np.vstack(pts), np.zeros(93)]
+        coord = coord[rng.permutation(93)]  # cells interleaved in index order
+        mat = np.zeros(93, np.int32)
+        o = oracle.Problem(**two_by_two_kwargs(coord, mat))
+        ref = RefSwarm(coord, mat)
+        o.population_control(upperT=10.0, lowerT=1.5, maxSplits=4, maxDeletions=5)
+        ref.run(maxS=4, maxD=5, upT=10.0, lowT=1.5)
+        _check_against_model(o, ref)
+
+
+def test_oracle_inflow_matches_swarm_model(oracle):
+    """The Inflow branch (PCDVC.cxx:1204-1460) on 2D layouts with a sparse
+    cell (below Threshold x particlesPerCell) and crowded ones (centroid far
+    from the last particle): random re-population, random deletions at table
+    slots, splits at random owned sub-cells; the oracle equals the swarm model
+    for several rand() seeds, the rand() calls included."""
+    for seed in (1, 7, 12345):
+        rng = np.random.default_rng(seed)
+        pts = []
+        for e, n in enumerate((4, 14, 9, 20)):
+            o_ = np.array([e % 2, e // 2], float)
+            c = rng.uniform(0.0, 1.0, (n, 2))
+            if e == 3:
+                c[:10] = [0.2, 0.2] + 0.1 * rng.uniform(0.0, 1.0, (10, 2))
+            pts.append(c + o_)
+        coord = np.c_[np.vstack(pts), np.zeros(47)]
+        mat = np.zeros(47, np.int32)
+        o = oracle.Problem(**two_by_two_kwargs(coord, mat))
+        ref = RefSwarm(coord, mat)
+        kw = dict(inflow=True, ppc=12, thresh=0.8, cpr=0.01, seed=seed, maxS=3, maxD=2)
+        draws = ref.run(**kw)
+        o.population_control(Inflow=True, particlesPerCell=12, Threshold=0.8, CentPosRatio=0.01, rand_seed=seed)
+        assert o.rand_draws == draws
+        assert 0 in ref.triggered  # 4 particles < 0.8 x 12
+        _check_against_model(o, ref)
+        # the cell that was below the threshold holds its original 4 (too few
+        # to delete, 4 <= 5) plus 12 new particles at sub-cell centres of the
+        # cell, before its own splits and deletions
+        owner, _, _, coord2 = o.particles()
+        assert np.sum(owner == 0) >= 16 - 2
+        # rand_skip: the pass on the stream advanced by 5 calls
+        o2 = oracle.Problem(**two_by_two_kwargs(coord, mat))
+        o2.population_control(Inflow=True, particlesPerCell=12, rand_seed=seed, rand_skip=5)
+        ref2 = RefSwarm(coord, mat)
+        assert ref2.run(**kw, skip=5) == o2.rand_draws
+        _check_against_model(o2, ref2)
 
 
 def test_oracle_population_control_interface_cell_untouched(oracle):
