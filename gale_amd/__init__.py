@@ -634,14 +634,18 @@ class Problem:
         return st
 
     def population_control(self, params=None):
-        """PCDVC splits and deletions, then DVC weights; returns (added,
-        deleted).  The particle count (self.P) changes: caller order is the
-        survivors in their previous order, then the new particles."""
+        """One PCDVC pass (splits, deletions, Inflow re-population when on),
+        then DVC weights; returns (added, deleted) and sets self.rand_draws to
+        the rand() calls made.  The particle count (self.P) changes; the
+        caller order afterwards is the reference swarm's array order (a
+        deleted particle's index taken by the array's last particle, new
+        particles appended)."""
         p = params if params is not None else pcdvc_params()
-        counts = np.zeros(2, np.int64)
+        counts = np.zeros(3, np.int64)
         st = lib().gs_problem_population_control(self._h, C.byref(p), _ptr(counts))
-        _check(st, "population_control", allow=(E_ESCAPED,))
+        _check(st, "population_control")
         self.P = self.info()["P"]
+        self.rand_draws = int(counts[2])
         return int(counts[0]), int(counts[1])
 
     def set_weights(self, kind=WEIGHTS_DVC, resolution=10):
@@ -693,15 +697,23 @@ class Problem:
 class PcdvcParams(C.Structure):
     _fields_ = [("upper_t", C.c_double), ("lower_t", C.c_double), ("max_splits", C.c_int32),
                 ("max_deletions", C.c_int32), ("split_in_interface_cells", C.c_int32),
-                ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("pad_", C.c_int32)]
+                ("delete_in_interface_cells", C.c_int32), ("res", C.c_int32 * 3), ("inflow", C.c_int32),
+                ("particles_per_cell", C.c_int32), ("pad_", C.c_int32), ("threshold", C.c_double),
+                ("cent_pos_ratio", C.c_double), ("rand_seed", C.c_int64), ("rand_skip", C.c_int64)]
 
 
 def pcdvc_params(upperT=25.0, lowerT=0.6, maxSplits=3, maxDeletions=2, splitInInterfaceCells=False,
-                 deleteInInterfaceCells=False, resolution=10):
-    """PCDVC component parameters with the reference's XML names and defaults."""
+                 deleteInInterfaceCells=False, resolution=10, Inflow=False, particlesPerCell=25, Threshold=0.8,
+                 CentPosRatio=0.01, rand_seed=1, rand_skip=0):
+    """PCDVC component parameters with the reference's XML names and defaults
+    (particlesPerCell is the root dictionary's); rand_seed / rand_skip place
+    the Inflow branch on the C library's rand() stream: srand(rand_seed),
+    then rand_skip calls already made."""
     r = _dims(3, [resolution] * 3 if np.isscalar(resolution) else resolution, 10)
     return PcdvcParams(upperT, lowerT, maxSplits, maxDeletions, int(bool(splitInInterfaceCells)),
-                       int(bool(deleteInInterfaceCells)), (C.c_int32 * 3)(*r), 0)
+                       int(bool(deleteInInterfaceCells)), (C.c_int32 * 3)(*r), int(bool(Inflow)),
+                       int(particlesPerCell), 0, float(Threshold), float(CentPosRatio), int(rand_seed),
+                       int(rand_skip))
 
 
 def dvc_max_particles(dim, resolution=10) -> int:

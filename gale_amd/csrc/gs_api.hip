@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <string.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "../../include/gale_stokes.h"
+#include "gs_crand.h"
 #include "gs_internal.h"
 #include "gs_learned.h"
 #include "gs_mg.h"
@@ -102,6 +104,10 @@ struct gs_problem {
   double* wp = nullptr;             // DVC weight of each (sorted) particle
   unsigned long long gen = 0;       // bumped by every relocation (solvers re-run their setup)
   int device = 0;                   // HIP device current at creation (the problem's buffers live there)
+  // the C library's rand() stream of PCDVC's Inflow branch, positioned after
+  // rng_pos calls since srand(rng_seed) (rng_seed < 0: not started)
+  CRand rng;
+  long long rng_seed = -1, rng_pos = 0;
   void* scratch = nullptr;          // that device's address of the no-allocation scratch array
 };
 
@@ -599,131 +605,447 @@ extern "C" int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const in
   return relocate(pb);
 }
 
+// One PCDVC pass (PCDVC.cxx:656-1090 / 1092-1512).  The cells run on the
+// device, one workgroup each, up to the deletions whose order depends on the
+// swarm's global indices; the host replays the swarm's array operations in
+// cell order (appends, last-particle moves; an integer walk over the cells
+// that changed) to fix that order and the new caller order; the device then
+// finishes the changed cells and rebuilds the particle arrays.
+//
+// With Inflow the cells share one rand() stream in cell order, and a
+// re-populated cell's split draws depend on the values it drew, so the
+// offset of a cell is known only once every earlier re-populated cell is
+// done.  The re-populated cells run in windows, each window assuming its
+// cells make no split draws; the first cell that makes some is still right
+// (its offset was), the window's later cells are run again from the
+// corrected offsets.
 extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_params* pp, int64_t* counts) {
   if (!pb || !pp || pp->max_splits < 0 || pp->max_deletions < 0) return GS_E_INVALID;
+  if (pb->escaped) return GS_E_ESCAPED;  // every particle must sit in a cell table
+  const int inflow = pp->inflow != 0;
+  if (inflow && (pp->particles_per_cell < 0 || pp->rand_seed < 0 || pp->rand_seed > 0xffffffffLL || pp->rand_skip < 0))
+    return GS_E_INVALID;
   int r[3] = {pp->res[0], pp->res[1], pb->dim == 3 ? pp->res[2] : 1};
-  if (dvc_max_particles(pb->dim, r) <= 0) return GS_E_INVALID;
+  PcParams A{};
+  if (!pc_grid(pb->dim, r, &A)) return GS_E_INVALID;
   double upT = pp->upper_t, lowT = pp->lower_t;
   if (upT < lowT) {  // PCDVC.cxx:265-270 resets inconsistent thresholds
     lowT = 0.6;
     upT = 25.0;
   }
+  const double V = pb->dim == 2 ? 4.0 : 8.0;
+  A.max_w = upT * V / 100.0;
+  A.min_w = lowT * V / 100.0;
+  A.thresh = pp->threshold;
+  A.cpr = pp->cent_pos_ratio;
+  A.max_splits = pp->max_splits;
+  A.max_deletions = pp->max_deletions;
+  A.split_if = pp->split_in_interface_cells;
+  A.delete_if = pp->delete_in_interface_cells;
+  A.inflow = inflow;
+  A.ppc = inflow ? pp->particles_per_cell : 0;
+  const double da = A.G.vol;
+  A.one_oda = (int)(1.0 / da + 0.5);
+  if (inflow)  // a region list longer than the region would hold entries the reference never wrote
+    for (int k = 1; k <= A.G.nc; k++)
+      if ((int)(k * da * A.one_oda + 0.5) > k) return GS_E_INVALID;
   hipStream_t st = pb->st;
   const long long P = pb->P, E = pb->g.E;
   CK(hipStreamSynchronize(st));
-  if (!pb->wp && P > 0) CK(dalloc(&pb->wp, P));
   std::vector<int> off(E + 1);
   CK(hipMemcpy(off.data(), pb->off, sizeof(int) * (E + 1), hipMemcpyDeviceToHost));
-  int maxp = 0;
-  for (long long e = 0; e < E; e++) maxp = std::max(maxp, off[e + 1] - off[e]);
-  if (maxp > dvc_max_particles(pb->dim, r)) return GS_E_INVALID;
-  // DVC of the particles' own local coordinates
-  launch_remap(pb->g, pb->coord, pb->off, pb->pk, E, st);
-  CKS(dvc_weights(pb->dim, r, E, pb->off, pb->pid, pb->pk, pb->wp, maxp, st));
-  const int slots = std::max(1, pp->max_splits);
-  unsigned char* del = nullptr;
-  int *nsplit = nullptr, *sparent = nullptr, *dmap = nullptr, *dbase = nullptr;
-  double* scoord = nullptr;
-  double *coordN = nullptr, *eta0N = nullptr, *pstrN = nullptr, *pincN = nullptr;
-  int* matN = nullptr;
-  int32_t rc = GS_OK;
-  auto cleanup = [&]() {
-    dfree(del); dfree(nsplit); dfree(sparent); dfree(dmap); dfree(dbase); dfree(scoord);
-    dfree(coordN); dfree(eta0N); dfree(matN); dfree(pstrN); dfree(pincN);
-  };
-  if (dalloc(&del, P > 0 ? P : 1) || dalloc(&nsplit, E) || dalloc(&sparent, E * slots) ||
-      dalloc(&scoord, E * slots * 3)) {
-    cleanup();
-    return GS_E_HIP;
-  }
-  const double V = pb->dim == 2 ? 4.0 : 8.0;
-  PcdvcArgs a{upT * V / 100.0, lowT * V / 100.0, pp->max_splits, pp->max_deletions,
-              pp->split_in_interface_cells, pp->delete_in_interface_cells, slots};
-  pcdvc_decide(pb->g, pb->off, pb->pid, pb->mat, pb->pk, pb->wp, a, del, nsplit, scoord, sparent, st);
-  // new caller order: survivors in their previous order, then the new particles
-  std::vector<unsigned char> hdel(P > 0 ? P : 1);
-  std::vector<int> pid(P > 0 ? P : 1), ns(E), base(E), map(P > 0 ? P : 1);
-  if (hipStreamSynchronize(st) || hipMemcpy(hdel.data(), del, P, hipMemcpyDeviceToHost) ||
-      hipMemcpy(pid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost) ||
-      hipMemcpy(ns.data(), nsplit, sizeof(int) * E, hipMemcpyDeviceToHost)) {
-    cleanup();
-    return GS_E_HIP;
-  }
-  std::vector<int> delc(P > 0 ? P : 1, 0), newid(P > 0 ? P : 1, -1);
-  for (long long i = 0; i < P; i++) delc[pid[i]] = hdel[i];
-  long long nsurv = 0;
-  for (long long c = 0; c < P; c++)
-    if (!delc[c]) newid[c] = (int)nsurv++;
-  for (long long i = 0; i < P; i++) map[i] = newid[pid[i]];
-  long long nnew = 0;
+  // staging: one slot per table entry a cell can reach
+  std::vector<long long> soff(E + 1, 0);
+  long long maxcap = 1, bound_draws = 0;
   for (long long e = 0; e < E; e++) {
-    base[e] = (int)nnew;
-    nnew += ns[e];
+    const long long n = off[e + 1] - off[e];
+    const long long cap = n > 0 ? n + A.ppc + A.max_splits : 0;
+    soff[e + 1] = soff[e] + cap;
+    maxcap = std::max(maxcap, cap);
+    if (n > 0 && inflow) bound_draws += A.ppc + n + A.max_splits;
   }
-  const long long P2 = nsurv + nnew;
-  if (P2 > 0x7fffffffLL) {
+  const long long S = soff[E];
+  if (maxcap > pc_max_capacity(A) || S >= 0x7fffffffLL) return GS_E_INVALID;
+  A.maxcap = (int)maxcap;
+  launch_remap(pb->g, pb->coord, pb->off, pb->pk, E, st);  // local coordinates of the particles
+
+  // device buffers
+  long long* d_soff = nullptr;
+  int *tbl = nullptr, *tcnt = nullptr, *rsrc = nullptr, *cnts = nullptr, *infdel = nullptr, *cand = nullptr;
+  double *rxi = nullptr, *scent = nullptr, *sw = nullptr;
+  int *rs = nullptr, *lcells = nullptr, *dlist = nullptr, *gout = nullptr, *fdel = nullptr;
+  long long *lroff = nullptr, *goff = nullptr, *fdoff = nullptr;
+  int *off2 = nullptr, *fidx = nullptr, *nbase_d = nullptr, *fidx_new = nullptr;
+  double *coordN = nullptr, *eta0N = nullptr, *pstrN = nullptr, *pincN = nullptr, *wN = nullptr;
+  int *matN = nullptr, *pidN = nullptr;
+  double4* pkN = nullptr;
+  unsigned* keysN = nullptr;
+  auto cleanup = [&]() {
+    dfree(d_soff); dfree(tbl); dfree(tcnt); dfree(rsrc); dfree(cnts); dfree(infdel); dfree(cand); dfree(rxi);
+    dfree(scent); dfree(sw); dfree(rs); dfree(lcells); dfree(dlist); dfree(gout); dfree(fdel); dfree(lroff);
+    dfree(goff); dfree(fdoff); dfree(off2); dfree(fidx); dfree(nbase_d); dfree(fidx_new); dfree(coordN);
+    dfree(eta0N); dfree(pstrN); dfree(pincN); dfree(wN); dfree(matN); dfree(pidN); dfree(pkN); dfree(keysN);
+  };
+  // a refused pass leaves the problem as it was (its weights recomputed)
+  auto refuse = [&]() -> int32_t {
+    const int32_t rr = relocate(pb);
+    return rr == GS_OK ? GS_E_INVALID : rr;
+  };
+#define PCK(x)                                                                                      \
+  do {                                                                                              \
+    hipError_t e_ = (x);                                                                            \
+    if (e_ != hipSuccess) {                                                                         \
+      std::fprintf(stderr, "gale_stokes: HIP error %s at %s:%d\n", hipGetErrorString(e_), __FILE__, __LINE__); \
+      cleanup();                                                                                    \
+      return GS_E_HIP;                                                                              \
+    }                                                                                               \
+  } while (0)
+  PCK(dalloc(&d_soff, E + 1));
+  PCK(dalloc(&tbl, S));
+  PCK(dalloc(&tcnt, E));
+  PCK(dalloc(&rsrc, S));
+  PCK(dalloc(&rxi, 3 * S));
+  PCK(dalloc(&cnts, 7 * E));
+  PCK(dalloc(&infdel, S));
+  PCK(dalloc(&cand, S));
+  PCK(dalloc(&scent, 3 * S));
+  PCK(dalloc(&sw, S));
+  PCK(hipMemcpyAsync(d_soff, soff.data(), sizeof(long long) * (E + 1), hipMemcpyHostToDevice, st));
+  // the rand() values this pass can use, from the stream position rand_skip
+  if (inflow) {
+    if (!(pb->rng_seed == pp->rand_seed && pb->rng_pos == pp->rand_skip)) {
+      pb->rng = CRand((unsigned)pp->rand_seed);
+      for (long long i = 0; i < pp->rand_skip; i++) (void)pb->rng.next();
+      pb->rng_seed = pp->rand_seed;
+      pb->rng_pos = pp->rand_skip;
+    }
+    std::vector<int> hv(bound_draws > 0 ? bound_draws : 1);
+    CRand g = pb->rng;
+    for (long long i = 0; i < bound_draws; i++) hv[i] = g.next();
+    PCK(dalloc(&rs, (long long)hv.size()));
+    PCK(hipMemcpyAsync(rs, hv.data(), sizeof(int) * hv.size(), hipMemcpyHostToDevice, st));
+    PCK(hipStreamSynchronize(st));
+  }
+  PcData D{};
+  D.off = pb->off;
+  D.pid = pb->pid;
+  D.mat = pb->mat;
+  D.pk = pb->pk;
+  D.soff = d_soff;
+  D.tbl = tbl;
+  D.tcnt = tcnt;
+  D.rsrc = rsrc;
+  D.rxi = rxi;
+  D.nins = cnts;
+  D.nspl = cnts + E;
+  D.ninfdel = cnts + 2 * E;
+  D.ncand = cnts + 3 * E;
+  D.cntdel = cnts + 4 * E;
+  D.ndraw = cnts + 5 * E;
+  D.flags = cnts + 6 * E;
+  D.infdel = infdel;
+  D.cand = cand;
+  D.scent = scent;
+  D.sw = sw;
+  D.rs = rs;
+  D.rs_n = inflow ? bound_draws : 0;
+  D.ndraw_list = nullptr;
+  std::vector<int> hc(7 * E);
+  auto fetch_counts = [&]() -> hipError_t {
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipMemcpy(hc.data(), cnts, sizeof(int) * 7 * E, hipMemcpyDeviceToHost);
+    return e;
+  };
+  if (!inflow) {
+    A.probe = 0;
+    pc_cells(A, D, E, nullptr, nullptr, st);
+    PCK(fetch_counts());
+  } else {
+    // probe: triggers, and the draw counts of the cells that are not re-populated
+    A.probe = 1;
+    pc_cells(A, D, E, nullptr, nullptr, st);
+    PCK(fetch_counts());
+    A.probe = 0;
+    const int* hdraw = hc.data() + 5 * E;
+    const int* hflag = hc.data() + 6 * E;
+    std::vector<long long> pre(E + 1, 0);  // offsets with no split draws in re-populated cells
+    std::vector<int> trig;
+    for (long long e = 0; e < E; e++) {
+      pre[e + 1] = pre[e] + hdraw[e];
+      if (hflag[e] & kPcTrig) trig.push_back((int)e);
+    }
+    std::vector<int> extra(trig.size(), 0);  // split draws of each re-populated cell
+    const long long T = (long long)trig.size();
+    if (T > 0) {
+      PCK(dalloc(&lcells, T));
+      PCK(dalloc(&lroff, T));
+      PCK(dalloc(&dlist, T));
+      PCK(hipMemcpyAsync(lcells, trig.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
+    }
+    long long pos = 0, delta = 0, win = 256;
+    std::vector<long long> ro;
+    std::vector<int> got;
+    while (pos < T) {
+      const long long w = std::min(win, T - pos);
+      ro.resize(w);
+      for (long long i = 0; i < w; i++) ro[i] = pre[trig[pos + i]] + delta;
+      PCK(hipMemcpyAsync(lroff, ro.data(), sizeof(long long) * w, hipMemcpyHostToDevice, st));
+      D.ndraw_list = dlist;
+      pc_cells(A, D, w, lcells + pos, lroff, st);
+      D.ndraw_list = nullptr;
+      got.resize(w);
+      PCK(hipMemcpyAsync(got.data(), dlist, sizeof(int) * w, hipMemcpyDeviceToHost, st));
+      PCK(hipStreamSynchronize(st));
+      long long k = 0;
+      for (; k < w; k++) {
+        const int n0 = off[trig[pos + k] + 1] - off[trig[pos + k]];
+        const int a = got[k] - (A.ppc + (n0 > 5 ? n0 : 0));
+        extra[pos + k] = a;
+        if (a != 0) break;  // later cells of the window drew from a wrong offset
+      }
+      if (k < w) {
+        delta += extra[pos + k];
+        pos += k + 1;
+        win = std::max<long long>(64, 2 * (k + 1));
+      } else {
+        pos += w;
+        win = std::min<long long>(16384, 2 * win);
+      }
+    }
+    // cells that split without re-population draw from their exact offsets now
+    std::vector<int> redo;
+    std::vector<long long> redo_off;
+    long long acc = 0, t = 0;
+    for (long long e = 0; e < E; e++) {
+      if (t < T && trig[t] == e) {
+        acc += extra[t];
+        t++;
+        continue;
+      }
+      if (hflag[e] & kPcRedo) {
+        redo.push_back((int)e);
+        redo_off.push_back(pre[e] + acc);
+      }
+    }
+    if (!redo.empty()) {
+      const long long R = (long long)redo.size();
+      dfree(lcells);
+      dfree(lroff);
+      PCK(dalloc(&lcells, R));
+      PCK(dalloc(&lroff, R));
+      PCK(hipMemcpyAsync(lcells, redo.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
+      PCK(hipMemcpyAsync(lroff, redo_off.data(), sizeof(long long) * R, hipMemcpyHostToDevice, st));
+      pc_cells(A, D, R, lcells, lroff, st);
+    }
+    PCK(fetch_counts());
+  }
+  const int* h_nins = hc.data();
+  const int* h_nspl = hc.data() + E;
+  const int* h_ninf = hc.data() + 2 * E;
+  const int* h_ncand = hc.data() + 3 * E;
+  const int* h_cntdel = hc.data() + 4 * E;
+  const int* h_draw = hc.data() + 5 * E;
+  const int* h_flag = hc.data() + 6 * E;
+  long long draws = 0, added = 0, nnew = 0;
+  for (long long e = 0; e < E; e++) {
+    if (h_flag[e] & kPcErr) {
+      cleanup();
+      return refuse();
+    }
+    draws += h_draw[e];
+    nnew += h_nins[e] + h_nspl[e];
+  }
+  added = nnew;
+  // the lists the host needs: Inflow deletions and deletion candidates
+  std::vector<int> gcells;
+  std::vector<long long> gofs;
+  long long gtot = 0;
+  for (long long e = 0; e < E; e++)
+    if (h_ninf[e] + h_ncand[e] > 0) {
+      gcells.push_back((int)e);
+      gofs.push_back(gtot);
+      gtot += 2LL * (h_ninf[e] + h_ncand[e]);
+    }
+  std::vector<int> hg(gtot > 0 ? gtot : 1);
+  if (!gcells.empty()) {
+    const long long G_ = (long long)gcells.size();
+    dfree(lcells);
+    PCK(dalloc(&lcells, G_));
+    PCK(dalloc(&goff, G_));
+    PCK(dalloc(&gout, gtot));
+    PCK(hipMemcpyAsync(lcells, gcells.data(), sizeof(int) * G_, hipMemcpyHostToDevice, st));
+    PCK(hipMemcpyAsync(goff, gofs.data(), sizeof(long long) * G_, hipMemcpyHostToDevice, st));
+    pc_gather(D, G_, lcells, goff, gout, st);
+    PCK(hipMemcpyAsync(hg.data(), gout, sizeof(int) * gtot, hipMemcpyDeviceToHost, st));
+    PCK(hipStreamSynchronize(st));
+  }
+  // the swarm's global array, cell by cell (PCDVC.cxx:339-610): ids < P are
+  // the caller's particles, P + k the k-th particle the pass created
+  if (P + nnew > 0x7fffffffLL) {
     cleanup();
-    return GS_E_INVALID;
+    return refuse();
   }
-  if (dalloc(&dmap, P > 0 ? P : 1) || dalloc(&dbase, E) || dalloc(&coordN, 3 * (P2 > 0 ? P2 : 1)) ||
-      dalloc(&matN, P2 > 0 ? P2 : 1) || (pb->eta0p && dalloc(&eta0N, P2 > 0 ? P2 : 1)) ||
-      (pb->pstr && (dalloc(&pstrN, P2 > 0 ? P2 : 1) || dalloc(&pincN, P2 > 0 ? P2 : 1))) ||
-      hipMemcpy(dmap, map.data(), sizeof(int) * P, hipMemcpyHostToDevice) ||
-      hipMemcpy(dbase, base.data(), sizeof(int) * E, hipMemcpyHostToDevice)) {
+  std::vector<int> arr(P), idx(P + nnew), nbase(E, 0);
+  for (long long i = 0; i < P; i++) arr[i] = idx[i] = (int)i;
+  auto del = [&](int id) {
+    const int k = idx[id], last = arr.back();
+    arr[k] = last;
+    idx[last] = k;
+    arr.pop_back();
+    idx[id] = -1;
+  };
+  std::vector<int> fcells, fdorder;
+  std::vector<long long> fdoffs;
+  std::vector<std::pair<int, int>> cl;  // (id, reference)
+  long long next = P, gi = 0, deleted = 0;
+  for (long long e = 0; e < E; e++) {
+    if (h_nins[e] + h_nspl[e] + h_ninf[e] + h_ncand[e] == 0) continue;
+    nbase[e] = (int)(next - P);
+    auto id_of = [&](int code) -> int {
+      return code >= 0 ? code : (int)(P + nbase[e] + ((-1 - (long long)code) - soff[e]));
+    };
+    const long long g0 = gi < (long long)gcells.size() && gcells[gi] == e ? gofs[gi] : -1;
+    if (g0 >= 0) gi++;
+    for (int k = 0; k < h_nins[e]; k++) {
+      idx[next] = (int)arr.size();
+      arr.push_back((int)next);
+      next++;
+    }
+    for (int k = 0; k < h_ninf[e]; k++) {
+      del(id_of(hg[g0 + 2 * k + 1]));
+      deleted++;
+    }
+    for (int k = 0; k < h_nspl[e]; k++) {
+      idx[next] = (int)arr.size();
+      arr.push_back((int)next);
+      next++;
+    }
+    if (h_flag[e] & kPcFinal) {
+      fcells.push_back((int)e);
+      fdoffs.push_back((long long)fdorder.size());
+    }
+    if (h_cntdel[e] > 0) {
+      // candidates sorted by their current index, highest first (compare_indexOnCPU)
+      cl.clear();
+      for (int k = 0; k < h_ncand[e]; k++) {
+        const long long q = g0 + 2LL * (h_ninf[e] + k);
+        cl.emplace_back(id_of(hg[q + 1]), hg[q]);
+      }
+      std::sort(cl.begin(), cl.end(), [&](const std::pair<int, int>& x, const std::pair<int, int>& y) {
+        return idx[x.first] > idx[y.first];
+      });
+      for (int k = 0; k < h_cntdel[e]; k++) {
+        del(cl[k].first);
+        fdorder.push_back(cl[k].second);
+        deleted++;
+      }
+    }
+  }
+  const long long P2 = (long long)arr.size();
+  if (!fcells.empty()) {
+    const long long F = (long long)fcells.size();
+    dfree(lcells);
+    PCK(dalloc(&lcells, F));
+    PCK(dalloc(&fdoff, F));
+    PCK(dalloc(&fdel, (long long)fdorder.size()));
+    PCK(hipMemcpyAsync(lcells, fcells.data(), sizeof(int) * F, hipMemcpyHostToDevice, st));
+    PCK(hipMemcpyAsync(fdoff, fdoffs.data(), sizeof(long long) * F, hipMemcpyHostToDevice, st));
+    if (!fdorder.empty())
+      PCK(hipMemcpyAsync(fdel, fdorder.data(), sizeof(int) * fdorder.size(), hipMemcpyHostToDevice, st));
+    pc_final(A, D, F, lcells, fdel, fdoff, st);
+    PCK(hipStreamSynchronize(st));
+    PCK(hipMemcpy(hc.data() + 6 * E, D.flags, sizeof(int) * E, hipMemcpyDeviceToHost));
+    for (long long e = 0; e < E; e++)
+      if (h_flag[e] & kPcErr) {
+        cleanup();
+        return refuse();
+      }
+  }
+  // new sorted order: cells in order, each in its final table order
+  std::vector<int> htc(E), hoff2(E + 1, 0);
+  PCK(hipMemcpy(htc.data(), tcnt, sizeof(int) * E, hipMemcpyDeviceToHost));
+  for (long long e = 0; e < E; e++) hoff2[e + 1] = hoff2[e] + htc[e];
+  if (hoff2[E] != P2) {  // the tables and the array must hold the same particles
     cleanup();
-    return GS_E_HIP;
+    return refuse();
   }
-  pcdvc_rebuild(P, E, dmap, pb->coord, pb->mat, PartDoubles{{pb->eta0p, pb->pstr, pb->pinc}}, nsplit, dbase, slots,
-                scoord, sparent, nsurv, coordN, matN, PartDoubles{{eta0N, pstrN, pincN}}, st);
-  if (hipStreamSynchronize(st)) {
-    cleanup();
-    return GS_E_HIP;
+  std::vector<int> hpid(P > 0 ? P : 1), hf(P > 0 ? P : 1), hfn(nnew > 0 ? nnew : 1);
+  PCK(hipMemcpy(hpid.data(), pb->pid, sizeof(int) * P, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < P; i++) hf[i] = idx[hpid[i]];
+  for (long long k = 0; k < nnew; k++) hfn[k] = idx[P + k];
+  const long long PA = P2 > 0 ? P2 : 1;
+  PCK(dalloc(&off2, E + 1));
+  PCK(dalloc(&fidx, P));
+  PCK(dalloc(&nbase_d, E));
+  PCK(dalloc(&fidx_new, nnew));
+  PCK(hipMemcpyAsync(off2, hoff2.data(), sizeof(int) * (E + 1), hipMemcpyHostToDevice, st));
+  PCK(hipMemcpyAsync(fidx, hf.data(), sizeof(int) * hf.size(), hipMemcpyHostToDevice, st));
+  PCK(hipMemcpyAsync(nbase_d, nbase.data(), sizeof(int) * E, hipMemcpyHostToDevice, st));
+  PCK(hipMemcpyAsync(fidx_new, hfn.data(), sizeof(int) * hfn.size(), hipMemcpyHostToDevice, st));
+  PCK(dalloc(&coordN, 3 * PA));
+  PCK(dalloc(&matN, PA));
+  PCK(dalloc(&pidN, PA));
+  PCK(dalloc(&pkN, PA));
+  PCK(dalloc(&wN, PA));
+  PCK(dalloc(&keysN, PA));
+  if (pb->eta0p) PCK(dalloc(&eta0N, PA));
+  if (pb->pstr) {
+    PCK(dalloc(&pstrN, PA));
+    PCK(dalloc(&pincN, PA));
   }
-  // resize the particle arrays of the problem
+  pc_rebuild(pb->g, D, off2, fidx, nbase_d, fidx_new, pb->coord, pb->mat, PartDoubles{{pb->eta0p, pb->pstr, pb->pinc}},
+             coordN, matN, pidN, PartDoubles{{eta0N, pstrN, pincN}}, pkN, wN, keysN, st);
+  PCK(hipMemcpyAsync(pb->off, off2, sizeof(int) * (E + 1), hipMemcpyDeviceToDevice, st));
+  PCK(hipStreamSynchronize(st));
+  // the problem takes the new particle arrays
   dfree(pb->coord); dfree(pb->coord2); dfree(pb->mat); dfree(pb->mat2); dfree(pb->pid); dfree(pb->pid2);
   dfree(pb->eta0p); dfree(pb->eta0p2); dfree(pb->pk); dfree(pb->pk32); dfree(pb->keys); dfree(pb->keys2);
   dfree(pb->vals); dfree(pb->vals2); dfree(pb->wp);
   dfree(pb->pstr); dfree(pb->pstr2); dfree(pb->pinc); dfree(pb->pinc2);
-  pb->pk32_valid = false;
-  pb->coord = coordN;
-  pb->mat = matN;
-  pb->eta0p = eta0N;
-  pb->pstr = pstrN;
-  pb->pinc = pincN;
-  coordN = nullptr;
-  matN = nullptr;
-  eta0N = nullptr;
-  pstrN = nullptr;
-  pincN = nullptr;
+  pb->coord = coordN; coordN = nullptr;
+  pb->mat = matN; matN = nullptr;
+  pb->pid = pidN; pidN = nullptr;
+  pb->pk = pkN; pkN = nullptr;
+  pb->wp = wN; wN = nullptr;
+  pb->keys = keysN; keysN = nullptr;
+  pb->eta0p = eta0N; eta0N = nullptr;
+  pb->pstr = pstrN; pstrN = nullptr;
+  pb->pinc = pincN; pincN = nullptr;
   cleanup();
+#undef PCK
   pb->P = P2;
-  const long long PA = P2 > 0 ? P2 : 1;
   CK(dalloc(&pb->coord2, 3 * PA));
   CK(dalloc(&pb->mat2, PA));
-  CK(dalloc(&pb->pid, PA));
   CK(dalloc(&pb->pid2, PA));
-  CK(dalloc(&pb->pk, PA));
-  CK(dalloc(&pb->keys, PA));
   CK(dalloc(&pb->keys2, PA));
   CK(dalloc(&pb->vals, PA));
   CK(dalloc(&pb->vals2, PA));
-  CK(dalloc(&pb->wp, PA));
   if (pb->eta0p) CK(dalloc(&pb->eta0p2, PA));
   if (pb->pstr) {
     CK(dalloc(&pb->pstr2, PA));
     CK(dalloc(&pb->pinc2, PA));
   }
-  std::vector<int> ids(PA);
-  for (long long i = 0; i < PA; i++) ids[i] = (int)i;
-  CK(hipMemcpy(pb->pid, ids.data(), sizeof(int) * PA, hipMemcpyHostToDevice));
+  pb->pk32_valid = false;
   pb->wkind = GS_WEIGHTS_DVC;
   for (int d = 0; d < 3; d++) pb->wres[d] = r[d];
-  if (counts) {
-    counts[0] = nnew;
-    counts[1] = P - nsurv;
+  pb->escaped = 0;
+  pb->min_one_particle = true;
+  for (long long e = 0; e < E; e++)
+    if (htc[e] == 0) {
+      pb->min_one_particle = false;
+      break;
+    }
+  pb->gen++;  // solvers re-run their setup on the new particles
+  if (inflow) {
+    for (long long i = 0; i < draws; i++) (void)pb->rng.next();
+    pb->rng_pos += draws;
   }
-  rc = relocate(pb);
-  return rc;
+  if (counts) {
+    counts[0] = added;
+    counts[1] = deleted;
+    counts[2] = draws;
+  }
+  return GS_OK;
 }
 
 extern "C" int32_t gs_problem_set_coords(gs_problem* pb, const double* coord) {

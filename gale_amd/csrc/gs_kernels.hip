@@ -958,22 +958,90 @@ void launch_offsets(const int* keys_sorted, long long P, long long E, int* off, 
 
 // CoincidentMapper global -> local: for an affine element the Newton
 // iteration of ElementType.cxx:221-333 converges to xi = (x - centre) * 2/h.
+// Local coordinates of a global point in element ijk by the reference's
+// Newton-Raphson iteration (ElementType.cxx:221-333): from the element centre,
+// J dxi = x - sum_n N_n(xi) x_n with J = sum_n dN_n/dxi x_n, solved by
+// Cramer's rule, until max |dxi| < 1e-4 (at most 100 iterations).  The
+// operations follow the oracle's restatement one for one (no contraction),
+// so both give the same bits; particles placed exactly on the sub-grid of
+// the Voronoi weights (PCDVC's Inflow) make ties that depend on them.
+__device__ void newton_local(const Geo& g, const long long* ijk, const double* x, double* xi) {
+#pragma clang fp contract(off)
+  const int dim = g.dim, nen = dim == 2 ? 4 : 8;
+  double xn[8][3];
+  for (int a = 0; a < nen; a++)
+    for (int d = 0; d < 3; d++) xn[a][d] = d < dim ? vcoord(g, d, ijk[d] + ((a >> d) & 1)) : 0.0;
+  for (int d = 0; d < 3; d++) xi[d] = 0.0;
+  for (int it = 0; it < 100; it++) {
+    double N[8], GNi[3][8], J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rhs[3] = {0, 0, 0}, inc[3] = {0, 0, 0};
+    if (dim == 2) {
+      N[0] = 0.25 * (1.0 - xi[0]) * (1.0 - xi[1]);
+      N[1] = 0.25 * (1.0 + xi[0]) * (1.0 - xi[1]);
+      N[2] = 0.25 * (1.0 - xi[0]) * (1.0 + xi[1]);
+      N[3] = 0.25 * (1.0 + xi[0]) * (1.0 + xi[1]);
+    } else {
+      for (int a = 0; a < 8; a++) {
+        const double sx = (a & 1) ? 1.0 : -1.0, sy = (a & 2) ? 1.0 : -1.0, sz = (a & 4) ? 1.0 : -1.0;
+        N[a] = 0.125 * (1.0 + sx * xi[0]) * (1.0 + sy * xi[1]) * (1.0 + sz * xi[2]);
+      }
+    }
+    const double c = dim == 2 ? 0.25 : 0.125;
+    for (int a = 0; a < nen; a++) {
+      double s[3], f[3];
+      for (int d = 0; d < dim; d++) {
+        s[d] = ((a >> d) & 1) ? 1.0 : -1.0;
+        f[d] = 1.0 + s[d] * xi[d];
+      }
+      for (int d = 0; d < dim; d++) {
+        double v = c * s[d];
+        for (int d2 = 0; d2 < dim; d2++)
+          if (d2 != d) v *= f[d2];
+        GNi[d][a] = v;
+      }
+    }
+    for (int a = 0; a < nen; a++)
+      for (int i = 0; i < dim; i++) {
+        for (int j = 0; j < dim; j++) J[i][j] += GNi[j][a] * xn[a][i];
+        rhs[i] -= N[a] * xn[a][i];
+      }
+    for (int i = 0; i < dim; i++) rhs[i] += x[i];
+    if (dim == 2) {
+      const double D = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      inc[0] = (rhs[0] * J[1][1] - J[0][1] * rhs[1]) / D;
+      inc[1] = (J[0][0] * rhs[1] - rhs[0] * J[1][0]) / D;
+    } else {
+      const double D = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                       J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      for (int cc = 0; cc < 3; cc++) {
+        double M[3][3];
+        for (int r = 0; r < 3; r++)
+          for (int q = 0; q < 3; q++) M[r][q] = q == cc ? rhs[r] : J[r][q];
+        inc[cc] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) -
+                   M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                   M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / D;
+      }
+    }
+    double mr = 0.0;
+    for (int d = 0; d < dim; d++) {
+      xi[d] += inc[d];
+      if (fabs(inc[d]) > mr) mr = fabs(inc[d]);
+    }
+    if (mr < 1e-4) break;
+  }
+}
+
 __global__ void k_remap(Geo g, const double* __restrict__ coord, const int* __restrict__ off,
                         double4* __restrict__ pk, long long E) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= E) return;
   const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
                             g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
-  double xc[3] = {0, 0, 0}, sc[3] = {0, 0, 0};
-  for (int d = 0; d < g.dim; d++) {
-    const double a = vcoord(g, d, ijk[d]), b = vcoord(g, d, ijk[d] + 1);
-    xc[d] = 0.5 * (a + b);
-    sc[d] = 2.0 / (b - a);
-  }
   for (int p = off[e]; p < off[e + 1]; p++) {
-    double xi[3] = {0, 0, 0};
-    for (int d = 0; d < g.dim; d++) xi[d] = (coord[3 * (long long)p + d] - xc[d]) * sc[d];
-    pk[p] = make_double4(xi[0], xi[1], xi[2], 0.0);
+    const double x[3] = {coord[3 * (long long)p], coord[3 * (long long)p + 1], coord[3 * (long long)p + 2]};
+    double xi[3];
+    newton_local(g, ijk, x, xi);
+    pk[p] = make_double4(xi[0], xi[1], g.dim == 3 ? xi[2] : 0.0, 0.0);
   }
 }
 void launch_remap(const Geo& g, const double* coord, const int* off, double4* pk, long long E, hipStream_t st) {

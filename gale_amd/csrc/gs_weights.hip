@@ -34,14 +34,6 @@ __device__ __forceinline__ double vcoord_w(const Geo& g, int d, long long idx) {
   return g.lo[d] + ((double)idx / (double)g.res[d]) * (g.hi[d] - g.lo[d]);
 }
 
-struct DvcGrid {
-  int dim;
-  int r[3];
-  int nc, nwords;
-  double h[3];
-  double vol;
-};
-
 // Closer-claimant test of ClaimCells (DVCWeights.cxx:629-634): positive when
 // q is strictly closer to the sub-cell centre c than the owner o.
 __device__ __forceinline__ double dvc_contest(int dim, const double* c, const double* o, const double* q) {
@@ -87,9 +79,9 @@ __device__ __forceinline__ int dvc_rebound(const DvcGrid& G, const short* owner,
   return added;
 }
 
-// LDS layout of one cell (bytes): centres, particle coordinates, order,
-// boundary sizes, sub-cell coordinates, owners, the current step's claims,
-// and one boundary bit set per particle.
+// LDS of one cell's growth (bytes): centres, particle coordinates, a table
+// of particle slots (order), boundary sizes, sub-cell coordinates, owners,
+// the current step's claims, and one boundary bit set per particle.
 __host__ __device__ inline size_t dvc_lds_bytes(const DvcGrid& G, int maxp) {
   size_t b = sizeof(double) * (G.r[0] + G.r[1] + G.r[2]);
   b += sizeof(double) * 3 * maxp;
@@ -99,6 +91,174 @@ __host__ __device__ inline size_t dvc_lds_bytes(const DvcGrid& G, int maxp) {
   b += sizeof(unsigned) * (G.nwords + 1);
   b += sizeof(unsigned) * (size_t)G.nwords * maxp;
   return b;
+}
+
+struct DvcLds {
+  double* ctr;      // sub-cell centres per axis
+  double* xs;       // 3 per particle, in table order
+  int* ord;         // table of the cell (its meaning is the caller's)
+  int* bsize;       // boundary size of each particle
+  unsigned* cxyz;   // packed sub-cell coordinates
+  short* owner;     // owning particle of each sub-cell
+  unsigned* took;   // the current step's claims
+  unsigned* bound;  // one boundary bit set per particle
+};
+
+__device__ __forceinline__ DvcLds dvc_carve(const DvcGrid& G, int maxp, unsigned char* base) {
+  DvcLds L;
+  L.ctr = reinterpret_cast<double*>(base);
+  L.xs = L.ctr + (G.r[0] + G.r[1] + G.r[2]);
+  L.ord = reinterpret_cast<int*>(L.xs + 3 * maxp);
+  L.bsize = L.ord + maxp;
+  L.cxyz = reinterpret_cast<unsigned*>(L.bsize + maxp);
+  L.owner = reinterpret_cast<short*>(L.cxyz + G.nc);
+  L.took = reinterpret_cast<unsigned*>(reinterpret_cast<unsigned char*>(L.owner) + (sizeof(short) * G.nc + 7) / 8 * 8);
+  L.bound = L.took + G.nwords + 1;
+  return L;
+}
+
+// Sub-cell centres, accumulated one step at a time (DVCWeights.cxx:525-619),
+// and the packed sub-cell coordinates.  The caller synchronises.
+__device__ void dvc_setup(const DvcGrid& G, const DvcLds& L) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x;
+  if (lane < G.dim) {
+    const int d = lane;
+    double* cd = L.ctr + (d == 0 ? 0 : (d == 1 ? G.r[0] : G.r[0] + G.r[1]));
+    double x = -1.0 - G.h[d] / 2.0;
+    for (int i = 0; i < G.r[d]; i++) {
+      x = x + G.h[d];
+      cd[i] = x;
+    }
+  }
+  for (int c = lane; c < G.nc; c += kDvcLanes) {
+    const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
+    L.cxyz[c] = (unsigned)ix | ((unsigned)iy << 10) | ((unsigned)iz << 20);
+  }
+}
+
+__device__ __forceinline__ void dvc_centre(const DvcGrid& G, const DvcLds& L, int c, double* x) {
+  const unsigned v = L.cxyz[c];
+  x[0] = L.ctr[dvc_ix(v)];
+  x[1] = L.ctr[G.r[0] + dvc_iy(v)];
+  x[2] = G.dim == 3 ? L.ctr[G.r[0] + G.r[1] + dvc_iz(v)] : 0.0;
+}
+
+// Grows the discrete Voronoi regions of particles 0..n-1 (coordinates in
+// L.xs, table order) as DVCWeights.cxx:693-786 does; L.owner holds the result.
+__device__ void dvc_grow(const DvcGrid& G, const DvcLds& L, int n) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x;
+  for (int c = lane; c < G.nc; c += kDvcLanes) L.owner[c] = -1;
+  for (int q = lane; q < n * G.nwords; q += kDvcLanes) L.bound[q] = 0;
+  __syncthreads();
+  // seeds, in order (a later particle in the same sub-cell takes it over)
+  if (lane == 0) {
+    for (int i = 0; i < n; i++) {
+      int idx[3] = {0, 0, 0};
+      for (int d = 0; d < G.dim; d++) {
+        int k = (int)((L.xs[3 * i + d] - (-1.0)) / G.h[d]);
+        if (k == G.r[d]) k--;
+        idx[d] = k;
+      }
+      const int c = idx[0] + G.r[0] * (idx[1] + G.r[1] * idx[2]);
+      L.owner[c] = (short)i;
+      L.bsize[i] = dvc_rebound(G, L.owner, L.cxyz, L.bound + (size_t)i * G.nwords, i, c);
+    }
+  }
+  __syncthreads();
+  const double* cx = L.ctr;
+  const double* cy = L.ctr + G.r[0];
+  const double* cz = L.ctr + G.r[0] + G.r[1];
+  // Rounds of claims until one claims nothing.  Lanes take the boundary's
+  // 32-bit words; a particle with an empty boundary (it claimed nothing last
+  // time) is skipped, as ClaimCells over an empty list claims nothing.
+  for (;;) {
+    int total = 0;
+    for (int i = 0; i < n; i++) {
+      if (L.bsize[i] == 0) continue;  // uniform across the wave (LDS value)
+      unsigned* bi = L.bound + (size_t)i * G.nwords;
+      const double* xi_ = L.xs + 3 * i;
+      int cnt = 0;
+      for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
+        unsigned bits = bi[wd], tk = 0;
+        while (bits) {
+          const int t = __ffs(bits) - 1;
+          bits &= bits - 1u;
+          const int c = wd * 32 + t;
+          const int o = L.owner[c];
+          bool take = o < 0;
+          if (!take && o != i) {
+            const unsigned v = L.cxyz[c];
+            const double c3[3] = {cx[dvc_ix(v)], cy[dvc_iy(v)], G.dim == 3 ? cz[dvc_iz(v)] : 0.0};
+            take = dvc_contest(G.dim, c3, L.xs + 3 * o, xi_) > 0.0;
+          }
+          if (take) {
+            L.owner[c] = (short)i;
+            tk |= 1u << t;
+            cnt++;
+          }
+        }
+        L.took[wd] = tk;
+        bi[wd] = 0u;
+      }
+      __syncthreads();
+      cnt = wave_sum_i(cnt);
+      total += cnt;
+      int added = 0;
+      if (cnt > 0)
+        for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
+          unsigned tk = L.took[wd];
+          while (tk) {
+            const int t = __ffs(tk) - 1;
+            tk &= tk - 1u;
+            added += dvc_rebound(G, L.owner, L.cxyz, bi, i, wd * 32 + t);
+          }
+        }
+      added = wave_sum_i(added);
+      if (lane == 0) L.bsize[i] = added;
+      __syncthreads();
+    }
+    if (total == 0) break;
+  }
+}
+
+// Centroid and volume of every region (DVCWeights.cxx:202-258): each particle
+// sums the centres of its sub-cells in index order, walking a bit set of the
+// sub-cells it owns (the boundary storage, no longer needed).  out(i, c0,
+// c1, c2, cnt) runs on the lane that handles particle i.
+template <class F>
+__device__ void dvc_regions(const DvcGrid& G, const DvcLds& L, int n, F out) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x;
+  for (int q = lane; q < n * G.nwords; q += kDvcLanes) L.bound[q] = 0;
+  __syncthreads();
+  for (int c = lane; c < G.nc; c += kDvcLanes) {
+    const int o = L.owner[c];
+    if (o >= 0) atomicOr(&L.bound[(size_t)o * G.nwords + (c >> 5)], 1u << (c & 31));
+  }
+  __syncthreads();
+  const double* cx = L.ctr;
+  const double* cy = L.ctr + G.r[0];
+  const double* cz = L.ctr + G.r[0] + G.r[1];
+  for (int i = lane; i < n; i += kDvcLanes) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    int cnt = 0;
+    const unsigned* own = L.bound + (size_t)i * G.nwords;
+    for (int wd = 0; wd < G.nwords; wd++) {
+      unsigned bits = own[wd];
+      while (bits) {
+        const int t = __ffs(bits) - 1;
+        bits &= bits - 1u;
+        const unsigned v = L.cxyz[wd * 32 + t];
+        s0 += cx[dvc_ix(v)];
+        s1 += cy[dvc_iy(v)];
+        if (G.dim == 3) s2 += cz[dvc_iz(v)];
+        cnt++;
+      }
+    }
+    out(i, cnt ? s0 / cnt : 0.0, cnt ? s1 / cnt : 0.0, cnt ? s2 / cnt : 0.0, cnt);
+  }
 }
 
 // One FE cell per workgroup.  Particles of cell e are pk[off[e] .. off[e+1])
@@ -113,149 +273,32 @@ __global__ __launch_bounds__(kDvcLanes) void k_dvc(DvcGrid G, const int* __restr
   const int b = off[e], n = off[e + 1] - b;
   if (n <= 0 || n > maxp) return;
   const int lane = threadIdx.x;
-  double* ctr = reinterpret_cast<double*>(dvc_smem);
-  double* xs = ctr + (G.r[0] + G.r[1] + G.r[2]);
-  int* ord = reinterpret_cast<int*>(xs + 3 * maxp);
-  int* bsize = ord + maxp;  // boundary size of each particle
-  unsigned* cxyz = reinterpret_cast<unsigned*>(bsize + maxp);
-  short* owner = reinterpret_cast<short*>(cxyz + G.nc);
-  unsigned* took = reinterpret_cast<unsigned*>(reinterpret_cast<unsigned char*>(owner) +
-                                               (sizeof(short) * G.nc + 7) / 8 * 8);
-  unsigned* bound = took + G.nwords + 1;
-  // sub-cell centres, accumulated one step at a time (DVCWeights.cxx:525-619)
-  if (lane < G.dim) {
-    const int d = lane;
-    double* cd = ctr + (d == 0 ? 0 : (d == 1 ? G.r[0] : G.r[0] + G.r[1]));
-    double x = -1.0 - G.h[d] / 2.0;
-    for (int i = 0; i < G.r[d]; i++) {
-      x = x + G.h[d];
-      cd[i] = x;
-    }
-  }
+  const DvcLds L = dvc_carve(G, maxp, dvc_smem);
+  dvc_setup(G, L);
   // rank of each particle by pid
   for (int j = lane; j < n; j += kDvcLanes) {
     const int pj = pid[b + j];
     int rk = 0;
     for (int k = 0; k < n; k++) rk += pid[b + k] < pj ? 1 : 0;
-    ord[rk] = j;
-  }
-  for (int c = lane; c < G.nc; c += kDvcLanes) {
-    owner[c] = -1;
-    const int ix = c % G.r[0], iy = (c / G.r[0]) % G.r[1], iz = c / (G.r[0] * G.r[1]);
-    cxyz[c] = (unsigned)ix | ((unsigned)iy << 10) | ((unsigned)iz << 20);
-  }
-  for (int q = lane; q < n * G.nwords; q += kDvcLanes) bound[q] = 0;
-  __syncthreads();
-  for (int i = lane; i < n; i += kDvcLanes) {
-    const double4 q = pk[b + ord[i]];
-    xs[3 * i] = q.x;
-    xs[3 * i + 1] = q.y;
-    xs[3 * i + 2] = G.dim == 3 ? q.z : 0.0;
-  }
-  __syncthreads();
-  // seeds, in order (a later particle in the same sub-cell takes it over)
-  if (lane == 0) {
-    for (int i = 0; i < n; i++) {
-      int idx[3] = {0, 0, 0};
-      for (int d = 0; d < G.dim; d++) {
-        int k = (int)((xs[3 * i + d] - (-1.0)) / G.h[d]);
-        if (k == G.r[d]) k--;
-        idx[d] = k;
-      }
-      const int c = idx[0] + G.r[0] * (idx[1] + G.r[1] * idx[2]);
-      owner[c] = (short)i;
-      bsize[i] = dvc_rebound(G, owner, cxyz, bound + (size_t)i * G.nwords, i, c);
-    }
-  }
-  __syncthreads();
-  const double* cx = ctr;
-  const double* cy = ctr + G.r[0];
-  const double* cz = ctr + G.r[0] + G.r[1];
-  // Rounds of claims until one claims nothing.  Lanes take the boundary's
-  // 32-bit words; a particle with an empty boundary (it claimed nothing last
-  // time) is skipped, as ClaimCells over an empty list claims nothing.
-  for (;;) {
-    int total = 0;
-    for (int i = 0; i < n; i++) {
-      if (bsize[i] == 0) continue;  // uniform across the wave (LDS value)
-      unsigned* bi = bound + (size_t)i * G.nwords;
-      const double* xi_ = xs + 3 * i;
-      int cnt = 0;
-      for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
-        unsigned bits = bi[wd], tk = 0;
-        while (bits) {
-          const int t = __ffs(bits) - 1;
-          bits &= bits - 1u;
-          const int c = wd * 32 + t;
-          const int o = owner[c];
-          bool take = o < 0;
-          if (!take && o != i) {
-            const unsigned v = cxyz[c];
-            const double c3[3] = {cx[dvc_ix(v)], cy[dvc_iy(v)], G.dim == 3 ? cz[dvc_iz(v)] : 0.0};
-            take = dvc_contest(G.dim, c3, xs + 3 * o, xi_) > 0.0;
-          }
-          if (take) {
-            owner[c] = (short)i;
-            tk |= 1u << t;
-            cnt++;
-          }
-        }
-        took[wd] = tk;
-        bi[wd] = 0u;
-      }
-      __syncthreads();
-      cnt = wave_sum_i(cnt);
-      total += cnt;
-      int added = 0;
-      if (cnt > 0)
-        for (int wd = lane; wd < G.nwords; wd += kDvcLanes) {
-          unsigned tk = took[wd];
-          while (tk) {
-            const int t = __ffs(tk) - 1;
-            tk &= tk - 1u;
-            added += dvc_rebound(G, owner, cxyz, bi, i, wd * 32 + t);
-          }
-        }
-      added = wave_sum_i(added);
-      if (lane == 0) bsize[i] = added;
-      __syncthreads();
-    }
-    if (total == 0) break;
-  }
-  // centroids and volumes (DVCWeights.cxx:202-258): each particle sums the
-  // centres of its sub-cells in index order, walking a bit set of the
-  // sub-cells it owns (the boundary storage, no longer needed)
-  for (int q = lane; q < n * G.nwords; q += kDvcLanes) bound[q] = 0;
-  __syncthreads();
-  for (int c = lane; c < G.nc; c += kDvcLanes) {
-    const int o = owner[c];
-    if (o >= 0) atomicOr(&bound[(size_t)o * G.nwords + (c >> 5)], 1u << (c & 31));
+    L.ord[rk] = j;
   }
   __syncthreads();
   for (int i = lane; i < n; i += kDvcLanes) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    int cnt = 0;
-    const unsigned* own = bound + (size_t)i * G.nwords;
-    for (int wd = 0; wd < G.nwords; wd++) {
-      unsigned bits = own[wd];
-      while (bits) {
-        const int t = __ffs(bits) - 1;
-        bits &= bits - 1u;
-        const unsigned v = cxyz[wd * 32 + t];
-        s0 += cx[dvc_ix(v)];
-        s1 += cy[dvc_iy(v)];
-        if (G.dim == 3) s2 += cz[dvc_iz(v)];
-        cnt++;
-      }
-    }
-    const int j = b + ord[i];
+    const double4 q = pk[b + L.ord[i]];
+    L.xs[3 * i] = q.x;
+    L.xs[3 * i + 1] = q.y;
+    L.xs[3 * i + 2] = G.dim == 3 ? q.z : 0.0;
+  }
+  dvc_grow(G, L, n);
+  dvc_regions(G, L, n, [&](int i, double c0, double c1, double c2, int cnt) {
+    const int j = b + L.ord[i];
     double4 q = pk[j];
-    q.x = cnt ? s0 / cnt : 0.0;
-    q.y = cnt ? s1 / cnt : 0.0;
-    if (G.dim == 3) q.z = cnt ? s2 / cnt : 0.0;
+    q.x = c0;
+    q.y = c1;
+    if (G.dim == 3) q.z = c2;
     pk[j] = q;
     w[j] = cnt * G.vol;
-  }
+  });
 }
 
 }  // namespace
@@ -312,124 +355,375 @@ namespace gs {
 
 namespace {
 
-__global__ void k_pcdvc_decide(Geo g, const int* __restrict__ off, const int* __restrict__ pid,
-                               const int* __restrict__ mat, const double4* __restrict__ pk,
-                               const double* __restrict__ w, PcdvcArgs a, unsigned char* __restrict__ del,
-                               int* __restrict__ nsplit, double* __restrict__ scoord, int* __restrict__ sparent) {
+// PCDVC, one wavefront per FE cell (PCDVC.cxx:656-1090 in 3D, 1092-1512 in
+// 2D).  The cell's table is an LDS list of particle references: r >= 0 is
+// an existing particle (its sorted index), r < 0 a particle the pass created,
+// record -1 - r of the staging area (its original source particle and its
+// exact local coordinate).  The table starts as the cell's particles in pid
+// order (CoincidentMapper.cxx:207-240); a split appends to it, a deletion
+// moves its last entry into the freed slot (SwarmClass.cxx:976-990).
+__device__ __forceinline__ int pc_src(const PcData& D, int ref) { return ref >= 0 ? ref : D.rsrc[-1 - ref]; }
+
+__device__ __forceinline__ void pc_load(const DvcGrid& G, const DvcLds& L, const PcData& D, int n) {
+  for (int i = threadIdx.x; i < n; i += kDvcLanes) {
+    const int ref = L.ord[i];
+    if (ref >= 0) {
+      const double4 q = D.pk[ref];
+      L.xs[3 * i] = q.x;
+      L.xs[3 * i + 1] = q.y;
+      L.xs[3 * i + 2] = G.dim == 3 ? q.z : 0.0;
+    } else {
+      const double* x = D.rxi + 3LL * (-1 - ref);
+      L.xs[3 * i] = x[0];
+      L.xs[3 * i + 1] = x[1];
+      L.xs[3 * i + 2] = G.dim == 3 ? x[2] : 0.0;
+    }
+  }
+}
+
+__host__ __device__ inline size_t pc_lds_extra(int maxcap) { return sizeof(double) * 4 * maxcap + sizeof(int) * 8; }
+
+// The pass over one cell, up to the deletions that need the swarm's global
+// order: pass-1 growth, the Inflow re-population (triggered cells), the
+// splits and the deletion candidates.  In probe mode a triggered cell stops
+// after the trigger test.  rand() values come from rs at roff (per listed
+// cell).
+__global__ __launch_bounds__(kDvcLanes) void k_pc_cell(PcParams A, PcData D, const int* __restrict__ cells,
+                                                       const long long* __restrict__ roff) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char pc_smem[];
+  const long long e = cells ? cells[blockIdx.x] : (long long)blockIdx.x;
+  const long long o = roff ? roff[blockIdx.x] : 0;
+  const DvcGrid& G = A.G;
+  const int lane = threadIdx.x;
+  const DvcLds L = dvc_carve(G, A.maxcap, pc_smem);
+  double* cent = reinterpret_cast<double*>(pc_smem + dvc_lds_bytes(G, A.maxcap));
+  double* wgt = cent + 3 * A.maxcap;
+  int* sh = reinterpret_cast<int*>(wgt + A.maxcap);
+  int* tb = L.ord;
+  const int b = D.off[e], n0 = D.off[e + 1] - b;
+  const long long so = D.soff[e];
+  if (n0 == 0) {  // an empty cell is left alone
+    if (lane == 0) {
+      D.tcnt[e] = 0;
+      D.nins[e] = D.nspl[e] = D.ninfdel[e] = D.ncand[e] = D.cntdel[e] = D.ndraw[e] = 0;
+      D.flags[e] = 0;
+    }
+    return;
+  }
+  dvc_setup(G, L);
+  for (int j = lane; j < n0; j += kDvcLanes) {
+    const int pj = D.pid[b + j];
+    int rk = 0;
+    for (int k = 0; k < n0; k++) rk += D.pid[b + k] < pj ? 1 : 0;
+    tb[rk] = b + j;
+  }
+  __syncthreads();
+  pc_load(G, L, D, n0);
+  auto regions = [&](int i, double c0, double c1, double c2, int cnt) {
+    cent[3 * i] = c0;
+    cent[3 * i + 1] = c1;
+    cent[3 * i + 2] = c2;
+    wgt[i] = cnt * G.vol;
+  };
+  dvc_grow(G, L, n0);
+  dvc_regions(G, L, n0, regions);
+  __syncthreads();
+  int n = n0, nd = 0, nrec = 0, ninf = 0, flags = 0, nins = 0;
+  bool err = false;
+  auto draw = [&]() -> double {
+    const long long i = o + nd++;
+    if (i >= D.rs_n) {
+      err = true;
+      return 0.0;
+    }
+    return (double)D.rs[i] / 2147483648.0;
+  };
+  if (A.inflow) {
+    double dist = 0.0;
+    for (int d = 0; d < G.dim; d++) {
+      const double t = cent[3 * (n0 - 1) + d] - L.xs[3 * (n0 - 1) + d];
+      dist += t * t;
+    }
+    if ((1.0 * n0) / A.ppc < A.thresh || dist > A.cpr * 2.0) {
+      flags |= kPcTrig;
+      if (A.probe) {
+        if (lane == 0) {
+          D.ndraw[e] = A.ppc + (n0 > 5 ? n0 : 0);
+          D.flags[e] = flags;
+        }
+        return;
+      }
+      if (lane == 0) {
+        for (int i = 0; i < A.ppc; i++) {  // PCDVC.cxx:824-832
+          const int j = (int)(G.nc * draw());
+          const long long r = so + nrec++;
+          D.rsrc[r] = pc_src(D, tb[L.owner[j]]);
+          dvc_centre(G, L, j, D.rxi + 3 * r);
+          tb[n++] = (int)(-1 - r);
+        }
+        nins = A.ppc;
+        if (n - A.ppc > 5) {  // PCDVC.cxx:838-846
+          const int del_num = n - A.ppc;
+          for (int i = 0; i < del_num; i++) {
+            const int j = (int)(n * draw());
+            D.infdel[so + ninf++] = tb[j];
+            tb[j] = tb[n - 1];
+            n--;
+          }
+        }
+        sh[0] = n;
+        sh[1] = nrec;
+        sh[2] = nd;
+        sh[3] = ninf;
+        sh[4] = err ? 1 : 0;
+      }
+      __syncthreads();
+      n = sh[0];
+      nrec = sh[1];
+      nd = sh[2];
+      ninf = sh[3];
+      err = sh[4] != 0;
+      nins = A.ppc;
+      if (A.ppc + ninf > 0) {  // redo the diagram (PCDVC.cxx:876-906)
+        if (n < 3) err = true;  // the reference's under-resolved-cell firewall
+        if (!err) {
+          pc_load(G, L, D, n);
+          dvc_grow(G, L, n);
+          dvc_regions(G, L, n, regions);
+          __syncthreads();
+        }
+      }
+    }
+  }
+  if (lane == 0) {
+    int mS = A.max_splits, mD = A.max_deletions;
+    const int m0 = D.mat[pc_src(D, tb[0])];
+    for (int i = 0; i < n; i++)
+      if (D.mat[pc_src(D, tb[i])] != m0) {
+        if (!A.delete_if) mD = 0;
+        if (!A.split_if) mS = 0;
+        break;
+      }
+    const int npre = n;
+    int ns = 0;
+    for (int q = 0; q < npre && ns < mS && !err; q++) {
+      if (!(wgt[q] > A.max_w)) continue;
+      const long long r = so + nrec++;
+      double* x = D.rxi + 3 * r;
+      if (!A.inflow) {
+        x[0] = cent[3 * q];
+        x[1] = cent[3 * q + 1];
+        x[2] = G.dim == 3 ? cent[3 * q + 2] : 0.0;
+      } else {
+        // a random entry of q's owned sub-cell list, which holds its first
+        // (int)(w / da' + 0.5) sub-cells in descending order (PCDVC.cxx:898-918)
+        const int ls = (int)(wgt[q] * A.one_oda + 0.5);
+        int k = ls - 1 - (int)(ls * draw());
+        int sub = -1;
+        for (int c = 0; c < G.nc && sub < 0; c++)
+          if (L.owner[c] == q) {
+            if (k == 0) sub = c;
+            k--;
+          }
+        if (sub < 0) err = true;
+        else dvc_centre(G, L, sub, x);
+      }
+      D.rsrc[r] = pc_src(D, tb[q]);
+      tb[n++] = (int)(-1 - r);
+      ns++;
+    }
+    int ncd = 0;
+    for (int q = 0; q < npre; q++)
+      if (wgt[q] < A.min_w) D.cand[so + ncd++] = tb[q];
+    if (mD > npre - 4) mD = (G.dim == 2 && A.inflow) ? npre - 4 : npre / 2;
+    int cd = mD < ncd ? mD : ncd;
+    if (cd < 0) cd = 0;
+    for (int i = 0; i < n; i++) D.tbl[so + i] = tb[i];
+    D.tcnt[e] = n;
+    D.nins[e] = nins;
+    D.nspl[e] = ns;
+    D.ninfdel[e] = ninf;
+    D.ncand[e] = ncd;
+    D.cntdel[e] = cd;
+    D.ndraw[e] = nd;
+    if (D.ndraw_list) D.ndraw_list[blockIdx.x] = nd;
+    if (ns > 0 || cd > 0) flags |= kPcFinal;
+    if (A.probe && A.inflow && ns > 0) flags |= kPcRedo;
+    if (err) flags |= kPcErr;
+    D.flags[e] = flags;
+    sh[5] = flags;
+  }
+  __syncthreads();
+  if (!(sh[5] & kPcFinal))
+    for (int i = lane; i < n; i += kDvcLanes) {
+      D.scent[3 * (so + i)] = cent[3 * i];
+      D.scent[3 * (so + i) + 1] = cent[3 * i + 1];
+      D.scent[3 * (so + i) + 2] = cent[3 * i + 2];
+      D.sw[so + i] = wgt[i];
+    }
+}
+
+// The deletions in the order the swarm's global indices give (decided on the
+// host), then the cell's final diagram (PCDVC.cxx:1015-1061).
+__global__ __launch_bounds__(kDvcLanes) void k_pc_final(PcParams A, PcData D, const int* __restrict__ cells,
+                                                        const int* __restrict__ dorder,
+                                                        const long long* __restrict__ doff) {
+#pragma clang fp contract(off)
+  extern __shared__ __align__(16) unsigned char pc_smem[];
+  const long long e = cells[blockIdx.x];
+  const DvcGrid& G = A.G;
+  const int lane = threadIdx.x;
+  const DvcLds L = dvc_carve(G, A.maxcap, pc_smem);
+  int* sh = reinterpret_cast<int*>(pc_smem + dvc_lds_bytes(G, A.maxcap) + sizeof(double) * 4 * A.maxcap);
+  int* tb = L.ord;
+  const long long so = D.soff[e];
+  int n = D.tcnt[e];
+  dvc_setup(G, L);
+  for (int i = lane; i < n; i += kDvcLanes) tb[i] = D.tbl[so + i];
+  __syncthreads();
+  if (lane == 0) {
+    const long long d0 = doff[blockIdx.x];
+    bool err = false;
+    for (int k = 0; k < D.cntdel[e]; k++) {
+      const int ref = dorder[d0 + k];
+      int s = 0;
+      while (s < n && tb[s] != ref) s++;
+      if (s == n) {
+        err = true;
+        break;
+      }
+      tb[s] = tb[n - 1];
+      n--;
+    }
+    if (n < 3) err = true;
+    if (err) D.flags[e] |= kPcErr;
+    sh[0] = n;
+    sh[1] = err ? 1 : 0;
+  }
+  __syncthreads();
+  n = sh[0];
+  if (sh[1]) return;
+  pc_load(G, L, D, n);
+  dvc_grow(G, L, n);
+  dvc_regions(G, L, n, [&](int i, double c0, double c1, double c2, int cnt) {
+    D.scent[3 * (so + i)] = c0;
+    D.scent[3 * (so + i) + 1] = c1;
+    D.scent[3 * (so + i) + 2] = c2;
+    D.sw[so + i] = cnt * G.vol;
+  });
+  for (int i = lane; i < n; i += kDvcLanes) D.tbl[so + i] = tb[i];
+  if (lane == 0) D.tcnt[e] = n;
+}
+
+// The deletion candidates and the Inflow deletions of the listed cells, as
+// caller indices (pid) for existing particles and records (-1 - r) for new
+// ones, packed for the host.
+__global__ void k_pc_gather(long long ncells, const int* __restrict__ cells, const long long* __restrict__ goff,
+                            PcData D, int* __restrict__ out) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= ncells) return;
+  const long long e = cells[k], so = D.soff[e];
+  long long w = goff[k];
+  for (int i = 0; i < D.ninfdel[e]; i++) {
+    const int ref = D.infdel[so + i];
+    out[w++] = ref;
+    out[w++] = ref >= 0 ? D.pid[ref] : ref;
+  }
+  for (int i = 0; i < D.ncand[e]; i++) {
+    const int ref = D.cand[so + i];
+    out[w++] = ref;
+    out[w++] = ref >= 0 ? D.pid[ref] : ref;
+  }
+}
+
+// Vertex coordinate of a regular mesh as vcoord_w; a new particle's global
+// position is 0.5 ((1 - xi) a + (1 + xi) b) per axis.
+__global__ void k_pc_rebuild(Geo g, PcData D, const int* __restrict__ off2, const int* __restrict__ fidx,
+                             const int* __restrict__ nbase, const int* __restrict__ fidx_new,
+                             const double* __restrict__ coord, const int* __restrict__ mat, PartDoubles x,
+                             double* __restrict__ coordN, int* __restrict__ matN, int* __restrict__ pidN,
+                             PartDoubles xN, double4* __restrict__ pkN, double* __restrict__ wN,
+                             unsigned* __restrict__ keysN) {
 #pragma clang fp contract(off)
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= g.E) return;
-  const int b = off[e], n = off[e + 1] - b;
-  for (int j = 0; j < n; j++) del[b + j] = 0;
-  nsplit[e] = 0;
-  if (n == 0) return;
-  // first particle in pid order and whether the cell holds several materials
-  int first = 0;
-  for (int j = 1; j < n; j++)
-    if (pid[b + j] < pid[b + first]) first = j;
-  bool interface = false;
-  for (int j = 0; j < n; j++) interface = interface || mat[b + j] != mat[b + first];
-  int mS = a.max_splits, mD = a.max_deletions;
-  if (interface) {
-    if (!a.delete_if) mD = 0;
-    if (!a.split_if) mS = 0;
-  }
-  if (mD > n - 4) mD = n / 2;
+  const long long so = D.soff[e];
   const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
                             g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
-  int ns = 0, last = -1;
-  for (int t = 0; t < n; t++) {
-    // next particle in pid order
-    int j = -1;
-    for (int k = 0; k < n; k++) {
-      const int pk_ = pid[b + k];
-      if (pk_ > last && (j < 0 || pk_ < pid[b + j])) j = k;
-    }
-    last = pid[b + j];
-    const double wj = w[b + j];
-    if (wj > a.max_w && ns < mS) {
-      const double4 c = pk[b + j];
-      const double xi[3] = {c.x, c.y, c.z};
-      double* out = scoord + ((long long)e * a.slots + ns) * 3;
+  for (int s = 0; s < D.tcnt[e]; s++) {
+    const int ref = D.tbl[so + s];
+    const long long dst = off2[e] + s;
+    const int src = pc_src(D, ref);
+    if (ref >= 0) {
+      for (int d = 0; d < 3; d++) coordN[3 * dst + d] = coord[3LL * ref + d];
+      pidN[dst] = fidx[ref];
+    } else {
+      const long long r = -1 - ref;
+      const double* xi = D.rxi + 3 * r;
       for (int d = 0; d < 3; d++) {
-        double x = 0.0;
+        double v = 0.0;
         if (d < g.dim) {
           const double lo = vcoord_w(g, d, ijk[d]), hi = vcoord_w(g, d, ijk[d] + 1);
-          x = 0.5 * ((1.0 - xi[d]) * lo + (1.0 + xi[d]) * hi);
+          v = 0.5 * ((1.0 - xi[d]) * lo + (1.0 + xi[d]) * hi);
         }
-        out[d] = x;
+        coordN[3 * dst + d] = v;
       }
-      sparent[(long long)e * a.slots + ns] = b + j;
-      ns++;
+      pidN[dst] = fidx_new[nbase[e] + (r - so)];
     }
-  }
-  // deletions: of the particles below minW, the reference sorts the list by
-  // swarm index in descending order and deletes its first maxDeletions
-  // (compare_indexOnCPU, PCDVC.cxx:611-620, :974, :1015-1021), i.e. the
-  // candidates with the highest indices
-  int ncand = 0;
-  for (int j = 0; j < n; j++) ncand += w[b + j] < a.min_w ? 1 : 0;
-  const int nD = mD < ncand ? mD : ncand;
-  if (nD > 0)
-    for (int j = 0; j < n; j++) {
-      if (!(w[b + j] < a.min_w)) continue;
-      int higher = 0;
-      for (int k = 0; k < n; k++) higher += (w[b + k] < a.min_w && pid[b + k] > pid[b + j]) ? 1 : 0;
-      if (higher < nD) del[b + j] = 1;
-    }
-  nsplit[e] = ns;
-}
-
-__global__ void k_pcdvc_compact(long long P, const int* __restrict__ map, const double* __restrict__ coord,
-                                const int* __restrict__ mat, PartDoubles x, double* __restrict__ coordN,
-                                int* __restrict__ matN, PartDoubles xN) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i >= P) return;
-  const int m = map[i];
-  if (m < 0) return;
-  for (int d = 0; d < 3; d++) coordN[3LL * m + d] = coord[3 * i + d];
-  matN[m] = mat[i];
-  for (int k = 0; k < kPartDoubles; k++)
-    if (xN.a[k]) xN.a[k][m] = x.a[k][i];
-}
-
-// A split copies its parent's particle state (Swarm_CopyParticleWithinSwarm
-// in PCDVC.cxx): material, per-particle eta0 and the StrainWeakening values.
-__global__ void k_pcdvc_append(long long E, const int* __restrict__ nsplit, const int* __restrict__ base, int slots,
-                               const double* __restrict__ scoord, const int* __restrict__ sparent,
-                               const int* __restrict__ mat, PartDoubles x, long long n_surv,
-                               double* __restrict__ coordN, int* __restrict__ matN, PartDoubles xN) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= E) return;
-  for (int k = 0; k < nsplit[e]; k++) {
-    const long long np = n_surv + base[e] + k;
-    const long long s = e * slots + k;
-    for (int d = 0; d < 3; d++) coordN[3 * np + d] = scoord[3 * s + d];
-    const int par = sparent[s];
-    matN[np] = mat[par];
+    matN[dst] = mat[src];
     for (int q = 0; q < kPartDoubles; q++)
-      if (xN.a[q]) xN.a[q][np] = x.a[q][par];
+      if (xN.a[q]) xN.a[q][dst] = x.a[q][src];
+    pkN[dst] = make_double4(D.scent[3 * (so + s)], D.scent[3 * (so + s) + 1], D.scent[3 * (so + s) + 2], 0.0);
+    wN[dst] = D.sw[so + s];
+    keysN[dst] = (unsigned)e;
   }
 }
 
 }  // namespace
 
-void pcdvc_decide(const Geo& g, const int* off, const int* pid, const int* mat, const double4* pk, const double* w,
-                  const PcdvcArgs& a, unsigned char* del, int* nsplit, double* scoord, int* sparent, hipStream_t st) {
-  if (g.E <= 0) return;
-  hipLaunchKernelGGL(k_pcdvc_decide, dim3((unsigned)((g.E + 255) / 256)), dim3(256), 0, st, g, off, pid, mat, pk, w,
-                     a, del, nsplit, scoord, sparent);
+size_t pc_lds_bytes(const PcParams& A) { return dvc_lds_bytes(A.G, A.maxcap) + pc_lds_extra(A.maxcap); }
+
+bool pc_grid(int dim, const int* res, PcParams* A) { return make_grid(dim, res, &A->G); }
+
+long long pc_max_capacity(const PcParams& A) {
+  long long lo = 0, hi = 32767;
+  while (lo < hi) {
+    const long long mid = (lo + hi + 1) / 2;
+    if (dvc_lds_bytes(A.G, (int)mid) + pc_lds_extra((int)mid) <= kDvcMaxLds)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
-void pcdvc_rebuild(long long P, long long E, const int* map, const double* coord, const int* mat, PartDoubles x,
-                   const int* nsplit, const int* base, int slots, const double* scoord, const int* sparent,
-                   long long n_surv, double* coordN, int* matN, PartDoubles xN, hipStream_t st) {
-  if (P > 0)
-    hipLaunchKernelGGL(k_pcdvc_compact, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, P, map, coord, mat, x,
-                       coordN, matN, xN);
-  if (E > 0)
-    hipLaunchKernelGGL(k_pcdvc_append, dim3((unsigned)((E + 255) / 256)), dim3(256), 0, st, E, nsplit, base, slots,
-                       scoord, sparent, mat, x, n_surv, coordN, matN, xN);
+void pc_cells(const PcParams& A, const PcData& D, long long ncells, const int* cells, const long long* roff,
+              hipStream_t st) {
+  if (ncells <= 0) return;
+  hipLaunchKernelGGL(k_pc_cell, dim3((unsigned)ncells), dim3(kDvcLanes), pc_lds_bytes(A), st, A, D, cells, roff);
+}
+
+void pc_final(const PcParams& A, const PcData& D, long long ncells, const int* cells, const int* dorder,
+              const long long* doff, hipStream_t st) {
+  if (ncells <= 0) return;
+  hipLaunchKernelGGL(k_pc_final, dim3((unsigned)ncells), dim3(kDvcLanes), pc_lds_bytes(A), st, A, D, cells, dorder,
+                     doff);
+}
+
+void pc_gather(const PcData& D, long long ncells, const int* cells, const long long* goff, int* out,
+               hipStream_t st) {
+  if (ncells <= 0) return;
+  hipLaunchKernelGGL(k_pc_gather, dim3((unsigned)((ncells + 255) / 256)), dim3(256), 0, st, ncells, cells, goff, D,
+                     out);
+}
+
+void pc_rebuild(const Geo& g, const PcData& D, const int* off2, const int* fidx, const int* nbase,
+                const int* fidx_new, const double* coord, const int* mat, PartDoubles x, double* coordN, int* matN,
+                int* pidN, PartDoubles xN, double4* pkN, double* wN, unsigned* keysN, hipStream_t st) {
+  if (g.E <= 0) return;
+  hipLaunchKernelGGL(k_pc_rebuild, dim3((unsigned)((g.E + 255) / 256)), dim3(256), 0, st, g, D, off2, fidx, nbase,
+                     fidx_new, coord, mat, x, coordN, matN, pidN, xN, pkN, wN, keysN);
 }
 
 }  // namespace gs

@@ -48,20 +48,45 @@ int32_t gs_problem_set_weights(gs_problem* pb, int32_t kind, const int32_t* res)
 int32_t gs_dvc_cells(int32_t dim, const int32_t* res, int64_t ncell, const int32_t* off, const double* xi,
                      double* centroid, double* weight);
 
-/* PCDVC population control (PCDVC.cxx:656-1090 / 1092-1512 with Inflow off)
- * followed by DVC weights.  Per cell, on the discrete Voronoi regions of its
- * particles: the first max_splits particles (in cell order) whose region
- * exceeds upper_t percent of the cell volume are split -- a copy of the
- * material point (material, eta0) is added at the centroid of the parent's
- * region -- and the first max_deletions whose region is below lower_t
- * percent are deleted; a cell holding more than one material splits and
- * deletes nothing unless split_in_interface_cells / delete_in_interface_cells;
- * max_deletions is cut to nump / 2 when it exceeds nump - 4.  The particle
- * count changes: afterwards the caller's particle order is the surviving
- * particles in their previous order, then the new ones (cell order, then
- * split order).  The problem is left with DVC weights of resolution res.
- * counts (may be NULL) = {particles added, particles deleted}.  Reference
- * defaults: upperT 25, lowerT 0.6, maxSplits 3, maxDeletions 2, resolution 10. */
+/* One PCDVC pass (PCDVC.cxx:656-1090 in 3D, 1092-1512 in 2D, Inflow
+ * included) followed by the DVC weights it leaves.  Replaces the pass that
+ * WeightsCalculator_CalculateAll runs over the cells in order
+ * (WeightsCalculator.cxx:195-197) with PCDVC as the calculator.
+ *
+ * The reference's swarm is a global particle array plus one table per cell
+ * (built in index order by the mapper); a split appends a copy of the parent
+ * to both, a deletion moves the cell's last table entry into the freed slot
+ * (SwarmClass.cxx:976-990) and the array's last particle into the freed
+ * index (PCDVC.cxx:339-545).  Per cell, on the discrete Voronoi regions of
+ * its particles in table order:
+ *   Inflow (inflow != 0): when the last particle's squared distance to its
+ *   centroid exceeds 2 cent_pos_ratio, or the count over particles_per_cell
+ *   is below threshold, particles_per_cell particles are added at the centres
+ *   of random sub-cells (copies of those sub-cells' owners) and, when the
+ *   original count is above 5, that many are deleted at random table slots;
+ *   the regions are redone.
+ *   Then the first max_splits particles whose region exceeds upper_t percent
+ *   of the cell volume split (at the region's centroid; with Inflow at the
+ *   centre of a random sub-cell of the region) and, of those below lower_t
+ *   percent, the ones with the highest indices are deleted up to
+ *   max_deletions; a cell holding more than one material splits and deletes
+ *   nothing unless split_in_interface_cells / delete_in_interface_cells;
+ *   max_deletions is cut to nump / 2 (nump - 4 in 2D with Inflow) when it
+ *   exceeds nump - 4.  The regions are redone after any change.
+ * Afterwards the caller's particle order is the reference's array order and
+ * the weights are those of the cells' final tables (until the next
+ * relocation, whose tables list particles in index order again).
+ * rand() is the C library's (glibc) stream after srand(rand_seed), entered
+ * after rand_skip calls; a pass that continues the previous pass's stream is
+ * resumed without replaying it.
+ * counts (may be NULL) = {particles added, particles deleted, rand() calls}.
+ * GS_E_INVALID for a cell left with fewer than 3 particles after a change
+ * (the reference's firewall), for a cell too large for the sub-grid's LDS
+ * budget, and with Inflow for sub-grids whose region lists
+ * ((int)(w / da' + 0.5) entries, da' = (int)(1 / da + 0.5)) could exceed the
+ * owned count.  Reference defaults: upperT 25, lowerT 0.6, maxSplits 3,
+ * maxDeletions 2, Inflow off, Threshold 0.8, CentPosRatio 0.01,
+ * particlesPerCell 25, resolution 10. */
 typedef struct gs_pcdvc_params {
   double upper_t;
   double lower_t;
@@ -70,7 +95,13 @@ typedef struct gs_pcdvc_params {
   int32_t split_in_interface_cells;
   int32_t delete_in_interface_cells;
   int32_t res[3];
+  int32_t inflow;
+  int32_t particles_per_cell;
   int32_t pad_;
+  double threshold;
+  double cent_pos_ratio;
+  int64_t rand_seed;
+  int64_t rand_skip;
 } gs_pcdvc_params;
 int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_params* p, int64_t* counts);
 
