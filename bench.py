@@ -63,7 +63,7 @@ def cpu_baseline(res, threads):
                 inner_iterations=int(rep.inner_iterations), E=cfg.E, P=cfg.P)
 
 
-def bench_dvc(G, pb, cfg, sample_cells=2000, res=10, population_control=False):
+def bench_dvc(G, pb, cfg, sample_cells=2000, res=10, population_control=False, ppc=16):
     """Discrete Voronoi cell weights (DVCWeights, resolution 10) of every
     element of the workload: GPU time of a relocation with DVC weights minus
     one with constant weights (the DVC growth kernel), and the oracle's
@@ -87,7 +87,7 @@ def bench_dvc(G, pb, cfg, sample_cells=2000, res=10, population_control=False):
             times.setdefault(kind, []).append(time.perf_counter() - t)
     pb.set_weights(G.WEIGHTS_CONSTANT)
     gpu_ms = 1e3 * (min(times[G.WEIGHTS_DVC]) - min(times[G.WEIGHTS_CONSTANT]))
-    pc = None
+    pc = pci = None
     if population_control:
         # one PCDVC pass with the reference defaults (changes the particle set)
         t = time.perf_counter()
@@ -95,12 +95,33 @@ def bench_dvc(G, pb, cfg, sample_cells=2000, res=10, population_control=False):
         pc = {"ms": 1e3 * (time.perf_counter() - t), "added": added, "deleted": deleted, "particles_after": pb.P,
               "params": "upperT 25, lowerT 0.6, maxSplits 3, maxDeletions 2, resolution 10"}
         print(f"# population control: {pc['ms']:.1f} ms, +{added} -{deleted} particles", file=sys.stderr, flush=True)
+        # then a pass with the Inflow branch on (defaults: Threshold 0.8,
+        # CentPosRatio 0.01, particlesPerCell = the layout's), and the oracle
+        # on a 16^3 field of the same generator for a per-cell CPU rate
+        t = time.perf_counter()
+        added, deleted = pb.population_control(G.pcdvc_params(resolution=res, Inflow=True,
+                                                              particlesPerCell=ppc))
+        pci = {"ms": 1e3 * (time.perf_counter() - t), "added": added, "deleted": deleted, "particles_after": pb.P,
+               "rand_calls": pb.rand_draws, "repopulated_cells": pb.repopulated_cells,
+               "repopulation_launches": pb.repopulation_launches,
+               "params": f"defaults + Inflow, particlesPerCell {ppc}"}
+        from gale_amd import configs as _configs
+        sc = _configs.random_viscosity_3d(res=16, ppc=ppc)
+        o = O.Problem(**sc.oracle_kwargs())
+        t = time.perf_counter()
+        o.population_control(Inflow=True, particlesPerCell=ppc)
+        cpu_cell_ms = 1e3 * (time.perf_counter() - t) / sc.E
+        pci.update(cpu_oracle_ms_per_cell=cpu_cell_ms, cpu_oracle_ms_projected=cpu_cell_ms * cfg.E,
+                   cpu_sample="oracle, 16^3 field of the same generator, 1 core")
+        print(f"# population control with Inflow: {pci['ms']:.1f} ms, +{added} -{deleted} particles, "
+              f"{pci['repopulated_cells']} re-populated cells in {pci['repopulation_launches']} launches; "
+              f"oracle {cpu_cell_ms * cfg.E:.0f} ms projected", file=sys.stderr, flush=True)
     cpu_ms_all = 1e3 * cpu_s / len(cells) * cfg.E
     out = {"elements": cfg.E, "particles": cfg.P, "resolution": res, "gpu_ms": gpu_ms,
            "gpu_elements_per_s": cfg.E / (gpu_ms * 1e-3) if gpu_ms > 0 else None,
            "relocate_constant_ms": 1e3 * min(times[G.WEIGHTS_CONSTANT]),
            "cpu_oracle_ms_projected": cpu_ms_all, "cpu_sample_elements": int(len(cells)), "cpu_cores": 1,
-           "population_control": pc}
+           "population_control": pc, "population_control_inflow": pci}
     print(f"# dvc weights: GPU {gpu_ms:.2f} ms for {cfg.E} elements, oracle (1 core) {cpu_ms_all:.0f} ms projected",
           file=sys.stderr, flush=True)
     return out
@@ -338,7 +359,7 @@ def main():
     print(f"# generated {cfg.name} res={args.res} E={cfg.E} P={cfg.P} in {t_gen:.1f}s", file=sys.stderr, flush=True)
     pb = G.Problem(**cfg.problem_kwargs())
     if args.only_dvc:
-        print(json.dumps({"dvc_weights": bench_dvc(G, pb, cfg, population_control=True)}), flush=True)
+        print(json.dumps({"dvc_weights": bench_dvc(G, pb, cfg, population_control=True, ppc=args.ppc)}), flush=True)
         return
     params = G.uzawa_params(tolerance=1e-6, maxIterations=args.plain_max_outer or args.max_outer,
                             inner_max_iterations=20000, mg_smooth=args.mg_smooth, monitor=500)

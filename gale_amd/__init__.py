@@ -641,11 +641,12 @@ class Problem:
         deleted particle's index taken by the array's last particle, new
         particles appended)."""
         p = params if params is not None else pcdvc_params()
-        counts = np.zeros(3, np.int64)
+        counts = np.zeros(5, np.int64)
         st = lib().gs_problem_population_control(self._h, C.byref(p), _ptr(counts))
         _check(st, "population_control")
         self.P = self.info()["P"]
         self.rand_draws = int(counts[2])
+        self.repopulated_cells, self.repopulation_launches = int(counts[3]), int(counts[4])
         return int(counts[0]), int(counts[1])
 
     def set_weights(self, kind=WEIGHTS_DVC, resolution=10):

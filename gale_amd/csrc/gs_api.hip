@@ -751,6 +751,7 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   D.rs_n = inflow ? bound_draws : 0;
   D.ndraw_list = nullptr;
   std::vector<int> hc(7 * E);
+  long long n_repop = 0, n_windows = 0;
   auto fetch_counts = [&]() -> hipError_t {
     hipError_t e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = hipMemcpy(hc.data(), cnts, sizeof(int) * 7 * E, hipMemcpyDeviceToHost);
@@ -783,6 +784,7 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
       PCK(hipMemcpyAsync(lcells, trig.data(), sizeof(int) * T, hipMemcpyHostToDevice, st));
     }
     long long pos = 0, delta = 0, win = 256;
+    n_repop = T;
     std::vector<long long> ro;
     std::vector<int> got;
     while (pos < T) {
@@ -793,6 +795,7 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
       D.ndraw_list = dlist;
       pc_cells(A, D, w, lcells + pos, lroff, st);
       D.ndraw_list = nullptr;
+      n_windows++;
       got.resize(w);
       PCK(hipMemcpyAsync(got.data(), dlist, sizeof(int) * w, hipMemcpyDeviceToHost, st));
       PCK(hipStreamSynchronize(st));
@@ -1044,6 +1047,8 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
     counts[0] = added;
     counts[1] = deleted;
     counts[2] = draws;
+    counts[3] = n_repop;
+    counts[4] = n_windows;
   }
   return GS_OK;
 }

@@ -79,7 +79,8 @@ int32_t gs_dvc_cells(int32_t dim, const int32_t* res, int64_t ncell, const int32
  * rand() is the C library's (glibc) stream after srand(rand_seed), entered
  * after rand_skip calls; a pass that continues the previous pass's stream is
  * resumed without replaying it.
- * counts (may be NULL) = {particles added, particles deleted, rand() calls}.
+ * counts (may be NULL) = {particles added, particles deleted, rand() calls,
+ * re-populated cells, device launches over re-populated cells}.
  * GS_E_INVALID for a cell left with fewer than 3 particles after a change
  * (the reference's firewall), for a cell too large for the sub-grid's LDS
  * budget, and with Inflow for sub-grids whose region lists
