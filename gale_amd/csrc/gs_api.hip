@@ -268,7 +268,7 @@ static int32_t relocate(gs_problem* pb) {
     std::swap(pb->pinc, pb->pinc2);
   }
   launch_offsets(reinterpret_cast<const int*>(pb->keys), P, pb->g.E, pb->off, st);
-  launch_remap(pb->g, pb->coord, pb->off, pb->pk, pb->g.E, st);
+  launch_remap(pb->g, pb->coord, pb->keys, pb->pk, P, st);
   pb->pk32_valid = false;
   CK(hipMemcpyAsync(pb->hS, pb->S, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, st));
   CK(hipStreamSynchronize(st));
@@ -667,7 +667,7 @@ extern "C" int32_t gs_problem_population_control(gs_problem* pb, const gs_pcdvc_
   const long long S = soff[E];
   if (maxcap > pc_max_capacity(A) || S >= 0x7fffffffLL) return GS_E_INVALID;
   A.maxcap = (int)maxcap;
-  launch_remap(pb->g, pb->coord, pb->off, pb->pk, E, st);  // local coordinates of the particles
+  launch_remap(pb->g, pb->coord, pb->keys, pb->pk, P, st);  // local coordinates of the particles
 
   // device buffers
   long long* d_soff = nullptr;

@@ -145,7 +145,7 @@ void launch_uz_uupdate(double* u, const double* v, long long n, const double* S,
 void launch_locate(const Geo& g, const double* coord, long long P, int* keys, int* vals, double* S,
                    hipStream_t st);
 void launch_offsets(const int* keys_sorted, long long P, long long E, int* off, hipStream_t st);
-void launch_remap(const Geo& g, const double* coord, const int* off, double4* pk, long long E,
+void launch_remap(const Geo& g, const double* coord, const unsigned* owner, double4* pk, long long P,
                   hipStream_t st);
 void launch_advect_rk2(const Geo& g, double* coord, long long P, const double* u, double dt, double* S,
                        hipStream_t st);

@@ -962,48 +962,43 @@ void launch_offsets(const int* keys_sorted, long long P, long long E, int* off, 
 // Newton-Raphson iteration (ElementType.cxx:221-333): from the element centre,
 // J dxi = x - sum_n N_n(xi) x_n with J = sum_n dN_n/dxi x_n, solved by
 // Cramer's rule, until max |dxi| < 1e-4 (at most 100 iterations).  The
-// operations follow the oracle's restatement one for one (no contraction),
-// so both give the same bits; particles placed exactly on the sub-grid of
-// the Voronoi weights (PCDVC's Inflow) make ties that depend on them.
+// operations follow the oracle's restatement one for one (nodes in order, no
+// contraction), so both give the same bits; particles placed exactly on the
+// sub-grid of the Voronoi weights (PCDVC's Inflow) make ties that depend on
+// them.  Node n of the box element sits at (lo or hi) per axis by bit d of n.
 __device__ void newton_local(const Geo& g, const long long* ijk, const double* x, double* xi) {
 #pragma clang fp contract(off)
   const int dim = g.dim, nen = dim == 2 ? 4 : 8;
-  double xn[8][3];
-  for (int a = 0; a < nen; a++)
-    for (int d = 0; d < 3; d++) xn[a][d] = d < dim ? vcoord(g, d, ijk[d] + ((a >> d) & 1)) : 0.0;
+  const double c = dim == 2 ? 0.25 : 0.125;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int d = 0; d < dim; d++) {
+    lo[d] = vcoord(g, d, ijk[d]);
+    hi[d] = vcoord(g, d, ijk[d] + 1);
+  }
   for (int d = 0; d < 3; d++) xi[d] = 0.0;
   for (int it = 0; it < 100; it++) {
-    double N[8], GNi[3][8], J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rhs[3] = {0, 0, 0}, inc[3] = {0, 0, 0};
-    if (dim == 2) {
-      N[0] = 0.25 * (1.0 - xi[0]) * (1.0 - xi[1]);
-      N[1] = 0.25 * (1.0 + xi[0]) * (1.0 - xi[1]);
-      N[2] = 0.25 * (1.0 - xi[0]) * (1.0 + xi[1]);
-      N[3] = 0.25 * (1.0 + xi[0]) * (1.0 + xi[1]);
-    } else {
-      for (int a = 0; a < 8; a++) {
-        const double sx = (a & 1) ? 1.0 : -1.0, sy = (a & 2) ? 1.0 : -1.0, sz = (a & 4) ? 1.0 : -1.0;
-        N[a] = 0.125 * (1.0 + sx * xi[0]) * (1.0 + sy * xi[1]) * (1.0 + sz * xi[2]);
-      }
-    }
-    const double c = dim == 2 ? 0.25 : 0.125;
+    double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, rhs[3] = {0, 0, 0}, inc[3] = {0, 0, 0};
     for (int a = 0; a < nen; a++) {
-      double s[3], f[3];
+      double s[3], f[3], xn[3];
       for (int d = 0; d < dim; d++) {
         s[d] = ((a >> d) & 1) ? 1.0 : -1.0;
         f[d] = 1.0 + s[d] * xi[d];
+        xn[d] = ((a >> d) & 1) ? hi[d] : lo[d];
       }
+      double N = c;
+      for (int d = 0; d < dim; d++) N *= f[d];
+      double gn[3];
       for (int d = 0; d < dim; d++) {
         double v = c * s[d];
         for (int d2 = 0; d2 < dim; d2++)
           if (d2 != d) v *= f[d2];
-        GNi[d][a] = v;
+        gn[d] = v;
+      }
+      for (int i = 0; i < dim; i++) {
+        for (int j = 0; j < dim; j++) J[i][j] += gn[j] * xn[i];
+        rhs[i] -= N * xn[i];
       }
     }
-    for (int a = 0; a < nen; a++)
-      for (int i = 0; i < dim; i++) {
-        for (int j = 0; j < dim; j++) J[i][j] += GNi[j][a] * xn[a][i];
-        rhs[i] -= N[a] * xn[a][i];
-      }
     for (int i = 0; i < dim; i++) rhs[i] += x[i];
     if (dim == 2) {
       const double D = J[0][0] * J[1][1] - J[0][1] * J[1][0];
@@ -1031,21 +1026,24 @@ __device__ void newton_local(const Geo& g, const long long* ijk, const double* x
   }
 }
 
-__global__ void k_remap(Geo g, const double* __restrict__ coord, const int* __restrict__ off,
-                        double4* __restrict__ pk, long long E) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= E) return;
+// One thread per (sorted) particle; particles outside every element (owner
+// E) keep their previous values.
+__global__ void k_remap(Geo g, const double* __restrict__ coord, const unsigned* __restrict__ owner,
+                        double4* __restrict__ pk, long long P) {
+  const long long p = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const long long e = owner[p];
+  if (e >= g.E) return;
   const long long ijk[3] = {e % g.res[0], (e / g.res[0]) % g.res[1],
                             g.dim == 3 ? e / ((long long)g.res[0] * g.res[1]) : 0};
-  for (int p = off[e]; p < off[e + 1]; p++) {
-    const double x[3] = {coord[3 * (long long)p], coord[3 * (long long)p + 1], coord[3 * (long long)p + 2]};
-    double xi[3];
-    newton_local(g, ijk, x, xi);
-    pk[p] = make_double4(xi[0], xi[1], g.dim == 3 ? xi[2] : 0.0, 0.0);
-  }
+  const double x[3] = {coord[3 * p], coord[3 * p + 1], coord[3 * p + 2]};
+  double xi[3];
+  newton_local(g, ijk, x, xi);
+  pk[p] = make_double4(xi[0], xi[1], g.dim == 3 ? xi[2] : 0.0, 0.0);
 }
-void launch_remap(const Geo& g, const double* coord, const int* off, double4* pk, long long E, hipStream_t st) {
-  hipLaunchKernelGGL(k_remap, dim3(grid_for(E)), dim3(kBlock), 0, st, g, coord, off, pk, E);
+void launch_remap(const Geo& g, const double* coord, const unsigned* owner, double4* pk, long long P, hipStream_t st) {
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_remap, dim3(grid_for(P)), dim3(kBlock), 0, st, g, coord, owner, pk, P);
 }
 
 // Velocity at a global point (FeVariable_InterpolateValueAt): 0 if outside.
