@@ -1782,8 +1782,8 @@ static double dot_host(gs_solver* s, const double* a, const double* b, long long
 // stagnation (residual down by less than 5% over the window) the iteration
 // restarts, and after learned_restarts restarts the rest of the solve uses Q.
 // With inner_relax_max > 0 the inner solves are relaxed as the outer
-// residual falls (inner rtol = min(inner_relax_max, irt |r_0| / |r_k|), the
-// inexact-Krylov relaxation rule); the recursive residual then drifts from
+// residual falls (inner rtol = min(inner_relax_max, gain irt |r_0| / |r_k|),
+// the inexact-Krylov relaxation rule); the recursive residual then drifts from
 // the true one, so convergence is re-checked with an accurate solve of
 // u = K^-1 (f - G p), and the loop resumes from the true residual if needed.
 // Stopping test, residual definition and firewalls are the reference's.
@@ -1798,6 +1798,7 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   if (m > kMaxWindow) m = kMaxWindow;
   const int win = s->up.stagnation_window > 0 ? s->up.stagnation_window : 25;
   const double relax = s->up.inner_relax_max > 0 ? s->up.inner_relax_max : 0.0;
+  const double gain = s->up.inner_relax_gain > 0 ? s->up.inner_relax_gain : 1.0;
   if (s->fl_m != m) {
     dfree(s->fl_s);
     dfree(s->fl_q);
@@ -1853,7 +1854,7 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
       launch_orthogonalise(s->z, s->fl_s, s->fl_q, sqv, c, slot, np, partials, beta, st);
       launch_apply_G(pb->g, sk, s->fT, pb->bcflag, st);
       double irt_k = irt;
-      if (relax > 0 && relRes > 0) irt_k = std::min(relax, std::max(irt, irt * relax_ref / relRes));
+      if (relax > 0 && relRes > 0) irt_k = std::min(relax, std::max(irt, gain * irt * relax_ref / relRes));
       CKS(inner_solve(s, s->fT, s->vstar, irt_k, &inner));
       launch_apply_Gt(pb->g, s->vstar, qk, st);
       if (recording && s->rec_every > 0 && it % s->rec_every == 0)
