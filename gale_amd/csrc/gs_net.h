@@ -83,7 +83,8 @@ struct Net {
   float* g = nullptr;     // gradients
   float* m = nullptr;     // Adam first moment
   float* v = nullptr;     // Adam second moment
-  void* wp[kLayers] = {};   // packed BF16 forward weights
+  void* wp[kLayers] = {};   // packed BF16 forward weights (16-wide K steps)
+  void* wp32[kLayers] = {}; // the same for 32-wide K steps (layers with cin a multiple of 8; null otherwise)
   void* wt[kLayers] = {};   // packed BF16 transposed weights (data gradient)
   long long adam_t = 0;
   Work work;

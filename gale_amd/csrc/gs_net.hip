@@ -2,21 +2,25 @@
 // preconditioner network (forward, data gradient, weight gradient), its
 // pooling / upsampling / loss kernels and the Adam optimiser.
 //
-// Convolutions are implicit GEMMs on v_mfma_f32_16x16x16_bf16: a wavefront
+// Convolutions are implicit GEMMs on the BF16 matrix cores: a wavefront
 // computes 16 consecutive voxels (flat index over batch x grid) times all
 // output channels; the K dimension is the flattened (tap, input channel)
-// index kk = tap*Cin + ci, consumed 16 at a time.  Operand layout of the
-// instruction (CDNA3/CDNA4 ISA): lane l holds A[l%16][4(l/16)+j] and
-// B[4(l/16)+j][l%16] for j = 0..3, and receives D[4(l/16)+i][l%16], i = 0..3.
-// Weights are pre-packed into exactly that per-lane B order, so a lane's B
-// operand is one 8-byte load; with Cin a multiple of 4 a lane's four A values
-// are four consecutive channels of one neighbour voxel (one 8-byte load).
+// index kk = tap*Cin + ci, consumed KW at a time -- 32 with gfx950's
+// v_mfma_f32_16x16x32_bf16 wherever Cin is a multiple of 8 (the forward
+// passes), else 16 with v_mfma_f32_16x16x16_bf16.  Lane l holds the KW/4 K
+// values KW/4 (l/16) .. of a step for row l%16 of A and column l%16 of B, and
+// receives D[4(l/16)+i][l%16], i = 0..3 (a product over K does not depend on
+// the order in which A and B share K).  Weights are pre-packed into exactly
+// that per-lane B order, so a lane's B operand is one 8- or 16-byte load;
+// with Cin a multiple of KW/4 a lane's A values are consecutive channels of
+// one neighbour voxel (one load of the same width).
 #include "gs_net.h"
 
 #include <math.h>
 #include <string.h>
 
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "../../include/gale_stokes.h"
@@ -26,7 +30,26 @@ namespace gs {
 namespace net {
 
 typedef short s4 __attribute__((ext_vector_type(4)));
+typedef short s8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+
+// One K step of the implicit GEMM (16 or 32 K values) on the matrix cores.
+__device__ __forceinline__ f4 mfma_bf16(const s4& a, const s4& b, const f4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma_bf16(const s8& a, const s8& b, const f4& c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8v, a), __builtin_bit_cast(bf8v, b), c, 0, 0,
+                                                 0);
+}
+template <int KW>
+struct KOp {
+  typedef s4 type;
+};
+template <>
+struct KOp<32> {
+  typedef s8 type;
+};
 
 static LayerSpec make_layer(int l) {
   // encoder / bottom
@@ -63,6 +86,7 @@ long long bias_offset(int l) { return weight_offset(l) + 27LL * kSpec[l].cin * k
 long long num_params() { return weight_offset(kLayers); }
 
 static inline int kchunks(int cin) { return (27 * cin + 15) / 16; }
+static inline int kchunks32(int cin) { return (27 * cin + 31) / 32; }
 static inline int ntiles(int cout) { return (cout + 15) / 16; }
 
 #define NCK(x)                                                                                        \
@@ -101,9 +125,10 @@ __device__ __forceinline__ void vox_xyz(long long vox, long long V, int nx, int 
 // Convolution (forward and data gradient share this kernel)
 // ---------------------------------------------------------------------------
 // out[vox][co] = act(bias[co] + sum_kk in[nbr(vox, tap(kk))][ci(kk)] * W[kk][co])
-// Wp: packed [KC][NT][64 lanes] of 4 BF16.  Grid: ceil(BV/64) blocks x 256.
-template <int CIN, int NT, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, const s4* __restrict__ Wp,
+// Wp: packed [KC][NT][64 lanes] of KW/4 BF16.  Grid: ceil(BV/64) blocks x 256.
+template <int CIN, int NT, bool OUT_F32, int KW>
+__global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in,
+                                              const typename KOp<KW>::type* __restrict__ Wp,
                                               const float* __restrict__ bias, void* __restrict__ out,
                                               int cout_store, int nx, int ny, int nz, long long BV, int relu) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -120,37 +145,37 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, c
   f4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  typedef typename KOp<KW>::type AT;
   constexpr int KT = 27 * CIN;
-  constexpr int KC = (KT + 15) / 16;
+  constexpr int KC = (KT + KW - 1) / KW;
   // K steps in groups of U: all operand loads of a group are issued before
   // its MFMAs, so a wave keeps U x (1 + NT) loads in flight instead of
   // waiting on one load per step.
   constexpr int U = KC < 9 ? KC : 9;
   for (int kc0 = 0; kc0 < KC; kc0 += U) {
-    s4 a[U];
-    s4 w[U][NT];
+    AT a[U];
+    AT w[U][NT];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int kc = kc0 + u;
-      const int kk = kc * 16 + kg * 4;
-      a[u] = s4{0, 0, 0, 0};
+      const int kk = kc * KW + kg * (KW / 4);
+      a[u] = AT{};
       if (valid && kc < KC && kk < KT) {
         const int tap = kk / CIN;
         const int ci = kk - tap * CIN;
         const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
         if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
           const long long nb = bbase + xx + (long long)nx * (yy + (long long)ny * zz);
-          a[u] = *reinterpret_cast<const s4*>(in + nb * CIN + ci);
+          a[u] = *reinterpret_cast<const AT*>(in + nb * CIN + ci);
         }
       }
 #pragma unroll
-      for (int t = 0; t < NT; t++) w[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : s4{0, 0, 0, 0};
+      for (int t = 0; t < NT; t++) w[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : AT{};
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], w[u][t], acc[t], 0, 0, 0);
+      for (int t = 0; t < NT; t++) acc[t] = mfma_bf16(a[u], w[u][t], acc[t]);
   }
 #pragma unroll
   for (int t = 0; t < NT; t++) {
@@ -179,8 +204,9 @@ __global__ __launch_bounds__(256) void k_conv(const uint16_t* __restrict__ in, c
 // steps in sequence: S waves share one 16-voxel tile, wave w taking K steps
 // in groups of four interleaved with the others, and their partial sums are
 // added in wave order through LDS.
-template <int CIN, int NT, bool OUT_F32, int S>
-__global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restrict__ in, const s4* __restrict__ Wp,
+template <int CIN, int NT, bool OUT_F32, int S, int KW>
+__global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restrict__ in,
+                                                        const typename KOp<KW>::type* __restrict__ Wp,
                                                         const float* __restrict__ bias, void* __restrict__ out,
                                                         int cout_store, int nx, int ny, int nz, long long BV,
                                                         int relu) {
@@ -197,34 +223,34 @@ __global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restri
   f4 acc[NT];
 #pragma unroll
   for (int t = 0; t < NT; t++) acc[t] = f4{0.f, 0.f, 0.f, 0.f};
+  typedef typename KOp<KW>::type AT;
   constexpr int KT = 27 * CIN;
-  constexpr int KC = (KT + 15) / 16;
+  constexpr int KC = (KT + KW - 1) / KW;
   constexpr int U = 4;
   for (int kc0 = w * U; kc0 < KC; kc0 += S * U) {
-    s4 a[U];
-    s4 wv[U][NT];
+    AT a[U];
+    AT wv[U][NT];
 #pragma unroll
     for (int u = 0; u < U; u++) {
       const int kc = kc0 + u;
-      const int kk = kc * 16 + kg * 4;
-      a[u] = s4{0, 0, 0, 0};
+      const int kk = kc * KW + kg * (KW / 4);
+      a[u] = AT{};
       if (valid && kc < KC && kk < KT) {
         const int tap = kk / CIN;
         const int ci = kk - tap * CIN;
         const int xx = x + tap % 3 - 1, yy = y + (tap / 3) % 3 - 1, zz = z + tap / 9 - 1;
         if (xx >= 0 && xx < nx && yy >= 0 && yy < ny && zz >= 0 && zz < nz) {
           const long long nb = bbase + xx + (long long)nx * (yy + (long long)ny * zz);
-          a[u] = *reinterpret_cast<const s4*>(in + nb * CIN + ci);
+          a[u] = *reinterpret_cast<const AT*>(in + nb * CIN + ci);
         }
       }
 #pragma unroll
-      for (int t = 0; t < NT; t++) wv[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : s4{0, 0, 0, 0};
+      for (int t = 0; t < NT; t++) wv[u][t] = kc < KC ? Wp[(kc * NT + t) * 64 + lane] : AT{};
     }
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int t = 0; t < NT; t++)
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[u], wv[u][t], acc[t], 0, 0, 0);
+      for (int t = 0; t < NT; t++) acc[t] = mfma_bf16(a[u], wv[u][t], acc[t]);
   }
   __shared__ f4 red[S][NT][64];
 #pragma unroll
@@ -258,15 +284,17 @@ __global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restri
 // each wave takes the four x-rows of one z-slice, so every input voxel is
 // read from memory once per tile instead of once per tap.  The K steps and
 // their order per output are those of k_conv (identical results).
-template <int CIN, int NT, bool OUT_F32>
-__global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__ in, const s4* __restrict__ Wp,
+template <int CIN, int NT, bool OUT_F32, int KW>
+__global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__ in,
+                                                    const typename KOp<KW>::type* __restrict__ Wp,
                                                     const float* __restrict__ bias, void* __restrict__ out,
                                                     int cout_store, int nx, int ny, int nz, int relu) {
+  typedef typename KOp<KW>::type AT;
   constexpr int BX = 18, BY = 6, BZ = 6, NBOX = BX * BY * BZ;
-  constexpr int CG = CIN / 16;   // K steps per tap
-  constexpr int KC = 27 * CG;
-  constexpr int Q = CIN / 4;     // s4 chunks per voxel
-  __shared__ s4 box[NBOX * Q];
+  constexpr int KT = 27 * CIN;
+  constexpr int KC = (KT + KW - 1) / KW;  // K steps (with KW = 32 and CIN = 16 a step spans two taps)
+  constexpr int Q = CIN / 4;              // s4 chunks per voxel
+  __shared__ __align__(16) s4 box[NBOX * Q];
   const int tx = nx / 16, ty = ny / 4, tz = nz / 4;
   long long t = blockIdx.x;
   const int bx = (int)(t % tx);
@@ -297,21 +325,23 @@ __global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__
     for (int u = 0; u < NT; u++) acc[j][u] = f4{0.f, 0.f, 0.f, 0.f};
   constexpr int U = 9;
   for (int kc0 = 0; kc0 < KC; kc0 += U) {
-    s4 wv[U][NT];
+    AT wv[U][NT];
 #pragma unroll
     for (int u = 0; u < U; u++)
 #pragma unroll
-      for (int v = 0; v < NT; v++) wv[u][v] = Wp[((kc0 + u) * NT + v) * 64 + lane];
+      for (int v = 0; v < NT; v++) wv[u][v] = kc0 + u < KC ? Wp[((kc0 + u) * NT + v) * 64 + lane] : AT{};
 #pragma unroll
     for (int u = 0; u < U; u++) {
-      const int kc = kc0 + u;
-      const int tap = kc / CG, cg = kc - tap * CG;
+      const int kk = (kc0 + u) * KW + kg * (KW / 4);  // this lane's first K index of the step
+      const bool in_k = kc0 + u < KC && kk < KT;
+      const int tap = kk / CIN, ci = kk - tap * CIN;
       const int dx = tap % 3, dy = (tap / 3) % 3, dz = tap / 9;
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const s4 a = box[(((w + dz) * BY + (j + dy)) * BX + (row + dx)) * Q + cg * 4 + kg];
+        const AT a = in_k ? *reinterpret_cast<const AT*>(&box[(((w + dz) * BY + (j + dy)) * BX + (row + dx)) * Q + ci / 4])
+                          : AT{};
 #pragma unroll
-        for (int v = 0; v < NT; v++) acc[j][v] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, wv[u][v], acc[j][v], 0, 0, 0);
+        for (int v = 0; v < NT; v++) acc[j][v] = mfma_bf16(a, wv[u][v], acc[j][v]);
       }
     }
   }
@@ -448,6 +478,33 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int trans
         if (ci < cout && col < cin) w = W[(long long)((26 - tap) * cin + col) * cout + ci];
       }
       v[j] = (short)f2bf(w);
+    }
+    out[i] = v;
+  }
+}
+
+// Forward weights for the 32-wide K step: lane l of chunk c, tile t holds
+// B[c*32 + 8(l/16) + j][t*16 + l%16], j = 0..7.  The 32-wide step halves the
+// matrix-core instructions and operand loads (measured at 96^3: the 48^3
+// convolutions 37.8 -> 22.6 us, the 96^3 ones 37.3 -> 36.1 us, the network
+// 0.48 -> about 0.37 ms per application).  It regroups the FP32 partial sums,
+// which flips some of the BF16 roundings of the activations between layers:
+// on a 48^3 input the two widths differ by 3.5e-3 relative, as far as either
+// is from the numpy oracle.  The learned solve's
+// iteration count moves by up to 30 % either way with such changes, and over
+// five 96^3 fields the two widths total 10.02 s (32) and 9.76 s (16), so the
+// 16-wide step is the default and GALE_LPC_K32=1 selects this one.
+__global__ void k_pack32(const float* __restrict__ W, int cin, int cout, int nt, int kc, s8* __restrict__ out) {
+  const long long n = (long long)kc * nt * 64;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const int lane = (int)(i % 64);
+    const long long r = i / 64;
+    const int t = (int)(r % nt), c = (int)(r / nt);
+    const int col = t * 16 + (lane & 15);
+    s8 v = s8{};
+    for (int j = 0; j < 8; j++) {
+      const int kk = c * 32 + (lane >> 4) * 8 + j;
+      if (kk < 27 * cin && col < cout) v[j] = (short)f2bf(W[(long long)kk * cout + col]);
     }
     out[i] = v;
   }
@@ -717,6 +774,11 @@ int create(Net* n, uint64_t seed, hipStream_t st) {
   for (int l = 0; l < kLayers; l++) {
     const LayerSpec& L = kSpec[l];
     NCK(hipMalloc(&n->wp[l], sizeof(s4) * 64LL * kchunks(L.cin) * ntiles(L.cout)));
+    // GALE_LPC_K32=1 runs the forward passes of layers with cin a multiple
+    // of 8 on the 32-wide step (see the note at k_pack32)
+    const char* k32 = std::getenv("GALE_LPC_K32");
+    if (L.cin % 8 == 0 && k32 && k32[0] == '1')
+      NCK(hipMalloc(&n->wp32[l], sizeof(s8) * 64LL * kchunks32(L.cin) * ntiles(L.cout)));
     const int cin_t = L.cout < 4 ? 4 : L.cout;  // transposed GEMM input channels (gradient tensor)
     NCK(hipMalloc(&n->wt[l], sizeof(s4) * 64LL * kchunks(cin_t) * ntiles(L.cin)));
   }
@@ -732,6 +794,9 @@ void pack(Net* n) {
     const int kc = kchunks(L.cin), nt = ntiles(L.cout);
     hipLaunchKernelGGL(k_pack, dim3(gridn(64LL * kc * nt)), dim3(kBlock), 0, n->st, W, L.cin, L.cout, 0, L.cin, nt,
                        kc, (s4*)n->wp[l]);
+    if (n->wp32[l])
+      hipLaunchKernelGGL(k_pack32, dim3(gridn(64LL * kchunks32(L.cin) * nt)), dim3(kBlock), 0, n->st, W, L.cin,
+                         L.cout, nt, kchunks32(L.cin), (s8*)n->wp32[l]);
     const int cin_t = L.cout < 4 ? 4 : L.cout;
     const int kct = kchunks(cin_t), ntt = ntiles(L.cin);
     hipLaunchKernelGGL(k_pack, dim3(gridn(64LL * kct * ntt)), dim3(kBlock), 0, n->st, W, L.cin, L.cout, 1, cin_t,
@@ -772,6 +837,7 @@ void destroy(Net* n) {
   f(n->v);
   for (int l = 0; l < kLayers; l++) {
     f(n->wp[l]);
+    f(n->wp32[l]);
     f(n->wt[l]);
   }
   *n = Net{};
@@ -828,20 +894,27 @@ int ensure_work(Net* n, const Shape& sh) {
 
 #define CONV_TILED_CASES(X) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3)
 
-static int launch_conv(int cin, int nt, bool f32, const uint16_t* in, const void* Wp, const float* bias, void* out,
-                       int cout_store, int nx, int ny, int nz, long long BV, int relu, hipStream_t st) {
+// KW: 16 or 32 K values per matrix-core step (32 needs cin a multiple of 8
+// and weights packed for it).
+template <int KW>
+static int launch_conv_kw(int cin, int nt, bool f32, const uint16_t* in, const void* Wp, const float* bias,
+                          void* out, int cout_store, int nx, int ny, int nz, long long BV, int relu, hipStream_t st) {
+  typedef typename KOp<KW>::type AT;
+  const AT* W = static_cast<const AT*>(Wp);
   if (nx % 16 == 0 && ny % 4 == 0 && nz % 4 == 0) {
     const long long V = (long long)nx * ny * nz;
     const dim3 tgrid((unsigned)(BV / V * (V / 256))), tblock(256);
-#define X(C, N)                                                                                                   \
-  if (cin == C && nt == N) {                                                                                      \
-    if (f32)                                                                                                      \
-      hipLaunchKernelGGL((k_conv_tiled<C, N, true>), tgrid, tblock, 0, st, in, (const s4*)Wp, bias, out,          \
-                         cout_store, nx, ny, nz, relu);                                                           \
-    else                                                                                                          \
-      hipLaunchKernelGGL((k_conv_tiled<C, N, false>), tgrid, tblock, 0, st, in, (const s4*)Wp, bias, out,         \
-                         cout_store, nx, ny, nz, relu);                                                           \
-    return GS_OK;                                                                                                 \
+#define X(C, N)                                                                                                  \
+  if constexpr (KW == 16 || (C) % 8 == 0) {                                                                      \
+    if (cin == C && nt == N) {                                                                                   \
+      if (f32)                                                                                                   \
+        hipLaunchKernelGGL((k_conv_tiled<C, N, true, KW>), tgrid, tblock, 0, st, in, W, bias, out, cout_store,   \
+                           nx, ny, nz, relu);                                                                    \
+      else                                                                                                       \
+        hipLaunchKernelGGL((k_conv_tiled<C, N, false, KW>), tgrid, tblock, 0, st, in, W, bias, out, cout_store,  \
+                           nx, ny, nz, relu);                                                                    \
+      return GS_OK;                                                                                              \
+    }                                                                                                            \
   }
     CONV_TILED_CASES(X)
 #undef X
@@ -849,43 +922,54 @@ static int launch_conv(int cin, int nt, bool f32, const uint16_t* in, const void
   const long long tiles16 = (BV + 15) / 16;
   if (tiles16 < 4096) {  // small level: split K over 4 (or 8) waves per 16-voxel tile
     const dim3 sgrid((unsigned)tiles16);
-#define X(C, N)                                                                                                   \
-  if (cin == C && nt == N) {                                                                                      \
-    if (tiles16 < 512) {                                                                                          \
-      if (f32)                                                                                                    \
-        hipLaunchKernelGGL((k_conv_splitk<C, N, true, 8>), sgrid, dim3(512), 0, st, in, (const s4*)Wp, bias, out, \
-                           cout_store, nx, ny, nz, BV, relu);                                                     \
-      else                                                                                                        \
-        hipLaunchKernelGGL((k_conv_splitk<C, N, false, 8>), sgrid, dim3(512), 0, st, in, (const s4*)Wp, bias,     \
-                           out, cout_store, nx, ny, nz, BV, relu);                                                \
-    } else {                                                                                                      \
-      if (f32)                                                                                                    \
-        hipLaunchKernelGGL((k_conv_splitk<C, N, true, 4>), sgrid, dim3(256), 0, st, in, (const s4*)Wp, bias, out, \
-                           cout_store, nx, ny, nz, BV, relu);                                                     \
-      else                                                                                                        \
-        hipLaunchKernelGGL((k_conv_splitk<C, N, false, 4>), sgrid, dim3(256), 0, st, in, (const s4*)Wp, bias,     \
-                           out, cout_store, nx, ny, nz, BV, relu);                                                \
-    }                                                                                                             \
-    return GS_OK;                                                                                                 \
+#define X(C, N)                                                                                                  \
+  if constexpr (KW == 16 || (C) % 8 == 0) {                                                                      \
+    if (cin == C && nt == N) {                                                                                   \
+      if (tiles16 < 512) {                                                                                       \
+        if (f32)                                                                                                 \
+          hipLaunchKernelGGL((k_conv_splitk<C, N, true, 8, KW>), sgrid, dim3(512), 0, st, in, W, bias, out,      \
+                             cout_store, nx, ny, nz, BV, relu);                                                  \
+        else                                                                                                     \
+          hipLaunchKernelGGL((k_conv_splitk<C, N, false, 8, KW>), sgrid, dim3(512), 0, st, in, W, bias, out,     \
+                             cout_store, nx, ny, nz, BV, relu);                                                  \
+      } else {                                                                                                   \
+        if (f32)                                                                                                 \
+          hipLaunchKernelGGL((k_conv_splitk<C, N, true, 4, KW>), sgrid, dim3(256), 0, st, in, W, bias, out,      \
+                             cout_store, nx, ny, nz, BV, relu);                                                  \
+        else                                                                                                     \
+          hipLaunchKernelGGL((k_conv_splitk<C, N, false, 4, KW>), sgrid, dim3(256), 0, st, in, W, bias, out,     \
+                             cout_store, nx, ny, nz, BV, relu);                                                  \
+      }                                                                                                          \
+      return GS_OK;                                                                                              \
+    }                                                                                                            \
   }
     CONV_CASES(X)
 #undef X
   }
   const dim3 grid((unsigned)((BV + 63) / 64)), block(256);
 #define X(C, N)                                                                                                  \
-  if (cin == C && nt == N) {                                                                                     \
-    if (f32)                                                                                                     \
-      hipLaunchKernelGGL((k_conv<C, N, true>), grid, block, 0, st, in, (const s4*)Wp, bias, out, cout_store, nx, \
-                         ny, nz, BV, relu);                                                                      \
-    else                                                                                                         \
-      hipLaunchKernelGGL((k_conv<C, N, false>), grid, block, 0, st, in, (const s4*)Wp, bias, out, cout_store,    \
-                         nx, ny, nz, BV, relu);                                                                  \
-    return GS_OK;                                                                                                \
+  if constexpr (KW == 16 || (C) % 8 == 0) {                                                                      \
+    if (cin == C && nt == N) {                                                                                   \
+      if (f32)                                                                                                   \
+        hipLaunchKernelGGL((k_conv<C, N, true, KW>), grid, block, 0, st, in, W, bias, out, cout_store, nx, ny,   \
+                           nz, BV, relu);                                                                        \
+      else                                                                                                       \
+        hipLaunchKernelGGL((k_conv<C, N, false, KW>), grid, block, 0, st, in, W, bias, out, cout_store, nx, ny,  \
+                           nz, BV, relu);                                                                        \
+      return GS_OK;                                                                                              \
+    }                                                                                                            \
   }
   CONV_CASES(X)
 #undef X
-  std::fprintf(stderr, "gale_stokes(net): no convolution kernel for cin %d, %d tiles\n", cin, nt);
+  std::fprintf(stderr, "gale_stokes(net): no convolution kernel for cin %d, %d tiles, K step %d\n", cin, nt, KW);
   return GS_E_INVALID;
+}
+
+static int launch_conv(int cin, int nt, bool f32, const uint16_t* in, const void* Wp, const float* bias, void* out,
+                       int cout_store, int nx, int ny, int nz, long long BV, int relu, hipStream_t st, int kw = 16) {
+  if (kw == 32)
+    return launch_conv_kw<32>(cin, nt, f32, in, Wp, bias, out, cout_store, nx, ny, nz, BV, relu, st);
+  return launch_conv_kw<16>(cin, nt, f32, in, Wp, bias, out, cout_store, nx, ny, nz, BV, relu, st);
 }
 
 static int launch_wgrad(int cin, int nt, const uint16_t* X, const uint16_t* G, int cg, float* partial, int nx,
@@ -918,6 +1002,9 @@ static int fwd_layer(Net* n, int l, const uint16_t* in, void* out) {
   int nx, ny, nz;
   lvl_dims(w.sh, L.level, &nx, &ny, &nz);
   const bool last = (l == kLayers - 1);
+  if (n->wp32[l])  // gfx950's 16x16x32 matrix-core step
+    return launch_conv(L.cin, ntiles(L.cout), last, in, n->wp32[l], n->w + bias_offset(l), out, L.cout, nx, ny,
+                       nz, w.BV[L.level], L.relu, n->st, 32);
   return launch_conv(L.cin, ntiles(L.cout), last, in, n->wp[l], n->w + bias_offset(l), out, L.cout, nx, ny, nz,
                      w.BV[L.level], L.relu, n->st);
 }
@@ -1095,20 +1182,40 @@ __global__ void k_mfma_selftest(float* __restrict__ out) {
   for (int i = 0; i < 4; i++) out[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
 }
 
+// The same for the 32-wide step (v_mfma_f32_16x16x32_bf16): lane l holds
+// K = 8 (l / 16) + j, j = 0..7, of A's row l % 16 and B's column l % 16.
+__global__ void k_mfma_selftest32(float* __restrict__ out) {
+  const int lane = threadIdx.x;
+  if (lane >= 64) return;
+  s8 a, b;
+  for (int j = 0; j < 8; j++) {
+    const int k = 8 * (lane >> 4) + j;
+    a[j] = (short)f2bf((float)((lane & 15) + 2 * k) / 32.f);
+    b[j] = (short)f2bf((float)(3 * k - (lane & 15)) / 32.f);
+  }
+  f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+  acc = mfma_bf16(a, b, acc);
+  for (int i = 0; i < 4; i++) out[(4 * (lane >> 4) + i) * 16 + (lane & 15)] = acc[i];
+}
+
+// Both step widths against the exact product (every value is exact in BF16
+// and every partial sum in FP32); returns the larger error.
 int selftest_mfma(float* max_err) {
   float* d = nullptr;
-  NCK(hipMalloc(&d, sizeof(float) * 256));
+  NCK(hipMalloc(&d, sizeof(float) * 512));
   hipLaunchKernelGGL(k_mfma_selftest, dim3(1), dim3(64), 0, 0, d);
-  float h[256];
+  hipLaunchKernelGGL(k_mfma_selftest32, dim3(1), dim3(64), 0, 0, d + 256);
+  float h[512];
   NCK(hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost));
   NCK(hipFree(d));
   float e = 0.f;
-  for (int i = 0; i < 16; i++)
-    for (int j = 0; j < 16; j++) {
-      double s = 0.0;
-      for (int k = 0; k < 16; k++) s += ((i + 2.0 * k) / 32.0) * ((3.0 * k - j) / 32.0);
-      e = std::fmax(e, (float)std::fabs(h[i * 16 + j] - s));
-    }
+  for (int kw = 16; kw <= 32; kw += 16)
+    for (int i = 0; i < 16; i++)
+      for (int j = 0; j < 16; j++) {
+        double s = 0.0;
+        for (int k = 0; k < kw; k++) s += ((i + 2.0 * k) / 32.0) * ((3.0 * k - j) / 32.0);
+        e = std::fmax(e, (float)std::fabs(h[(kw == 32 ? 256 : 0) + i * 16 + j] - s));
+      }
   *max_err = e;
   return GS_OK;
 }

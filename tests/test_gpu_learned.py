@@ -56,6 +56,43 @@ def test_forward_matches_oracle(gale):
     assert rel(out, exact) < 5e-2
 
 
+def test_forward_32_wide_step_matches(gale, monkeypatch):
+    """The forward pass on gfx950's 16x16x32 BF16 step (GALE_LPC_K32=1) matches
+    the 16-wide one and the numpy oracle, on a grid whose levels take the
+    LDS-tiled, split-K and generic convolutions.  The two steps only regroup
+    FP32 partial sums, but activations are stored in BF16 between layers, so
+    a regrouped sum that lands on the other side of a BF16 rounding boundary
+    moves that activation by one BF16 ulp (2^-8 relative) and the change
+    travels through the later layers: the two outputs agree to the BF16 scale,
+    not to FP32 rounding, and each is as close to the oracle as the other."""
+    torch = _torch()
+    W = None
+    outs = {}
+    x = _rand_in(1, 48, 48, 48, 21)
+    for k32 in ("0", "1"):
+        monkeypatch.setenv("GALE_LPC_K32", k32)
+        net = gale.LearnedPreconditioner(seed=3)
+        if W is None:
+            P = net.get_params()
+            sl = N.layer_slices()[0]
+            last = N.NL - 1
+            P[sl[last][0]] *= 100.0
+            P[sl[last][1]] = 0.05
+            W = P
+        net.set_params(W)
+        d_out = torch.empty((1, 48, 48, 48), dtype=torch.float32, device="cuda")
+        net.forward(torch.tensor(x, device="cuda"), d_out)
+        outs[k32] = d_out.cpu().numpy()
+    monkeypatch.setenv("GALE_LPC_K32", "0")
+    ref, _ = N.forward(W, x, mimic=True)
+    e0, e1 = rel(outs["0"], ref), rel(outs["1"], ref)
+    print(f"16-wide vs oracle {e0:.2e}, 32-wide vs oracle {e1:.2e}, "
+          f"between {rel(outs['1'], outs['0']):.2e}", flush=True)
+    assert np.isfinite(outs["1"]).all()
+    assert rel(outs["1"], outs["0"]) < 1e-2
+    assert e1 < 1e-2 and e1 < 1.5 * e0 + 1e-3
+
+
 def _grads(gale, net, P, x, gout):
     torch = _torch()
     g = net.backward(torch.tensor(x, device="cuda"), torch.tensor(gout, device="cuda"))
