@@ -48,7 +48,9 @@ def build(verbose: bool = False, jobs: int = 2) -> str:
         if p.wait() != 0:
             raise RuntimeError("hipcc failed")
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
+    # rocSOLVER (with rocBLAS) factors the dense coarsest multigrid operator
+    # at setup when it is too large for the host inverse
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-lrocsolver", "-lrocblas"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocsolver/rocsolver.h>
 
 #include "../../include/gale_stokes.h"
 #include "gs_crand.h"
@@ -132,12 +133,26 @@ struct MGLevel {
   float* h32 = nullptr;
 };
 
+// Free dofs up to which the 3D hierarchy stops for a dense coarsest solve.
+// 96^3 coarsens 96-48-24-12 and solves the 12^3 level (5577 free dofs with
+// free-slip walls) densely: one matrix-vector product replaces the 12^3 and
+// 6^3 levels' smoothing, transfer and solve launches of every V-cycle.
+// GALE_MG_DENSE_MAX overrides (A/B timing); at most kDenseWideMax.
+static constexpr int kDenseHostMax = 2000;  // larger inverses are formed on the device
+static int dense_max_free() {
+  const char* e = std::getenv("GALE_MG_DENSE_MAX");
+  int v = e ? std::atoi(e) : 6000;
+  if (v < 0) v = 0;
+  return std::min(v, kDenseWideMax);
+}
+
 struct MGData {
   bool structured = false;
   bool bottom = false;  // single-workgroup bottom of the cycle (3D) where levels fit
   int L = 0;  // index of the coarsest level
   std::vector<MGLevel> lv;
   int nfc = 0;
+  int ldk = 0;  // 0: dense inverse packed nfc x nfc; > 0: rows padded to ldk (wide solve)
   int* freeidx = nullptr;
   double* Kinv64 = nullptr;
   float* Kinv32 = nullptr;
@@ -1199,6 +1214,7 @@ static void mg_free(MGData& mg) {
   mg.structured = false;
   mg.L = 0;
   mg.nfc = 0;
+  mg.ldk = 0;
 }
 
 static int32_t mg_alloc_level(MGLevel& L) {
@@ -1230,22 +1246,23 @@ static int32_t mg_structure(gs_solver* s) {
   std::vector<uint8_t> flag(pb->g.N);
   CK(hipMemcpy(flag.data(), pb->bcflag, pb->g.N, hipMemcpyDeviceToHost));
   // 3D: coarsening stops at the first level small enough for the dense
-  // coarsest solve (at most 2000 free dofs): an exact solve there needs fewer
-  // inner iterations than smoothing further down, and its matrix-vector
-  // product is one wide launch (96^3: 96-48-24-12-6, 6^3 dense; 9 % less time
-  // per inner iteration and 8 % fewer inner iterations than going on to 3^3).
+  // coarsest solve (dense_max_free(), 6000 free dofs): an exact solve there
+  // needs fewer inner iterations than smoothing further down, and its
+  // matrix-vector product is one wide launch where the levels below would
+  // take a dozen small ones per V-cycle (96^3: 96-48-24-12, 12^3 dense).
   // 2D levels coarsen as far as the mesh allows (the batched solver runs each
   // instance's dense coarsest solve inside one workgroup, where a small one
   // is cheaper).  GALE_MG_DENSE_STOP=0 / 1 overrides (A/B timing).
   const char* dstop = std::getenv("GALE_MG_DENSE_STOP");
   const bool dense_stop = dstop ? std::atoi(dstop) != 0 : dim == 3;
+  const int dense_max = dense_max_free();
   for (int lev = 0; lev < 16; lev++) {
     const Geo& gf = mg.lv.back().g;
     if (dense_stop && lev > 0) {
       long long nfree = 0;
       for (long long n = 0; n < gf.N; n++)
         for (int d = 0; d < dim; d++) nfree += ((flag[n] >> d) & 1u) ? 0 : 1;
-      if (nfree <= 2000) break;
+      if (nfree <= dense_max) break;
     }
     bool ok = true;
     for (int d = 0; d < dim; d++) ok = ok && gf.res[d] % 2 == 0 && gf.res[d] / 2 >= 2;
@@ -1301,11 +1318,16 @@ static int32_t mg_structure(gs_solver* s) {
     for (int d = 0; d < dim; d++)
       if (!((flag[n] >> d) & 1u)) fidx.push_back((int)(n * dim + d));
   mg.nfc = (int)fidx.size();
-  if (mg.nfc <= 2000) {
+  if (mg.nfc <= dense_max) {
     CK(dalloc(&mg.freeidx, mg.nfc));
     CK(hipMemcpy(mg.freeidx, fidx.data(), sizeof(int) * mg.nfc, hipMemcpyHostToDevice));
-    CK(dalloc(&mg.Kinv64, (long long)mg.nfc * mg.nfc));
-    CK(dalloc(&mg.Kinv32, (long long)mg.nfc * mg.nfc));
+    // up to kDenseHostMax free dofs the inverse is formed on the host and
+    // stored packed; larger ones are inverted on the device and stored with
+    // padded rows for the wide solve
+    mg.ldk = mg.nfc > kDenseHostMax ? (mg.nfc + 3) & ~3 : 0;
+    const long long cols = mg.ldk > 0 ? mg.ldk : mg.nfc;
+    CK(dalloc(&mg.Kinv64, (long long)mg.nfc * cols));
+    CK(dalloc(&mg.Kinv32, (long long)mg.nfc * cols));
   } else {
     mg.nfc = -1;  // no dense coarse solve: Chebyshev smoothing on the coarsest level
   }
@@ -1322,6 +1344,39 @@ static int32_t mg_structure(gs_solver* s) {
 }
 
 // In-place inverse of an SPD matrix by Cholesky (host, coarsest level only).
+// Inverse of the SPD n x n matrix held column-major in A (leading dimension
+// ld) on the device: Cholesky factor and inverse from it (rocSOLVER potrf,
+// potri on the lower triangle), then the upper triangle filled from the
+// lower.  GS_E_NAN when the matrix is not positive definite.
+static int32_t spd_inverse_device(double* A, int n, int ld, hipStream_t st) {
+  rocblas_handle h = nullptr;
+  if (rocblas_create_handle(&h) != rocblas_status_success) return GS_E_HIP;
+  int32_t ret = GS_OK;
+  rocblas_int* dinfo = nullptr;
+  rocblas_int info = 0;
+  if (rocblas_set_stream(h, st) != rocblas_status_success || hipMalloc(&dinfo, sizeof(rocblas_int)) != hipSuccess) {
+    ret = GS_E_HIP;
+  } else if (rocsolver_dpotrf(h, rocblas_fill_lower, n, A, ld, dinfo) != rocblas_status_success ||
+             hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess) {
+    ret = GS_E_HIP;
+  } else if (info != 0) {
+    ret = GS_E_NAN;
+  } else if (rocsolver_dpotri(h, rocblas_fill_lower, n, A, ld, dinfo) != rocblas_status_success ||
+             hipMemcpyAsync(&info, dinfo, sizeof(info), hipMemcpyDeviceToHost, st) != hipSuccess ||
+             hipStreamSynchronize(st) != hipSuccess) {
+    ret = GS_E_HIP;
+  } else if (info != 0) {
+    ret = GS_E_NAN;
+  } else {
+    launch_symmetrize_lower(A, n, ld, st);
+    if (hipStreamSynchronize(st) != hipSuccess) ret = GS_E_HIP;
+  }
+  if (dinfo) (void)hipFree(dinfo);
+  rocblas_destroy_handle(h);
+  return ret;
+}
+
 static bool spd_inverse(std::vector<double>& A, int n) {
   std::vector<double> Lm(A);
   for (int j = 0; j < n; j++) {
@@ -1437,11 +1492,23 @@ static int32_t mg_numeric(gs_solver* s) {
                   }
             }
           }
-    if (!spd_inverse(Kd, nf)) return GS_E_NAN;
-    std::vector<float> Kf(Kd.begin(), Kd.end());
-    CK(hipMemcpyAsync(mg.Kinv64, Kd.data(), sizeof(double) * Kd.size(), hipMemcpyHostToDevice, st));
-    CK(hipMemcpyAsync(mg.Kinv32, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice, st));
-    CK(hipStreamSynchronize(st));
+    if (mg.ldk == 0) {
+      if (!spd_inverse(Kd, nf)) return GS_E_NAN;
+      std::vector<float> Kf(Kd.begin(), Kd.end());
+      CK(hipMemcpyAsync(mg.Kinv64, Kd.data(), sizeof(double) * Kd.size(), hipMemcpyHostToDevice, st));
+      CK(hipMemcpyAsync(mg.Kinv32, Kf.data(), sizeof(float) * Kf.size(), hipMemcpyHostToDevice, st));
+      CK(hipStreamSynchronize(st));
+    } else {
+      // Kd is symmetric, so its row-major layout is also the column-major
+      // matrix; padded rows (ld = mg.ldk), padding zero
+      const int ld = mg.ldk;
+      launch_fill(mg.Kinv64, (long long)nf * ld, 0.0, st);
+      CK(hipMemcpy2DAsync(mg.Kinv64, sizeof(double) * ld, Kd.data(), sizeof(double) * nf, sizeof(double) * nf, nf,
+                          hipMemcpyHostToDevice, st));
+      CKS(spd_inverse_device(mg.Kinv64, nf, ld, st));
+      launch_d2f(mg.Kinv64, mg.Kinv32, (long long)nf * ld, st);
+      CK(hipStreamSynchronize(st));
+    }
   }
   return GS_OK;
 }
@@ -1525,7 +1592,7 @@ static T* vcycle(gs_solver* s, int l) {
   if (l == mg.L) {
     if (mg.nfc > 0) {
       launch_dense_solve<T>(sizeof(T) == 8 ? (const T*)(void*)mg.Kinv64 : (const T*)(void*)mg.Kinv32, mg.freeidx,
-                            mg.nfc, P.b, P.x, L.n, st);
+                            mg.nfc, mg.ldk, P.b, P.x, L.n, st);
       return P.x;
     }
     return cheb_smooth<T>(L, 8, (T*)nullptr, st);
@@ -1651,6 +1718,7 @@ int32_t gs::batch_copy_level(gs_solver* s, int l, float* M32, float* dinv32, uin
 
 int32_t gs::batch_copy_dense(gs_solver* s, float* Kinv32, int* freeidx, hipStream_t st) {
   const int nf = s->mg.nfc;
+  if (s->mg.ldk != 0) return GS_E_INVALID;  // the batched solver takes packed inverses only
   if (nf <= 0) return GS_OK;
   CK(hipMemcpyAsync(Kinv32, s->mg.Kinv32, sizeof(float) * nf * nf, hipMemcpyDeviceToDevice, st));
   CK(hipMemcpyAsync(freeidx, s->mg.freeidx, sizeof(int) * nf, hipMemcpyDeviceToDevice, st));

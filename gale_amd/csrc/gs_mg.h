@@ -26,8 +26,18 @@ void launch_cheb(const T* b, const T* Ax, const T* dinv, T* d, T* x, long long n
                  hipStream_t st);
 template <typename T>
 void launch_resid(const T* b, const T* Ax, T* r, long long n, hipStream_t st);
+// Dense coarsest solve x = Kinv b on the free dofs.  ld == 0: Kinv packed
+// nf x nf (one wavefront per row, the right-hand side gathered per row);
+// ld > 0: rows padded to ld (a multiple of 4) for 16-byte row loads, the
+// gathered right-hand side staged once per workgroup in LDS (nf at most
+// kDenseWideMax).
+constexpr int kDenseWideMax = 8192;
 template <typename T>
-void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, const T* b, T* x, long long n, hipStream_t st);
+void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, int ld, const T* b, T* x, long long n,
+                        hipStream_t st);
+// Fills the upper triangle of a column-major n x n matrix (leading dimension
+// ld) from its lower triangle.
+void launch_symmetrize_lower(double* A, int n, int ld, hipStream_t st);
 
 inline int n_moments(int dim) { return dim == 2 ? 9 : 27; }
 

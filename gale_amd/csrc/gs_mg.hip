@@ -417,14 +417,62 @@ __global__ void k_dense_solve(const T* __restrict__ Kinv, const int* __restrict_
   for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
   if (lane == 0) x[freeidx[row]] = s;
 }
+// Wide form: 16 rows per 1024-thread workgroup, one wavefront per row; the
+// gathered right-hand side b[free[j]] is read once per workgroup into LDS,
+// and each row is read with 16-byte loads (rows padded to ld, a multiple of
+// 4 values, so every row starts 16-byte aligned).
 template <typename T>
-void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, const T* b, T* x, long long n, hipStream_t st) {
+__global__ __launch_bounds__(1024) void k_dense_solve_wide(const T* __restrict__ Kinv, int ld,
+                                                           const int* __restrict__ freeidx, int nf,
+                                                           const T* __restrict__ b, T* __restrict__ x) {
+  constexpr int E = 16 / (int)sizeof(T);
+  typedef T vec_t __attribute__((ext_vector_type(E)));
+  __shared__ __align__(16) T sb[kDenseWideMax];
+  for (int j = threadIdx.x; j < nf; j += blockDim.x) sb[j] = b[freeidx[j]];
+  __syncthreads();
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= nf) return;
+  const vec_t* r = reinterpret_cast<const vec_t*>(Kinv + (long long)row * ld);
+  T s = 0;
+  for (int j0 = lane * E; j0 < nf; j0 += 64 * E) {
+    const vec_t v = r[j0 / E];
+#pragma unroll
+    for (int e = 0; e < E; e++)
+      if (j0 + e < nf) s += v[e] * sb[j0 + e];
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) x[freeidx[row]] = s;
+}
+
+template <typename T>
+void launch_dense_solve(const T* Kinv, const int* freeidx, int nf, int ld, const T* b, T* x, long long n,
+                        hipStream_t st) {
   unsigned gb = gridn(n);
   if (gb > 8192) gb = 8192;
   hipLaunchKernelGGL(k_fill_mg<T>, dim3(gb), dim3(kBlock), 0, st, x, n, (T)0);
-  if (nf > 0) hipLaunchKernelGGL(k_dense_solve<T>, dim3((nf + 3) / 4), dim3(kBlock), 0, st, Kinv, freeidx, nf, b, x);
+  if (nf <= 0) return;
+  if (ld > 0)
+    hipLaunchKernelGGL(k_dense_solve_wide<T>, dim3((nf + 15) / 16), dim3(1024), 0, st, Kinv, ld, freeidx, nf, b, x);
+  else
+    hipLaunchKernelGGL(k_dense_solve<T>, dim3((nf + 3) / 4), dim3(kBlock), 0, st, Kinv, freeidx, nf, b, x);
 }
-template void launch_dense_solve<double>(const double*, const int*, int, const double*, double*, long long, hipStream_t);
-template void launch_dense_solve<float>(const float*, const int*, int, const float*, float*, long long, hipStream_t);
+template void launch_dense_solve<double>(const double*, const int*, int, int, const double*, double*, long long,
+                                         hipStream_t);
+template void launch_dense_solve<float>(const float*, const int*, int, int, const float*, float*, long long,
+                                        hipStream_t);
+
+__global__ void k_symmetrize_lower(double* __restrict__ A, int n, int ld) {
+  const long long nn = (long long)n * n;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < nn; t += (long long)gridDim.x * blockDim.x) {
+    const int i = (int)(t % n), j = (int)(t / n);
+    if (i < j) A[i + (long long)j * ld] = A[j + (long long)i * ld];
+  }
+}
+void launch_symmetrize_lower(double* A, int n, int ld, hipStream_t st) {
+  unsigned gb = gridn((long long)n * n);
+  if (gb > 65536) gb = 65536;
+  hipLaunchKernelGGL(k_symmetrize_lower, dim3(gb), dim3(kBlock), 0, st, A, n, ld);
+}
 
 }  // namespace gs
