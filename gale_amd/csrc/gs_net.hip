@@ -490,10 +490,11 @@ __global__ void k_pack(const float* __restrict__ W, int cin, int cout, int trans
 // 0.48 -> about 0.37 ms per application).  It regroups the FP32 partial sums,
 // which flips some of the BF16 roundings of the activations between layers:
 // on a 48^3 input the two widths differ by 3.5e-3 relative, as far as either
-// is from the numpy oracle.  The learned solve's
-// iteration count moves by up to 30 % either way with such changes, and over
-// five 96^3 fields the two widths total 10.02 s (32) and 9.76 s (16), so the
-// 16-wide step is the default and GALE_LPC_K32=1 selects this one.
+// is from the numpy oracle.  The learned solve's iteration count moves by up
+// to 30 % either way with such changes, so the two widths were compared over
+// ten 96^3 fields of the generator: 19.83 s (32) against 20.59 s (16) in
+// total, 5 % less time per Uzawa iteration, faster on eight.  The 32-wide
+// step is the default; GALE_LPC_K32=0 selects the 16-wide one.
 __global__ void k_pack32(const float* __restrict__ W, int cin, int cout, int nt, int kc, s8* __restrict__ out) {
   const long long n = (long long)kc * nt * 64;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
@@ -774,10 +775,10 @@ int create(Net* n, uint64_t seed, hipStream_t st) {
   for (int l = 0; l < kLayers; l++) {
     const LayerSpec& L = kSpec[l];
     NCK(hipMalloc(&n->wp[l], sizeof(s4) * 64LL * kchunks(L.cin) * ntiles(L.cout)));
-    // GALE_LPC_K32=1 runs the forward passes of layers with cin a multiple
-    // of 8 on the 32-wide step (see the note at k_pack32)
+    // forward passes of layers with cin a multiple of 8 run on the 32-wide
+    // step unless GALE_LPC_K32=0 (see the note at k_pack32)
     const char* k32 = std::getenv("GALE_LPC_K32");
-    if (L.cin % 8 == 0 && k32 && k32[0] == '1')
+    if (L.cin % 8 == 0 && !(k32 && k32[0] == '0'))
       NCK(hipMalloc(&n->wp32[l], sizeof(s8) * 64LL * kchunks32(L.cin) * ntiles(L.cout)));
     const int cin_t = L.cout < 4 ? 4 : L.cout;  // transposed GEMM input channels (gradient tensor)
     NCK(hipMalloc(&n->wt[l], sizeof(s4) * 64LL * kchunks(cin_t) * ntiles(L.cin)));

@@ -57,8 +57,8 @@ def test_forward_matches_oracle(gale):
 
 
 def test_forward_32_wide_step_matches(gale, monkeypatch):
-    """The forward pass on gfx950's 16x16x32 BF16 step (GALE_LPC_K32=1) matches
-    the 16-wide one and the numpy oracle, on a grid whose levels take the
+    """The forward pass on gfx950's 16x16x32 BF16 step (the default) matches
+    the 16-wide one (GALE_LPC_K32=0) and the numpy oracle, on a grid whose levels take the
     LDS-tiled, split-K and generic convolutions.  The two steps only regroup
     FP32 partial sums, but activations are stored in BF16 between layers, so
     a regrouped sum that lands on the other side of a BF16 rounding boundary
