@@ -1866,7 +1866,7 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   if (m > kMaxWindow) m = kMaxWindow;
   const int win = s->up.stagnation_window > 0 ? s->up.stagnation_window : 25;
   const double relax = s->up.inner_relax_max > 0 ? s->up.inner_relax_max : 0.0;
-  const double gain = s->up.inner_relax_gain > 0 ? s->up.inner_relax_gain : 1.0;
+  const double gain = s->up.inner_relax_gain > 0 ? s->up.inner_relax_gain : 3.0;
   if (s->fl_m != m) {
     dfree(s->fl_s);
     dfree(s->fl_q);

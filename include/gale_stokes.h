@@ -117,7 +117,7 @@ typedef struct gs_uzawa_params {
   double inner_relax_max;       /* learned preconditioner: > 0 relaxes the inner tolerance as the outer
                                    residual falls, up to this rtol, and re-checks the true residual */
   double inner_relax_gain;      /* the relaxed inner rtol is min(inner_relax_max, gain * rtol0 * |r_0| / |r_k|),
-                                   rtol0 the unrelaxed inner rtol (0 = 1) */
+                                   rtol0 the unrelaxed inner rtol (0 = 3) */
 } gs_uzawa_params;
 
 /* SystemLinearEquations nonlinear keys (SystemLinearEquations.cxx:346-356). */
