@@ -278,8 +278,8 @@ __global__ __launch_bounds__(64 * S) void k_conv_splitk(const uint16_t* __restri
   }
 }
 
-// LDS-tiled form of k_conv for the large levels (CIN a multiple of 16; nx,
-// ny, nz multiples of 16, 4, 4).  A block computes a 16 x 4 x 4 voxel tile:
+// LDS-tiled form of k_conv for the large levels (CIN a multiple of 4, of 8
+// on the 32-wide step; nx, ny, nz multiples of 16, 4, 4).  A block computes a 16 x 4 x 4 voxel tile:
 // it stages the 18 x 6 x 6 input box (zero outside the grid) in LDS once, and
 // each wave takes the four x-rows of one z-slice, so every input voxel is
 // read from memory once per tile instead of once per tap.  The K steps and
@@ -893,7 +893,7 @@ int ensure_work(Net* n, const Shape& sh) {
 #define CONV_CASES(X) \
   X(4, 1) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3) X(48, 2) X(48, 3) X(48, 4) X(64, 3) X(64, 4) X(64, 6) X(96, 4)
 
-#define CONV_TILED_CASES(X) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3)
+#define CONV_TILED_CASES(X) X(4, 1) X(16, 1) X(16, 2) X(32, 1) X(32, 2) X(32, 3)
 
 // KW: 16 or 32 K values per matrix-core step (32 needs cin a multiple of 8
 // and weights packed for it).
