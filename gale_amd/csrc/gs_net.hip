@@ -294,7 +294,8 @@ __global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__
   constexpr int KT = 27 * CIN;
   constexpr int KC = (KT + KW - 1) / KW;  // K steps (with KW = 32 and CIN = 16 a step spans two taps)
   constexpr int Q = CIN / 4;              // s4 chunks per voxel
-  __shared__ __align__(16) s4 box[NBOX * Q];
+  constexpr int STG = 4 * 4 * 16 * NT * 16 / 4;  // output staging (s4 units), reuses the box
+  __shared__ __align__(16) s4 box[NBOX * Q > STG ? NBOX * Q : STG];
   const int tx = nx / 16, ty = ny / 4, tz = nz / 4;
   long long t = blockIdx.x;
   const int bx = (int)(t % tx);
@@ -305,15 +306,31 @@ __global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__
   const long long b = t / tz;
   const int x0 = bx * 16, y0 = by * 4, z0 = bz * 4;
   const long long bbase = b * (long long)nx * ny * nz;
-  const s4* in4 = reinterpret_cast<const s4*>(in);
-  for (int q = threadIdx.x; q < NBOX * Q; q += 256) {
-    const int c4 = q % Q, vx = q / Q;
-    const int xx = vx % BX, yy = (vx / BX) % BY, zz = vx / (BX * BY);
-    const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
-    s4 v = s4{0, 0, 0, 0};
-    if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
-      v = in4[(bbase + gx + (long long)nx * (gy + (long long)ny * gz)) * Q + c4];
-    box[q] = v;
+  if constexpr (Q % 2 == 0) {
+    // 16-byte pieces (8 channels) when a voxel holds a whole number of them
+    constexpr int Q8 = Q / 2;
+    const s8* in8 = reinterpret_cast<const s8*>(in);
+    s8* box8 = reinterpret_cast<s8*>(box);
+    for (int q = threadIdx.x; q < NBOX * Q8; q += 256) {
+      const int c8 = q % Q8, vx = q / Q8;
+      const int xx = vx % BX, yy = (vx / BX) % BY, zz = vx / (BX * BY);
+      const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+      s8 v = s8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+        v = in8[(bbase + gx + (long long)nx * (gy + (long long)ny * gz)) * Q8 + c8];
+      box8[q] = v;
+    }
+  } else {
+    const s4* in4 = reinterpret_cast<const s4*>(in);
+    for (int q = threadIdx.x; q < NBOX * Q; q += 256) {
+      const int c4 = q % Q, vx = q / Q;
+      const int xx = vx % BX, yy = (vx / BX) % BY, zz = vx / (BX * BY);
+      const int gx = x0 - 1 + xx, gy = y0 - 1 + yy, gz = z0 - 1 + zz;
+      s4 v = s4{0, 0, 0, 0};
+      if (gx >= 0 && gx < nx && gy >= 0 && gy < ny && gz >= 0 && gz < nz)
+        v = in4[(bbase + gx + (long long)nx * (gy + (long long)ny * gz)) * Q + c4];
+      box[q] = v;
+    }
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -344,6 +361,37 @@ __global__ __launch_bounds__(256) void k_conv_tiled(const uint16_t* __restrict__
         for (int v = 0; v < NT; v++) acc[j][v] = mfma_bf16(a, wv[u][v], acc[j][v]);
       }
     }
+  }
+  if (!OUT_F32 && cout_store == NT * 16) {
+    // BF16 output of every channel: the wave's 64 voxels x NT*16 channels
+    // go through LDS (the box is free once every wave is past its K loop)
+    // and leave as 16-byte stores of whole voxel rows, instead of one
+    // 2-byte store per lane per voxel (same values, 10 % less time)
+    __syncthreads();
+    uint16_t* stg = reinterpret_cast<uint16_t*>(box) + w * (4 * 16 * NT * 16);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+#pragma unroll
+      for (int v = 0; v < NT; v++) {
+        const int co = v * 16 + row;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          float val = acc[j][v][i] + bv;
+          if (relu) val = fmaxf(val, 0.f);
+          stg[(j * 16 + kg * 4 + i) * (NT * 16) + co] = f2bf(val);
+        }
+      }
+    __syncthreads();
+    const s8* st8 = reinterpret_cast<const s8*>(stg);
+    constexpr int CH8 = NT * 2;  // 16-byte pieces per voxel
+    for (int c = lane; c < 4 * 16 * CH8; c += 64) {
+      const int vx = c / CH8, part = c - vx * CH8;
+      const int j = vx / 16, xx = vx - j * 16;
+      const long long vo = bbase + (x0 + xx) + (long long)nx * ((y0 + j) + (long long)ny * (z0 + w));
+      reinterpret_cast<s8*>(reinterpret_cast<uint16_t*>(out) + vo * cout_store)[part] = st8[c];
+    }
+    return;
   }
 #pragma unroll
   for (int j = 0; j < 4; j++)
