@@ -348,8 +348,12 @@ __device__ __forceinline__ void element2(const T (&u)[4][2], T ix, T iy, int for
 // ---------------------------------------------------------------------------
 // 3D tile kernel
 // ---------------------------------------------------------------------------
+// The FP32 moment form is held to 5 waves per SIMD (86 VGPRs, no spills;
+// 118 and 4 waves unconstrained); the other forms keep the compiler's choice.
 template <typename T, int FORM>
-__global__ __launch_bounds__(kTileThreads) void k_tile_apply3(Geo g, TileGrid tg, const T* __restrict__ x,
+__global__ __launch_bounds__(kTileThreads)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && FORM == 0 ? 5 : 1))) void
+k_tile_apply3(Geo g, TileGrid tg, const T* __restrict__ x,
                                                               T* __restrict__ y, T* __restrict__ halo,
                                                               const T* __restrict__ Mom,
                                                               const typename V4<T>::type* __restrict__ pk,
