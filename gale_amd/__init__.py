@@ -88,7 +88,7 @@ class UzawaParams(C.Structure):
         ("inner_rtol_override", C.c_double), ("monitor", C.c_int32), ("check_every", C.c_int32),
         ("inner_preconditioner", C.c_int32), ("mg_smooth", C.c_int32), ("flexible_window", C.c_int32),
         ("stagnation_window", C.c_int32), ("learned_restarts", C.c_int32), ("learned_q_every", C.c_int32),
-        ("inner_relax_max", C.c_double), ("inner_relax_gain", C.c_double),
+        ("inner_relax_max", C.c_double), ("inner_relax_gain", C.c_double), ("mg_smooth_coarse", C.c_int32),
     ]
 
 
@@ -740,7 +740,7 @@ def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsolut
                  Preconditioner=True, precision=PREC_FP64, inner_max_iterations=0, inner_rtol_factor=0.1,
                  inner_rtol_override=0.0, monitor=False, check_every=4, inner_preconditioner=INNER_MULTIGRID,
                  mg_smooth=2, learned=False, flexible_window=16, stagnation_window=50, learned_restarts=5,
-                 learned_q_every=0, inner_relax_max=0.0, inner_relax_gain=0.0) -> UzawaParams:
+                 learned_q_every=0, inner_relax_max=0.0, inner_relax_gain=0.0, mg_smooth_coarse=0) -> UzawaParams:
     """Stokes_SLE_UzawaSolver parameters under their XML names
     (Stokes_SLE_UzawaSolver.cxx:251-257 defaults) plus the inner-solve controls.
     ``learned=True`` selects the learned Schur preconditioner (attach a network
@@ -750,7 +750,7 @@ def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsolut
                        pc, precision, inner_max_iterations,
                        inner_rtol_factor, inner_rtol_override, int(monitor), check_every,
                        inner_preconditioner, mg_smooth, flexible_window, stagnation_window, learned_restarts,
-                       learned_q_every, inner_relax_max, inner_relax_gain)
+                       learned_q_every, inner_relax_max, inner_relax_gain, mg_smooth_coarse)
 
 
 class Stokes_SLE_UzawaSolver:

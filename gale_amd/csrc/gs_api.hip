@@ -1120,6 +1120,11 @@ extern "C" int32_t gs_solver_set_params(gs_solver* s, const gs_uzawa_params* up)
   if (s->up.inner_max_iterations <= 0) s->up.inner_max_iterations = 50000;
   if (s->up.inner_rtol_factor <= 0) s->up.inner_rtol_factor = 0.1;
   if (s->up.mg_smooth <= 0) s->up.mg_smooth = 2;
+  // Below the finest level, one smoothing step when the inner solves are
+  // relaxed (two or three V-cycles each): over ten 96^3 fields the learned
+  // solve took 8 % less time with one step than with two, faster on every
+  // field, while the plain solve (tight inner solves) took 15 % more.
+  if (s->up.mg_smooth_coarse <= 0) s->up.mg_smooth_coarse = s->up.inner_relax_max > 0 ? 1 : s->up.mg_smooth;
   if (s->up.inner_preconditioner != GS_INNER_JACOBI && s->up.inner_preconditioner != GS_INNER_MULTIGRID)
     return GS_E_INVALID;
   if (s->up.preconditioner != GS_PC_NONE && s->up.preconditioner != GS_PC_DIAGONAL &&
@@ -1571,7 +1576,7 @@ static bool vcycle_bottom(gs_solver* s, int l) {
     a.lv[k - l] = BotLevel<T>{Lk.g, P.M, P.dinv, Lk.bcflag, P.b, P.x, P.x2, P.r, P.d, Lk.lmax};
   }
   a.nlev = mg.L - l + 1;
-  a.nu = s->up.mg_smooth > 0 ? s->up.mg_smooth : 2;
+  a.nu = l == 0 ? s->up.mg_smooth : s->up.mg_smooth_coarse;
   a.Kinv = mg.nfc > 0 ? (sizeof(T) == 8 ? (const T*)(void*)mg.Kinv64 : (const T*)(void*)mg.Kinv32) : nullptr;
   a.freeidx = mg.freeidx;
   a.nfc = mg.nfc > 0 ? mg.nfc : 0;
@@ -1586,7 +1591,7 @@ static T* vcycle(gs_solver* s, int l) {
   MGData& mg = s->mg;
   MGLevel& L = mg.lv[l];
   hipStream_t st = s->pb->st;
-  const int nu = s->up.mg_smooth > 0 ? s->up.mg_smooth : 2;
+  const int nu = l == 0 ? s->up.mg_smooth : s->up.mg_smooth_coarse;
   LvPtr<T> P = lvp(L, (T*)nullptr);
   if (vcycle_bottom<T>(s, l)) return P.x;
   if (l == mg.L) {

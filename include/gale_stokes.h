@@ -118,6 +118,8 @@ typedef struct gs_uzawa_params {
                                    residual falls, up to this rtol, and re-checks the true residual */
   double inner_relax_gain;      /* the relaxed inner rtol is min(inner_relax_max, gain * rtol0 * |r_0| / |r_k|),
                                    rtol0 the unrelaxed inner rtol (0 = 3) */
+  int32_t mg_smooth_coarse;     /* Chebyshev-Jacobi steps on the levels below the finest (0 = 1 when the
+                                   inner tolerance is relaxed, inner_relax_max > 0, else mg_smooth) */
 } gs_uzawa_params;
 
 /* SystemLinearEquations nonlinear keys (SystemLinearEquations.cxx:346-356). */
