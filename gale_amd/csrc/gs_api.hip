@@ -1800,19 +1800,19 @@ static int32_t inner_solve_mg(gs_solver* s, const double* b, double* x, double r
   MGLevel& L0 = s->mg.lv[0];
   launch_fill(x, n, 0.0, st);
   CK(hipMemcpyAsync(s->cg_r, b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+  // b.b is read back with the first convergence check, not before the first
+  // iteration (one host round trip fewer per inner solve).  With b = 0 every
+  // iterate stays exactly zero (the V-cycle of 0 is 0 and alpha, beta are
+  // guarded), so that case still returns x = 0 after no counted iteration.
   launch_dot(b, b, n, s->S, S_BB, s->rd, st);
-  CKS(read_scalars(s));
-  const double bb = s->hS[S_BB];
-  if (bb == 0.0) return GS_OK;
-  if (!std::isfinite(bb)) return GS_E_NAN;
-  const double target = rtol * rtol * bb;
   mg_precond(s, s->cg_r, s->cg_z);
   CK(hipMemcpyAsync(s->cg_p, s->cg_z, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
   launch_dot(s->cg_r, s->cg_z, n, s->S, S_RZ, s->rd, st);
   const int maxit = s->up.inner_max_iterations;
   const int every = s->up.check_every > 0 ? std::min(s->up.check_every, 2) : 1;
   int k = 0;
-  bool conv = false;
+  bool conv = false, first = true;
+  double target = 0.0;
   while (k < maxit) {
     launch_mom_apply<double>(L0.g, s->cg_p, s->cg_Ap, L0.M64, pb->bcflag, L0.h64, st);
     s->k_applies++;
@@ -1821,6 +1821,13 @@ static int32_t inner_solve_mg(gs_solver* s, const double* b, double* x, double r
     k++;
     if (k % every == 0 || k >= maxit) {
       CKS(read_scalars(s));
+      if (first) {
+        first = false;
+        const double bb = s->hS[S_BB];
+        if (bb == 0.0) return GS_OK;
+        if (!std::isfinite(bb)) return GS_E_NAN;
+        target = rtol * rtol * bb;
+      }
       const double rr = s->hS[S_RR];
       if (!std::isfinite(rr)) return GS_E_NAN;
       if (rr <= target) {
@@ -1892,6 +1899,8 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
   const int q_every = s->up.learned_q_every > 0 ? s->up.learned_q_every : 0;
   constexpr int kMaxBad = 5;
   int bad_left = kMaxBad;
+  long long nbad_seen = 0;
+  launch_fill(s->S + S_NBAD, 1, 0.0, st);
   int32_t e2 = GS_OK;
   double relRes = std::sqrt(dot_host(s, s->r, s->r, np, &e2)) * qRecip / uzawaRhsScale;
   CKS(e2);
@@ -1903,22 +1912,18 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
     std::vector<double> hist;
     bool converged = (s->up.use_absolute_tolerance ? absRes : relRes) <= tol && it >= s->up.min_iterations;
     while (!converged) {
-      bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
+      const bool use_net = lp_on && !(q_every > 0 && (it + 1) % q_every == 0);
       if (use_net) {
+        // a network output that is not a descent direction (z.r <= 0 or
+        // not finite) is replaced by Q^-1 r on the device, without waiting
+        // for z.r; the replacements are counted in S[S_NBAD] and read with
+        // this iteration's update, and after kMaxBad of them the network is
+        // dropped for the solve
         CKS(lpc_precond(s->lpc, pb->g, s->r, s->Qinv, s->logeta, nullptr, s->z, s->S, s->rd, st));
-        CKS(read_scalars(s));
-        const double zr = s->hS[S_ZR];
-        if (!(zr > 0.0) || !std::isfinite(zr)) {
-          // not a descent direction: this iteration takes Q^-1 r; after
-          // kMaxBad such iterations the network is dropped for the solve
-          use_net = false;
-          if (--bad_left <= 0) {
-            lp_on = false;
-            rep->learned_fallback = it + 1;
-          }
-        }
+        launch_uz_precond_if_bad(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
+      } else {
+        launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
       }
-      if (!use_net) launch_uz_precond(s->r, s->Qinv, s->z, np, s->S, s->rd, st);
       const int k = it - base;
       const int c = k < m ? k : m;
       const int slot = k % m;
@@ -1936,6 +1941,16 @@ static int32_t uzawa_learned_loop(gs_solver* s, gs_report* rep, double uzawaRhsS
       launch_uz_update(s->p, sk, s->r, qk, np, s->S, s->rd, st);
       launch_uz_uupdate(s->u, s->vstar, n, s->S, st);
       CKS(read_scalars(s));
+      if (use_net) {
+        const long long nbad = (long long)s->hS[S_NBAD];
+        if (nbad > nbad_seen) {  // this iteration took Q^-1 r
+          nbad_seen = nbad;
+          if (--bad_left <= 0) {
+            lp_on = false;
+            rep->learned_fallback = it + 1;
+          }
+        }
+      }
       const double sq = s->hS[S_SQ], alpha = s->hS[S_TMP0];
       if (!std::isfinite(sq) || !std::isfinite(alpha)) {
         err = GS_E_NAN;

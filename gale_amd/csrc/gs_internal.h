@@ -41,6 +41,7 @@ enum Slot {
   S_UBETA = 10,  // Uzawa beta
   S_SQ = 11,     // s . q
   S_ORR = 12,    // Uzawa r . r
+  S_NBAD = 13,   // learned Uzawa: network outputs replaced by Q^-1 r (stored as double)
   S_TMP0 = 16,
   S_TMP1 = 17,
   S_TMP2 = 18,
@@ -136,6 +137,10 @@ void launch_fcg_p(double* p, const double* z, long long n, const double* S, hipS
 // Uzawa outer loop.
 void launch_uz_precond(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,
                        hipStream_t st);
+// launch_uz_precond only where S[S_ZR] (z.r of a network output) is not
+// positive and finite; counts those cases in S[S_NBAD].
+void launch_uz_precond_if_bad(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,
+                              hipStream_t st);
 void launch_uz_sdir(double* s, const double* z, long long n, const double* S, int first, hipStream_t st);
 void launch_uz_update(double* p, const double* s, double* r, const double* q, long long n, double* S,
                       Reducer rd, hipStream_t st);

@@ -362,6 +362,40 @@ void launch_uz_precond(const double* r, const double* Qinv, double* z, long long
   reduce_finish<1, false>(rd, nb, OpUzPrecond{S}, st);
 }
 
+// launch_uz_precond in place of a network output that is not a descent
+// direction: the blocks and the finishing op test S[S_ZR] (z.r of the
+// network pass) the same way, so either both do exactly the work and the
+// scalar updates of launch_uz_precond, counting the case in S[S_NBAD], or
+// neither changes z or the scalars.
+__device__ __forceinline__ bool uz_descent(double zr) { return zr > 0.0 && isfinite(zr); }
+__global__ void k_uz_precond_if_bad(const double* __restrict__ r, const double* __restrict__ Qinv,
+                                    double* __restrict__ z, long long n, double* S, Reducer rd) {
+  double v[1] = {0.0};
+  if (!uz_descent(S[S_ZR])) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x) {
+      const double zi = Qinv ? r[i] * Qinv[i] : r[i];
+      z[i] = zi;
+      v[0] += zi * r[i];
+    }
+  }
+  block_partials<1, false>(v, rd);
+}
+struct OpUzPrecondIfBad {
+  double* S;
+  __device__ void operator()(const double* t) const {
+    if (uz_descent(S[S_ZR])) return;
+    OpUzPrecond{S}(t);
+    S[S_NBAD] += 1.0;
+  }
+};
+void launch_uz_precond_if_bad(const double* r, const double* Qinv, double* z, long long n, double* S, Reducer rd,
+                              hipStream_t st) {
+  const int nb = reduce_grid(n);
+  hipLaunchKernelGGL(k_uz_precond_if_bad, dim3(nb), dim3(kBlock), 0, st, r, Qinv, z, n, S, rd);
+  reduce_finish<1, false>(rd, nb, OpUzPrecondIfBad{S}, st);
+}
+
 __global__ void k_uz_sdir(double* __restrict__ s, const double* __restrict__ z, long long n, const double* S,
                           int first) {
   const double beta = first ? 0.0 : S[S_UBETA];
