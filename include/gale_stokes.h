@@ -103,7 +103,8 @@ typedef struct gs_uzawa_params {
   double inner_rtol_factor;     /* inner rtol = factor * tolerance (reference: 0.1, :591,596) */
   double inner_rtol_override;   /* > 0: this rtol for every inner solve */
   int32_t monitor;              /* "monitor": print the outer residual to stderr every monitor iterations (0 = off) */
-  int32_t check_every;          /* inner CG convergence check interval (0 = every iteration; at most 2) */
+  int32_t check_every;          /* Jacobi inner CG convergence check interval (0 = 4); the multigrid
+                                   inner CG checks after every iteration */
   int32_t inner_preconditioner; /* GS_INNER_JACOBI or GS_INNER_MULTIGRID */
   int32_t mg_smooth;            /* Chebyshev-Jacobi steps before and after each coarse correction (0 = 2) */
   int32_t flexible_window;      /* learned preconditioner: previous directions each new one is
