@@ -428,15 +428,22 @@ __global__ __launch_bounds__(1024) void k_dense_solve_wide(const T* __restrict__
   constexpr int E = 16 / (int)sizeof(T);
   typedef T vec_t __attribute__((ext_vector_type(E)));
   __shared__ __align__(16) T sb[kDenseWideMax];
-  for (int j = threadIdx.x; j < nf; j += blockDim.x) sb[j] = b[freeidx[j]];
-  __syncthreads();
   const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (row >= nf) return;
-  const vec_t* r = reinterpret_cast<const vec_t*>(Kinv + (long long)row * ld);
+  const bool live = row < nf;
+  const vec_t* r = reinterpret_cast<const vec_t*>(Kinv + (long long)(live ? row : 0) * ld);
+  // the row's first segment is requested before the right-hand side is
+  // staged, and each later one while the previous one is being used (the
+  // products and their order are unchanged)
+  vec_t vn = {};
+  if (live && lane * E < nf) vn = r[lane];
+  for (int j = threadIdx.x; j < nf; j += blockDim.x) sb[j] = b[freeidx[j]];
+  __syncthreads();
+  if (!live) return;
   T s = 0;
   for (int j0 = lane * E; j0 < nf; j0 += 64 * E) {
-    const vec_t v = r[j0 / E];
+    const vec_t v = vn;
+    if (j0 + 64 * E < nf) vn = r[(j0 + 64 * E) / E];
 #pragma unroll
     for (int e = 0; e < E; e++)
       if (j0 + e < nf) s += v[e] * sb[j0 + e];
