@@ -749,8 +749,7 @@ def uzawa_params(tolerance=1e-5, maxIterations=1000, minIterations=1, useAbsolut
     ``inner_relax_gain``, 0 = 3) and then also selects one smoothing step on
     the multigrid levels below the finest (``mg_smooth_coarse``, 0 = that
     rule; otherwise ``mg_smooth``).  ``check_every`` paces the Jacobi inner
-    CG; the multigrid inner CG checks after every iteration.  Zero means the
-    library default for every count and factor here."""
+    CG; the multigrid inner CG checks after every iteration."""
     pc = PC_LEARNED if learned else (PC_DIAGONAL if Preconditioner else PC_NONE)
     return UzawaParams(tolerance, maxIterations, minIterations, int(bool(useAbsoluteTolerance)),
                        pc, precision, inner_max_iterations,
